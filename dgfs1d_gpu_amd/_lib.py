@@ -1,0 +1,79 @@
+"""ctypes binding of libdgfs_hip.so (the C ABI declared in include/dgfs_hip.h).
+
+The library is the only compute path of this package: there is no CPU
+fallback.  ``lib()`` raises if the shared object is missing or a symbol is
+absent, and every wrapper raises ``DGFSError`` with the library's message when
+a call returns a non-zero status.
+"""
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, 'libdgfs_hip.so')
+
+F64, F32 = 0, 1
+VHS, VHS_GLL = 0, 1
+
+_vp, _i, _d, _sz = C.c_void_p, C.c_int, C.c_double, C.c_size_t
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+_szp = C.POINTER(C.c_size_t)
+
+# name -> (restype, argtypes); the authoritative list of exported entry points
+SIGNATURES = {
+    'dgfs_last_error': (C.c_char_p, []),
+    'dgfs_collision_plan_create': (_i, [_i, _i, _i, _d, _dp, _dp, _dp, _d, _d, _d, _i, _i, _i,
+                                        C.POINTER(_vp)]),
+    'dgfs_collision_plan_destroy': (_i, [_vp]),
+    'dgfs_collision_plan_query': (_i, [_vp, _ip, _ip, _ip, _szp]),
+    'dgfs_collide': (_i, [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _vp]),
+}
+
+
+class DGFSError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def lib():
+    """Load (once) and return the ctypes handle; raise if unavailable."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise DGFSError(
+                'libdgfs_hip.so not found at %s: build it with '
+                '`python -m dgfs1d_gpu_amd.build` (no CPU fallback exists)' % LIB_PATH)
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = h
+    return _LIB
+
+
+def check(status):
+    if status != 0:
+        msg = lib().dgfs_last_error()
+        raise DGFSError((msg or b'unknown error').decode())
+
+
+def ptr(t):
+    """Device pointer of a contiguous torch tensor."""
+    if not t.is_contiguous():
+        raise DGFSError('tensor must be contiguous')
+    return C.c_void_p(t.data_ptr())
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def dbl_array(a):
+    import numpy as np
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)

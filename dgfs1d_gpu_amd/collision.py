@@ -1,0 +1,109 @@
+"""Batched fast-spectral collision operator on the GPU (host side).
+
+``CollisionPlan`` owns a ``dgfs_collision_plan_t`` (tables + scratch on the
+device) and a device-resident task table.  A *task* evaluates one collision
+integral Q(f_t, g_t), where f_t and g_t are linear combinations of slices of
+the coefficient array u (K, Ne, Nv) -- the U[m]^T / V[m] contractions of the
+SVD-decomposed H tensor (reference dgfs1D/basis.py:163-177, std/std.py:352-356)
+-- and accumulates prefac * (gain - f_t * loss) into one output slice.
+"""
+import numpy as np
+
+from . import _lib
+
+
+class CollisionPlan:
+    def __init__(self, N, M, Nrho, L, qz, qw, sz, sw, gamma, prefac, dtype, variant,
+                 ntask_max, device='cuda'):
+        import torch
+        self.torch = torch
+        self.N, self.M, self.Nrho = int(N), int(M), int(Nrho)
+        self.n = self.N ** 3
+        self.dtype = dtype
+        self.tdtype = torch.float64 if dtype == _lib.F64 else torch.float32
+        self.variant = variant
+        self.prefac = float(prefac)
+        self.device = device
+        L_ = _lib.lib()
+        qz_a, qz_p = _lib.dbl_array(qz)
+        qw_a, qw_p = _lib.dbl_array(qw)
+        sz_a, sz_p = _lib.dbl_array(np.asarray(sz).reshape(-1, 3))
+        if sz_a.shape[0] != self.M or len(qz_a) != self.Nrho or len(qw_a) != self.Nrho:
+            raise ValueError('inconsistent quadrature table sizes')
+        h = _lib.C.c_void_p()
+        _lib.check(L_.dgfs_collision_plan_create(
+            self.N, self.M, self.Nrho, float(L), qz_p, qw_p, sz_p, float(sw), float(gamma),
+            self.prefac, dtype, variant, int(ntask_max), _lib.C.byref(h)))
+        self._h = h
+        self.ntask_max = int(ntask_max)
+        self.tasks = None
+
+    def query(self):
+        s, qg, pg = _lib.C.c_int(), _lib.C.c_int(), _lib.C.c_int()
+        b = _lib.C.c_size_t()
+        _lib.check(_lib.lib().dgfs_collision_plan_query(
+            self._h, _lib.C.byref(s), _lib.C.byref(qg), _lib.C.byref(pg), _lib.C.byref(b)))
+        return dict(split=s.value, qgroups=qg.value, pgroups=pg.value, scratch_bytes=b.value)
+
+    def set_tasks(self, K, Ne, tasks):
+        """tasks: list of (out_slice, f_terms, g_terms), terms = [(slice, coef), ...].
+        Output slices are u-layout slice indices mode*Ne + elem."""
+        torch = self.torch
+        tasks = sorted(tasks, key=lambda t: t[0])
+        T = len(tasks)
+        if T > self.ntask_max:
+            raise ValueError('too many collision tasks (%d > %d)' % (T, self.ntask_max))
+        nterm = np.zeros((T, 2), np.int32)
+        sl = np.zeros((T, 2, K), np.int32)
+        cf = np.zeros((T, 2, K), np.float64)
+        gstart, gslice = [0], []
+        fg_same = True
+        for t, (out, fterms, gterms) in enumerate(tasks):
+            for w, terms in enumerate((fterms, gterms)):
+                if len(terms) > K:
+                    raise ValueError('more than K terms in a task')
+                nterm[t, w] = len(terms)
+                for a, (s, c) in enumerate(terms):
+                    sl[t, w, a] = s
+                    cf[t, w, a] = c
+            if sorted(fterms) != sorted(gterms):
+                fg_same = False
+            if t > 0 and out != tasks[t - 1][0]:
+                gstart.append(t)
+                gslice.append(tasks[t - 1][0])
+        if T:
+            gstart.append(T)
+            gslice.append(tasks[-1][0])
+        dev = self.device
+        self.tasks = dict(
+            K=int(K), Ne=int(Ne), ntask=T, ngroup=len(gslice), fg_same=int(fg_same),
+            nterm=torch.tensor(nterm, dtype=torch.int32, device=dev),
+            slice=torch.tensor(sl, dtype=torch.int32, device=dev),
+            coef=torch.tensor(cf, dtype=self.tdtype, device=dev),
+            gstart=torch.tensor(np.array(gstart, np.int32), device=dev),
+            gslice=torch.tensor(np.array(gslice, np.int32), device=dev))
+        return self.tasks
+
+    def __call__(self, u, Q, stream=None):
+        """Q += collision(u) for every task (stream-ordered, no host sync)."""
+        t = self.tasks
+        if t is None:
+            raise RuntimeError('set_tasks() must be called first')
+        if u.dtype != self.tdtype or Q.dtype != self.tdtype:
+            raise TypeError('u and Q must be %s' % self.tdtype)
+        _lib.check(_lib.lib().dgfs_collide(
+            self._h, _lib.ptr(u), _lib.ptr(Q), t['K'], t['Ne'], t['ntask'],
+            _lib.ptr(t['nterm']), _lib.ptr(t['slice']), _lib.ptr(t['coef']), t['ngroup'],
+            _lib.ptr(t['gstart']), _lib.ptr(t['gslice']), t['fg_same'],
+            _lib.stream_ptr(stream)))
+
+    def close(self):
+        if getattr(self, '_h', None) is not None and self._h.value:
+            _lib.lib().dgfs_collision_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,194 @@
+// Collision plan (tables, scratch) and the C ABI entry points.  The kernels
+// live in collision_kernels.h; each (dtype, N) instance is compiled in its own
+// translation unit (coll_<dtype>_<N>.hip) so the library builds in parallel.
+#include "collision_kernels.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace dgfs {
+
+static thread_local std::string g_err;
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+const char* last_error() { return g_err.c_str(); }
+
+template <typename T>
+static int dispatch(CollisionPlan* pl, const void* u, void* Q, TaskView tv, int ntask,
+                    int ngroup, const int* gs, const int* gsl, int fgs, hipStream_t st) {
+    switch (pl->N) {
+        case 4: return collide_launch<T, 4>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        case 6: return collide_launch<T, 6>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        case 8: return collide_launch<T, 8>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        case 12: return collide_launch<T, 12>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        case 16: return collide_launch<T, 16>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        case 20: return collide_launch<T, 20>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        case 24: return collide_launch<T, 24>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        case 32: return collide_launch<T, 32>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        default: break;
+    }
+    set_error("collision: unsupported velocity grid N=%d", pl->N);
+    return 2;
+}
+
+static bool supported_N(int N) {
+    return N == 4 || N == 6 || N == 8 || N == 12 || N == 16 || N == 20 || N == 24 || N == 32;
+}
+
+template <typename T>
+static int split_for(int N) { return pick_split<T>(N); }
+
+}  // namespace dgfs
+
+// ================================================================ C ABI
+using dgfs::CollisionPlan;
+
+extern "C" const char* dgfs_last_error(void) { return dgfs::last_error(); }
+
+extern "C" int dgfs_collision_plan_create(int N, int M, int Nrho, double L, const double* qz,
+                                          const double* qw, const double* sz, double sw,
+                                          double gamma, double prefac, int dtype, int variant,
+                                          int ntask_max, dgfs_collision_plan_t* out) {
+    DGFS_REQUIRE(out != nullptr, "collision_plan_create: null output handle");
+    *out = nullptr;
+    DGFS_REQUIRE(dgfs::supported_N(N), "collision_plan_create: unsupported N=%d", N);
+    DGFS_REQUIRE(M > 0 && Nrho > 0 && ntask_max > 0, "collision_plan_create: bad sizes");
+    DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "collision_plan_create: bad dtype");
+    DGFS_REQUIRE(variant == DGFS_VHS || variant == DGFS_VHS_GLL,
+                 "collision_plan_create: bad variant");
+    DGFS_REQUIRE(qz && qw && sz, "collision_plan_create: null table");
+
+    auto* pl = new CollisionPlan();
+    pl->N = N; pl->M = M; pl->Nrho = Nrho; pl->dtype = dtype; pl->variant = variant;
+    pl->ntask_max = ntask_max; pl->prefac = prefac;
+    const size_t n = size_t(N) * N * N;
+    pl->n = n;
+    const int S = dtype == DGFS_F64 ? dgfs::split_for<double>(N) : dgfs::split_for<float>(N);
+    pl->SA = pl->SB = pl->SC = pl->SD = S;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) pl->num_cu = prop.multiProcessorCount;
+    // q-groups: enough gain work-groups to give every CU ~3 of them
+    {
+        const long base = long(ntask_max) * Nrho * S;
+        int QG = 1;
+        while (QG < M && base * QG < 3L * pl->num_cu) ++QG;
+        pl->QG = QG;
+        int PG = 1;
+        const long cb = long(ntask_max) * S;
+        while (PG < Nrho && cb * PG < 2L * pl->num_cu) ++PG;
+        pl->PG = PG;
+    }
+
+    // ---- host tables in FP64 (vhs-gll.mako:43-99 restated with own code)
+    std::vector<double> l0(N);
+    for (int i = 0; i < N; ++i) l0[i] = (i < N / 2) ? double(i) : double(i - N);
+    std::vector<double> ph(size_t(Nrho) * M * 3 * N * 2);
+    for (int p = 0; p < Nrho; ++p) {
+        const double fac = M_PI / L * qz[p] / 2.0;
+        for (int q = 0; q < M; ++q)
+            for (int ax = 0; ax < 3; ++ax)
+                for (int i = 0; i < N; ++i) {
+                    const double a = fac * sz[q * 3 + ax] * l0[i];
+                    const size_t o = (((size_t(p) * M + q) * 3 + ax) * N + i) * 2;
+                    ph[o] = std::cos(a);
+                    ph[o + 1] = std::sin(a);
+                }
+    }
+    auto sinc = [](double s) { return std::sin(s + 1e-15) / (s + 1e-15); };
+    std::vector<double> wt(size_t(Nrho) * n), b2(n, 0.0);
+    for (int i1 = 0; i1 < N; ++i1)
+        for (int i2 = 0; i2 < N; ++i2)
+            for (int i3 = 0; i3 < N; ++i3) {
+                const double ln = std::sqrt(l0[i1] * l0[i1] + l0[i2] * l0[i2] + l0[i3] * l0[i3]);
+                const size_t o = (size_t(i3) * N + i1) * N + i2;   // [l3][l1][l2]
+                double bsum = 0.0;
+                for (int p = 0; p < Nrho; ++p) {
+                    const double zg = std::pow(qz[p], gamma + 2.0);
+                    const double bb1 = zg * 4.0 * M_PI * sinc(M_PI / L * qz[p] / 2.0 * ln);
+                    wt[size_t(p) * n + o] = 2.0 * qw[p] * sw / double(n) * bb1;
+                    bsum += qw[p] * zg * 16.0 * M_PI * M_PI * sinc(M_PI / L * qz[p] * ln);
+                }
+                b2[o] = bsum;
+            }
+    const size_t es = dtype == DGFS_F64 ? 8 : 4;
+    auto upload = [&](void** dptr, const std::vector<double>& h) -> int {
+        DGFS_HIP_CHECK(hipMalloc(dptr, h.size() * es));
+        if (es == 8) {
+            DGFS_HIP_CHECK(hipMemcpy(*dptr, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(h.begin(), h.end());
+            DGFS_HIP_CHECK(hipMemcpy(*dptr, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+        }
+        return 0;
+    };
+    int rc = upload(&pl->phase, ph);
+    if (!rc) rc = upload(&pl->wtab, wt);
+    if (!rc) rc = upload(&pl->bb2, b2);
+    if (rc) { dgfs_collision_plan_destroy(pl); return rc; }
+    const size_t cs = 2 * es;
+    hipError_t e1 = hipMalloc(&pl->Fhat, size_t(ntask_max) * 2 * n * cs);
+    hipError_t e2 = hipMalloc(&pl->acc, size_t(ntask_max) * Nrho * pl->QG * n * cs);
+    hipError_t e3 = hipMalloc(&pl->Gp, size_t(ntask_max) * pl->PG * n * cs);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        dgfs::set_error("collision_plan_create: scratch allocation failed");
+        dgfs_collision_plan_destroy(pl);
+        return 1;
+    }
+    *out = pl;
+    return 0;
+}
+
+extern "C" int dgfs_collision_plan_destroy(dgfs_collision_plan_t plan) {
+    auto* pl = static_cast<CollisionPlan*>(plan);
+    if (!pl) return 0;
+    for (void* p : {pl->phase, pl->wtab, pl->bb2, pl->Fhat, pl->acc, pl->Gp})
+        if (p) (void)hipFree(p);
+    delete pl;
+    return 0;
+}
+
+extern "C" int dgfs_collision_plan_query(dgfs_collision_plan_t plan, int* split, int* qgroups,
+                                         int* pgroups, size_t* scratch_bytes) {
+    auto* pl = static_cast<CollisionPlan*>(plan);
+    DGFS_REQUIRE(pl, "collision_plan_query: null plan");
+    const size_t cs = pl->dtype == DGFS_F64 ? 16 : 8;
+    if (split) *split = pl->SB;
+    if (qgroups) *qgroups = pl->QG;
+    if (pgroups) *pgroups = pl->PG;
+    if (scratch_bytes)
+        *scratch_bytes = size_t(pl->ntask_max) * pl->n * cs * (2 + pl->Nrho * pl->QG + pl->PG);
+    return 0;
+}
+
+extern "C" int dgfs_collide(dgfs_collision_plan_t plan, const void* u, void* Q, int K, int Ne,
+                            int ntask, const int* task_nterm, const int* task_slice,
+                            const void* task_coef, int ngroup, const int* group_start,
+                            const int* group_slice, int fg_same, hipStream_t stream) {
+    auto* pl = static_cast<CollisionPlan*>(plan);
+    DGFS_REQUIRE(pl, "collide: null plan");
+    DGFS_REQUIRE(ntask >= 0 && ntask <= pl->ntask_max, "collide: ntask=%d exceeds plan max %d",
+                 ntask, pl->ntask_max);
+    DGFS_REQUIRE(ngroup >= 0 && ngroup <= ntask, "collide: bad ngroup=%d", ngroup);
+    DGFS_REQUIRE(K > 0 && Ne > 0, "collide: bad K/Ne");
+    if (ntask == 0) return 0;
+    DGFS_REQUIRE(u && Q && task_nterm && task_slice && task_coef && group_start && group_slice,
+                 "collide: null pointer argument");
+    if (pl->variant == DGFS_VHS_GLL) fg_same = 1;   // vhs-gll evaluates Q(f, f)
+    dgfs::TaskView tv{K, Ne, int(pl->n), task_nterm, task_slice, task_coef};
+    if (pl->dtype == DGFS_F64)
+        return dgfs::dispatch<double>(pl, u, Q, tv, ntask, ngroup, group_start, group_slice,
+                                      fg_same, stream);
+    return dgfs::dispatch<float>(pl, u, Q, tv, ntask, ngroup, group_start, group_slice, fg_same,
+                                 stream);
+}

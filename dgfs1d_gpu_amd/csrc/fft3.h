@@ -1,0 +1,245 @@
+// Register-resident mixed-radix DFTs and the LDS-staged "partial 3D FFT"
+// engine shared by every collision kernel.
+//
+// RegDFT<T,L,DIR>: out[k] = sum_j in[S*j+O] * exp(DIR*2*pi*i*j*k/L), natural
+// order, fully unrolled (radix 4/2/3/5/7 decimation in time).  All twiddles
+// are compile-time literals (cx_unit), trivial angles are special-cased.
+//
+// Fft3Engine<T,N,S,NT,DIR>: one work-group transforms an N^3 complex volume
+// X[a1][a2][a3] along a3, a2, a1 (in that order) but keeps only the outputs
+// whose a3-frequency/point o3 satisfies o3 = S*j + r (residue r = per-WG),
+// which costs 1/S of a full transform (decimation in frequency on the first
+// axis).  Pass 1 streams pencils from global memory through a caller load
+// functor, pass 2 and pass 3 work in LDS (N*N*(N/S+1) complex), and pass 3
+// hands each thread one finished a1-pencil (o1 = 0..N-1) in registers.
+#pragma once
+
+#include "dgfs_common.h"
+
+namespace dgfs {
+
+// ------------------------------------------------------------ twiddle multiply
+// a * exp(2*pi*i*E/L) with E, L compile-time.
+template <typename T, long E, long L>
+DGFS_DEV cplx<T> twiddle(cplx<T> a) {
+    constexpr long e = ((E % L) + L) % L;
+    if constexpr (e == 0) {
+        return a;
+    } else if constexpr (2 * e == L) {
+        return mk(-a.x, -a.y);
+    } else if constexpr (4 * e == L) {
+        return mul_i<1>(a);
+    } else if constexpr (4 * e == 3 * L) {
+        return mul_i<-1>(a);
+    } else if constexpr ((8 * e) % L == 0) {
+        // odd multiple of pi/4: (+-1 +- i)/sqrt(2)
+        constexpr cs_pair w = cx_unit(e, L);
+        constexpr T h = T(0.70710678118654752440084436210485);
+        constexpr T sc = w.c > 0 ? h : -h;
+        constexpr T ss = w.s > 0 ? h : -h;
+        // (x + iy)(sc + i ss)
+        if constexpr (sc == ss) {
+            return mk(sc * (a.x - a.y), sc * (a.x + a.y));
+        } else {
+            return mk(sc * (a.x + a.y), sc * (a.y - a.x));
+        }
+    } else {
+        constexpr cs_pair w = cx_unit(e, L);
+        return a * mk(T(w.c), T(w.s));
+    }
+}
+
+// --------------------------------------------------------------- butterflies
+template <typename T, int R, int DIR>
+DGFS_DEV void butterfly(cplx<T>* t) {
+    if constexpr (R == 1) {
+    } else if constexpr (R == 2) {
+        cplx<T> a = t[0], b = t[1];
+        t[0] = a + b;
+        t[1] = a - b;
+    } else if constexpr (R == 4) {
+        cplx<T> a = t[0] + t[2], b = t[0] - t[2];
+        cplx<T> c = t[1] + t[3], d = mul_i<DIR>(t[1] - t[3]);
+        t[0] = a + c;
+        t[2] = a - c;
+        t[1] = b + d;
+        t[3] = b - d;
+    } else if constexpr (R == 3) {
+        constexpr T h = T(0.86602540378443864676372317075294);  // sqrt(3)/2
+        cplx<T> s = t[1] + t[2], d = t[1] - t[2];
+        cplx<T> m = t[0] - T(0.5) * s;
+        cplx<T> e = mul_i<DIR>(h * d);
+        t[0] = t[0] + s;
+        t[1] = m + e;
+        t[2] = m - e;
+    } else {
+        // direct R-point DFT (R = 5, 7, other primes)
+        cplx<T> in[R];
+        sfor<0, R>([&](auto I) { in[I] = t[I]; });
+        sfor<0, R>([&](auto K) {
+            constexpr int k = K;
+            cplx<T> acc = in[0];
+            sfor<1, R>([&](auto J) {
+                constexpr int j = J;
+                acc = acc + twiddle<T, long(DIR) * j * k, R>(in[j]);
+            });
+            t[k] = acc;
+        });
+    }
+}
+
+// ---------------------------------------------------------- register DFT
+template <typename T, int L, int DIR>
+struct RegDFT {
+    // out[k] = sum_{j<L} in[S*j+O] * w_L^{DIR j k}
+    template <int S, int O>
+    static DGFS_DEV void run(const cplx<T>* in, cplx<T>* out) {
+        if constexpr (L == 1) {
+            out[0] = in[O];
+        } else {
+            constexpr int R = smallest_radix(L);
+            constexpr int L2 = L / R;
+            if constexpr (L2 == 1) {
+                cplx<T> t[R];
+                sfor<0, R>([&](auto I) { t[I] = in[O + S * I]; });
+                butterfly<T, R, DIR>(t);
+                sfor<0, R>([&](auto I) { out[I] = t[I]; });
+            } else {
+                cplx<T> sub[L];
+                sfor<0, R>([&](auto Ri) {
+                    constexpr int r = Ri;
+                    RegDFT<T, L2, DIR>::template run<S * R, O + S * r>(in, sub + r * L2);
+                });
+                sfor<0, L2>([&](auto Ki) {
+                    constexpr int k2 = Ki;
+                    cplx<T> t[R];
+                    sfor<0, R>([&](auto Ri) {
+                        constexpr int r = Ri;
+                        t[r] = twiddle<T, long(DIR) * r * k2, L>(sub[r * L2 + k2]);
+                    });
+                    butterfly<T, R, DIR>(t);
+                    sfor<0, R>([&](auto Ri) {
+                        constexpr int k1 = Ri;
+                        out[k2 + L2 * k1] = t[k1];
+                    });
+                });
+            }
+        }
+    }
+};
+
+// ------------------------------------------------------------- 3-pass engine
+template <typename T, int N_, int S_, int NT_, int DIR_>
+struct Fft3Engine {
+    static constexpr int N = N_;
+    static constexpr int S = S_;
+    static constexpr int NT = NT_;
+    static constexpr int DIR = DIR_;
+    static constexpr int NS = N / S;
+    static constexpr int NSP = NS + 1;          // padded LDS row (bank spread)
+    static constexpr int NPEN = N * NS;         // pass-2 columns == pass-3 pencils
+    static constexpr int LDS_ELEMS = N * N * NSP;
+    static_assert(N % S == 0, "S must divide N");
+    static_assert(NT >= NPEN, "one pass-3 pencil per thread");
+
+    static DGFS_DEV int vidx(int i1, int i2, int j) { return (i1 * N + i2) * NSP + j; }
+
+    // per-work-group twiddles of the residue-r split: wS[s] = w_S^{DIR s r},
+    // wN[m] = w_N^{DIR m r}
+    struct Split {
+        cplx<T> wS[S];
+        cplx<T> wN[NS];
+        DGFS_DEV void init(int r) {
+            sfor<0, S>([&](auto Si) {
+                constexpr int s = Si;
+                T sn, cs;
+                sincospi_t(T(2 * DIR * ((s * r) % S)) / T(S), &sn, &cs);
+                wS[s] = mk(cs, sn);
+            });
+            sfor<0, NS>([&](auto Mi) {
+                constexpr int m = Mi;
+                T sn, cs;
+                sincospi_t(T(2 * DIR * ((m * r) % N)) / T(N), &sn, &cs);
+                wN[m] = mk(cs, sn);
+            });
+        }
+        static DGFS_DEV void sincospi_t(float a, float* s, float* c) { sincospif(a, s, c); }
+        static DGFS_DEV void sincospi_t(double a, double* s, double* c) { sincospi(a, s, c); }
+    };
+
+    // Pass 1: pencils (i1,i2) along a3 -> outputs o3 = S*j + r into V[i1][i2][j].
+    // map1(p, i1, i2) assigns pencil p in [0, N*N) ; load(i1, i2, i3) -> cplx<T>
+    template <class Map1, class Load>
+    static DGFS_DEV void pass1(cplx<T>* V, const Split& sp, Map1&& map1, Load&& load) {
+        for (int p = threadIdx.x; p < N * N; p += NT) {
+            int i1, i2;
+            map1(p, i1, i2);
+            cplx<T> a[N];
+            sfor<0, N>([&](auto I) { a[I] = load(i1, i2, int(I)); });
+            cplx<T> b[NS];
+            if constexpr (S == 1) {
+                sfor<0, N>([&](auto I) { b[I] = a[I]; });
+            } else {
+                sfor<0, NS>([&](auto Mi) {
+                    constexpr int m = Mi;
+                    cplx<T> acc = a[m];
+                    sfor<1, S>([&](auto Si) {
+                        constexpr int s = Si;
+                        acc = acc + a[m + s * NS] * sp.wS[s];
+                    });
+                    b[m] = (m == 0) ? acc : acc * sp.wN[m];
+                });
+            }
+            cplx<T> o[NS];
+            RegDFT<T, NS, DIR>::template run<1, 0>(b, o);
+            sfor<0, NS>([&](auto J) { V[vidx(i1, i2, J)] = o[J]; });
+        }
+    }
+
+    // Pass 2: columns (i1, j) along a2, in place. pre2(i2_const, v) -> v'
+    template <class Pre2>
+    static DGFS_DEV void pass2(cplx<T>* V, Pre2&& pre2) {
+        for (int c = threadIdx.x; c < NPEN; c += NT) {
+            const int i1 = c / NS, j = c - (c / NS) * NS;
+            cplx<T> a[N], o[N];
+            sfor<0, N>([&](auto I) { a[I] = pre2(I, V[vidx(i1, I, j)]); });
+            RegDFT<T, N, DIR>::template run<1, 0>(a, o);
+            sfor<0, N>([&](auto I) { V[vidx(i1, I, j)] = o[I]; });
+        }
+    }
+
+    // Pass 3: pencil (o2, j) along a1 for this thread (tid < NPEN).
+    // map3(P, o2, j); pre3(i1_const, v); result o[o1] in registers.
+    template <class Map3, class Pre3>
+    static DGFS_DEV void pass3(const cplx<T>* V, int P, Map3&& map3, Pre3&& pre3,
+                               int& o2, int& j, cplx<T>* o) {
+        map3(P, o2, j);
+        cplx<T> a[N];
+        sfor<0, N>([&](auto I) { a[I] = pre3(I, V[vidx(I, o2, j)]); });
+        RegDFT<T, N, DIR>::template run<1, 0>(a, o);
+    }
+
+    // Whole transform; epi(o2, j, o[]) is called by threads tid < NPEN.
+    template <class Map1, class Load, class Pre2, class Map3, class Pre3, class Epi>
+    static DGFS_DEV void run(cplx<T>* V, const Split& sp, Map1&& map1, Load&& load,
+                             Pre2&& pre2, Map3&& map3, Pre3&& pre3, Epi&& epi) {
+        __syncthreads();
+        pass1(V, sp, map1, load);
+        __syncthreads();
+        pass2(V, pre2);
+        __syncthreads();
+        if (threadIdx.x < NPEN) {
+            int o2, j;
+            cplx<T> o[N];
+            pass3(V, threadIdx.x, map3, pre3, o2, j, o);
+            epi(o2, j, o);
+        }
+    }
+};
+
+struct IdentityPre {
+    template <class I, class V>
+    DGFS_DEV V operator()(I, V v) const { return v; }
+};
+
+}  // namespace dgfs

@@ -1,0 +1,91 @@
+"""GPU parity of the batched collision operator against the FP64 oracle.
+
+The oracle (oracle/collision.py) restates std/scattering.py:251-313 ('vhs-gll')
+and :346-429 ('vhs'); it is itself pinned by the brute-force spectral sum
+(tests/test_oracle_collision.py).  Tolerances (relative to max|Q|):
+FP64 1e-11, FP32 2e-5.
+"""
+import numpy as np
+import pytest
+
+from oracle import collision as OC
+from oracle import velocitymesh as OVM
+
+pytestmark = pytest.mark.gpu
+
+
+def _mesh(N, M, Nrho):
+    return OVM.VelocityMesh(H0=1e-3, T0=273.0, rho0=2.3e-4, molarMass0=39.96e-3, Nv=N,
+                            Nrho=Nrho, cmax=0.0, Tmax=273.0, dev=5.0, M=M)
+
+
+def _field(vm, seed, u=(0.3, 0.0, 0.0), T=0.8):
+    rng = np.random.default_rng(seed)
+    c = vm.cv - np.array(u)[:, None]
+    f = np.exp(-(c * c).sum(0) / T) / (np.pi * T) ** 1.5
+    return f * (1.0 + 0.05 * rng.uniform(-1.0, 1.0, f.shape))
+
+
+def _run(cuda, N, M, Nrho, dtype, variant, tasks_spec, K=2, Ne=2, omega=0.81):
+    import torch
+    from dgfs1d_gpu_amd import _lib
+    from dgfs1d_gpu_amd.collision import CollisionPlan
+    vm = _mesh(N, M, Nrho)
+    gamma = 2.0 * (1.0 - omega)
+    pre = OC.Precomp(N, vm.L, vm.qz, vm.qw, vm.sz, vm.sw, gamma)
+    prefac = 0.37
+    n = N ** 3
+    u = np.stack([_field(vm, 10 + s, u=(0.3 - 0.2 * s, 0.1 * s, 0.0), T=0.8 + 0.1 * s)
+                  for s in range(K * Ne)])
+    q0 = np.random.default_rng(99).standard_normal((K * Ne, n)) * 1e-3
+    plan = CollisionPlan(N, M, Nrho, vm.L, vm.qz, vm.qw, vm.sz, vm.sw, gamma, prefac,
+                         dtype, variant, ntask_max=len(tasks_spec), device=cuda)
+    plan.set_tasks(K, Ne, tasks_spec)
+    td = torch.float64 if dtype == _lib.F64 else torch.float32
+    ud = torch.tensor(u, dtype=td, device=cuda)
+    Qd = torch.tensor(q0, dtype=td, device=cuda)
+    plan(ud, Qd)
+    torch.cuda.synchronize()
+    got = Qd.double().cpu().numpy()
+    ref = q0.copy()
+    for out, ft, gt in tasks_spec:
+        f = sum(c * u[s] for s, c in ft)
+        g = sum(c * u[s] for s, c in gt)
+        if variant == _lib.VHS:
+            ref[out] += OC.q_vhs(pre, f, g, prefac)
+        else:
+            ref[out] += OC.q_vhs_gll(pre, f, prefac)
+    scale = np.abs(ref - q0).max()
+    err = np.abs(got - ref).max() / scale
+    plan.close()
+    return err
+
+
+NODAL = [(s, [(s, 1.0)], [(s, 1.0)]) for s in range(4)]
+BILINEAR = [(0, [(0, 1.0)], [(1, 1.0)]), (1, [(2, 0.5), (3, -0.25)], [(1, 1.0), (0, 0.3)]),
+            (1, [(3, 1.0)], [(2, -1.0)]), (3, [(1, 0.7)], [(1, 0.7), (2, 0.1)])]
+
+
+@pytest.mark.parametrize('N,M,Nrho', [(8, 6, 4), (16, 6, 8)])
+@pytest.mark.parametrize('variant', [0, 1])
+def test_collision_fp64_nodal(cuda, N, M, Nrho, variant):
+    from dgfs1d_gpu_amd import _lib
+    err = _run(cuda, N, M, Nrho, _lib.F64, variant, NODAL)
+    print('fp64 nodal', N, variant, err)
+    assert err < 1e-11
+
+
+@pytest.mark.parametrize('N,M,Nrho', [(6, 3, 4), (8, 6, 4), (12, 6, 6)])
+def test_collision_fp64_bilinear_grouped(cuda, N, M, Nrho):
+    from dgfs1d_gpu_amd import _lib
+    err = _run(cuda, N, M, Nrho, _lib.F64, _lib.VHS, BILINEAR)
+    print('fp64 bilinear', N, err)
+    assert err < 1e-11
+
+
+@pytest.mark.parametrize('N,M,Nrho', [(16, 6, 8), (24, 6, 6), (32, 6, 8)])
+def test_collision_fp32(cuda, N, M, Nrho):
+    from dgfs1d_gpu_amd import _lib
+    err = _run(cuda, N, M, Nrho, _lib.F32, _lib.VHS, NODAL)
+    print('fp32 nodal', N, err)
+    assert err < 2e-5
