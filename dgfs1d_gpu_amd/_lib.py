@@ -13,12 +13,33 @@ LIB_PATH = os.path.join(_PKG, 'libdgfs_hip.so')
 
 F64, F32 = 0, 1
 VHS, VHS_GLL = 0, 1
+BC_COPY, BC_DIFFUSE, BC_DIFFUSE_EXPR, BC_INLET, BC_VANISH = 0, 1, 2, 3, 4
 
 _vp, _i, _d, _sz = C.c_void_p, C.c_int, C.c_double, C.c_size_t
 _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
 _szp = C.POINTER(C.c_size_t)
 
+
+
+class Wall(C.Structure):
+    """dgfs_wall_t"""
+    _fields_ = [('kind', C.c_int), ('nl', C.c_double), ('ushift', C.c_double),
+                ('f0', C.c_void_p), ('ux', C.c_double), ('uy', C.c_double),
+                ('uz', C.c_double), ('T', C.c_double)]
+
+
+class Transport(C.Structure):
+    """dgfs_transport_t"""
+    _fields_ = [('dtype', C.c_int), ('K', C.c_int), ('Ne', C.c_int), ('Nv', C.c_int),
+                ('D', C.c_void_p), ('BL', C.c_void_p), ('BR', C.c_void_p),
+                ('gLD', C.c_void_p), ('gRD', C.c_void_p), ('invjac', C.c_void_p),
+                ('cx', C.c_void_p), ('cy', C.c_void_p), ('cz', C.c_void_p),
+                ('cw', C.c_double), ('left', Wall), ('right', Wall),
+                ('work', C.c_void_p), ('nw', C.c_void_p)]
+
+
+_l = C.c_long
 # name -> (restype, argtypes); the authoritative list of exported entry points
 SIGNATURES = {
     'dgfs_last_error': (C.c_char_p, []),
@@ -26,7 +47,16 @@ SIGNATURES = {
                                         C.POINTER(_vp)]),
     'dgfs_collision_plan_destroy': (_i, [_vp]),
     'dgfs_collision_plan_query': (_i, [_vp, _ip, _ip, _ip, _szp]),
-    'dgfs_collide': (_i, [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _vp]),
+    'dgfs_collide': (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _i,
+                          _vp]),
+    'dgfs_transport_work_bytes': (_sz, [_i]),
+    'dgfs_transport_rhs': (_i, [C.POINTER(Transport), _vp, _vp, _d, _vp]),
+    'dgfs_matop': (_i, [_i, _i, _i, _l, _vp, _vp, _vp, _i, _vp]),
+    'dgfs_axnpby': (_i, [_i, _l, _i, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _vp]),
+    'dgfs_moments_work_bytes': (_sz, [_i, _i]),
+    'dgfs_moments': (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
+    'dgfs_residual_work_bytes': (_sz, []),
+    'dgfs_residual': (_i, [_i, _l, _vp, _vp, _vp, _vp, _vp]),
 }
 
 
@@ -61,10 +91,21 @@ def check(status):
 
 
 def ptr(t):
-    """Device pointer of a contiguous torch tensor."""
+    """Device pointer of a contiguous torch tensor (None -> NULL)."""
+    if t is None:
+        return C.c_void_p(None)
     if not t.is_contiguous():
         raise DGFSError('tensor must be contiguous')
     return C.c_void_p(t.data_ptr())
+
+
+def dtype_code(torch_dtype):
+    import torch
+    if torch_dtype == torch.float64:
+        return F64
+    if torch_dtype == torch.float32:
+        return F32
+    raise DGFSError('unsupported dtype %s' % torch_dtype)
 
 
 def stream_ptr(stream=None):

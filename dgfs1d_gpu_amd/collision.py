@@ -84,17 +84,23 @@ class CollisionPlan:
             gslice=torch.tensor(np.array(gslice, np.int32), device=dev))
         return self.tasks
 
-    def __call__(self, u, Q, stream=None):
-        """Q += collision(u) for every task (stream-ordered, no host sync)."""
+    def __call__(self, u, Q, stream=None, ug=None):
+        """Q += collision(u) for every task (stream-ordered, no host sync).
+        g-terms read ``ug`` when given (the separate d_g array of fs())."""
         t = self.tasks
         if t is None:
             raise RuntimeError('set_tasks() must be called first')
-        if u.dtype != self.tdtype or Q.dtype != self.tdtype:
-            raise TypeError('u and Q must be %s' % self.tdtype)
+        for x in (u, Q) + ((ug,) if ug is not None else ()):
+            if x.dtype != self.tdtype:
+                raise TypeError('collision operands must be %s' % self.tdtype)
+        n = t['K'] * t['Ne'] * self.n
+        if u.numel() < n or Q.numel() < n or (ug is not None and ug.numel() < n):
+            raise ValueError('collision operands smaller than K*Ne*Nv')
+        fg_same = t['fg_same'] if ug is None else 0
         _lib.check(_lib.lib().dgfs_collide(
-            self._h, _lib.ptr(u), _lib.ptr(Q), t['K'], t['Ne'], t['ntask'],
+            self._h, _lib.ptr(u), _lib.ptr(ug), _lib.ptr(Q), t['K'], t['Ne'], t['ntask'],
             _lib.ptr(t['nterm']), _lib.ptr(t['slice']), _lib.ptr(t['coef']), t['ngroup'],
-            _lib.ptr(t['gstart']), _lib.ptr(t['gslice']), t['fg_same'],
+            _lib.ptr(t['gstart']), _lib.ptr(t['gslice']), fg_same,
             _lib.stream_ptr(stream)))
 
     def close(self):

@@ -23,17 +23,18 @@ void set_error(const char* fmt, ...) {
 const char* last_error() { return g_err.c_str(); }
 
 template <typename T>
-static int dispatch(CollisionPlan* pl, const void* u, void* Q, TaskView tv, int ntask,
-                    int ngroup, const int* gs, const int* gsl, int fgs, hipStream_t st) {
+static int dispatch(CollisionPlan* pl, const void* u, const void* ug, void* Q, TaskView tv,
+                    int ntask, int ngroup, const int* gs, const int* gsl, int fgs,
+                    hipStream_t st) {
     switch (pl->N) {
-        case 4: return collide_launch<T, 4>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
-        case 6: return collide_launch<T, 6>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
-        case 8: return collide_launch<T, 8>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
-        case 12: return collide_launch<T, 12>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
-        case 16: return collide_launch<T, 16>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
-        case 20: return collide_launch<T, 20>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
-        case 24: return collide_launch<T, 24>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
-        case 32: return collide_launch<T, 32>(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        case 4: return collide_launch<T, 4>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        case 6: return collide_launch<T, 6>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        case 8: return collide_launch<T, 8>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        case 12: return collide_launch<T, 12>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        case 16: return collide_launch<T, 16>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        case 20: return collide_launch<T, 20>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        case 24: return collide_launch<T, 24>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        case 32: return collide_launch<T, 32>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
         default: break;
     }
     set_error("collision: unsupported velocity grid N=%d", pl->N);
@@ -171,7 +172,8 @@ extern "C" int dgfs_collision_plan_query(dgfs_collision_plan_t plan, int* split,
     return 0;
 }
 
-extern "C" int dgfs_collide(dgfs_collision_plan_t plan, const void* u, void* Q, int K, int Ne,
+extern "C" int dgfs_collide(dgfs_collision_plan_t plan, const void* u, const void* ug, void* Q,
+                            int K, int Ne,
                             int ntask, const int* task_nterm, const int* task_slice,
                             const void* task_coef, int ngroup, const int* group_start,
                             const int* group_slice, int fg_same, hipStream_t stream) {
@@ -185,10 +187,12 @@ extern "C" int dgfs_collide(dgfs_collision_plan_t plan, const void* u, void* Q, 
     DGFS_REQUIRE(u && Q && task_nterm && task_slice && task_coef && group_start && group_slice,
                  "collide: null pointer argument");
     if (pl->variant == DGFS_VHS_GLL) fg_same = 1;   // vhs-gll evaluates Q(f, f)
+    if (ug == nullptr) ug = u;
+    if (ug != u) fg_same = 0;
     dgfs::TaskView tv{K, Ne, int(pl->n), task_nterm, task_slice, task_coef};
     if (pl->dtype == DGFS_F64)
-        return dgfs::dispatch<double>(pl, u, Q, tv, ntask, ngroup, group_start, group_slice,
-                                      fg_same, stream);
-    return dgfs::dispatch<float>(pl, u, Q, tv, ntask, ngroup, group_start, group_slice, fg_same,
-                                 stream);
+        return dgfs::dispatch<double>(pl, u, ug, Q, tv, ntask, ngroup, group_start,
+                                      group_slice, fg_same, stream);
+    return dgfs::dispatch<float>(pl, u, ug, Q, tv, ntask, ngroup, group_start, group_slice,
+                                 fg_same, stream);
 }

@@ -70,7 +70,8 @@ DGFS_DEV int xcd_remap(int bid, int nblk) {
 // ================================================================ kernel A
 template <typename T, int N, int S>
 __global__ void __launch_bounds__(round64(N * (N / S)))
-k_forward(const T* __restrict__ u, TaskView tv, int nfg, cplx<T>* __restrict__ Fhat) {
+k_forward(const T* __restrict__ u, const T* __restrict__ ug, TaskView tv, int nfg,
+          cplx<T>* __restrict__ Fhat) {
     using E = Fft3Engine<T, N, S, round64(N * (N / S)), -1>;
     constexpr int n = N * N * N;
     __shared__ cplx<T> V[E::LDS_ELEMS];
@@ -82,12 +83,13 @@ k_forward(const T* __restrict__ u, TaskView tv, int nfg, cplx<T>* __restrict__ F
     sp.init(r);
     const T inv_n = T(1) / T(n);
     cplx<T>* out = Fhat + (size_t(t) * 2 + which) * n;
+    const T* src = which ? ug : u;
     // abstract axes: a3 = x (pass 1, residue on l1), a2 = z, a1 = y
     E::run(V, sp,
         [](int p, int& i1, int& i2) { i1 = p / N; i2 = p - (p / N) * N; },  // (y, z)
         [&](int i1, int i2, int i3) {
             const int v = (i3 * N + i1) * N + i2;   // f[x][y][z]
-            return mk(task_value<T>(tv, u, t, which, v) * inv_n, T(0));
+            return mk(task_value<T>(tv, src, t, which, v) * inv_n, T(0));
         },
         IdentityPre{},
         [](int P, int& o2, int& j) { o2 = P / (N / S); j = P - (P / (N / S)) * (N / S); },
@@ -303,12 +305,12 @@ struct Impl {
     static constexpr int S = pick_split<T>(N);
     static constexpr int NT = round64(N * (N / S));
 
-    static int launch(CollisionPlan* pl, const void* u, void* Q, TaskView tv, int ntask,
-                      int ngroup, const int* gstart, const int* gslice, int fg_same,
+    static int launch(CollisionPlan* pl, const void* u, const void* ug, void* Q, TaskView tv,
+                      int ntask, int ngroup, const int* gstart, const int* gslice, int fg_same,
                       hipStream_t st) {
         const int nfg = fg_same ? 1 : 2;
         hipLaunchKernelGGL((k_forward<T, N, S>), dim3(ntask * nfg * S), dim3(NT), 0, st,
-                           (const T*)u, tv, nfg, (cplx<T>*)pl->Fhat);
+                           (const T*)u, (const T*)ug, tv, nfg, (cplx<T>*)pl->Fhat);
         DGFS_HIP_CHECK(hipGetLastError());
         const int nb = ntask * pl->Nrho * S * pl->QG;
         if (fg_same)
@@ -335,18 +337,19 @@ struct Impl {
 
 
 template <typename T, int N>
-int collide_launch(CollisionPlan* pl, const void* u, void* Q, TaskView tv, int ntask, int ngroup,
-                   const int* gs, const int* gsl, int fgs, hipStream_t st) {
-    return Impl<T, N>::launch(pl, u, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+int collide_launch(CollisionPlan* pl, const void* u, const void* ug, void* Q, TaskView tv,
+                   int ntask, int ngroup, const int* gs, const int* gsl, int fgs, hipStream_t st) {
+    return Impl<T, N>::launch(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
 }
 
 #define DGFS_COLLIDE_EXTERN(T, N)                                                        \
-    extern template int collide_launch<T, N>(CollisionPlan*, const void*, void*, TaskView, \
-                                             int, int, const int*, const int*, int,     \
+    extern template int collide_launch<T, N>(CollisionPlan*, const void*, const void*, void*, \
+                                             TaskView, int, int, const int*, const int*, int, \
                                              hipStream_t);
 #define DGFS_COLLIDE_INST(T, N)                                                          \
-    template int collide_launch<T, N>(CollisionPlan*, const void*, void*, TaskView, int,  \
-                                      int, const int*, const int*, int, hipStream_t);
+    template int collide_launch<T, N>(CollisionPlan*, const void*, const void*, void*,     \
+                                      TaskView, int, int, const int*, const int*, int,     \
+                                      hipStream_t);
 #define DGFS_COLLIDE_SIZES(X, T) X(T, 4) X(T, 6) X(T, 8) X(T, 12) X(T, 16) X(T, 20) X(T, 24) X(T, 32)
 
 DGFS_COLLIDE_SIZES(DGFS_COLLIDE_EXTERN, float)

@@ -62,8 +62,9 @@ int dgfs_collision_plan_query(dgfs_collision_plan_t plan, int* split, int* qgrou
  * of rhs() (dgfs1D/std/std.py:352-356) and fs() itself
  * (dgfs1D/std/scattering.py:251-313 'vhs-gll', :346-429 'vhs').
  * Task t evaluates Q(f_t, g_t) with
- *     f_t = sum_a coef[t][0][a] * u[slice[t][0][a]],  a < nterm[t][0]
- *     g_t = sum_b coef[t][1][b] * u[slice[t][1][b]],  b < nterm[t][1]
+ *     f_t = sum_a coef[t][0][a] * u [slice[t][0][a]],  a < nterm[t][0]
+ *     g_t = sum_b coef[t][1][b] * ug[slice[t][1][b]],  b < nterm[t][1]
+ * (ug = NULL means ug = u; a distinct ug is the d_f / d_g pair of fs()).
  * (the U[m]^T / V[m] contraction of the SVD-decomposed H tensor fused into
  * the load, basis.py:163-177) and tasks are grouped by output slice:
  *     Q[group_slice[g]] += sum_{t in [group_start[g], group_start[g+1])}
@@ -73,10 +74,88 @@ int dgfs_collision_plan_query(dgfs_collision_plan_t plan, int* split, int* qgrou
  * group_slice [ngroup] int.  fg_same != 0 declares f_t == g_t for every task
  * (halves the work; forced for DGFS_VHS_GLL).  Distinct groups must have
  * distinct output slices. */
-int dgfs_collide(dgfs_collision_plan_t plan, const void* u, void* Q, int K, int Ne, int ntask,
+int dgfs_collide(dgfs_collision_plan_t plan, const void* u, const void* ug, void* Q, int K,
+                 int Ne, int ntask,
                  const int* task_nterm, const int* task_slice, const void* task_coef,
                  int ngroup, const int* group_start, const int* group_slice, int fg_same,
                  hipStream_t stream);
+
+/* ---------------------------------------------------------------- transport
+ * Boundary kinds (dgfs1D/std/bc.py):
+ *   COPY          'dgfs-periodic' / 'dgfs-cyclic' on one rank: ghost = interior
+ *                 (bcs/dgfs-periodic.mako:12, dgfs-cyclic.mako:12)
+ *   DIFFUSE       'dgfs-wall-diffuse': ghost = fac<0 ? n_w f0 : interior,
+ *                 n_w = -sum_{fac>=0} fac f cw / sum_{fac<0} fac f0 cw   (bc.py:25-78)
+ *   DIFFUSE_EXPR  'dgfs-wall-expr-diffuse': as DIFFUSE with the unnormalised
+ *                 exp(-|c-u|^2/T) of (ux,uy,uz,T) evaluated in the kernel (bc.py:136-198)
+ *   INLET         'dgfs-inlet' / 'dgfs-inlet-nondim': ghost = fac<0 ? f0 : interior
+ *   VANISH        'dgfs-vanish': ghost = fac>=0 ? 0 : interior
+ * with fac = (c_x - ushift) * nl, nl = -1 (left) / +1 (right).  The n_w sums
+ * accumulate in FP64. */
+#define DGFS_BC_COPY 0
+#define DGFS_BC_DIFFUSE 1
+#define DGFS_BC_DIFFUSE_EXPR 2
+#define DGFS_BC_INLET 3
+#define DGFS_BC_VANISH 4
+
+typedef struct {
+    int kind;
+    double nl, ushift;
+    const void* f0;            /* device, Nv values of dtype (DIFFUSE, INLET) */
+    double ux, uy, uz, T;      /* DIFFUSE_EXPR, non-dimensional */
+} dgfs_wall_t;
+
+typedef struct {
+    int dtype, K, Ne, Nv;
+    const double* D;           /* device K*K: derivative operator, ux = D u (std.py:109-114) */
+    const double* BL;          /* device K: left-face reconstruction  Bqf[:,0]      */
+    const double* BR;          /* device K: right-face reconstruction Bqf[:,Nqf-1]  */
+    const double* gLD;         /* device K: lift vectors (basis.py gLD, gRD) */
+    const double* gRD;
+    const double* invjac;      /* device Ne: 2/h_e (mesh.py:121) */
+    const void* cx;            /* device Nv velocity components (dtype) */
+    const void* cy;
+    const void* cz;
+    double cw;                 /* velocity cell volume (velocitymesh.py:60) */
+    dgfs_wall_t left, right;
+    double* work;              /* device, dgfs_transport_work_bytes(Nv) */
+    double* nw;                /* device, 2 doubles: wall number densities */
+} dgfs_transport_t;
+
+size_t dgfs_transport_work_bytes(int Nv);
+
+/* Fused transport of rhs() (dgfs1D/std/std.py:283-348): face reconstruction
+ * (bwdTransFace + extract_left/right), boundary update/apply/insert,
+ * upwind flux, -c_x D u, lift with gLD/gRD and the 1/J scaling
+ * (dgfs1D/std/kernels/std.mako:4-190).  Writes ux (K, Ne, Nv); t is the
+ * stage time (kept for time-dependent boundaries evaluated on the host). */
+int dgfs_transport_rhs(const dgfs_transport_t* tp, const void* u, void* ux, double t,
+                       hipStream_t stream);
+
+/* out[k][j] (+)= sum_m A[k][m] in[m][j], j < ncol: the GiMMiK operators of
+ * dgfs1D/util.py:20-25 (fwdTrans, bwdTrans, invMass, U/V, interpolation).
+ * A is a device array of Kout*Kin doubles (row-major). */
+int dgfs_matop(int dtype, int Kout, int Kin, long ncol, const double* A, const void* in,
+               void* out, int accumulate, hipStream_t stream);
+
+/* x0 = a0 x0 + a1 x1 + a2 x2 + a3 x3 (first nv terms; a0 == 0 never reads x0)
+ * -- dgfs1D/axnpby.py:13-43 with the K sub-arrays flattened (n = K Ne Nv). */
+int dgfs_axnpby(int dtype, long n, int nv, void* x0, const void* x1, const void* x2,
+                const void* x3, double a0, double a1, double a2, double a3, hipStream_t stream);
+
+/* FP64 moments at Nqr reconstruction points per element
+ * (dgfs1D/std/moments.py:11-62, non-dimensional, before the dimensional
+ * scaling of :63-70).  im: device K*Nqr (interpMat); out: device
+ * [Nqr*Ne][9] = {rho, ux, uy, T, qx, qy, Pxx, Pyy, Pxy}, node = r*Ne + e. */
+size_t dgfs_moments_work_bytes(int Nv, int nnode);
+int dgfs_moments(int dtype, int K, int Ne, int Nqr, int Nv, const double* im, const void* u,
+                 const void* cx, const void* cy, const void* cz, double cw, double* work,
+                 double* out, hipStream_t stream);
+
+/* out2 = {sum (cur-prev)^2, sum prev^2} in FP64 (dgfs1D/std/residual.py:21-22). */
+size_t dgfs_residual_work_bytes(void);
+int dgfs_residual(int dtype, long n, const void* cur, const void* prev, double* work,
+                  double* out2, hipStream_t stream);
 
 #ifdef __cplusplus
 }

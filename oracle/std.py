@@ -1,0 +1,157 @@
+"""Oracle: the dgfsStd1D RHS, boundaries, integrators, moments (FP64).
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+Restates, with explicit face arrays as the reference kernels use them:
+  face maps mapL/mapR          dgfs1D/std/std.py:62-66
+  rhs()                        dgfs1D/std/std.py:281-366
+  extract/transfer/insert      dgfs1D/std/kernels/std.mako:4-90
+  flux / mul_by_adv /
+  totalFlux / mul_by_invjac    dgfs1D/std/kernels/std.mako:92-190
+  boundaries                   dgfs1D/std/bc.py:25-324 + std/kernels/bcs/*.mako
+  collision loop               dgfs1D/std/std.py:352-356
+  integrators                  dgfs1D/std/integrator.py:36-157
+  moments                      dgfs1D/std/moments.py:11-70
+  residual                     dgfs1D/std/residual.py:21-29
+"""
+import numpy as np
+
+from . import collision as OC
+
+
+def maxwellian(cv, rho, ux, uy, uz, T):
+    """std/initcond.py:56-58"""
+    u = np.array([ux, uy, uz]).reshape(3, 1)
+    return rho / (np.pi * T) ** 1.5 * np.exp(-((cv - u) ** 2).sum(0) / T)
+
+
+class Wall:
+    """kind: 'copy' | 'diffuse' | 'inlet' | 'vanish' (bc.py)."""
+
+    def __init__(self, kind, nl, cv, cw, f0=None, ushift=0.0):
+        self.kind, self.nl, self.cv, self.cw = kind, nl, cv, cw
+        self.f0, self.ushift = f0, ushift
+        self.nw = None
+
+    def apply(self, ul):
+        c = self.cv[0]
+        fac = (c - self.ushift) * self.nl
+        if self.kind == 'copy':
+            return ul.copy()
+        if self.kind == 'diffuse':                                  # wall-diffuse.mako:24-37
+            num = np.sum(np.where(fac >= 0, fac * ul * self.cw, 0.0))
+            den = np.sum(np.where(fac < 0, fac * self.f0 * self.cw, 0.0))
+            self.nw = -num / den                                    # bc.py:265-267
+            return np.where(fac < 0, self.f0 * self.nw, ul)         # mako:10-21
+        if self.kind == 'inlet':
+            return np.where(fac < 0, self.f0, ul)
+        if self.kind == 'vanish':
+            return np.where(fac >= 0, 0.0, ul)
+        raise ValueError(self.kind)
+
+
+class StdOracle:
+    """FP64 dgfsStd1D operator.  u has shape (K, Ne, n)."""
+
+    def __init__(self, basis, invjac, vm, wall_l, wall_r, pre, prefac, model='vhs'):
+        self.b, self.invjac, self.vm = basis, np.asarray(invjac).ravel(), vm
+        self.wl, self.wr = wall_l, wall_r
+        self.pre, self.prefac, self.model = pre, prefac, model
+        self.K = basis.K
+        self.Ne = len(self.invjac)
+        self.Nqf = basis.face.shape[0]
+
+    def transport(self, t, u):
+        K, Ne, Nqf = self.K, self.Ne, self.Nqf
+        n = u.shape[-1]
+        c = self.vm.cv[0]
+        usolF = (self.b.face @ u.reshape(K, -1)).reshape(Nqf * Ne, n)   # bwdTransFace
+        mapL = np.arange(Ne + 1) + (Nqf - 1) * Ne - 1
+        mapR = np.arange(Ne + 1)
+        mapL[0], mapR[-1] = 0, Ne * Nqf - 1
+        uL, uR = usolF[mapL].copy(), usolF[mapR].copy()                 # extract_left/right
+        bcL, bcR = uL[0].copy(), uR[Ne].copy()                          # transfer_bc_*
+        uL[0] = self.wl.apply(bcL)                                      # update/apply/insert
+        uR[Ne] = self.wr.apply(bcR)
+        jL = 0.5 * np.abs(c) * (uL - uR) + 0.5 * c * (uR - uL)          # flux
+        jR = 0.5 * np.abs(c) * (uL - uR) + 0.5 * c * (uL - uR)
+        ux = (self.b.deriv @ u.reshape(K, -1)).reshape(K, Ne, n)        # deriv
+        ux *= -c                                                        # mul_by_adv
+        for k in range(K):                                              # totalFlux
+            for e in range(Ne):
+                ux[k, e] += -jL[e + 1] * self.b.gRD[k] + jR[e] * self.b.gLD[k]
+        ux *= self.invjac[None, :, None]                                # mul_by_invjac
+        return ux
+
+    def collisions(self, u, ux):
+        K, Ne = self.K, self.Ne
+        for m in range(K):
+            f = np.einsum('ra,aen->ren', self.b.U[m].T, u)              # trans_U
+            g = np.einsum('rb,ben->ren', self.b.V[m], u)                # trans_V
+            for r in self.b.sig[m]:
+                for e in range(Ne):
+                    if self.model == 'vhs':
+                        ux[m, e] += OC.q_vhs(self.pre, f[r, e], g[r, e], self.prefac)
+                    else:
+                        ux[m, e] += OC.q_vhs_gll(self.pre, f[r, e], self.prefac)
+        return ux
+
+    def rhs(self, t, u):
+        ux = self.collisions(u, self.transport(t, u))
+        K = self.K
+        return (self.b.invmass @ ux.reshape(K, -1)).reshape(ux.shape)   # invMass
+
+
+def integrate(op, scheme, t, dt, u):
+    """One step of the explicit schemes (std/integrator.py:36-157)."""
+    if scheme == 'euler':
+        return u + dt * op.rhs(t, u)
+    if scheme == 'ssp-rk2':
+        r0 = u + dt * op.rhs(t, u)
+        return 0.5 * u + 0.5 * r0 + 0.5 * dt * op.rhs(t + dt, r0)
+    if scheme == 'ssp-rk3':
+        r0 = u + dt * op.rhs(t, u)
+        r0 = 0.75 * u + 0.25 * r0 + 0.25 * dt * op.rhs(t + dt, r0)
+        return u / 3.0 + 2.0 / 3.0 * r0 + 2.0 / 3.0 * dt * op.rhs(t + 0.5 * dt, r0)
+    if scheme in ('lse-rk45', 'lesrk-45'):
+        a = [0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+             -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0]
+        b = [1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+             1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+             2277821191437.0 / 14882151754819.0]
+        c = [0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+             2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0]
+        r1 = np.zeros_like(u)
+        for i in range(5):
+            r1 = a[i] * r1 + dt * op.rhs(t + c[i] * dt, u)
+            u = u + b[i] * r1
+        return u
+    raise ValueError(scheme)
+
+
+def moments(u, interp, vm):
+    """Non-dimensional (Nqr, Ne, 9): rho, ux, uy, T, qx, qy, Pxx, Pyy, Pxy
+    (std/moments.py:26-61; u_z is taken as 0 in the peculiar velocity)."""
+    cv, cw = vm.cv, vm.cw
+    s = np.einsum('mr,mej->rej', interp, u)
+    out = np.zeros(s.shape[:2] + (9,))
+    rho = s.sum(-1) * cw
+    ux = (s @ cv[0]) * cw / rho
+    uy = (s @ cv[1]) * cw / rho
+    cx = cv[0][None, None, :] - ux[..., None]
+    cy = cv[1][None, None, :] - uy[..., None]
+    cz = cv[2][None, None, :]
+    c2 = cx * cx + cy * cy + cz * cz
+    out[..., 0], out[..., 1], out[..., 2] = rho, ux, uy
+    out[..., 3] = (s * c2).sum(-1) * (2.0 / 3.0 * cw) / rho
+    out[..., 4] = (s * c2 * cx).sum(-1) * cw
+    out[..., 5] = (s * c2 * cy).sum(-1) * cw
+    out[..., 6] = 2 * (s * cx * cx).sum(-1) * cw
+    out[..., 7] = 2 * (s * cy * cy).sum(-1) * cw
+    out[..., 8] = 2 * (s * cx * cy).sum(-1) * cw
+    return out
+
+
+def residual(cur, prev):
+    d = cur - prev
+    return np.sqrt(np.sum(d * d) / np.sum(prev * prev))
