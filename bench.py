@@ -1,0 +1,147 @@
+"""Benchmark: phase-space updates/s of the production Couette case on one MI355X.
+
+Workload (dgfs1d_gpu_amd/configs/couette_prod.ini): single-species VHS
+Couette, Ne=4, K=3 (nodal-sem-gll), N=32^3, M=12, N_rho=16, SSP-RK2,
+dt=5e-4, FP32 spectral arithmetic, FP64 wall reductions.  One step = one
+SSP-RK2 time step (2 RHS evaluations, each with 12 batched collision
+integrals).  Inputs are resident in HBM before timing starts.
+
+    python bench.py [--gpus 1] [--steps 52] [--warmup 3] [--no-cpu-baseline]
+
+Prints one JSON line.  ``roofline`` is for the dominant kernel (the gain
+kernel of the collision operator), timed with HIP events recorded around it
+on its launch stream over the timed region.  ``cpu_baseline`` is the FP64
+NumPy/SciPy oracle on the host cores, timed on a bounded sample (2 collision
+integrals + 1 transport evaluation of the same case) and scaled to one step.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3      # MI355X published dense FP32
+PEAK_HBM_GBS = 8000.0
+
+
+def fft_flops(n):
+    return 5.0 * n * math.log2(n)
+
+
+def cpu_baseline(deck_text, cores, nq=2):
+    import numpy as np
+    from oracle import collision as OC
+    from oracle.deck import Deck
+    OC.set_fft_workers(cores)
+    d = Deck(deck_text)
+    u = d.u0
+    t0 = time.perf_counter()
+    d.op.transport(0.0, u)
+    t_tr = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for i in range(nq):
+        f = u[i % d.K, i % d.Ne]
+        OC.q_vhs(d.pre, f, f, d.prefac)
+    t_q = (time.perf_counter() - t0) / nq
+    OC.set_fft_workers(None)
+    ncoll = d.K * d.Ne                                   # nodal basis: one Q per node
+    step_s = 2 * (ncoll * t_q + t_tr)                    # SSP-RK2: 2 RHS per step
+    pts = d.Ne * d.K * d.vm.vsize
+    return dict(value=pts / step_s, unit='phase-space point-steps/s', cores=cores,
+                kind='port',
+                sample='%d FP64 collision integrals + 1 transport of the same deck '
+                       '(%.2f s/collision), scaled to one SSP-RK2 step (%d collisions)'
+                       % (nq, t_q, 2 * ncoll))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=52)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--config', default=os.path.join(ROOT, 'dgfs1d_gpu_amd', 'configs',
+                                                     'couette_prod.ini'))
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    args = ap.parse_args(argv)
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
+        print('bench.py measures one GPU only: this build has no multi-GPU path '
+              '(--gpus %d requested)' % args.gpus, file=sys.stderr)
+        return 2
+
+    import torch
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    from dgfs1d_gpu_amd.std.std import DGFSStd1D
+
+    text = open(args.config).read()
+    cfg = Dictionary(text)
+    dev = torch.device('cuda:0')
+    s = DGFSStd1D(cfg, device=dev, writers=False)
+    plan = s.sm.plan
+    for _ in range(args.warmup):
+        s.step()
+    torch.cuda.synchronize()
+
+    nrhs = {'euler': 1, 'ssp-rk2': 2, 'ssp-rk3': 3, 'lse-rk45': 5, 'lesrk-45': 5}[
+        cfg.lookup('time-integrator', 'scheme')]
+    plan.timing(args.steps * nrhs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    stage_ms, ncalls = plan.timing_read()
+    plan.timing(0)
+
+    N, Ne, K = s.vm.Nv(), s.Ne, s.K
+    n = N ** 3
+    B = s.vm.M() * s.vm.Nrho()
+    ntask = s.ntasks
+    fg_same = plan.tasks['fg_same']
+    nvar, pw = (2, 14) if fg_same else (4, 26)
+    gain_flops = ntask * B * (nvar * fft_flops(n) + pw * n)
+    gain_ms = stage_ms[1] / max(ncalls, 1)
+    achieved = gain_flops / (gain_ms * 1e-3) / 1e12
+    coll_ms = sum(stage_ms) / max(ncalls, 1)
+    q_survey = (4 + 5 * B) * fft_flops(n) + 26 * B * n   # per Q, reference-structured count
+    value = Ne * K * n * args.steps / wall
+    q = plan.query()
+    out = {
+        'metric': 'phase-space updates/sec at Ne/K/N^3',
+        'value': value,
+        'unit': 'phase-space point-steps/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': wall / args.steps * 1e3,
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32' if cfg.dtype.__name__ == 'float32' else 'f64',
+        'data': 'synthetic (deterministic Maxwellian initial state, diffuse walls)',
+        'config': {'workload': 'couette Ne=%d K=%d N=%d^3 M=%d Nrho=%d %s dt=%g' % (
+            Ne, K, N, s.vm.M(), s.vm.Nrho(), cfg.lookup('time-integrator', 'scheme'), s.dt),
+            'Ne': Ne, 'K': K, 'N': N, 'M': s.vm.M(), 'Nrho': s.vm.Nrho(),
+            'collisions_per_rhs': ntask, 'split_S': q['split'], 'qgroups': q['qgroups'],
+            'pgroups': q['pgroups']},
+        'roofline': {'bound': 'fp32-valu', 'kernel': 'k_gain', 'achieved': achieved,
+                     'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': achieved / PEAK_FP32_TFLOPS, 'traffic': None,
+                     'flops_per_launch': gain_flops, 'avg_launch_ms': gain_ms},
+        'collision_stage_ms': {'forward': stage_ms[0] / max(ncalls, 1), 'gain': gain_ms,
+                               'weight': stage_ms[2] / max(ncalls, 1),
+                               'output': stage_ms[3] / max(ncalls, 1)},
+        'collision_ms_per_rhs': coll_ms,
+        'collision_integrals_per_s': ntask * ncalls / (sum(stage_ms) * 1e-3),
+        'reference_equivalent_tflops': ntask * q_survey / (coll_ms * 1e-3) / 1e12,
+    }
+    if not args.no_cpu_baseline:
+        cores = min(16, os.cpu_count() or 1)
+        out['cpu_baseline'] = cpu_baseline(text, cores)
+        out['gpu_over_cpu'] = value / out['cpu_baseline']['value']
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

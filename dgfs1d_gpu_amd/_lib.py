@@ -49,6 +49,8 @@ SIGNATURES = {
     'dgfs_collision_plan_query': (_i, [_vp, _ip, _ip, _ip, _szp]),
     'dgfs_collide': (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _i,
                           _vp]),
+    'dgfs_collision_timing': (_i, [_vp, _i]),
+    'dgfs_collision_timing_read': (_i, [_vp, _dp, _ip]),
     'dgfs_transport_work_bytes': (_sz, [_i]),
     'dgfs_transport_rhs': (_i, [C.POINTER(Transport), _vp, _vp, _d, _vp]),
     'dgfs_matop': (_i, [_i, _i, _i, _l, _vp, _vp, _vp, _i, _vp]),

@@ -103,6 +103,18 @@ class CollisionPlan:
             _lib.ptr(t['gstart']), _lib.ptr(t['gslice']), fg_same,
             _lib.stream_ptr(stream)))
 
+    def timing(self, max_calls):
+        """Record per-stage HIP events for the next max_calls calls (0 = off)."""
+        _lib.check(_lib.lib().dgfs_collision_timing(self._h, int(max_calls)))
+
+    def timing_read(self):
+        """(ms per stage [forward, gain, weight, output] summed, ncalls)."""
+        ms = (_lib.C.c_double * 4)()
+        nc = _lib.C.c_int()
+        _lib.check(_lib.lib().dgfs_collision_timing_read(
+            self._h, _lib.C.cast(ms, _lib.C.POINTER(_lib.C.c_double)), _lib.C.byref(nc)))
+        return list(ms), nc.value
+
     def close(self):
         if getattr(self, '_h', None) is not None and self._h.value:
             _lib.lib().dgfs_collision_plan_destroy(self._h)

@@ -150,9 +150,48 @@ extern "C" int dgfs_collision_plan_create(int N, int M, int Nrho, double L, cons
     return 0;
 }
 
+static void free_events(CollisionPlan* pl) {
+    if (!pl->ev) return;
+    for (int i = 0; i < pl->ev_max * 5; ++i) (void)hipEventDestroy(pl->ev[i]);
+    delete[] pl->ev;
+    pl->ev = nullptr;
+    pl->ev_max = pl->ev_used = 0;
+}
+
+extern "C" int dgfs_collision_timing(dgfs_collision_plan_t plan, int max_calls) {
+    auto* pl = static_cast<CollisionPlan*>(plan);
+    DGFS_REQUIRE(pl, "collision_timing: null plan");
+    DGFS_REQUIRE(max_calls >= 0, "collision_timing: bad max_calls");
+    free_events(pl);
+    if (max_calls == 0) return 0;
+    pl->ev = new hipEvent_t[size_t(max_calls) * 5];
+    for (int i = 0; i < max_calls * 5; ++i) DGFS_HIP_CHECK(hipEventCreate(&pl->ev[i]));
+    pl->ev_max = max_calls;
+    pl->ev_used = 0;
+    return 0;
+}
+
+extern "C" int dgfs_collision_timing_read(dgfs_collision_plan_t plan, double* ms, int* ncalls) {
+    auto* pl = static_cast<CollisionPlan*>(plan);
+    DGFS_REQUIRE(pl && ms && ncalls, "collision_timing_read: null argument");
+    for (int s = 0; s < 4; ++s) ms[s] = 0.0;
+    *ncalls = pl->ev_used;
+    for (int c = 0; c < pl->ev_used; ++c) {
+        DGFS_HIP_CHECK(hipEventSynchronize(pl->ev[c * 5 + 4]));
+        for (int s = 0; s < 4; ++s) {
+            float t = 0.f;
+            DGFS_HIP_CHECK(hipEventElapsedTime(&t, pl->ev[c * 5 + s], pl->ev[c * 5 + s + 1]));
+            ms[s] += t;
+        }
+    }
+    pl->ev_used = 0;
+    return 0;
+}
+
 extern "C" int dgfs_collision_plan_destroy(dgfs_collision_plan_t plan) {
     auto* pl = static_cast<CollisionPlan*>(plan);
     if (!pl) return 0;
+    free_events(pl);
     for (void* p : {pl->phase, pl->wtab, pl->bb2, pl->Fhat, pl->acc, pl->Gp})
         if (p) (void)hipFree(p);
     delete pl;

@@ -296,7 +296,15 @@ struct CollisionPlan {
     void* acc = nullptr;    // [ntask][Nrho][QG][n] cplx<T>
     void* Gp = nullptr;     // [ntask][PG][n] cplx<T>
     int num_cu = 256;
+    // optional stage timing: 5 events per call (before A, B, C, D, after D)
+    hipEvent_t* ev = nullptr;
+    int ev_max = 0, ev_used = 0;
 };
+
+inline void timing_mark(CollisionPlan* pl, int stage, hipStream_t st) {
+    if (pl->ev && pl->ev_used < pl->ev_max)
+        (void)hipEventRecord(pl->ev[pl->ev_used * 5 + stage], st);
+}
 
 // -------------------------------------------------------------- launch
 
@@ -309,10 +317,12 @@ struct Impl {
                       int ntask, int ngroup, const int* gstart, const int* gslice, int fg_same,
                       hipStream_t st) {
         const int nfg = fg_same ? 1 : 2;
+        timing_mark(pl, 0, st);
         hipLaunchKernelGGL((k_forward<T, N, S>), dim3(ntask * nfg * S), dim3(NT), 0, st,
                            (const T*)u, (const T*)ug, tv, nfg, (cplx<T>*)pl->Fhat);
         DGFS_HIP_CHECK(hipGetLastError());
         const int nb = ntask * pl->Nrho * S * pl->QG;
+        timing_mark(pl, 1, st);
         if (fg_same)
             hipLaunchKernelGGL((k_gain<T, N, S, true>), dim3(nb), dim3(NT), 0, st,
                                (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->phase, pl->M,
@@ -322,15 +332,19 @@ struct Impl {
                                (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->phase, pl->M,
                                pl->Nrho, pl->QG, (cplx<T>*)pl->acc);
         DGFS_HIP_CHECK(hipGetLastError());
+        timing_mark(pl, 2, st);
         hipLaunchKernelGGL((k_weight<T, N, S, S>), dim3(ntask * pl->PG * S), dim3(NT), 0, st,
                            (const cplx<T>*)pl->acc, (const T*)pl->wtab, pl->Nrho, pl->QG,
                            pl->PG, (cplx<T>*)pl->Gp);
         DGFS_HIP_CHECK(hipGetLastError());
+        timing_mark(pl, 3, st);
         hipLaunchKernelGGL((k_output<T, N, S>), dim3(ngroup * S), dim3(NT), 0, st,
                            (const T*)u, tv, (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->Gp,
                            (const T*)pl->bb2, pl->PG, fg_same, gstart, gslice,
                            T(pl->prefac), (T*)Q);
         DGFS_HIP_CHECK(hipGetLastError());
+        timing_mark(pl, 4, st);
+        if (pl->ev && pl->ev_used < pl->ev_max) ++pl->ev_used;
         return 0;
     }
 };

@@ -73,14 +73,33 @@ class Precomp:
         return (2.0 * self.qw * self.sw / self.n)[:, None] * self.bb1
 
 
+# FFT backend: numpy (default) or scipy.fft with a worker count (used for the
+# timed CPU baseline, where all host cores should be busy).
+FFT_WORKERS = None
+
+
+def set_fft_workers(workers):
+    global FFT_WORKERS
+    FFT_WORKERS = workers
+
+
 def _fft(x, N):
-    return np.fft.fftn(x.reshape(x.shape[:-1] + (N, N, N)), axes=(-3, -2, -1)).reshape(x.shape)
+    shp = x.shape[:-1] + (N, N, N)
+    if FFT_WORKERS:
+        import scipy.fft as sf
+        return sf.fftn(x.reshape(shp), axes=(-3, -2, -1), workers=FFT_WORKERS).reshape(x.shape)
+    return np.fft.fftn(x.reshape(shp), axes=(-3, -2, -1)).reshape(x.shape)
 
 
 def _ifft_u(x, N):
     """Unnormalised inverse transform (cuFFT CUFFT_INVERSE)."""
     n = N ** 3
-    return n * np.fft.ifftn(x.reshape(x.shape[:-1] + (N, N, N)), axes=(-3, -2, -1)).reshape(x.shape)
+    shp = x.shape[:-1] + (N, N, N)
+    if FFT_WORKERS:
+        import scipy.fft as sf
+        return n * sf.ifftn(x.reshape(shp), axes=(-3, -2, -1),
+                            workers=FFT_WORKERS).reshape(x.shape)
+    return n * np.fft.ifftn(x.reshape(shp), axes=(-3, -2, -1)).reshape(x.shape)
 
 
 def q_vhs(pre, f, g, prefac):

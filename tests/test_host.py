@@ -1,0 +1,103 @@
+"""Host-side logic without a GPU: INI dictionary semantics, CLI overrides,
+restricted expressions, plugin registry, spherical rules, velocity mesh."""
+import numpy as np
+import pytest
+
+from dgfs1d_gpu_amd.dictionary import Dictionary
+from dgfs1d_gpu_amd.initialize import overrides_to_dict
+from dgfs1d_gpu_amd.nputil import npeval, subclass_where
+
+
+def test_dictionary_and_overrides():
+    ini = '[config]\nprecision = float\n[mesh]\nNe = 2 ; comment\nxlo = -0.5\n'
+    cfg = Dictionary(ini, defaults=overrides_to_dict([('mesh::Ne', '4'), ('basis::K', '5')]))
+    assert cfg.lookupint('mesh', 'Ne') == 4 and cfg.lookupint('basis', 'K') == 5
+    assert cfg.dtype == np.float32 and cfg.dtypename == 'float'
+    assert cfg.lookupfloat('mesh', 'xlo') == np.float32(-0.5)
+    assert cfg.lookupordefault('mesh', 'missing', 7) == 7
+    assert cfg.lookupexpr('mesh', 'xlo') == '(-0.5)'
+
+
+def test_npeval_restrictions():
+    assert npeval('pow(x, 2) + sin(0)', {'x': 3.0}) == 9.0
+    for bad in ('x**2', 'x.__class__', 'x; 1'):
+        with pytest.raises(ValueError):
+            npeval(bad, {'x': 1.0})
+    with pytest.raises(SyntaxError):          # statements are not expressions
+        npeval('import os', {'x': 1.0})
+
+
+def test_registries():
+    from dgfs1d_gpu_amd.basis import Basis
+    from dgfs1d_gpu_amd.std.bc import DGFSBCStd
+    from dgfs1d_gpu_amd.std.integrator import DGFSIntegratorStd
+    from dgfs1d_gpu_amd.std.scattering import DGFSScatteringModelStd
+    assert subclass_where(Basis, basis_kind='nodal-sem-gll').__name__ == 'NodalSemGLL'
+    for k in ('euler', 'ssp-rk2', 'ssp-rk3', 'lse-rk45', 'lesrk-45'):
+        subclass_where(DGFSIntegratorStd, intg_kind=k)
+    for k in ('dgfs-wall-diffuse', 'dgfs-inlet', 'dgfs-periodic', 'dgfs-cyclic', 'dgfs-vanish',
+              'dgfs-wall-expr-diffuse', 'dgfs-inlet-nondim'):
+        subclass_where(DGFSBCStd, type=k)
+    for k in ('vhs', 'vhs-gll'):
+        subclass_where(DGFSScatteringModelStd, scattering_model=k)
+    with pytest.raises(KeyError):
+        subclass_where(Basis, basis_kind='nope')
+
+
+@pytest.mark.parametrize('npts', [2, 6, 12, 32, 48, 70, 94, 120, 156, 192])
+def test_stored_designs_antipodal(npts):
+    from dgfs1d_gpu_amd.sphericaldesign import get_sphquadrule
+    p = get_sphquadrule('symmetric', 'womersley', npts)
+    M = npts // 2
+    assert p.shape == (npts, 3)
+    assert np.abs(np.linalg.norm(p, axis=1) - 1).max() < 1e-12
+    assert np.abs(p[M:] + p[:M]).max() < 1e-12
+
+
+def test_octahedral_design():
+    """24-point antipodal set, spherical 5-design (monomial averages exact)."""
+    from math import gamma
+    from itertools import product
+    from dgfs1d_gpu_amd.sphericaldesign import get_sphquadrule
+    from oracle.deck import octahedral24_oracle
+    p = get_sphquadrule('symmetric', 'octahedral', 24)
+    assert np.abs(p - octahedral24_oracle()).max() < 1e-15
+    assert np.abs(p[12:] + p[:12]).max() == 0.0
+
+    def avg(i, j, k):
+        if i % 2 or j % 2 or k % 2:
+            return 0.0
+        return 2 * gamma((i + 1) / 2) * gamma((j + 1) / 2) * gamma((k + 1) / 2) / \
+            gamma((i + j + k + 3) / 2) / (4 * np.pi)
+    for i, j, k in product(range(6), repeat=3):
+        if i + j + k <= 5:
+            assert abs((p[:, 0] ** i * p[:, 1] ** j * p[:, 2] ** k).mean() - avg(i, j, k)) < 1e-14
+
+
+def test_tabulated_rule():
+    from dgfs1d_gpu_amd.sphericaldesign import get_sphquadrule
+    p = get_sphquadrule('symmetric', '(0, 0, 1) (0, 0, -1)', 2)
+    assert p.shape == (2, 3)
+
+
+def test_velocity_mesh_matches_oracle():
+    from dgfs1d_gpu_amd.std.velocitymesh import DGFSVelocityMeshStd
+    from oracle.velocitymesh import VelocityMesh
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cfg = Dictionary.load(os.path.join(root, 'dgfs1d_gpu_amd', 'configs', 'couette_small.ini'))
+    p = DGFSVelocityMeshStd(cfg)
+    o = VelocityMesh(1.0, 273.0, 8.588900796810341e-07, 39.96e-3, 16, 8, 0.0, 273.0, 5.0, 6)
+    assert abs(p.L() - o.L) < 1e-14 and abs(p.cw() - o.cw) < 1e-14
+    assert np.abs(p.cv() - o.cv).max() < 1e-14
+    assert np.abs(p.qz() - o.qz).max() < 1e-13 and np.abs(p.qw() - o.qw).max() < 1e-13
+    assert np.abs(p.sz() - o.sz).max() == 0.0
+
+
+def test_mesh_types():
+    from dgfs1d_gpu_amd.mesh import Mesh
+    base = '[mesh]\nxlo=0\nxhi=2\nNe=4\nH0=2\n'
+    m = Mesh(Dictionary(base))
+    assert np.allclose(m.xmesh, [0, 0.25, 0.5, 0.75, 1.0]) and np.allclose(m.invjac, 8.0)
+    m = Mesh(Dictionary(base + 'type=bump\nmu=1.0\n'))
+    assert m.Ne == 4 and np.isclose(m.xmesh[0], 0) and np.isclose(m.xmesh[-1], 1)

@@ -1,0 +1,110 @@
+"""Pins the FP64 collision oracle (oracle/collision.py) without the reference:
+
+* pipeline == brute-force spectral sum (no FFT of products) at N = 4, 6, 8,
+  for 'vhs' (bilinear) and 'vhs-gll' (complex squares) -- pins step order,
+  1/n scalings, the aliased index arithmetic, cos/sin vs complex square;
+* Q(f, f) of 'vhs' equals 'vhs-gll' exactly (t1 t2 + t3 t4 with f = g);
+* bilinearity in (f, g) and the prefactor scaling;
+* mass / momentum / energy of Q vanish to spectral accuracy, and Q(M, M) of a
+  centred Maxwellian is small, on a grid that resolves the Maxwellian;
+* reflection covariance c -> -c (Q(Rf, Rg) = R Q(f, g)).
+A planted mistake (e.g. dropping the 1/n of the forward transform or the sign
+of one phase) fails the brute-force comparison; see test_planted_mistake.
+"""
+import numpy as np
+import pytest
+
+from oracle import collision as OC
+from oracle import velocitymesh as OVM
+
+
+def _setup(N, M, Nrho, omega=0.81):
+    vm = OVM.VelocityMesh(1e-3, 273.0, 2.3e-4, 39.96e-3, N, Nrho, 0.0, 273.0, 5.0, M)
+    pre = OC.Precomp(N, vm.L, vm.qz, vm.qw, vm.sz, vm.sw, 2 * (1 - omega))
+    return vm, pre
+
+
+def _f(vm, u, T, seed):
+    rng = np.random.default_rng(seed)
+    c = vm.cv - np.array(u)[:, None]
+    return np.exp(-(c * c).sum(0) / T) * (1 + 0.05 * rng.uniform(-1, 1, vm.vsize))
+
+
+@pytest.mark.parametrize('N,M,Nrho', [(4, 6, 3), (6, 3, 4), (8, 6, 4)])
+def test_pipeline_equals_bruteforce(N, M, Nrho):
+    vm, pre = _setup(N, M, Nrho)
+    f = _f(vm, (0.3, 0, 0), 0.8, 1)
+    g = _f(vm, (-0.3, 0.2, 0), 1.2, 2)
+    ref = OC.q_bruteforce(pre, f, g, 0.7)
+    assert np.abs(OC.q_vhs(pre, f, g, 0.7) - ref).max() < 1e-13 * np.abs(ref).max()
+    refg = OC.q_bruteforce(pre, f, f, 0.7)
+    assert np.abs(OC.q_vhs_gll(pre, f, 0.7) - refg).max() < 1e-13 * np.abs(refg).max()
+
+
+def test_vhs_with_f_equal_g_is_vhs_gll():
+    vm, pre = _setup(8, 6, 4)
+    f = _f(vm, (0.2, 0.1, 0), 1.0, 3)
+    assert np.array_equal(OC.q_vhs(pre, f, f, 1.0), OC.q_vhs_gll(pre, f, 1.0))
+
+
+def test_bilinear_and_prefactor():
+    vm, pre = _setup(8, 6, 4)
+    f1, f2 = _f(vm, (0.2, 0, 0), 1.0, 4), _f(vm, (0, 0.3, 0), 0.9, 5)
+    g = _f(vm, (0, 0, 0.1), 1.1, 6)
+    lhs = OC.q_vhs(pre, 2.0 * f1 - 0.5 * f2, g, 1.0)
+    rhs = 2.0 * OC.q_vhs(pre, f1, g, 1.0) - 0.5 * OC.q_vhs(pre, f2, g, 1.0)
+    assert np.abs(lhs - rhs).max() < 1e-13 * np.abs(lhs).max()
+    assert np.allclose(OC.q_vhs(pre, f1, g, 3.0), 3.0 * OC.q_vhs(pre, f1, g, 1.0), rtol=1e-14,
+                       atol=0)
+
+
+def test_conservation_and_equilibrium():
+    vm, pre = _setup(24, 6, 12)
+    c = vm.cv
+    M0 = np.exp(-(c * c).sum(0) / 1.0) / np.pi ** 1.5
+    f = 0.5 * np.exp(-((c - np.array([[0.5], [0], [0]])) ** 2).sum(0) / 0.8) \
+        + 0.5 * np.exp(-((c + np.array([[0.5], [0], [0]])) ** 2).sum(0) / 1.2)
+    Q = OC.q_vhs(pre, f, f, 1.0)
+    scale = np.abs(Q).max() * vm.vsize * vm.cw
+    for phi in (np.ones_like(c[0]), c[0], c[1], c[2], (c * c).sum(0)):
+        assert abs(np.sum(Q * phi) * vm.cw) < 1e-3 * scale * max(1.0, np.abs(phi).max())
+    Qeq = OC.q_vhs(pre, M0, M0, 1.0)
+    assert np.abs(Qeq).max() < 2e-2 * np.abs(Q).max()
+
+
+def test_reflection_covariance():
+    vm, pre = _setup(6, 3, 4)
+    N = vm.Nv
+    f = _f(vm, (0.3, 0.1, -0.2), 0.9, 7)
+    g = _f(vm, (-0.1, 0.2, 0.1), 1.1, 8)
+    R = lambda h: h.reshape(N, N, N)[::-1, ::-1, ::-1].ravel()
+    # cell-centred symmetric grid: c -> -c is index reversal; the half-sphere
+    # set is antipodal, so the operator is covariant up to the Nyquist modes
+    a = OC.q_bruteforce(pre, R(f), R(g), 1.0)
+    b = R(OC.q_bruteforce(pre, f, g, 1.0))
+    assert np.abs(a - b).max() < 5e-2 * np.abs(b).max()
+
+
+def test_planted_mistake(monkeypatch):
+    """Plausible pipeline mistakes must fail the brute-force pin: a normalised
+    inverse transform (missing the cuFFT factor n) and a conjugated product."""
+    vm, pre = _setup(6, 3, 4)
+    f, g = _f(vm, (0.3, 0, 0), 0.8, 1), _f(vm, (-0.3, 0.2, 0), 1.2, 2)
+    ref = OC.q_bruteforce(pre, f, g, 1.0)
+    good = OC.q_vhs(pre, f, g, 1.0)
+    assert np.abs(good - ref).max() < 1e-13 * np.abs(ref).max()
+    orig = OC._ifft_u
+    monkeypatch.setattr(OC, '_ifft_u', lambda x, N: orig(x, N) / N ** 3)
+    bad1 = OC.q_vhs(pre, f, g, 1.0)
+    monkeypatch.setattr(OC, '_ifft_u', orig)
+    assert np.abs(bad1 - ref).max() > 1e-3 * np.abs(ref).max()
+    # sign error in the cos/sin product sum (cplxMulAdd subtracting)
+    N, n = pre.N, pre.n
+    FTf, FTg = OC._fft(f.astype(complex), N) / n, OC._fft(g.astype(complex), N) / n
+    a = pre.a.reshape(-1, n)
+    t5 = OC._ifft_u(np.cos(a) * FTf, N) * OC._ifft_u(np.cos(a) * FTg, N) \
+        - OC._ifft_u(np.sin(a) * FTf, N) * OC._ifft_u(np.sin(a) * FTg, N)
+    t1 = OC._fft(t5, N).reshape(pre.Nrho, pre.M, n)
+    QG = OC._ifft_u((pre.weight()[:, None, :] * t1).sum(axis=(0, 1)), N)
+    bad2 = QG.real - f * OC._ifft_u(pre.bb2 * FTg, N).real
+    assert np.abs(bad2 - ref).max() > 1e-3 * np.abs(ref).max()
