@@ -74,7 +74,8 @@ extern "C" int dgfs_collision_plan_create(int N, int M, int Nrho, double L, cons
     const size_t n = size_t(N) * N * N;
     pl->n = n;
     const int S = dtype == DGFS_F64 ? dgfs::split_for<double>(N) : dgfs::split_for<float>(N);
-    pl->SA = pl->SB = pl->SC = pl->SD = S;
+    pl->SB = S;
+    pl->SA = pl->SC = pl->SD = dgfs::planes_per_wg(N);
     int dev = 0;
     (void)hipGetDevice(&dev);
     hipDeviceProp_t prop;
@@ -85,10 +86,6 @@ extern "C" int dgfs_collision_plan_create(int N, int M, int Nrho, double L, cons
         int QG = 1;
         while (QG < M && base * QG < 3L * pl->num_cu) ++QG;
         pl->QG = QG;
-        int PG = 1;
-        const long cb = long(ntask_max) * S;
-        while (PG < Nrho && cb * PG < 2L * pl->num_cu) ++PG;
-        pl->PG = PG;
     }
 
     // ---- host tables in FP64 (vhs-gll.mako:43-99 restated with own code)
@@ -112,7 +109,7 @@ extern "C" int dgfs_collision_plan_create(int N, int M, int Nrho, double L, cons
         for (int i2 = 0; i2 < N; ++i2)
             for (int i3 = 0; i3 < N; ++i3) {
                 const double ln = std::sqrt(l0[i1] * l0[i1] + l0[i2] * l0[i2] + l0[i3] * l0[i3]);
-                const size_t o = (size_t(i3) * N + i1) * N + i2;   // [l3][l1][l2]
+                const size_t o = (size_t(i1) * N + i2) * N + i3;   // natural [l1][l2][l3]
                 double bsum = 0.0;
                 for (int p = 0; p < Nrho; ++p) {
                     const double zg = std::pow(qz[p], gamma + 2.0);
@@ -140,8 +137,9 @@ extern "C" int dgfs_collision_plan_create(int N, int M, int Nrho, double L, cons
     const size_t cs = 2 * es;
     hipError_t e1 = hipMalloc(&pl->Fhat, size_t(ntask_max) * 2 * n * cs);
     hipError_t e2 = hipMalloc(&pl->acc, size_t(ntask_max) * Nrho * pl->QG * n * cs);
-    hipError_t e3 = hipMalloc(&pl->Gp, size_t(ntask_max) * pl->PG * n * cs);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+    hipError_t e3 = hipMalloc(&pl->Iw, size_t(ntask_max) * 2 * n * cs);
+    hipError_t e4 = hipMalloc(&pl->H, size_t(ntask_max) * 2 * n * cs);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
         dgfs::set_error("collision_plan_create: scratch allocation failed");
         dgfs_collision_plan_destroy(pl);
         return 1;
@@ -192,7 +190,7 @@ extern "C" int dgfs_collision_plan_destroy(dgfs_collision_plan_t plan) {
     auto* pl = static_cast<CollisionPlan*>(plan);
     if (!pl) return 0;
     free_events(pl);
-    for (void* p : {pl->phase, pl->wtab, pl->bb2, pl->Fhat, pl->acc, pl->Gp})
+    for (void* p : {pl->phase, pl->wtab, pl->bb2, pl->Fhat, pl->acc, pl->Iw, pl->H})
         if (p) (void)hipFree(p);
     delete pl;
     return 0;
@@ -205,9 +203,9 @@ extern "C" int dgfs_collision_plan_query(dgfs_collision_plan_t plan, int* split,
     const size_t cs = pl->dtype == DGFS_F64 ? 16 : 8;
     if (split) *split = pl->SB;
     if (qgroups) *qgroups = pl->QG;
-    if (pgroups) *pgroups = pl->PG;
+    if (pgroups) *pgroups = pl->SD;
     if (scratch_bytes)
-        *scratch_bytes = size_t(pl->ntask_max) * pl->n * cs * (2 + pl->Nrho * pl->QG + pl->PG);
+        *scratch_bytes = size_t(pl->ntask_max) * pl->n * cs * (6 + pl->Nrho * pl->QG);
     return 0;
 }
 

@@ -7,18 +7,22 @@
 // Here one launch of each of four kernels covers every node of an RHS:
 //
 //   A  fwd   : F^ = FFT3(f)/n, G^ = FFT3(g)/n for every task (f, g are the
-//              U/V contractions of the coefficient array, fused into the load)
+//              U/V contractions of the coefficient array, fused into the load):
+//              k_fwd_plane (2D over y,z per x-plane) + k_fwd_line (1D over x)
 //   B  gain  : for every (task, radius p, residue r, q-group):
 //                acc_p(x) = sum_q 1/2 (u+f u-g + u-f u+g)(x),
-//                u+-h = IFFT3(exp(+-i a_pq) H^)   (3 LDS passes, registers)
+//                u+-h = IFFT3(exp(+-i a_pq) H^)   (Fft3Engine: 3 LDS passes)
 //              cos/sin products of the reference are the same quantity:
 //              t1 t2 + t3 t4 = 1/2 (u+f u-g + u-f u+g).  The q-sum is taken
-//              in physical space before the forward transform (linearity).
-//   C  weight: Gp(k) = sum_{p in group} (2 w_p sw/n) bb1_p(k) FFT3(acc_p)(k)
-//   D  out   : Q[out] += prefac (Re IFFT3(sum G) - f Re IFFT3(bb2 G^))
+//              in physical space before the forward transform (linearity);
+//              the forward transform of acc_p along x is done in registers.
+//   C  plane : per (task, k1-plane): G^ = sum_p (2 w_p sw/n) bb1_p FFT2(acc_p),
+//              then the inverse plane transforms of G^ and of bb2 G^_g
+//   D  line  : inverse along k1 per (y, z) line and
+//              Q[out] += prefac (Re IFFT3(sum G^) - f Re IFFT3(bb2 G^_g))
 //
-// Every kernel is an instance of Fft3Engine (fft3.h).  Tables (phases, weights,
-// bb2) are built once in FP64 on the host at plan creation.
+// Tables (phases, weights, bb2) are built once in FP64 on the host at plan
+// creation.
 #pragma once
 #include "fft3.h"
 
@@ -39,6 +43,9 @@ constexpr int pick_split(int N) {
     }
     return N;
 }
+
+// planes per work-group of the 2D plane kernels (PL * N threads)
+constexpr int planes_per_wg(int N) { return N >= 128 ? 2 : (256 / N); }
 
 // task description (device arrays, see dgfs_hip.h)
 struct TaskView {
@@ -67,38 +74,81 @@ DGFS_DEV int xcd_remap(int bid, int nblk) {
     return bid;
 }
 
+// ============================================================ plane helpers
+// P points at PL planes of N x (N+1) complex in LDS; thread (pl, i) owns row
+// or column i of plane pl.
+template <typename T, int N, int DIR>
+DGFS_DEV void lds_fft_row(cplx<T>* P, int i) {
+    cplx<T> a[N], o[N];
+    sfor<0, N>([&](auto J) { a[J] = P[i * (N + 1) + J]; });
+    RegDFT<T, N, DIR>::template run<1, 0>(a, o);
+    sfor<0, N>([&](auto J) { P[i * (N + 1) + J] = o[J]; });
+}
+template <typename T, int N, int DIR>
+DGFS_DEV void lds_fft_col(cplx<T>* P, int i) {
+    cplx<T> a[N], o[N];
+    sfor<0, N>([&](auto J) { a[J] = P[J * (N + 1) + i]; });
+    RegDFT<T, N, DIR>::template run<1, 0>(a, o);
+    sfor<0, N>([&](auto J) { P[J * (N + 1) + i] = o[J]; });
+}
+
 // ================================================================ kernel A
-template <typename T, int N, int S>
-__global__ void __launch_bounds__(round64(N * (N / S)))
-k_forward(const T* __restrict__ u, const T* __restrict__ ug, TaskView tv, int nfg,
-          cplx<T>* __restrict__ Fhat) {
-    using E = Fft3Engine<T, N, S, round64(N * (N / S)), -1>;
-    constexpr int n = N * N * N;
-    __shared__ cplx<T> V[E::LDS_ELEMS];
-    const int bid = blockIdx.x;
-    const int r = bid % S;
-    const int which = (bid / S) % nfg;
-    const int t = bid / (S * nfg);
-    typename E::Split sp;
-    sp.init(r);
-    const T inv_n = T(1) / T(n);
-    cplx<T>* out = Fhat + (size_t(t) * 2 + which) * n;
-    const T* src = which ? ug : u;
-    // abstract axes: a3 = x (pass 1, residue on l1), a2 = z, a1 = y
-    E::run(V, sp,
-        [](int p, int& i1, int& i2) { i1 = p / N; i2 = p - (p / N) * N; },  // (y, z)
-        [&](int i1, int i2, int i3) {
-            const int v = (i3 * N + i1) * N + i2;   // f[x][y][z]
-            return mk(task_value<T>(tv, src, t, which, v) * inv_n, T(0));
-        },
-        IdentityPre{},
-        [](int P, int& o2, int& j) { o2 = P / (N / S); j = P - (P / (N / S)) * (N / S); },
-        IdentityPre{},
-        [&](int o2, int j, const cplx<T>* o) {
-            // F^ layout [l3][l1][l2]: o2 = l3, l1 = S*j + r, o1 = l2
-            cplx<T>* dst = out + (size_t(o2) * N + (S * j + r)) * N;
-            sfor<0, N>([&](auto I) { dst[I] = o[I]; });
-        });
+// A1: plane x of task t's f (or g): 2D forward transform over (y, z);
+//     Iw[t][which][x][l3][l2]
+template <typename T, int N>
+__global__ void __launch_bounds__(planes_per_wg(N) * N)
+k_fwd_plane(const T* __restrict__ u, const T* __restrict__ ug, TaskView tv, int nfg,
+            long nplanes, cplx<T>* __restrict__ Iw) {
+    constexpr int PL = planes_per_wg(N), NT = PL * N, NN = N * N;
+    __shared__ cplx<T> L[PL * N * (N + 1)];
+    const long plane0 = long(blockIdx.x) * PL;
+    for (int i = threadIdx.x; i < PL * NN; i += NT) {
+        const int pl = i / NN, yz = i - pl * NN;
+        const long pid = plane0 + pl;
+        cplx<T> v = mk(T(0), T(0));
+        if (pid < nplanes) {
+            const int x = int(pid % N);
+            const long tw = pid / N;
+            const int which = int(tw % nfg), t = int(tw / nfg);
+            v.x = task_value<T>(tv, which ? ug : u, t, which, x * NN + yz);
+        }
+        L[pl * N * (N + 1) + (yz / N) * (N + 1) + yz % N] = v;
+    }
+    __syncthreads();
+    const int pl = threadIdx.x / N, i = threadIdx.x - pl * N;
+    cplx<T>* P = L + pl * N * (N + 1);
+    lds_fft_row<T, N, -1>(P, i);     // z -> l3
+    __syncthreads();
+    lds_fft_col<T, N, -1>(P, i);     // y -> l2
+    __syncthreads();
+    for (int k = threadIdx.x; k < PL * NN; k += NT) {
+        const int q = k / NN, r = k - q * NN;
+        const long pid = plane0 + q;
+        if (pid >= nplanes) continue;
+        const int l3 = r / N, l2 = r - l3 * N;
+        Iw[pid * NN + r] = L[q * N * (N + 1) + l2 * (N + 1) + l3];   // [l3][l2]
+    }
+}
+
+// A2: line (l3, l2) of task-which tw: transform over x, scale 1/n;
+//     F^[t][which][l3][l1][l2]
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+k_fwd_line(long nlines, int nfg, const cplx<T>* __restrict__ Iw, cplx<T>* __restrict__ Fhat) {
+    constexpr int NN = N * N;
+    const long gid = long(blockIdx.x) * 256 + threadIdx.x;
+    if (gid >= nlines) return;
+    const long tw = gid / NN;
+    const int r = int(gid - tw * NN);             // r = l3*N + l2
+    const int t = int(tw / nfg), which = int(tw % nfg);
+    const cplx<T>* src = Iw + tw * size_t(N) * NN + r;
+    cplx<T> a[N], o[N];
+    sfor<0, N>([&](auto X) { a[X] = src[size_t(X) * NN]; });
+    RegDFT<T, N, -1>::template run<1, 0>(a, o);
+    const T inv_n = T(1) / T(N * NN);
+    const int l3 = r / N, l2 = r - l3 * N;
+    cplx<T>* dst = Fhat + (size_t(t) * 2 + which) * size_t(N) * NN + size_t(l3) * NN + l2;
+    sfor<0, N>([&](auto K1) { dst[size_t(K1) * N] = inv_n * o[K1]; });
 }
 
 // ================================================================ kernel B
@@ -162,131 +212,127 @@ k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
         int o2, j;
         map3(threadIdx.x, o2, j);
         const T scale = FGSAME ? T(1) : T(0.5);
-        // acc layout [t][p][qg][x][y][pos], pos = r*NS + j (z = S*j + r)
+        // forward transform along x of this thread's pencil (first axis of
+        // FFT3(acc_p)); the (y, z) plane transforms follow in k_plane_weight
+        sfor<0, N>([&](auto I) { R1[I] = scale * acc[I]; });
+        RegDFT<T, N, -1>::template run<1, 0>(R1, acc);
+        // acc layout [t][p][qg][k1][r][y][j] with z = S*j + r
         cplx<T>* dst = acc_out + ((size_t(t) * Nrho + p) * QG + qg) * n;
         sfor<0, N>([&](auto I) {
-            dst[(size_t(I) * N + o2) * N + r * NS + j] = scale * acc[I];
+            dst[((size_t(I) * S + r) * N + o2) * NS + j] = acc[I];
         });
     }
 }
 
 // ================================================================ kernel C
-template <typename T, int N, int S, int SB>
-__global__ void __launch_bounds__(round64(N * (N / S)))
-k_weight(const cplx<T>* __restrict__ acc_in, const T* __restrict__ wtab,
-         int Nrho, int QG, int PG, cplx<T>* __restrict__ Gp) {
-    using E = Fft3Engine<T, N, S, round64(N * (N / S)), -1>;
-    constexpr int n = N * N * N;
-    constexpr int NS = N / S;
-    constexpr int NSB = N / SB;     // gain-kernel residue layout of z
-    __shared__ cplx<T> V[E::LDS_ELEMS];
-    const int bid = blockIdx.x;
-    const int r = bid % S;
-    const int pg = (bid / S) % PG;
-    const int t = bid / (S * PG);
-    typename E::Split sp;
-    sp.init(r);
-    const int pchunk = (Nrho + PG - 1) / PG;
-    const int p0 = pg * pchunk;
-    const int p1 = (p0 + pchunk < Nrho) ? p0 + pchunk : Nrho;
-
-    cplx<T> accG[N];
-    sfor<0, N>([&](auto I) { accG[I] = mk(T(0), T(0)); });
-    // pass-1 pencil (i1 = y, i2 = z), thread order follows the stored position
-    auto map1 = [](int pp, int& i1, int& i2) {
-        i1 = pp / N;
-        const int pos = pp - (pp / N) * N;
-        i2 = (pos % NSB) * SB + pos / NSB;
-    };
-    auto map3 = [](int P, int& o2, int& j) { o2 = P / NS; j = P - (P / NS) * NS; };  // (k3, j)
-    for (int p = p0; p < p1; ++p) {
-        const cplx<T>* src = acc_in + (size_t(t) * Nrho + p) * QG * n;
-        const T* w = wtab + size_t(p) * n;
-        E::run(V, sp, map1,
-            [&](int i1, int i2, int i3) {
-                // acc[x = i3][y = i1][pos(z = i2)]
-                const size_t off = (size_t(i3) * N + i1) * N + (i2 % SB) * NSB + i2 / SB;
-                cplx<T> s = src[off];
-                for (int g = 1; g < QG; ++g) s = s + src[size_t(g) * n + off];
-                return s;
-            },
-            IdentityPre{}, map3, IdentityPre{},
-            [&](int o2, int j, const cplx<T>* o) {
-                // k3 = o2, k1 = S*j + r, k2 = o1; weight layout [p][k3][k1][k2]
-                const T* wr = w + (size_t(o2) * N + (S * j + r)) * N;
-                sfor<0, N>([&](auto I) { accG[I] = accG[I] + wr[I] * o[I]; });
-            });
+// One work-group per plane (t, k1).  Its PL slots of N threads take radii
+// p = slot, slot+PL, ...:  G^(k1,:,:) = sum_p wtab_p(k1,:,:) FFT2_{y,z}(sum_qg acc[t][p][qg][k1]),
+// reduced over the slots through LDS; then Hg = IFFT2(G^(k1)) (slot 0) and
+// Hl = IFFT2(bb2 G^_g(k1)) (slot 1).  H[t][0 = loss | 1 = gain][k1][y][z].
+template <typename T, int N>
+__global__ void __launch_bounds__(planes_per_wg(N) * N)
+k_plane_weight(const cplx<T>* __restrict__ acc_in, const T* __restrict__ wtab,
+               const cplx<T>* __restrict__ Fhat, const T* __restrict__ bb2, int fg_same,
+               int Nrho, int QG, int SB, cplx<T>* __restrict__ H) {
+    constexpr int PL = planes_per_wg(N), NT = PL * N, NN = N * N, n = N * NN, PS = N * (N + 1);
+    static_assert(PL >= 2, "plane kernel needs two slots");
+    __shared__ cplx<T> L[PL * PS];
+    const int k1 = blockIdx.x % N, t = blockIdx.x / N;
+    const int slot = threadIdx.x / N, c = threadIdx.x - slot * N;
+    cplx<T>* P = L + slot * PS;
+    const int NSB = N / SB;
+    cplx<T> G[N];
+    sfor<0, N>([&](auto I) { G[I] = mk(T(0), T(0)); });
+    const int nround = (Nrho + PL - 1) / PL;
+    for (int rd = 0; rd < nround; ++rd) {
+        const int p = rd * PL + slot;
+        const bool live = p < Nrho;
+        __syncthreads();
+        if (live) {
+            const cplx<T>* src = acc_in + ((size_t(t) * Nrho + p) * QG) * n + size_t(k1) * NN;
+            for (int pos = c; pos < NN; pos += N) {
+                cplx<T> v = src[pos];
+                for (int g = 1; g < QG; ++g) v = v + src[size_t(g) * n + pos];
+                // stored position (r, y, j) with z = SB*j + r
+                const int r = pos / (N * NSB), rem = pos - r * N * NSB;
+                const int y = rem / NSB, z = (rem - y * NSB) * SB + r;
+                P[y * (N + 1) + z] = v;
+            }
+        }
+        __syncthreads();
+        if (live) lds_fft_row<T, N, -1>(P, c);     // z -> k3
+        __syncthreads();
+        if (live) {
+            lds_fft_col<T, N, -1>(P, c);            // y -> k2 (column c = k3)
+            const T* w = wtab + size_t(p) * n + size_t(k1) * NN + c;   // [k1][k2][k3 = c]
+            sfor<0, N>([&](auto K2) { G[K2] = G[K2] + w[size_t(K2) * N] * P[K2 * (N + 1) + c]; });
+        }
     }
-    if (threadIdx.x < E::NPEN) {
-        int o2, j;
-        map3(threadIdx.x, o2, j);
-        cplx<T>* dst = Gp + ((size_t(t) * PG + pg) * N + o2) * N * N + size_t(S * j + r) * N;
-        sfor<0, N>([&](auto I) { dst[I] = accG[I]; });
+    // reduce the slots' partial columns: slot 0 ends with G^(k1, :, c)
+    __syncthreads();
+    sfor<0, N>([&](auto K2) { P[K2 * (N + 1) + c] = G[K2]; });
+    __syncthreads();
+    if (slot == 0)
+        for (int o = 1; o < PL; ++o)
+            sfor<0, N>([&](auto K2) { G[K2] = G[K2] + L[o * PS + K2 * (N + 1) + c]; });
+    __syncthreads();   // every slot's partial has been read before slot 1 reuses its plane
+    if (slot == 0) {
+        sfor<0, N>([&](auto K2) { P[K2 * (N + 1) + c] = G[K2]; });
+    } else if (slot == 1) {
+        // loss plane: bb2 * G^_g at l1 = k1, column c = l3
+        const cplx<T>* Fg = Fhat + (size_t(t) * 2 + (fg_same ? 0 : 1)) * n;
+        sfor<0, N>([&](auto L2) {
+            P[L2 * (N + 1) + c] = bb2[size_t(k1) * NN + L2 * N + c] *
+                                  Fg[(size_t(c) * N + k1) * N + L2];
+        });
+    }
+    __syncthreads();
+    if (slot < 2) lds_fft_col<T, N, +1>(P, c);      // k2 -> y
+    __syncthreads();
+    if (slot < 2) lds_fft_row<T, N, +1>(P, c);      // k3 -> z
+    __syncthreads();
+    for (int i = threadIdx.x; i < 2 * NN; i += NT) {
+        const int which = i / NN, r = i - which * NN;     // slot 0 = gain, slot 1 = loss
+        H[((size_t(t) * 2 + (which == 0 ? 1 : 0)) * N + k1) * NN + r] =
+            L[which * PS + (r / N) * (N + 1) + r % N];
     }
 }
 
 // ================================================================ kernel D
-template <typename T, int N, int S>
-__global__ void __launch_bounds__(round64(N * (N / S)))
-k_output(const T* __restrict__ u, TaskView tv, const cplx<T>* __restrict__ Fhat,
-         const cplx<T>* __restrict__ Gp, const T* __restrict__ bb2, int PG, int fg_same,
-         const int* __restrict__ gstart, const int* __restrict__ gslice, T prefac,
-         T* __restrict__ Q) {
-    using E = Fft3Engine<T, N, S, round64(N * (N / S)), +1>;
-    constexpr int n = N * N * N;
-    constexpr int NS = N / S;
-    __shared__ cplx<T> V[E::LDS_ELEMS];
-    const int bid = blockIdx.x;
-    const int r = bid % S;
-    const int g = bid / S;
-    typename E::Split sp;
-    sp.init(r);
+// Line (g, y, z): gain(x) = Re IFFT_k1(sum_{t in g} Hg[t](:,y,z)),
+// loss(x) = sum_{t in g} f_t(x,y,z) Re IFFT_k1(Hl[t](:,y,z));
+// Q[gslice[g]](x,y,z) += prefac (gain - loss).
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+k_line_output(const T* __restrict__ u, TaskView tv, const cplx<T>* __restrict__ H, long nlines,
+              const int* __restrict__ gstart, const int* __restrict__ gslice, T prefac,
+              T* __restrict__ Q) {
+    constexpr int NN = N * N, n = N * NN;
+    const long gid = long(blockIdx.x) * 256 + threadIdx.x;
+    if (gid >= nlines) return;
+    const int g = int(gid / NN), yz = int(gid - long(g) * NN);
     const int ta = gstart[g], tb = gstart[g + 1];
-
-    auto map1 = [](int pp, int& i1, int& i2) { i1 = pp / N; i2 = pp - (pp / N) * N; };  // (l1, l2)
-    auto map3 = [](int P, int& o2, int& j) { o2 = P / NS; j = P - (P / NS) * NS; };      // (y, j)
-
+    cplx<T> a[N], o[N];
     T loss[N];
-    sfor<0, N>([&](auto I) { loss[I] = T(0); });
+    sfor<0, N>([&](auto I) { loss[I] = T(0); a[I] = mk(T(0), T(0)); });
     for (int t = ta; t < tb; ++t) {
-        const cplx<T>* Fg = Fhat + (size_t(t) * 2 + (fg_same ? 0 : 1)) * n;
-        E::run(V, sp, map1,
-            [&](int i1, int i2, int i3) {
-                const size_t off = (size_t(i3) * N + i1) * N + i2;   // [l3][l1][l2]
-                return bb2[off] * Fg[off];
-            },
-            IdentityPre{}, map3, IdentityPre{},
-            [&](int o2, int j, const cplx<T>* o) {
-                const int z = S * j + r;
-                sfor<0, N>([&](auto I) {
-                    const int v = (int(I) * N + o2) * N + z;
-                    loss[I] += task_value<T>(tv, u, t, 0, v) * o[I].x;
-                });
-            });
+        const cplx<T>* hl = H + (size_t(t) * 2) * n + yz;
+        cplx<T> b[N];
+        sfor<0, N>([&](auto K1) { b[K1] = hl[size_t(K1) * NN]; });
+        RegDFT<T, N, +1>::template run<1, 0>(b, o);
+        sfor<0, N>([&](auto X) { loss[X] += task_value<T>(tv, u, t, 0, X * NN + yz) * o[X].x; });
+        const cplx<T>* hg = hl + n;
+        sfor<0, N>([&](auto K1) { a[K1] = a[K1] + hg[size_t(K1) * NN]; });
     }
-    E::run(V, sp, map1,
-        [&](int i1, int i2, int i3) {
-            const size_t off = (size_t(i3) * N + i1) * N + i2;
-            cplx<T> s = mk(T(0), T(0));
-            for (int t = ta; t < tb; ++t)
-                for (int pg = 0; pg < PG; ++pg) s = s + Gp[(size_t(t) * PG + pg) * n + off];
-            return s;
-        },
-        IdentityPre{}, map3, IdentityPre{},
-        [&](int o2, int j, const cplx<T>* o) {
-            const int z = S * j + r;
-            T* dst = Q + size_t(gslice[g]) * n;
-            sfor<0, N>([&](auto I) {
-                const int v = (int(I) * N + o2) * N + z;
-                dst[v] += prefac * (o[I].x - loss[I]);
-            });
-        });
+    RegDFT<T, N, +1>::template run<1, 0>(a, o);
+    T* dst = Q + size_t(gslice[g]) * n + yz;
+    sfor<0, N>([&](auto X) { dst[size_t(X) * NN] += prefac * (o[X].x - loss[X]); });
 }
 
 // ================================================================ the plan
 struct CollisionPlan {
     int N, M, Nrho, dtype, variant, ntask_max, ngroup_max;
-    int SA, SB, SC, SD, QG, PG;
+    int SA, SB, SC, SD, QG;
     double prefac;
     size_t n;
     void* phase = nullptr;  // [Nrho][M][3][N] cplx<T>
@@ -294,7 +340,8 @@ struct CollisionPlan {
     void* bb2 = nullptr;    // [l3][l1][l2] T
     void* Fhat = nullptr;   // [ntask][2][n] cplx<T>
     void* acc = nullptr;    // [ntask][Nrho][QG][n] cplx<T>
-    void* Gp = nullptr;     // [ntask][PG][n] cplx<T>
+    void* Iw = nullptr;     // [ntask][2][n] cplx<T>  forward-transform intermediate
+    void* H = nullptr;      // [ntask][2][n] cplx<T>  plane-inverted loss / gain
     int num_cu = 256;
     // optional stage timing: 5 events per call (before A, B, C, D, after D)
     hipEvent_t* ev = nullptr;
@@ -310,16 +357,23 @@ inline void timing_mark(CollisionPlan* pl, int stage, hipStream_t st) {
 
 template <typename T, int N>
 struct Impl {
-    static constexpr int S = pick_split<T>(N);
+    static constexpr int S = pick_split<T>(N);          // gain kernel residue split
     static constexpr int NT = round64(N * (N / S));
+    static constexpr int PL = planes_per_wg(N);
 
     static int launch(CollisionPlan* pl, const void* u, const void* ug, void* Q, TaskView tv,
                       int ntask, int ngroup, const int* gstart, const int* gslice, int fg_same,
                       hipStream_t st) {
         const int nfg = fg_same ? 1 : 2;
+        constexpr long NN = long(N) * N;
         timing_mark(pl, 0, st);
-        hipLaunchKernelGGL((k_forward<T, N, S>), dim3(ntask * nfg * S), dim3(NT), 0, st,
-                           (const T*)u, (const T*)ug, tv, nfg, (cplx<T>*)pl->Fhat);
+        const long npl = long(ntask) * nfg * N;
+        hipLaunchKernelGGL((k_fwd_plane<T, N>), dim3((npl + PL - 1) / PL), dim3(PL * N), 0, st,
+                           (const T*)u, (const T*)ug, tv, nfg, npl, (cplx<T>*)pl->Iw);
+        DGFS_HIP_CHECK(hipGetLastError());
+        const long nli = long(ntask) * nfg * NN;
+        hipLaunchKernelGGL((k_fwd_line<T, N>), dim3((nli + 255) / 256), dim3(256), 0, st, nli,
+                           nfg, (const cplx<T>*)pl->Iw, (cplx<T>*)pl->Fhat);
         DGFS_HIP_CHECK(hipGetLastError());
         const int nb = ntask * pl->Nrho * S * pl->QG;
         timing_mark(pl, 1, st);
@@ -333,14 +387,15 @@ struct Impl {
                                pl->Nrho, pl->QG, (cplx<T>*)pl->acc);
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 2, st);
-        hipLaunchKernelGGL((k_weight<T, N, S, S>), dim3(ntask * pl->PG * S), dim3(NT), 0, st,
-                           (const cplx<T>*)pl->acc, (const T*)pl->wtab, pl->Nrho, pl->QG,
-                           pl->PG, (cplx<T>*)pl->Gp);
+        hipLaunchKernelGGL((k_plane_weight<T, N>), dim3(ntask * N), dim3(PL * N), 0, st,
+                           (const cplx<T>*)pl->acc, (const T*)pl->wtab,
+                           (const cplx<T>*)pl->Fhat, (const T*)pl->bb2, fg_same, pl->Nrho,
+                           pl->QG, S, (cplx<T>*)pl->H);
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 3, st);
-        hipLaunchKernelGGL((k_output<T, N, S>), dim3(ngroup * S), dim3(NT), 0, st,
-                           (const T*)u, tv, (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->Gp,
-                           (const T*)pl->bb2, pl->PG, fg_same, gstart, gslice,
+        const long nlo = long(ngroup) * NN;
+        hipLaunchKernelGGL((k_line_output<T, N>), dim3((nlo + 255) / 256), dim3(256), 0, st,
+                           (const T*)u, tv, (const cplx<T>*)pl->H, nlo, gstart, gslice,
                            T(pl->prefac), (T*)Q);
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 4, st);
@@ -348,7 +403,6 @@ struct Impl {
         return 0;
     }
 };
-
 
 template <typename T, int N>
 int collide_launch(CollisionPlan* pl, const void* u, const void* ug, void* Q, TaskView tv,

@@ -53,7 +53,8 @@ int dgfs_collision_plan_create(int N, int M, int Nrho, double L, const double* q
                                double gamma, double prefac, int dtype, int variant,
                                int ntask_max, dgfs_collision_plan_t* plan);
 int dgfs_collision_plan_destroy(dgfs_collision_plan_t plan);
-/* split factor S of the residue decomposition, q/p group counts, scratch */
+/* split S of the gain kernel's residue decomposition, its q-group count, the
+ * split of the forward/weight/output kernels, and the scratch bytes */
 int dgfs_collision_plan_query(dgfs_collision_plan_t plan, int* split, int* qgroups,
                               int* pgroups, size_t* scratch_bytes);
 
