@@ -102,11 +102,15 @@ def main(argv=None):
     B = s.vm.M() * s.vm.Nrho()
     ntask = s.ntasks
     fg_same = plan.tasks['fg_same']
-    nvar, pw = (2, 14) if fg_same else (4, 26)
+    # k_gain: one packed inverse 3D transform per (task, p, q) and function
+    # (f, and g when f != g) plus the survey's pointwise count per (p, q, k)
+    nvar, pw = (1, 14) if fg_same else (2, 26)
     gain_flops = ntask * B * (nvar * fft_flops(n) + pw * n)
-    gain_ms = stage_ms[1] / max(ncalls, 1)
+    nc = max(ncalls, 1)
+    stages = dict(zip(plan.STAGES, (m / nc for m in stage_ms)))
+    gain_ms = stages['gain']
     achieved = gain_flops / (gain_ms * 1e-3) / 1e12
-    coll_ms = sum(stage_ms) / max(ncalls, 1)
+    coll_ms = sum(stage_ms) / nc
     q_survey = (4 + 5 * B) * fft_flops(n) + 26 * B * n   # per Q, reference-structured count
     value = Ne * K * n * args.steps / wall
     q = plan.query()
@@ -128,9 +132,7 @@ def main(argv=None):
                      'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / PEAK_FP32_TFLOPS, 'traffic': None,
                      'flops_per_launch': gain_flops, 'avg_launch_ms': gain_ms},
-        'collision_stage_ms': {'forward': stage_ms[0] / max(ncalls, 1), 'gain': gain_ms,
-                               'weight': stage_ms[2] / max(ncalls, 1),
-                               'output': stage_ms[3] / max(ncalls, 1)},
+        'collision_stage_ms': stages,
         'collision_ms_per_rhs': coll_ms,
         'collision_integrals_per_s': ntask * ncalls / (sum(stage_ms) * 1e-3),
         'reference_equivalent_tflops': ntask * q_survey / (coll_ms * 1e-3) / 1e12,

@@ -107,9 +107,11 @@ class CollisionPlan:
         """Record per-stage HIP events for the next max_calls calls (0 = off)."""
         _lib.check(_lib.lib().dgfs_collision_timing(self._h, int(max_calls)))
 
+    STAGES = ('forward', 'nyquist', 'gain', 'plane', 'line')
+
     def timing_read(self):
-        """(ms per stage [forward, gain, weight, output] summed, ncalls)."""
-        ms = (_lib.C.c_double * 4)()
+        """(ms per stage [forward, nyquist, gain, plane, line] summed, ncalls)."""
+        ms = (_lib.C.c_double * 5)()
         nc = _lib.C.c_int()
         _lib.check(_lib.lib().dgfs_collision_timing_read(
             self._h, _lib.C.cast(ms, _lib.C.POINTER(_lib.C.c_double)), _lib.C.byref(nc)))

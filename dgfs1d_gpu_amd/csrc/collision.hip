@@ -74,6 +74,8 @@ extern "C" int dgfs_collision_plan_create(int N, int M, int Nrho, double L, cons
     const size_t n = size_t(N) * N * N;
     pl->n = n;
     const int S = dtype == DGFS_F64 ? dgfs::split_for<double>(N) : dgfs::split_for<float>(N);
+    // (the gain kernel's own split is a compile-time choice per (dtype, N, f==g);
+    //  S here sizes the q-groups)
     pl->SB = S;
     pl->SA = pl->SC = pl->SD = dgfs::planes_per_wg(N);
     int dev = 0;
@@ -139,7 +141,9 @@ extern "C" int dgfs_collision_plan_create(int N, int M, int Nrho, double L, cons
     hipError_t e2 = hipMalloc(&pl->acc, size_t(ntask_max) * Nrho * pl->QG * n * cs);
     hipError_t e3 = hipMalloc(&pl->Iw, size_t(ntask_max) * 2 * n * cs);
     hipError_t e4 = hipMalloc(&pl->H, size_t(ntask_max) * 2 * n * cs);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+    hipError_t e5 = hipMalloc(&pl->nyq, size_t(ntask_max) * 2 * Nrho * M * 6 * N * N * es);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess ||
+        e5 != hipSuccess) {
         dgfs::set_error("collision_plan_create: scratch allocation failed");
         dgfs_collision_plan_destroy(pl);
         return 1;
@@ -150,7 +154,7 @@ extern "C" int dgfs_collision_plan_create(int N, int M, int Nrho, double L, cons
 
 static void free_events(CollisionPlan* pl) {
     if (!pl->ev) return;
-    for (int i = 0; i < pl->ev_max * 5; ++i) (void)hipEventDestroy(pl->ev[i]);
+    for (int i = 0; i < pl->ev_max * dgfs::kTimingEvents; ++i) (void)hipEventDestroy(pl->ev[i]);
     delete[] pl->ev;
     pl->ev = nullptr;
     pl->ev_max = pl->ev_used = 0;
@@ -162,8 +166,9 @@ extern "C" int dgfs_collision_timing(dgfs_collision_plan_t plan, int max_calls) 
     DGFS_REQUIRE(max_calls >= 0, "collision_timing: bad max_calls");
     free_events(pl);
     if (max_calls == 0) return 0;
-    pl->ev = new hipEvent_t[size_t(max_calls) * 5];
-    for (int i = 0; i < max_calls * 5; ++i) DGFS_HIP_CHECK(hipEventCreate(&pl->ev[i]));
+    constexpr int E = dgfs::kTimingEvents;
+    pl->ev = new hipEvent_t[size_t(max_calls) * E];
+    for (int i = 0; i < max_calls * E; ++i) DGFS_HIP_CHECK(hipEventCreate(&pl->ev[i]));
     pl->ev_max = max_calls;
     pl->ev_used = 0;
     return 0;
@@ -172,13 +177,14 @@ extern "C" int dgfs_collision_timing(dgfs_collision_plan_t plan, int max_calls) 
 extern "C" int dgfs_collision_timing_read(dgfs_collision_plan_t plan, double* ms, int* ncalls) {
     auto* pl = static_cast<CollisionPlan*>(plan);
     DGFS_REQUIRE(pl && ms && ncalls, "collision_timing_read: null argument");
-    for (int s = 0; s < 4; ++s) ms[s] = 0.0;
+    constexpr int E = dgfs::kTimingEvents;
+    for (int s = 0; s < E - 1; ++s) ms[s] = 0.0;
     *ncalls = pl->ev_used;
     for (int c = 0; c < pl->ev_used; ++c) {
-        DGFS_HIP_CHECK(hipEventSynchronize(pl->ev[c * 5 + 4]));
-        for (int s = 0; s < 4; ++s) {
+        DGFS_HIP_CHECK(hipEventSynchronize(pl->ev[c * E + E - 1]));
+        for (int s = 0; s < E - 1; ++s) {
             float t = 0.f;
-            DGFS_HIP_CHECK(hipEventElapsedTime(&t, pl->ev[c * 5 + s], pl->ev[c * 5 + s + 1]));
+            DGFS_HIP_CHECK(hipEventElapsedTime(&t, pl->ev[c * E + s], pl->ev[c * E + s + 1]));
             ms[s] += t;
         }
     }
@@ -190,7 +196,7 @@ extern "C" int dgfs_collision_plan_destroy(dgfs_collision_plan_t plan) {
     auto* pl = static_cast<CollisionPlan*>(plan);
     if (!pl) return 0;
     free_events(pl);
-    for (void* p : {pl->phase, pl->wtab, pl->bb2, pl->Fhat, pl->acc, pl->Iw, pl->H})
+    for (void* p : {pl->phase, pl->wtab, pl->bb2, pl->Fhat, pl->acc, pl->Iw, pl->H, pl->nyq})
         if (p) (void)hipFree(p);
     delete pl;
     return 0;

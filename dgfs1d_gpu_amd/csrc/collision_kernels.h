@@ -152,14 +152,109 @@ k_fwd_line(long nlines, int nfg, const cplx<T>* __restrict__ Iw, cplx<T>* __rest
 }
 
 // ================================================================ kernel B
-template <typename T, int N, int S, bool FGSAME>
+// Nyquist planes.  f is real, so F^(-l) = conj F^(l) and a(-l) = -a(l) for
+// every l off the Nyquist planes (no index equal to N/2): there
+// u+-_h = IFFT3(e^{+-ia} F^) is real, and one packed transform
+// w = IFFT3((1+i)(cos a + sin a) F^) = u+ + i u- carries both.  On the set
+// {l1 = N/2} U {l2 = N/2} U {l3 = N/2} the symmetry fails; only the imaginary
+// parts of that set's contribution are needed to unpack w exactly:
+//   beta  = Im u+_nyq,  delta = Im u-_nyq,
+//   u+ = (Re w + delta) + i beta,   u- = (Im w - beta) + i delta.
+// The set splits into three disjoint planes, each a 2D transform times a
+// (-1)^coordinate factor:
+//   A: l1 = N/2                 -> (-1)^x A(y, z)
+//   B: l2 = N/2, l1 != N/2      -> (-1)^y B(x, z)
+//   C: l3 = N/2, l1, l2 != N/2  -> (-1)^z C(x, y)
+// k_nyq writes Im of A+-, B+-, C+- for every (task, which, p, q):
+// nyq[t][w][p][q][6][N*N] with planes A+ A- [y][z], B+ B- [z][x], C+ C- [y][x].
+template <typename T, int N>
+__global__ void __launch_bounds__(planes_per_wg(N) * N)
+k_nyq(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase, int M, int Nrho,
+      int NW, T* __restrict__ nyq) {
+    constexpr int PL = planes_per_wg(N), NN = N * N, n = N * NN, PS = N * (N + 1);
+    constexpr int H = N / 2;
+    __shared__ cplx<T> L[PL * PS];
+    const int bid = blockIdx.x;
+    const int q = bid % M, p = (bid / M) % Nrho, w = (bid / (M * Nrho)) % NW;
+    const int t = bid / (M * Nrho * NW);
+    const cplx<T>* F = Fhat + (size_t(t) * 2 + w) * n;
+    const cplx<T>* ph = phase + (size_t(p) * M + q) * 3 * N;
+    const int slot = threadIdx.x / N, c = threadIdx.x - slot * N;
+    cplx<T>* P = L + slot * PS;
+    T* out = nyq + ((((size_t(t) * NW + w) * Nrho + p) * M + q) * 6) * NN;
+    constexpr int NIT = (6 + PL - 1) / PL;
+    for (int it = 0; it < NIT; ++it) {
+        const int pl = it * PL + slot;
+        const bool live = pl < 6;
+        const int type = pl / 2;
+        const bool minus = (pl & 1) != 0;
+        __syncthreads();
+        if (live) {
+            sfor<0, N>([&](auto A) {
+                constexpr int a = A;
+                int l1, l2, l3;
+                bool zero = false;
+                if (type == 0) { l1 = H; l2 = a; l3 = c; }
+                else if (type == 1) { l1 = a; l2 = H; l3 = c; zero = (a == H); }
+                else { l1 = a; l2 = c; l3 = H; zero = (a == H) || (c == H); }
+                const cplx<T> e = ph[l1] * ph[N + l2] * ph[2 * N + l3];
+                const cplx<T> v = cmul_conj_if(F[(size_t(l3) * N + l1) * N + l2], e, minus);
+                P[a * (N + 1) + c] = zero ? mk(T(0), T(0)) : v;
+            });
+        }
+        __syncthreads();
+        if (live) lds_fft_row<T, N, +1>(P, c);      // over b = column index
+        __syncthreads();
+        if (live) lds_fft_col<T, N, +1>(P, c);      // over a = row index
+        __syncthreads();
+        if (live) {
+            T* dst = out + size_t(pl) * NN;
+            // thread c writes row c of the stored plane
+            sfor<0, N>([&](auto K) {
+                constexpr int k = K;
+                if (type == 0) dst[c * N + k] = P[c * (N + 1) + k].y;        // A[y=c][z=k]
+                else dst[c * N + k] = P[k * (N + 1) + c].y;                  // B[z=c][x=k], C[y=c][x=k]
+            });
+        }
+    }
+}
+
+// LDS image of one (which)'s Nyquist data for residue r, row pitch BP
+template <int N, int S>
+struct NyLayout {
+    static constexpr int NS = N / S, BP = N + 4;
+    static constexpr int A = 0;                   // [sgn][y][j]
+    static constexpr int B = 2 * N * NS;          // [sgn][j][x]  (pitch BP)
+    static constexpr int C = B + 2 * NS * BP;     // [sgn][y][x]  (pitch BP)
+    static constexpr int SIZE = C + 2 * N * BP;
+};
+
+template <typename T, int N, int S>
+constexpr int gain_lds_bytes(int NW) {
+    return N * N * (N / S + 1) * 2 * int(sizeof(T)) + NW * NyLayout<N, S>::SIZE * int(sizeof(T));
+}
+template <typename T, int N, int NW>
+constexpr int pick_split_gain() {
+    if constexpr (N % 1 == 0 && gain_lds_bytes<T, N, 1>(NW) <= 160 * 1024) return 1;
+    else if constexpr (N % 2 == 0 && gain_lds_bytes<T, N, 2>(NW) <= 160 * 1024) return 2;
+    else if constexpr (N % 3 == 0 && gain_lds_bytes<T, N, 3>(NW) <= 160 * 1024) return 3;
+    else if constexpr (N % 4 == 0 && gain_lds_bytes<T, N, 4>(NW) <= 160 * 1024) return 4;
+    else if constexpr (N % 6 == 0 && gain_lds_bytes<T, N, 6>(NW) <= 160 * 1024) return 6;
+    else return 8;
+}
+
+// Gain: for (task, radius p, z-residue r, q-group), acc = sum_q t1 t2 + t3 t4
+// with t1 t2 + t3 t4 = u+f u-f (NW = 1, f = g) or 1/2 (u+f u-g + u-f u+g).
+template <typename T, int N, int S, int NW>
 __global__ void __launch_bounds__(round64(N * (N / S)))
 k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
-       int M, int Nrho, int QG, cplx<T>* __restrict__ acc_out) {
+       const T* __restrict__ nyq, int M, int Nrho, int QG, cplx<T>* __restrict__ acc_out) {
     using E = Fft3Engine<T, N, S, round64(N * (N / S)), +1>;
-    constexpr int n = N * N * N;
-    constexpr int NS = N / S;
+    using NL = NyLayout<N, S>;
+    constexpr int n = N * N * N, NN = N * N;
+    constexpr int NS = N / S, BP = NL::BP;
     __shared__ cplx<T> V[E::LDS_ELEMS];
+    __shared__ T Ny[NW * NL::SIZE];
     const int nblk = gridDim.x;
     const int lb = xcd_remap(blockIdx.x, nblk);
     const int qg = lb % QG;
@@ -170,39 +265,92 @@ k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
     sp.init(r);
 
     const cplx<T>* Ff = Fhat + size_t(t) * 2 * n;
-    const cplx<T>* Fg = FGSAME ? Ff : Ff + n;
     const int qchunk = (M + QG - 1) / QG;
     const int q0 = qg * qchunk;
     const int q1 = (q0 + qchunk < M) ? q0 + qchunk : M;
 
-    cplx<T> acc[N], R1[N];
-    sfor<0, N>([&](auto I) { acc[I] = mk(T(0), T(0)); R1[I] = acc[I]; });
+    cplx<T> acc[N];
+    [[maybe_unused]] cplx<T> R1[NW == 2 ? N : 1];
+    sfor<0, N>([&](auto I) { acc[I] = mk(T(0), T(0)); });
 
     auto map1 = [](int pp, int& i1, int& i2) { i1 = pp / N; i2 = pp - (pp / N) * N; };  // (l1, l2)
     auto map3 = [](int P, int& o2, int& j) { o2 = P / NS; j = P - (P / NS) * NS; };      // (y, j)
 
     for (int q = q0; q < q1; ++q) {
         const cplx<T>* ph = phase + (size_t(p) * M + q) * 3 * N;   // [axis][l]
-        constexpr int NVAR = FGSAME ? 2 : 4;
-        sfor<0, NVAR>([&](auto Vi) {
-            constexpr int var = Vi;
-            // variants: 0 (f,+) 1 (g,-) 2 (f,-) 3 (g,+); FGSAME: 0 (f,+) 1 (f,-)
-            constexpr bool minus = (var == 1 || var == 2);
-            const cplx<T>* F = (var == 1 || var == 3) ? Fg : Ff;
-            E::run(V, sp, map1,
-                [&](int i1, int i2, int i3) {
-                    return cmul_conj_if(F[(size_t(i3) * N + i1) * N + i2], ph[2 * N + i3], minus);
+        sfor<0, NW>([&](auto Wi) {
+            constexpr int wv = Wi;
+            const cplx<T>* F = Ff + size_t(wv) * n;
+            E::run(V, sp,
+                [&] {
+                    if constexpr (wv != 0) return;
+                    // stage this q's Nyquist imaginary parts for residue r
+                    for (int i = threadIdx.x; i < NW * 6 * N * NS + NW * 2 * NN;
+                         i += E::NT) {
+                        if (i < NW * 6 * N * NS) {
+                            // A+- [y][j] and B+- [j][x] (both N*NS per sign)
+                            const int w = i / (6 * N * NS), k = i - w * 6 * N * NS;
+                            const int seg = k / (N * NS), e = k - seg * (N * NS);  // seg 0..5
+                            if (seg >= 4) continue;
+                            const T* src = nyq + ((((size_t(t) * NW + w) * Nrho + p) * M + q) * 6 +
+                                                  seg) * NN;
+                            T* dstw = Ny + w * NL::SIZE;
+                            if (seg < 2) {       // A: e = y*NS + j
+                                const int y = e / NS, j = e - y * NS;
+                                dstw[NL::A + seg * N * NS + e] = src[y * N + S * j + r];
+                            } else {             // B: e = j*N + x
+                                const int j = e / N, x = e - j * N;
+                                dstw[NL::B + (seg - 2) * NS * BP + j * BP + x] =
+                                    src[(S * j + r) * N + x];
+                            }
+                        } else {
+                            // C+- [y][x]
+                            const int k0 = i - NW * 6 * N * NS;
+                            const int w = k0 / (2 * NN), k = k0 - w * 2 * NN;
+                            const int sg = k / NN, e = k - sg * NN;
+                            const T* src = nyq + ((((size_t(t) * NW + w) * Nrho + p) * M + q) * 6 +
+                                                  4 + sg) * NN;
+                            Ny[w * NL::SIZE + NL::C + sg * N * BP + (e / N) * BP + e % N] = src[e];
+                        }
+                    }
                 },
-                [&](auto I, cplx<T> v) { return cmul_conj_if(v, ph[N + int(I)], minus); },
-                map3,
-                [&](auto I, cplx<T> v) { return cmul_conj_if(v, ph[int(I)], minus); },
-                [&](int, int, const cplx<T>* o) {
-                    if constexpr (var == 0 || var == 2) {
+                map1,
+                [&](int i1, int i2, int i3) {
+                    const cplx<T> e = ph[i1] * ph[N + i2] * ph[2 * N + i3];
+                    return (e.x + e.y) * F[(size_t(i3) * N + i1) * N + i2];
+                },
+                IdentityPre{}, map3, IdentityPre{},
+                [&](int o2, int j, const cplx<T>* o) {
+                    const int z = S * j + r;
+                    const T sy = (o2 & 1) ? T(-1) : T(1), sz = (z & 1) ? T(-1) : T(1);
+                    auto unpack = [&](int w, auto I, cplx<T> v, cplx<T>& up, cplx<T>& um) {
+                        const T* nw = Ny + w * NL::SIZE;
+                        constexpr T sx = (int(I) & 1) ? T(-1) : T(1);
+                        const T beta = sx * nw[NL::A + o2 * NS + j] +
+                                       sy * nw[NL::B + j * BP + int(I)] +
+                                       sz * nw[NL::C + o2 * BP + int(I)];
+                        const T delta = sx * nw[NL::A + N * NS + o2 * NS + j] +
+                                        sy * nw[NL::B + NS * BP + j * BP + int(I)] +
+                                        sz * nw[NL::C + N * BP + o2 * BP + int(I)];
+                        // w = (1 + i) v = u+ + i u-
+                        const T wr = v.x - v.y, wi = v.x + v.y;
+                        up = mk(wr + delta, beta);
+                        um = mk(wi - beta, delta);
+                    };
+                    if constexpr (NW == 1) {
+                        sfor<0, N>([&](auto I) {
+                            cplx<T> up, um;
+                            unpack(0, I, o[I], up, um);
+                            acc[I] = acc[I] + up * um;
+                        });
+                    } else if constexpr (wv == 0) {
                         sfor<0, N>([&](auto I) { R1[I] = o[I]; });
                     } else {
                         sfor<0, N>([&](auto I) {
-                            acc[I].x += R1[I].x * o[I].x - R1[I].y * o[I].y;
-                            acc[I].y += R1[I].x * o[I].y + R1[I].y * o[I].x;
+                            cplx<T> upf, umf, upg, umg;
+                            unpack(0, I, R1[I], upf, umf);
+                            unpack(1, I, o[I], upg, umg);
+                            acc[I] = acc[I] + upf * umg + umf * upg;
                         });
                     }
                 });
@@ -211,11 +359,12 @@ k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
     if (threadIdx.x < E::NPEN) {
         int o2, j;
         map3(threadIdx.x, o2, j);
-        const T scale = FGSAME ? T(1) : T(0.5);
+        const T scale = NW == 1 ? T(1) : T(0.5);
         // forward transform along x of this thread's pencil (first axis of
         // FFT3(acc_p)); the (y, z) plane transforms follow in k_plane_weight
-        sfor<0, N>([&](auto I) { R1[I] = scale * acc[I]; });
-        RegDFT<T, N, -1>::template run<1, 0>(R1, acc);
+        cplx<T> tmp[N];
+        sfor<0, N>([&](auto I) { tmp[I] = scale * acc[I]; });
+        RegDFT<T, N, -1>::template run<1, 0>(tmp, acc);
         // acc layout [t][p][qg][k1][r][y][j] with z = S*j + r
         cplx<T>* dst = acc_out + ((size_t(t) * Nrho + p) * QG + qg) * n;
         sfor<0, N>([&](auto I) {
@@ -342,23 +491,27 @@ struct CollisionPlan {
     void* acc = nullptr;    // [ntask][Nrho][QG][n] cplx<T>
     void* Iw = nullptr;     // [ntask][2][n] cplx<T>  forward-transform intermediate
     void* H = nullptr;      // [ntask][2][n] cplx<T>  plane-inverted loss / gain
+    void* nyq = nullptr;    // [ntask][2][Nrho][M][6][N*N] T  Nyquist-plane imaginary parts
     int num_cu = 256;
-    // optional stage timing: 5 events per call (before A, B, C, D, after D)
+    // optional stage timing: 6 events per call (before forward, nyquist,
+    // gain, plane, line, and after line)
     hipEvent_t* ev = nullptr;
     int ev_max = 0, ev_used = 0;
 };
 
+constexpr int kTimingEvents = 6;
+
 inline void timing_mark(CollisionPlan* pl, int stage, hipStream_t st) {
     if (pl->ev && pl->ev_used < pl->ev_max)
-        (void)hipEventRecord(pl->ev[pl->ev_used * 5 + stage], st);
+        (void)hipEventRecord(pl->ev[pl->ev_used * kTimingEvents + stage], st);
 }
 
 // -------------------------------------------------------------- launch
 
 template <typename T, int N>
 struct Impl {
-    static constexpr int S = pick_split<T>(N);          // gain kernel residue split
-    static constexpr int NT = round64(N * (N / S));
+    static constexpr int S1 = pick_split_gain<T, N, 1>();   // gain residue split, f = g
+    static constexpr int S2 = pick_split_gain<T, N, 2>();   // gain residue split, bilinear
     static constexpr int PL = planes_per_wg(N);
 
     static int launch(CollisionPlan* pl, const void* u, const void* ug, void* Q, TaskView tv,
@@ -375,30 +528,36 @@ struct Impl {
         hipLaunchKernelGGL((k_fwd_line<T, N>), dim3((nli + 255) / 256), dim3(256), 0, st, nli,
                            nfg, (const cplx<T>*)pl->Iw, (cplx<T>*)pl->Fhat);
         DGFS_HIP_CHECK(hipGetLastError());
-        const int nb = ntask * pl->Nrho * S * pl->QG;
         timing_mark(pl, 1, st);
-        if (fg_same)
-            hipLaunchKernelGGL((k_gain<T, N, S, true>), dim3(nb), dim3(NT), 0, st,
-                               (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->phase, pl->M,
-                               pl->Nrho, pl->QG, (cplx<T>*)pl->acc);
-        else
-            hipLaunchKernelGGL((k_gain<T, N, S, false>), dim3(nb), dim3(NT), 0, st,
-                               (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->phase, pl->M,
-                               pl->Nrho, pl->QG, (cplx<T>*)pl->acc);
+        hipLaunchKernelGGL((k_nyq<T, N>), dim3(ntask * nfg * pl->Nrho * pl->M), dim3(PL * N), 0,
+                           st, (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->phase, pl->M,
+                           pl->Nrho, nfg, (T*)pl->nyq);
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 2, st);
+        const int S = fg_same ? S1 : S2;
+        const int nb = ntask * pl->Nrho * S * pl->QG;
+        if (fg_same)
+            hipLaunchKernelGGL((k_gain<T, N, S1, 1>), dim3(nb), dim3(round64(N * (N / S1))), 0,
+                               st, (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->phase,
+                               (const T*)pl->nyq, pl->M, pl->Nrho, pl->QG, (cplx<T>*)pl->acc);
+        else
+            hipLaunchKernelGGL((k_gain<T, N, S2, 2>), dim3(nb), dim3(round64(N * (N / S2))), 0,
+                               st, (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->phase,
+                               (const T*)pl->nyq, pl->M, pl->Nrho, pl->QG, (cplx<T>*)pl->acc);
+        DGFS_HIP_CHECK(hipGetLastError());
+        timing_mark(pl, 3, st);
         hipLaunchKernelGGL((k_plane_weight<T, N>), dim3(ntask * N), dim3(PL * N), 0, st,
                            (const cplx<T>*)pl->acc, (const T*)pl->wtab,
                            (const cplx<T>*)pl->Fhat, (const T*)pl->bb2, fg_same, pl->Nrho,
                            pl->QG, S, (cplx<T>*)pl->H);
         DGFS_HIP_CHECK(hipGetLastError());
-        timing_mark(pl, 3, st);
+        timing_mark(pl, 4, st);
         const long nlo = long(ngroup) * NN;
         hipLaunchKernelGGL((k_line_output<T, N>), dim3((nlo + 255) / 256), dim3(256), 0, st,
                            (const T*)u, tv, (const cplx<T>*)pl->H, nlo, gstart, gslice,
                            T(pl->prefac), (T*)Q);
         DGFS_HIP_CHECK(hipGetLastError());
-        timing_mark(pl, 4, st);
+        timing_mark(pl, 5, st);
         if (pl->ev && pl->ev_used < pl->ev_max) ++pl->ev_used;
         return 0;
     }

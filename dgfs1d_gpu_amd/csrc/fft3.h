@@ -223,7 +223,18 @@ struct Fft3Engine {
     template <class Map1, class Load, class Pre2, class Map3, class Pre3, class Epi>
     static DGFS_DEV void run(cplx<T>* V, const Split& sp, Map1&& map1, Load&& load,
                              Pre2&& pre2, Map3&& map3, Pre3&& pre3, Epi&& epi) {
+        run(V, sp, [] {}, map1, load, pre2, map3, pre3, epi);
+    }
+
+    // As above; prologue() runs by every thread after the opening barrier
+    // (LDS owned by the caller may be refilled there: readers of the previous
+    // call have passed the barrier, readers in this call's epilogue follow
+    // the pass-1 barrier).
+    template <class Pro, class Map1, class Load, class Pre2, class Map3, class Pre3, class Epi>
+    static DGFS_DEV void run(cplx<T>* V, const Split& sp, Pro&& prologue, Map1&& map1,
+                             Load&& load, Pre2&& pre2, Map3&& map3, Pre3&& pre3, Epi&& epi) {
         __syncthreads();
+        prologue();
         pass1(V, sp, map1, load);
         __syncthreads();
         pass2(V, pre2);

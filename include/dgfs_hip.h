@@ -83,12 +83,13 @@ int dgfs_collide(dgfs_collision_plan_t plan, const void* u, const void* ug, void
 
 /* Stage timing (no counterpart in the reference, which only times the whole
  * loop, std/std.py:394-423): dgfs_collision_timing(plan, max_calls) creates
- * 5*max_calls HIP events (0 frees them); the next max_calls dgfs_collide calls
- * record an event before each of the four kernels (forward, gain, weight,
- * output) and after the last, on the launch stream.  timing_read waits for
- * the recorded events and returns the summed elapsed ms per stage. */
+ * 6*max_calls HIP events (0 frees them); the next max_calls dgfs_collide calls
+ * record an event before each of the five stages (forward transform, Nyquist
+ * planes, gain, plane weight, line output) and after the last, on the launch
+ * stream.  timing_read waits for the recorded events and returns the summed
+ * elapsed ms per stage. */
 int dgfs_collision_timing(dgfs_collision_plan_t plan, int max_calls);
-int dgfs_collision_timing_read(dgfs_collision_plan_t plan, double* ms /*[4]*/, int* ncalls);
+int dgfs_collision_timing_read(dgfs_collision_plan_t plan, double* ms /*[5]*/, int* ncalls);
 
 /* ---------------------------------------------------------------- transport
  * Boundary kinds (dgfs1D/std/bc.py):
