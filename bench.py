@@ -27,6 +27,15 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3      # MI355X published dense FP32
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per k_gain dispatch, from two rocprofv3 counter passes over this
+# script (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what
+# the same counter reads for a device-to-device copy of 512 MiB (--copy-calibration):
+# FETCH x 2.000, WRITE x 1.000.  Keyed by workload; re-measured whenever the
+# kernel changes, null for workloads not measured.
+GAIN_TRAFFIC_KIB = {
+    'couette Ne=4 K=3 N=32^3 M=12 Nrho=16 ssp-rk2 dt=0.0005': (32023.5, 2.000, 98304.0, 1.000),
+}
+
 
 def fft_flops(n):
     return 5.0 * n * math.log2(n)
@@ -66,6 +75,9 @@ def main(argv=None):
     ap.add_argument('--config', default=os.path.join(ROOT, 'dgfs1d_gpu_amd', 'configs',
                                                      'couette_prod.ini'))
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
+                    help='only run 5 device-to-device copies of MIB MiB (the known-size '
+                         'kernel that scales the FETCH_SIZE / WRITE_SIZE counters)')
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py measures one GPU only: this build has no multi-GPU path '
@@ -73,6 +85,15 @@ def main(argv=None):
         return 2
 
     import torch
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device='cuda:0')
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_calibration_bytes': nbytes, 'launches': 5}))
+        return 0
     from dgfs1d_gpu_amd.dictionary import Dictionary
     from dgfs1d_gpu_amd.std.std import DGFSStd1D
 
@@ -114,6 +135,10 @@ def main(argv=None):
     q_survey = (4 + 5 * B) * fft_flops(n) + 26 * B * n   # per Q, reference-structured count
     value = Ne * K * n * args.steps / wall
     q = plan.query()
+    workload = 'couette Ne=%d K=%d N=%d^3 M=%d Nrho=%d %s dt=%g' % (
+        Ne, K, N, s.vm.M(), s.vm.Nrho(), cfg.lookup('time-integrator', 'scheme'), s.dt)
+    tr = GAIN_TRAFFIC_KIB.get(workload)
+    traffic = None if tr is None else (tr[0] * tr[1] + tr[2] * tr[3]) * 1024.0
     out = {
         'metric': 'phase-space updates/sec at Ne/K/N^3',
         'value': value,
@@ -123,14 +148,13 @@ def main(argv=None):
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'f32' if cfg.dtype.__name__ == 'float32' else 'f64',
         'data': 'synthetic (deterministic Maxwellian initial state, diffuse walls)',
-        'config': {'workload': 'couette Ne=%d K=%d N=%d^3 M=%d Nrho=%d %s dt=%g' % (
-            Ne, K, N, s.vm.M(), s.vm.Nrho(), cfg.lookup('time-integrator', 'scheme'), s.dt),
+        'config': {'workload': workload,
             'Ne': Ne, 'K': K, 'N': N, 'M': s.vm.M(), 'Nrho': s.vm.Nrho(),
             'collisions_per_rhs': ntask, 'split_S': q['split'], 'qgroups': q['qgroups'],
             'pgroups': q['pgroups']},
         'roofline': {'bound': 'fp32-valu', 'kernel': 'k_gain', 'achieved': achieved,
                      'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': achieved / PEAK_FP32_TFLOPS, 'traffic': None,
+                     'frac': achieved / PEAK_FP32_TFLOPS, 'traffic': traffic,
                      'flops_per_launch': gain_flops, 'avg_launch_ms': gain_ms},
         'collision_stage_ms': stages,
         'collision_ms_per_rhs': coll_ms,
