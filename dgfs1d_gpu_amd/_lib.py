@@ -59,6 +59,11 @@ SIGNATURES = {
     'dgfs_moments': (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     'dgfs_residual_work_bytes': (_sz, []),
     'dgfs_residual': (_i, [_i, _l, _vp, _vp, _vp, _vp, _vp]),
+    'dgfs_projection_work_bytes': (_sz, [_i, _i]),
+    'dgfs_project_conservative': (_i, [_i, _i, _i, _vp, _vp, _vp, _dp, _vp, _vp, _i, _vp, _vp]),
+    'dgfs_shakov_work_bytes': (_sz, [_i, _i]),
+    'dgfs_shakov': (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp,
+                         _d, _d, _d, _d, _vp, _vp, _vp]),
 }
 
 

@@ -12,6 +12,51 @@ import numpy as np
 from . import _lib
 
 
+def task_table(K, Ne, tasks, tdtype, device):
+    """Device task table shared by the collision and relaxation entry points.
+
+    tasks: (out_slice, f_terms, g_terms) with terms [(slice, coef), ...]; the
+    table is sorted by output slice and grouped (one group per distinct output
+    slice): nterm [T][2], slice [T][2][K], coef [T][2][K] (tdtype), group_start
+    [G+1], group_slice [G], and fg_same = every task has identical f and g terms."""
+    import torch
+    tasks = sorted(tasks, key=lambda t: t[0])
+    T = len(tasks)
+    nterm = np.zeros((T, 2), np.int32)
+    sl = np.zeros((T, 2, K), np.int32)
+    cf = np.zeros((T, 2, K), np.float64)
+    gstart, gslice = [0], []
+    fg_same = True
+    nsl = int(K) * int(Ne)
+    for t, (out, fterms, gterms) in enumerate(tasks):
+        if not 0 <= out < nsl:
+            raise ValueError('output slice %d outside [0, K*Ne)' % out)
+        for w, terms in enumerate((fterms, gterms)):
+            if len(terms) > K:
+                raise ValueError('more than K terms in a task')
+            if any(not 0 <= s < nsl for s, _ in terms):
+                raise ValueError('input slice outside [0, K*Ne)')
+            nterm[t, w] = len(terms)
+            for a, (s, c) in enumerate(terms):
+                sl[t, w, a] = s
+                cf[t, w, a] = c
+        if sorted(fterms) != sorted(gterms):
+            fg_same = False
+        if t > 0 and out != tasks[t - 1][0]:
+            gstart.append(t)
+            gslice.append(tasks[t - 1][0])
+    if T:
+        gstart.append(T)
+        gslice.append(tasks[-1][0])
+    return dict(
+        K=int(K), Ne=int(Ne), ntask=T, ngroup=len(gslice), fg_same=int(fg_same),
+        nterm=torch.tensor(nterm, dtype=torch.int32, device=device),
+        slice=torch.tensor(sl, dtype=torch.int32, device=device),
+        coef=torch.tensor(cf, dtype=tdtype, device=device),
+        gstart=torch.tensor(np.array(gstart, np.int32), device=device),
+        gslice=torch.tensor(np.array(gslice, np.int32), device=device))
+
+
 class CollisionPlan:
     def __init__(self, N, M, Nrho, L, qz, qw, sz, sw, gamma, prefac, dtype, variant,
                  ntask_max, device='cuda'):
@@ -48,48 +93,20 @@ class CollisionPlan:
     def set_tasks(self, K, Ne, tasks):
         """tasks: list of (out_slice, f_terms, g_terms), terms = [(slice, coef), ...].
         Output slices are u-layout slice indices mode*Ne + elem."""
-        torch = self.torch
-        tasks = sorted(tasks, key=lambda t: t[0])
-        T = len(tasks)
-        if T > self.ntask_max:
-            raise ValueError('too many collision tasks (%d > %d)' % (T, self.ntask_max))
-        nterm = np.zeros((T, 2), np.int32)
-        sl = np.zeros((T, 2, K), np.int32)
-        cf = np.zeros((T, 2, K), np.float64)
-        gstart, gslice = [0], []
-        fg_same = True
-        for t, (out, fterms, gterms) in enumerate(tasks):
-            for w, terms in enumerate((fterms, gterms)):
-                if len(terms) > K:
-                    raise ValueError('more than K terms in a task')
-                nterm[t, w] = len(terms)
-                for a, (s, c) in enumerate(terms):
-                    sl[t, w, a] = s
-                    cf[t, w, a] = c
-            if sorted(fterms) != sorted(gterms):
-                fg_same = False
-            if t > 0 and out != tasks[t - 1][0]:
-                gstart.append(t)
-                gslice.append(tasks[t - 1][0])
-        if T:
-            gstart.append(T)
-            gslice.append(tasks[-1][0])
-        dev = self.device
-        self.tasks = dict(
-            K=int(K), Ne=int(Ne), ntask=T, ngroup=len(gslice), fg_same=int(fg_same),
-            nterm=torch.tensor(nterm, dtype=torch.int32, device=dev),
-            slice=torch.tensor(sl, dtype=torch.int32, device=dev),
-            coef=torch.tensor(cf, dtype=self.tdtype, device=dev),
-            gstart=torch.tensor(np.array(gstart, np.int32), device=dev),
-            gslice=torch.tensor(np.array(gslice, np.int32), device=dev))
+        if len(tasks) > self.ntask_max:
+            raise ValueError('too many collision tasks (%d > %d)' % (len(tasks), self.ntask_max))
+        self.tasks = task_table(K, Ne, tasks, self.tdtype, self.device)
         return self.tasks
 
-    def __call__(self, u, Q, stream=None, ug=None):
+    def __call__(self, u, Q, stream=None, ug=None, tasks=None):
         """Q += collision(u) for every task (stream-ordered, no host sync).
-        g-terms read ``ug`` when given (the separate d_g array of fs())."""
-        t = self.tasks
+        g-terms read ``ug`` when given (the separate d_g array of fs()).
+        ``tasks`` (a task_table) overrides the table of set_tasks()."""
+        t = self.tasks if tasks is None else tasks
         if t is None:
             raise RuntimeError('set_tasks() must be called first')
+        if t['ntask'] > self.ntask_max:
+            raise ValueError('too many collision tasks (%d > %d)' % (t['ntask'], self.ntask_max))
         for x in (u, Q) + ((ug,) if ug is not None else ()):
             if x.dtype != self.tdtype:
                 raise TypeError('collision operands must be %s' % self.tdtype)

@@ -1,0 +1,113 @@
+"""Host side of the relaxation-type collision term and the conservation projection.
+
+``Shakov`` evaluates the reference's 'shakov-direct-gll' model
+(dgfs1D/std/scattering.py:1636-1833) for a whole batch of (mode, element)
+tasks in two launches; ``ConservativeProjection`` removes the mass, momentum
+and energy moments of collision output slices in FP64 (no reference
+counterpart).  Both call the C-ABI of libdgfs_hip.so and never fall back to
+host arithmetic.
+"""
+import numpy as np
+
+from . import _lib
+from .collision import task_table
+
+
+def _grid_as_device(cv, tdtype):
+    """The velocity grid exactly as the kernels see it (rounded to tdtype)."""
+    import torch
+    np_dt = np.float64 if tdtype == torch.float64 else np.float32
+    return np.asarray(cv, dtype=np_dt).astype(np.float64)
+
+
+def collision_invariants(cv):
+    """Phi = {1, c_x, c_y, c_z, |c|^2} as a (5, n) FP64 array."""
+    cv = np.asarray(cv, dtype=np.float64)
+    return np.vstack([np.ones(cv.shape[1]), cv, (cv * cv).sum(0)])
+
+
+class ConservativeProjection:
+    """out (+)= Q - Phi^T (Phi Phi^T)^{-1} Phi Q per slice (FP64 moments and solve)."""
+
+    def __init__(self, cv, cx, cy, cz, tdtype, device):
+        import torch
+        self.torch = torch
+        self.tdtype = tdtype
+        self.device = device
+        self.n = int(np.asarray(cv).shape[1])
+        phi = collision_invariants(_grid_as_device(cv, tdtype))
+        self.gram_inv = np.ascontiguousarray(np.linalg.inv(phi @ phi.T))
+        self._gp = self.gram_inv.ctypes.data_as(_lib.C.POINTER(_lib.C.c_double))
+        self.cx, self.cy, self.cz = cx, cy, cz
+        self._work = None
+        self._buf = None
+
+    def work(self, nslice):
+        nb = _lib.lib().dgfs_projection_work_bytes(int(nslice), self.n)
+        if self._work is None or self._work.numel() * 8 < nb:
+            self._work = self.torch.empty(max(1, nb // 8), dtype=self.torch.float64,
+                                          device=self.device)
+        return self._work
+
+    def buffer(self, like):
+        """A zeroed scratch array shaped like ``like`` (kept between calls)."""
+        if self._buf is None or self._buf.numel() != like.numel():
+            self._buf = self.torch.empty_like(like)
+        self._buf.zero_()
+        return self._buf
+
+    def __call__(self, Q, out, nslice, accumulate=True, stream=None):
+        if Q.dtype != self.tdtype or out.dtype != self.tdtype:
+            raise TypeError('projection operands must be %s' % self.tdtype)
+        if Q.numel() < nslice * self.n or out.numel() < nslice * self.n:
+            raise ValueError('projection operands smaller than nslice*Nv')
+        w = self.work(nslice)
+        _lib.check(_lib.lib().dgfs_project_conservative(
+            _lib.dtype_code(self.tdtype), int(nslice), self.n, _lib.ptr(self.cx),
+            _lib.ptr(self.cy), _lib.ptr(self.cz), self._gp, _lib.ptr(Q), _lib.ptr(out),
+            int(bool(accumulate)), _lib.ptr(w), _lib.stream_ptr(stream)))
+
+
+class Shakov:
+    """Batched Shakov relaxation term Q += prefac rho T^{1-omega} (f_S - f)."""
+
+    def __init__(self, n, cx, cy, cz, cw, prefac, omega, Pr, tdtype, device):
+        import torch
+        self.torch = torch
+        self.n = int(n)
+        self.cx, self.cy, self.cz = cx, cy, cz
+        self.cw, self.prefac, self.omega, self.Pr = float(cw), float(prefac), float(omega), float(Pr)
+        self.tdtype = tdtype
+        self.device = device
+        self.tasks = None
+        self._work = None
+
+    def set_tasks(self, K, Ne, tasks):
+        """tasks: (out_slice, f_terms[, g_terms]); only the f terms are used
+        (the reference's fs() reads d_arr_in only, scattering.py:1786-1790)."""
+        tasks = [(t[0], t[1], t[1]) for t in tasks]
+        self.tasks = task_table(K, Ne, tasks, self.tdtype, self.device)
+        return self.tasks
+
+    def work(self, ntask):
+        nb = _lib.lib().dgfs_shakov_work_bytes(max(1, int(ntask)), self.n)
+        if self._work is None or self._work.numel() * 8 < nb:
+            self._work = self.torch.empty(max(1, nb // 8), dtype=self.torch.float64,
+                                          device=self.device)
+        return self._work
+
+    def __call__(self, u, Q, stream=None, tasks=None):
+        t = self.tasks if tasks is None else tasks
+        if t is None:
+            raise RuntimeError('set_tasks() must be called first')
+        if u.dtype != self.tdtype or Q.dtype != self.tdtype:
+            raise TypeError('Shakov operands must be %s' % self.tdtype)
+        nmin = t['K'] * t['Ne'] * self.n
+        if u.numel() < nmin or Q.numel() < nmin:
+            raise ValueError('Shakov operands smaller than K*Ne*Nv')
+        _lib.check(_lib.lib().dgfs_shakov(
+            _lib.dtype_code(self.tdtype), t['K'], t['Ne'], self.n, t['ntask'],
+            _lib.ptr(t['nterm']), _lib.ptr(t['slice']), _lib.ptr(t['coef']), t['ngroup'],
+            _lib.ptr(t['gstart']), _lib.ptr(t['gslice']), _lib.ptr(u), _lib.ptr(self.cx),
+            _lib.ptr(self.cy), _lib.ptr(self.cz), self.cw, self.prefac, self.omega, self.Pr,
+            _lib.ptr(Q), _lib.ptr(self.work(t['ntask'])), _lib.stream_ptr(stream)))

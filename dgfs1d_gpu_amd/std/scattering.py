@@ -1,17 +1,28 @@
 """Scattering-model plugins (reference dgfs1D/std/scattering.py).
 
-``vhs-gll`` (:51-313) and ``vhs`` (:317-429) keep the reference constructor
-``(cfg, vm, Ne=Ne)``, the ``prefac``/``omega`` properties and the single-node
+``vhs-gll`` (:51-313), ``vhs`` (:317-429) and ``shakov-direct-gll``
+(:1636-1833) keep the reference constructor ``(cfg, vm, Ne=Ne)``, the
+``prefac``/``omega`` properties and the single-node
 ``fs(d_f, d_g, d_Q, elem, modein, modeout)``; the solver uses the batched
 ``collide(u, Q)``, which evaluates every (m, r in sigModes[m], e) collision of
 one RHS in one set of launches with the U/V contraction fused (std.py:352-356).
+
+``[scattering-model] projection = conservative`` (default ``none``, the
+reference behaviour) routes the collision output through the FP64
+mass/momentum/energy projection before it is added to the RHS.
 """
 from math import gamma
 
 import numpy as np
 
 from .. import _lib
-from ..collision import CollisionPlan
+from ..collision import CollisionPlan, task_table
+from ..relax import ConservativeProjection, Shakov
+
+
+def _torch_dtype(cfg):
+    import torch
+    return torch.float64 if cfg.dtype == np.float64 else torch.float32
 
 
 class DGFSScatteringModelStd:
@@ -20,8 +31,34 @@ class DGFSScatteringModelStd:
     def __init__(self, cfg, velocitymesh, **kwargs):
         self.cfg = cfg
         self.vm = velocitymesh
+        self._Ne = kwargs.get('Ne')
+        self._device = kwargs.get('device')
+        self._ntask_max = kwargs.get('ntask_max')
         self.load_parameters()
         self.perform_precomputation()
+        self._projection = self._make_projection()
+
+    def _make_projection(self):
+        mode = self.cfg.lookupordefault('scattering-model', 'projection', 'none')
+        if mode == 'none':
+            return None
+        if mode != 'conservative':
+            raise ValueError("scattering-model::projection must be 'none' or 'conservative'")
+        vm = self.vm
+        return ConservativeProjection(vm.cv(), vm.d_cvx(), vm.d_cvy(), vm.d_cvz(),
+                                      _torch_dtype(self.cfg), self._device)
+
+    def collide(self, u, Q, stream=None):
+        """Q += the collision term of every task (projected when enabled)."""
+        if self._projection is None:
+            self._collide(u, Q, stream)
+            return
+        buf = self._projection.buffer(Q)
+        self._collide(u, buf, stream)
+        self._projection(buf, Q, Q.numel() // self.vm.vsize(), accumulate=True, stream=stream)
+
+    def _collide(self, u, Q, stream=None):
+        raise NotImplementedError
 
     def load_parameters(self):
         raise NotImplementedError
@@ -43,9 +80,6 @@ class DGFSVHSGLLScatteringModelStd(DGFSScatteringModelStd):
     _variant = _lib.VHS_GLL
 
     def __init__(self, cfg, velocitymesh, **kwargs):
-        self._Ne = kwargs.get('Ne')
-        self._device = kwargs.get('device')
-        self._ntask_max = kwargs.get('ntask_max')
         kind = cfg.lookupordefault('basis', 'kind', 'nodal-sem-gll')
         if self.scattering_model == 'vhs-gll' and kind not in (
                 'nodal-sem-gll', 'nodal-gll', 'nodal-sem-gll-2'):
@@ -96,7 +130,7 @@ class DGFSVHSGLLScatteringModelStd(DGFSScatteringModelStd):
         self.plan.set_tasks(K, Ne, _fold_same_slice(tasks))
         self._single = {}
 
-    def collide(self, u, Q, stream=None):
+    def _collide(self, u, Q, stream=None):
         self.plan(u, Q, stream)
 
     # ---------------------------------------------------- single node fs()
@@ -104,21 +138,77 @@ class DGFSVHSGLLScatteringModelStd(DGFSScatteringModelStd):
         """d_Q[modeout, elem] += Q(d_f[modein, elem], d_g[modein, elem]) on
         (K, Ne, Nv) arrays -- the reference's per-node call
         (std/scattering.py:251-313 'vhs-gll' uses d_f only, :346-429 'vhs').
-        Slow path (one task per call); the solver uses collide()."""
-        K = d_Q.numel() // (self._Ne * self.vm.vsize())
-        key = (K, elem, modein, modeout)
-        if self._single.get('key') != key:
-            s = modein * self._Ne + elem
-            self.plan.set_tasks(K, self._Ne, [(modeout * self._Ne + elem, [(s, 1.0)],
-                                               [(s, 1.0)])])
-            self._single['key'] = key
+        Slow path (one task per call, never projected); the solver uses collide()."""
+        tab = _single_task(self._single, self._Ne, self.vm.vsize(), d_Q, elem, modein, modeout,
+                           self.plan.tdtype, self._device)
         gl = self._variant == _lib.VHS_GLL or d_g is d_f
-        self.plan(d_f, d_Q, ug=None if gl else d_g)
+        self.plan(d_f, d_Q, ug=None if gl else d_g, tasks=tab)
 
 
 class DGFSVHSScatteringModelStd(DGFSVHSGLLScatteringModelStd):
     scattering_model = 'vhs'
     _variant = _lib.VHS
+
+
+class DGFSShakovDirectGLLScatteringModelStd(DGFSScatteringModelStd):
+    """BGK-Shakov relaxation, iteration-free (std/scattering.py:1636-1833)."""
+    scattering_model = 'shakov-direct-gll'
+
+    def __init__(self, cfg, velocitymesh, **kwargs):
+        kind = cfg.lookupordefault('basis', 'kind', 'nodal-sem-gll')
+        if kind not in ('nodal-sem-gll', 'nodal-gll'):
+            raise RuntimeError('Only tested for nodal basis')
+        super().__init__(cfg, velocitymesh, **kwargs)
+
+    def load_parameters(self):
+        """prefac = t0 p0 / mu(T0), t0 = H0/u0 (std/scattering.py:1645-1661)."""
+        cfg, vm = self.cfg, self.vm
+        Pr = float(cfg.lookupordefault('scattering-model', 'Pr', 2. / 3.))
+        omega = float(cfg.lookupfloat('scattering-model', 'omega'))
+        muRef = float(cfg.lookupfloat('scattering-model', 'muRef'))
+        Tref = float(cfg.lookupfloat('scattering-model', 'Tref'))
+        t0 = vm.H0() / vm.u0()
+        visc = muRef * ((vm.T0() / Tref) ** omega)
+        p0 = vm.n0() * vm.R0 / vm.NA * vm.T0()
+        self._prefactor = t0 * p0 / visc
+        self._omega = omega
+        self._Pr = Pr
+
+    @property
+    def Pr(self):
+        return self._Pr
+
+    def perform_precomputation(self):
+        vm = self.vm
+        self.model = Shakov(vm.vsize(), vm.d_cvx(), vm.d_cvy(), vm.d_cvz(), vm.cw(),
+                            self._prefactor, self._omega, self._Pr, _torch_dtype(self.cfg),
+                            self._device)
+        self._single = {}
+
+    def set_tasks(self, K, Ne, tasks):
+        self.model.set_tasks(K, Ne, tasks)
+
+    def _collide(self, u, Q, stream=None):
+        self.model(u, Q, stream)
+
+    def fs(self, d_f, d_g, d_Q, elem, modein, modeout):
+        """d_Q[modeout, elem] += Q_S(d_f[modein, elem]); d_g is not read
+        (std/scattering.py:1786-1833).  Slow path, never projected."""
+        tab = _single_task(self._single, self._Ne, self.vm.vsize(), d_Q, elem, modein, modeout,
+                           self.model.tdtype, self._device)
+        self.model(d_f, d_Q, tasks=tab)
+
+
+def _single_task(cache, Ne, Nv, d_Q, elem, modein, modeout, tdtype, device):
+    """Cached one-task table: slice (modein, elem) -> slice (modeout, elem)."""
+    K = d_Q.numel() // (Ne * Nv)
+    key = (K, elem, modein, modeout)
+    if cache.get('key') != key:
+        s = modein * Ne + elem
+        cache['tab'] = task_table(K, Ne, [(modeout * Ne + elem, [(s, 1.0)], [(s, 1.0)])],
+                                  tdtype, device)
+        cache['key'] = key
+    return cache['tab']
 
 
 def _fold_same_slice(tasks):

@@ -168,6 +168,44 @@ size_t dgfs_residual_work_bytes(void);
 int dgfs_residual(int dtype, long n, const void* cur, const void* prev, double* work,
                   double* out2, hipStream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * Conservation projection (FP64).  No reference counterpart: the reference's
+ * spectral Q conserves mass, momentum and energy only to spectral accuracy
+ * (std/scattering.py:251-429); its one discrete-conservation mechanism is the
+ * BGK Newton solve (std/scattering.py:647-695).
+ *
+ * For each of nslice slices Q_s (Nv points), with Phi = {1, cx, cy, cz, |c|^2}
+ * (5 x Nv) and gram_inv = (Phi Phi^T)^{-1} (5x5 row-major, host memory):
+ *     out_s (+)= Q_s - Phi^T gram_inv Phi Q_s
+ * the Euclidean-nearest slice whose five moments vanish.  Moments, the 5x5
+ * solve and the correction are evaluated in FP64.  out may alias Q when
+ * accumulate == 0.  work: dgfs_projection_work_bytes(nslice, Nv) bytes. */
+size_t dgfs_projection_work_bytes(int nslice, int Nv);
+int dgfs_project_conservative(int dtype, int nslice, int Nv, const void* cx, const void* cy,
+                              const void* cz, const double* gram_inv, const void* Q, void* out,
+                              int accumulate, double* work, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Shakov relaxation model 'shakov-direct-gll' (std/scattering.py:1636-1833,
+ * std/kernels/scattering/shakov-direct-gll.mako:15-167): its fs(d_f, d_g, d_Q,
+ * elem, modein, modeout) for every task of a batch in two launches.
+ *
+ * Task table: the layout of dgfs_collide, f-terms only (task_nterm[t*2],
+ * task_slice[(t*2)*K + a], task_coef[(t*2)*K + a]); tasks sorted by output
+ * slice, group g = tasks [group_start[g], group_start[g+1]) writing slice
+ * group_slice[g].  For each task, with f_t = sum_a coef_a u[slice_a]:
+ *   rho = cw sum f, u_i = sum f c_i / sum f, T = sum f |c-u|^2 cw / (1.5 rho),
+ *   q_i = cw sum f |c-u|^2 (c-u)_i,
+ *   f_S = rho/(pi T)^{3/2} exp(-|c-u|^2/T) (1 + 4(1-Pr)/5 (q.(c-u))/(rho T^2) (|c-u|^2/T - 5/2))
+ *   Q[group slice] += prefac rho T^{1-omega} (f_S - f_t)
+ * Moments accumulate in FP64.  work: dgfs_shakov_work_bytes(ntask, Nv) bytes. */
+size_t dgfs_shakov_work_bytes(int ntask, int Nv);
+int dgfs_shakov(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nterm,
+                const int* task_slice, const void* task_coef, int ngroup,
+                const int* group_start, const int* group_slice, const void* u, const void* cx,
+                const void* cy, const void* cz, double cw, double prefac, double omega,
+                double Pr, void* Q, double* work, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,7 @@ from configparser import ConfigParser
 import numpy as np
 
 from . import collision as OC
+from . import relax as OR
 from . import std as OS
 from .basis import OracleBasis
 from .velocitymesh import VelocityMesh
@@ -55,14 +56,27 @@ class Deck:
         # scattering
         omega = g('scattering-model', 'omega')
         self.model = cp.get('scattering-model', 'type')
-        self.prefac, self.invKn = OC.vhs_prefactor(H0, vm.n0, vm.T0, omega,
-                                                   g('scattering-model', 'dRef'),
-                                                   g('scattering-model', 'Tref'))
-        self.pre = OC.Precomp(Nv, vm.L, vm.qz, vm.qw, vm.sz, vm.sw, 2.0 * (1.0 - omega))
+        shakov = None
+        if self.model == 'shakov-direct-gll':
+            Pr = g('scattering-model', 'Pr') if cp.has_option('scattering-model', 'Pr') \
+                else 2.0 / 3.0
+            self.prefac = OR.shakov_prefactor(H0, vm.u0, vm.T0, vm.n0,
+                                              g('scattering-model', 'muRef'),
+                                              g('scattering-model', 'Tref'), omega)
+            shakov = (omega, Pr)
+            self.pre = None
+        else:
+            self.prefac, self.invKn = OC.vhs_prefactor(H0, vm.n0, vm.T0, omega,
+                                                       g('scattering-model', 'dRef'),
+                                                       g('scattering-model', 'Tref'))
+            self.pre = OC.Precomp(Nv, vm.L, vm.qz, vm.qw, vm.sz, vm.sw, 2.0 * (1.0 - omega))
+        project = cp.has_option('scattering-model', 'projection') and \
+            cp.get('scattering-model', 'projection') == 'conservative'
         # boundaries
         self.walls = [self._wall('soln-bcs-xlo', -1.0), self._wall('soln-bcs-xhi', 1.0)]
         self.op = OS.StdOracle(self.basis, self.invjac, vm, self.walls[0], self.walls[1],
-                               self.pre, self.prefac, model=self.model)
+                               self.pre, self.prefac, model=self.model, shakov=shakov,
+                               project=project)
         # time
         ti, dt, tf = g('time-integrator', 'tstart'), g('time-integrator', 'dt'), \
             g('time-integrator', 'tend')

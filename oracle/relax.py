@@ -1,0 +1,67 @@
+"""Oracle: Shakov relaxation term and the conservation projection (FP64).
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+``q_shakov`` restates, step for step, the reference's per-node
+'shakov-direct-gll' fs() (dgfs1D/std/scattering.py:1786-1833) with the
+kernels of std/kernels/scattering/shakov-direct-gll.mako:
+  flocKern                       :15-29    f = d_f[(modein, elem)]
+  mom1 + sums + mom01Norm        :32-61    u_i = sum f c_i / sum f, rho = cw sum f
+  mom23Sk + sums + mom23SkNorm   :63-104   T = cw sum f C^2 / (1.5 rho),
+                                           q_i = cw sum f C^2 C_i
+  equiSkDistCompute              :106-142  f_S
+  output                         :145-167  Q += prefac rho / T^(omega-1) (f_S - f)
+
+``project_conservative`` is the Euclidean-nearest slice with vanishing
+mass, momentum and energy moments (no reference counterpart), computed with
+a least-squares solve so that it does not share the product's Gram-inverse
+formulation.
+"""
+import numpy as np
+
+
+def shakov_moments(f, cv, cw):
+    """(rho, u (3,), T, q (3,)) by the reference's two-pass sequence."""
+    s0 = f.sum()
+    u = np.array([(f * cv[0]).sum(), (f * cv[1]).sum(), (f * cv[2]).sum()]) / s0
+    rho = s0 * cw
+    C = cv - u[:, None]
+    C2 = (C * C).sum(0)
+    T = (f * C2).sum() * cw / (1.5 * rho)
+    q = np.array([(f * C2 * C[i]).sum() for i in range(3)]) * cw
+    return rho, u, T, q
+
+
+def shakov_fS(f, cv, cw, Pr):
+    """Shakov target distribution (mako:106-142)."""
+    rho, u, T, q = shakov_moments(f, cv, cw)
+    C = cv - u[:, None]
+    C2 = (C * C).sum(0)
+    fe = rho / (np.pi * T) ** 1.5 * np.exp(-C2 / T)
+    qc = q[0] * C[0] + q[1] * C[1] + q[2] * C[2]
+    return fe * (1.0 + (1.0 - Pr) / 5.0 * 4.0 * qc / (rho * T * T) * (C2 / T - 2.5))
+
+
+def q_shakov(f, cv, cw, prefac, omega, Pr):
+    """Increment added to Q[modeout, elem] by one fs() call."""
+    rho, _, T, _ = shakov_moments(f, cv, cw)
+    return prefac * (rho / T ** (omega - 1.0)) * (shakov_fS(f, cv, cw, Pr) - f)
+
+
+def shakov_prefactor(H0, u0, T0, n0, muRef, Tref, omega, R0=8.3144598, NA=6.0221409e+23):
+    """t0 p0 / mu(T0) with t0 = H0/u0, p0 = n0 R0/NA T0 (std/scattering.py:1645-1661)."""
+    t0 = H0 / u0
+    visc = muRef * (T0 / Tref) ** omega
+    p0 = n0 * R0 / NA * T0
+    return t0 * p0 / visc
+
+
+def invariants(cv):
+    return np.vstack([np.ones(cv.shape[1]), cv, (cv * cv).sum(0)])
+
+
+def project_conservative(Q, cv):
+    """Q - Phi^T lam with lam the least-squares coefficients of Q on span(Phi)."""
+    phi = invariants(np.asarray(cv, dtype=np.float64))
+    lam, *_ = np.linalg.lstsq(phi.T, np.asarray(Q, dtype=np.float64), rcond=None)
+    return Q - phi.T @ lam

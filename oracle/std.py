@@ -17,6 +17,7 @@ Restates, with explicit face arrays as the reference kernels use them:
 import numpy as np
 
 from . import collision as OC
+from . import relax as OR
 
 
 def maxwellian(cv, rho, ux, uy, uz, T):
@@ -53,10 +54,14 @@ class Wall:
 class StdOracle:
     """FP64 dgfsStd1D operator.  u has shape (K, Ne, n)."""
 
-    def __init__(self, basis, invjac, vm, wall_l, wall_r, pre, prefac, model='vhs'):
+    def __init__(self, basis, invjac, vm, wall_l, wall_r, pre, prefac, model='vhs',
+                 shakov=None, project=False):
+        """shakov = (omega, Pr) for model 'shakov-direct-gll'; project applies
+        the conservation projection to each output slice's collision sum."""
         self.b, self.invjac, self.vm = basis, np.asarray(invjac).ravel(), vm
         self.wl, self.wr = wall_l, wall_r
         self.pre, self.prefac, self.model = pre, prefac, model
+        self.shakov, self.project = shakov, project
         self.K = basis.K
         self.Ne = len(self.invjac)
         self.Nqf = basis.face.shape[0]
@@ -85,15 +90,23 @@ class StdOracle:
 
     def collisions(self, u, ux):
         K, Ne = self.K, self.Ne
+        vm = self.vm
         for m in range(K):
             f = np.einsum('ra,aen->ren', self.b.U[m].T, u)              # trans_U
             g = np.einsum('rb,ben->ren', self.b.V[m], u)                # trans_V
-            for r in self.b.sig[m]:
-                for e in range(Ne):
+            for e in range(Ne):
+                acc = np.zeros(u.shape[-1])
+                for r in self.b.sig[m]:
                     if self.model == 'vhs':
-                        ux[m, e] += OC.q_vhs(self.pre, f[r, e], g[r, e], self.prefac)
+                        acc += OC.q_vhs(self.pre, f[r, e], g[r, e], self.prefac)
+                    elif self.model == 'vhs-gll':
+                        acc += OC.q_vhs_gll(self.pre, f[r, e], self.prefac)
+                    elif self.model == 'shakov-direct-gll':
+                        omega, Pr = self.shakov
+                        acc += OR.q_shakov(f[r, e], vm.cv, vm.cw, self.prefac, omega, Pr)
                     else:
-                        ux[m, e] += OC.q_vhs_gll(self.pre, f[r, e], self.prefac)
+                        raise ValueError(self.model)
+                ux[m, e] += OR.project_conservative(acc, vm.cv) if self.project else acc
         return ux
 
     def rhs(self, t, u):

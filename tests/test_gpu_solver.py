@@ -148,6 +148,45 @@ def test_rhs_boundaries(cuda, lo, hi, model):
     assert err < 1e-11
 
 
+@pytest.mark.parametrize('kind', ['nodal-sem-gll', 'nodal-gll'])
+def test_rhs_shakov(cuda, kind):
+    """'shakov-direct-gll' plugin in the full RHS (std/scattering.py:1636-1833)."""
+    import torch
+    deck, solver, u = _pair(cuda, {('scattering-model', 'type'): 'shakov-direct-gll',
+                                   ('scattering-model', 'muRef'): 2.117e-5,
+                                   ('basis', 'kind'): kind})
+    out = torch.zeros_like(solver.u)
+    solver.rhs(0.0, solver.u, out)
+    ref = deck.op.rhs(0.0, u)
+    err = _rel(out.cpu().numpy().reshape(ref.shape), ref)
+    print('shakov', kind, err)
+    assert err < 1e-11
+
+
+@pytest.mark.parametrize('model', ['vhs', 'shakov-direct-gll'])
+def test_rhs_projection(cuda, model):
+    """projection = conservative: RHS parity and vanishing collision moments."""
+    import torch
+    over = {('scattering-model', 'type'): model, ('scattering-model', 'muRef'): 2.117e-5,
+            ('scattering-model', 'projection'): 'conservative'}
+    deck, solver, u = _pair(cuda, over)
+    out = torch.zeros_like(solver.u)
+    solver.rhs(0.0, solver.u, out)
+    ref = deck.op.rhs(0.0, u)
+    err = _rel(out.cpu().numpy().reshape(ref.shape), ref)
+    print('projection rhs', model, err)
+    assert err < 1e-11
+    # collision part alone (nodal basis: invMass is diagonal, apply it back)
+    Q = torch.zeros_like(solver.u)
+    solver.sm.collide(solver.u, Q)
+    q = Q.cpu().numpy().reshape(solver.K * solver.Ne, -1)
+    from oracle import relax as OR
+    phi = OR.invariants(deck.vm.cv)
+    m = np.abs(phi @ q.T) / (np.abs(phi) @ np.abs(q).T)
+    print('collision moments after projection', m.max())
+    assert m.max() < 1e-12
+
+
 @pytest.mark.parametrize('scheme', ['euler', 'ssp-rk2', 'ssp-rk3', 'lse-rk45'])
 def test_integrators(cuda, scheme):
     deck, solver, u = _pair(cuda, {('time-integrator', 'scheme'): scheme})

@@ -1,0 +1,134 @@
+"""Pins of the Shakov and projection oracles (CPU) by facts the mathematics fixes.
+
+Shakov: on a grid wide enough for the Gaussian tails, the discrete moments of
+f_S equal (rho, u, T) of f and its heat flux equals (1 - Pr) q -- for the
+reference's non-dimensionalisation (variance T/2 per axis) this fixes the
+factor 4 (1 - Pr)/5 and the 5/2 of the correction; a Maxwellian is a fixed
+point (Q_S = 0); Q_S conserves mass, momentum and energy.  Planted mistakes
+in those constants fail the moment test.
+Projection: the output has vanishing invariant moments, differs from the input
+by an element of span{1, c, |c|^2}, returns inputs orthogonal to that span
+unchanged and annihilates elements of the span.
+"""
+import numpy as np
+import pytest
+
+from oracle import relax as OR
+
+
+def grid(N=32, L=7.0):
+    c0 = np.linspace(-L + L / N, L - L / N, N)
+    I, J, K = np.meshgrid(c0, c0, c0, indexing='ij')
+    cv = np.stack([I.ravel(), J.ravel(), K.ravel()])
+    return cv, (2 * L / N) ** 3
+
+
+def maxw(cv, rho, u, T):
+    u = np.asarray(u, float).reshape(3, 1)
+    return rho / (np.pi * T) ** 1.5 * np.exp(-((cv - u) ** 2).sum(0) / T)
+
+
+def bimodal(cv):
+    return 0.6 * maxw(cv, 1.0, (0.3, 0.1, 0.0), 0.8) + 0.5 * maxw(cv, 0.9, (-0.2, 0.25, 0.1), 1.3)
+
+
+def moments_of(g, cv, cw):
+    """Direct discrete moments (rho, u, T, q) of any g (independent loop)."""
+    rho = g.sum() * cw
+    u = (g[None, :] * cv).sum(1) * cw / rho
+    C = cv - u[:, None]
+    C2 = (C * C).sum(0)
+    T = 2.0 / 3.0 * (g * C2).sum() * cw / rho
+    q = (g[None, :] * C2[None, :] * C).sum(1) * cw
+    return rho, u, T, q
+
+
+@pytest.mark.parametrize('Pr', [2.0 / 3.0, 0.5, 1.0])
+def test_shakov_target_moments(Pr):
+    cv, cw = grid()
+    f = bimodal(cv)
+    rho, u, T, q = moments_of(f, cv, cw)
+    assert np.abs(q).max() > 1e-2                       # non-trivial heat flux
+    fS = OR.shakov_fS(f, cv, cw, Pr)
+    rS, uS, TS, qS = moments_of(fS, cv, cw)
+    assert abs(rS - rho) / rho < 1e-11
+    assert np.abs(uS - u).max() < 1e-11
+    assert abs(TS - T) / T < 1e-11
+    assert np.abs(qS - (1.0 - Pr) * q).max() < 1e-10 * np.abs(q).max() + 1e-14
+
+
+def test_shakov_moment_state_matches_direct():
+    cv, cw = grid()
+    f = bimodal(cv)
+    rho, u, T, q = OR.shakov_moments(f, cv, cw)
+    r2, u2, T2, q2 = moments_of(f, cv, cw)
+    assert abs(rho - r2) / r2 < 1e-14 and np.abs(u - u2).max() < 1e-14
+    assert abs(T - T2) / T2 < 1e-13 and np.abs(q - q2).max() < 1e-13
+
+
+def _mutant_fS(f, cv, cw, Pr, k4=4.0, half5=2.5):
+    rho, u, T, q = OR.shakov_moments(f, cv, cw)
+    C = cv - u[:, None]
+    C2 = (C * C).sum(0)
+    fe = rho / (np.pi * T) ** 1.5 * np.exp(-C2 / T)
+    qc = (q[:, None] * C).sum(0)
+    return fe * (1.0 + (1.0 - Pr) / 5.0 * k4 * qc / (rho * T * T) * (C2 / T - half5))
+
+
+@pytest.mark.parametrize('k4,half5', [(2.0, 2.5), (4.0, 5.0), (4.0, 1.5)])
+def test_planted_shakov_constant_fails(k4, half5):
+    cv, cw = grid()
+    f = bimodal(cv)
+    _, _, T, q = moments_of(f, cv, cw)
+    Pr = 2.0 / 3.0
+    _, uS, TS, qS = moments_of(_mutant_fS(f, cv, cw, Pr, k4, half5), cv, cw)
+    bad = np.abs(qS - (1.0 - Pr) * q).max() > 1e-6 * np.abs(q).max() or \
+        abs(TS - T) / T > 1e-6
+    assert bad
+
+
+def test_shakov_maxwellian_fixed_point():
+    cv, cw = grid()
+    f = maxw(cv, 1.3, (0.2, -0.1, 0.05), 0.9)
+    Q = OR.q_shakov(f, cv, cw, prefac=2.0, omega=0.81, Pr=2.0 / 3.0)
+    assert np.abs(Q).max() < 1e-10 * np.abs(f).max()
+
+
+def test_shakov_conserves():
+    cv, cw = grid()
+    f = bimodal(cv)
+    Q = OR.q_shakov(f, cv, cw, prefac=1.7, omega=0.81, Pr=2.0 / 3.0)
+    phi = OR.invariants(cv)
+    m = phi @ Q * cw
+    scale = np.abs(phi) @ np.abs(Q) * cw
+    assert np.abs(m / scale).max() < 1e-11
+
+
+def test_shakov_collision_frequency_scaling():
+    # Q_S(f) carries rho T^{1-omega}: doubling rho at fixed shape doubles the
+    # frequency and the amplitude (Q(2f) = 4 Q(f)).
+    cv, cw = grid()
+    f = bimodal(cv)
+    Q1 = OR.q_shakov(f, cv, cw, 1.0, 0.7, 2.0 / 3.0)
+    Q2 = OR.q_shakov(2.0 * f, cv, cw, 1.0, 0.7, 2.0 / 3.0)
+    assert np.abs(Q2 - 4.0 * Q1).max() < 1e-12 * np.abs(Q2).max()
+
+
+def test_projection_properties():
+    rng = np.random.default_rng(5)
+    cv, _ = grid(N=12, L=5.0)
+    phi = OR.invariants(cv)
+    Q = rng.standard_normal(cv.shape[1])
+    Qc = OR.project_conservative(Q, cv)
+    assert np.abs(phi @ Qc).max() < 1e-12 * np.abs(phi).max() * np.abs(Q).sum()
+    # the correction lies in span(phi)
+    d = Q - Qc
+    lam, *_ = np.linalg.lstsq(phi.T, d, rcond=None)
+    assert np.abs(phi.T @ lam - d).max() < 1e-12 * np.abs(d).max()
+    # idempotent, annihilates the span, keeps the orthogonal complement
+    assert np.abs(OR.project_conservative(Qc, cv) - Qc).max() < 1e-12 * np.abs(Qc).max()
+    span = phi.T @ rng.standard_normal(5)
+    assert np.abs(OR.project_conservative(span, cv)).max() < 1e-10 * np.abs(span).max()
+    # minimal norm: any other conservative candidate is farther from Q
+    other = Qc + 1e-3 * OR.project_conservative(rng.standard_normal(Q.shape), cv)
+    assert np.linalg.norm(Q - other) > np.linalg.norm(Q - Qc)
