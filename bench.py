@@ -2,13 +2,16 @@
 
 Workload (dgfs1d_gpu_amd/configs/couette_prod.ini): single-species VHS
 Couette, Ne=4, K=3 (nodal-sem-gll), N=32^3, M=12, N_rho=16, SSP-RK2,
-dt=5e-4, FP32 spectral arithmetic, FP64 wall reductions.  One step = one
+dt=5e-4, FP32 spectral arithmetic, FP64 wall reductions and FP64
+conservation projection of the collision output.  One step = one
 SSP-RK2 time step (2 RHS evaluations, each with 12 batched collision
 integrals).  Inputs are resident in HBM before timing starts.
 
     python bench.py [--gpus 1] [--steps 52] [--warmup 3] [--no-cpu-baseline]
 
-Prints one JSON line.  ``roofline`` is for the dominant kernel (the gain
+Prints one JSON line.  ``accuracy`` compares the moments of the final FP32
+state with an FP64 run of the same steps (after the timed region).
+``roofline`` is for the dominant kernel (the gain
 kernel of the collision operator), timed with HIP events recorded around it
 on its launch stream over the timed region.  ``cpu_baseline`` is the FP64
 NumPy/SciPy oracle on the host cores, timed on a bounded sample (2 collision
@@ -67,6 +70,42 @@ def cpu_baseline(deck_text, cores, nq=2):
                        % (nq, t_q, 2 * ncoll))
 
 
+MOMENT_NAMES = ('rho', 'ux', 'uy', 'T', 'qx', 'qy', 'Pxx', 'Pyy', 'Pxy')
+
+
+def fp64_accuracy(text, nsteps, s32, dev):
+    """Moments of the FP32 run against the FP64 path after the same steps, and
+    the invariant moments of the collision output of both final states."""
+    import numpy as np
+    import torch
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    from dgfs1d_gpu_amd.std.std import DGFSStd1D
+    from dgfs1d_gpu_amd.transport import Moments
+    cfg = Dictionary(text)
+    cfg.set('config', 'precision', 'double')
+    s64 = DGFSStd1D(cfg, device=dev, writers=False)
+    for _ in range(nsteps):
+        s64.step()
+    mom = Moments(s64.basis.interpMat, s64.K, s64.Ne, s64.vm, dev)
+    m64 = mom(s64.state_fp64()).reshape(-1, 9)
+    m32 = mom(s32.state_fp64()).reshape(-1, 9)      # FP32 state (hi + lo), FP64 moments
+    err = {}
+    for i, name in enumerate(MOMENT_NAMES):
+        scale = float(np.abs(m64[:, i]).max())
+        err[name] = {'max_abs': float(np.abs(m32[:, i] - m64[:, i]).max()), 'field_max': scale}
+        err[name]['rel'] = err[name]['max_abs'] / scale if scale > 0 else None
+    cv = s64.vm.cv()
+    phi = np.vstack([np.ones(cv.shape[1]), cv, (cv * cv).sum(0)])
+    cons = {}
+    for tag, s in (('f64', s64), ('f32', s32)):
+        Q = torch.zeros_like(s.u)
+        s.sm.collide(s.u, Q)
+        q = Q.double().cpu().numpy().reshape(s.K * s.Ne, -1)
+        cons[tag] = float((np.abs(phi @ q.T) / (np.abs(phi) @ np.abs(q).T)).max())
+    return {'steps': nsteps, 'moments_fp32_vs_fp64': err,
+            'collision_invariant_moments_rel': cons}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -75,6 +114,8 @@ def main(argv=None):
     ap.add_argument('--config', default=os.path.join(ROOT, 'dgfs1d_gpu_amd', 'configs',
                                                      'couette_prod.ini'))
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-accuracy', action='store_true',
+                    help='skip the FP64 comparison run after the timed region')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run 5 device-to-device copies of MIB MiB (the known-size '
                          'kernel that scales the FETCH_SIZE / WRITE_SIZE counters)')
@@ -161,6 +202,8 @@ def main(argv=None):
         'collision_integrals_per_s': ntask * ncalls / (sum(stage_ms) * 1e-3),
         'reference_equivalent_tflops': ntask * q_survey / (coll_ms * 1e-3) / 1e12,
     }
+    if not args.no_accuracy:
+        out['accuracy'] = fp64_accuracy(text, args.warmup + args.steps, s, dev)
     if not args.no_cpu_baseline:
         cores = min(16, os.cpu_count() or 1)
         out['cpu_baseline'] = cpu_baseline(text, cores)

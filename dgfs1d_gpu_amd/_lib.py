@@ -41,6 +41,29 @@ class Transport(C.Structure):
 
 _l = C.c_long
 # name -> (restype, argtypes); the authoritative list of exported entry points
+IMEX_MAXS = 8
+
+
+class LM(C.Structure):
+    _fields_ = [('nstages', C.c_int), ('dt', C.c_double), ('prefac', C.c_double),
+                ('omega', C.c_double), ('Pr', C.c_double),
+                ('a', C.c_double * IMEX_MAXS), ('b', C.c_double * IMEX_MAXS),
+                ('c', C.c_double * (IMEX_MAXS + 1)),
+                ('U', C.c_void_p * IMEX_MAXS), ('LU', C.c_void_p * IMEX_MAXS),
+                ('LM', C.c_void_p * IMEX_MAXS), ('Unew', C.c_void_p),
+                ('F', C.c_void_p * IMEX_MAXS), ('L', C.c_void_p * IMEX_MAXS),
+                ('Pn', C.c_void_p * IMEX_MAXS), ('Mnew', C.c_void_p), ('fnew', C.c_void_p)]
+
+
+class ARS(C.Structure):
+    _fields_ = [('stage', C.c_int), ('dt', C.c_double), ('prefac', C.c_double),
+                ('omega', C.c_double), ('Pr', C.c_double),
+                ('ah', C.c_double * IMEX_MAXS), ('ai', C.c_double * (IMEX_MAXS + 1)),
+                ('U', C.c_void_p * (IMEX_MAXS + 1)), ('LU', C.c_void_p * IMEX_MAXS),
+                ('F', C.c_void_p * (IMEX_MAXS + 1)), ('L', C.c_void_p * IMEX_MAXS),
+                ('M', C.c_void_p * (IMEX_MAXS + 1))]
+
+
 SIGNATURES = {
     'dgfs_last_error': (C.c_char_p, []),
     'dgfs_collision_plan_create': (_i, [_i, _i, _i, _d, _dp, _dp, _dp, _d, _d, _d, _i, _i, _i,
@@ -55,6 +78,8 @@ SIGNATURES = {
     'dgfs_transport_rhs': (_i, [C.POINTER(Transport), _vp, _vp, _d, _vp]),
     'dgfs_matop': (_i, [_i, _i, _i, _l, _vp, _vp, _vp, _i, _vp]),
     'dgfs_axnpby': (_i, [_i, _l, _i, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _vp]),
+    'dgfs_axnpby_compensated': (_i, [_l, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d,
+                                     _d, _d, _vp]),
     'dgfs_moments_work_bytes': (_sz, [_i, _i]),
     'dgfs_moments': (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     'dgfs_residual_work_bytes': (_sz, []),
@@ -62,6 +87,12 @@ SIGNATURES = {
     'dgfs_projection_work_bytes': (_sz, [_i, _i]),
     'dgfs_project_conservative': (_i, [_i, _i, _i, _vp, _vp, _vp, _dp, _vp, _vp, _i, _vp, _vp]),
     'dgfs_shakov_work_bytes': (_sz, [_i, _i]),
+    'dgfs_mom14_work_bytes': (_sz, [_i, _i]),
+    'dgfs_mom14': (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
+    'dgfs_shakov_target': (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _d, _vp, _vp]),
+    'dgfs_relax_collide': (_i, [_i, _i, _i, _vp, _vp, _vp, _d, _d, _vp, _vp]),
+    'dgfs_imex_lm_update': (_i, [_i, _i, _i, C.POINTER(LM), _i, _vp]),
+    'dgfs_imex_ars_update': (_i, [_i, _i, _i, C.POINTER(ARS), _i, _vp]),
     'dgfs_shakov': (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                          _d, _d, _d, _d, _vp, _vp, _vp]),
 }

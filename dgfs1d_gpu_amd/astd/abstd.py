@@ -1,0 +1,63 @@
+"""dgfsAstd1D: the IMEX (asymptotic-preserving) single-species driver.
+
+Same INI schema as the reference dgfs1D/astd/abstd.py:37-480: the mesh,
+basis, velocity mesh, [scattering-model] (the Boltzmann operator Q, any std
+plugin), [penalized-scattering-model] (type shakov-direct-gll, parameters
+from [scattering-model]) and [time-integrator] scheme = one of the IMEX kinds
+(imex-lm-bdf1..4, -cn2, -ad2, -sg2, -ad3, -tvb3, ars-111, ars-222, ars-443).
+
+    explicit(t, f, out)   = invMass(transport(f))               (abstd.py:285-381)
+    explicitQ(t, f, out)  = sum_{r,e} Q(f[r,e]) -> out[r,e]     (abstd.py:395-413)
+
+Usage:  python -m dgfs1d_gpu_amd.astd.abstd run dgfs1D.ini [-v sect::key value]...
+"""
+import logging
+import sys
+
+from ..initialize import initialize
+from ..nputil import subclass_where
+from ..std.std import DGFSStd1D
+from .integrator import DGFSIntegratorAstd
+from .scattering import DGFSScatteringModelAstd
+
+log = logging.getLogger('dgfs1d_gpu_amd')
+
+
+class DGFSAstd1D(DGFSStd1D):
+    def _make_integrator(self, cfg, shape):
+        K, Ne, Nv = shape
+        self.sm.set_tasks(K, Ne, [(s, [(s, 1.0)], [(s, 1.0)]) for s in range(K * Ne)])
+        pcls = subclass_where(DGFSScatteringModelAstd,
+                              scattering_model=cfg.lookup('penalized-scattering-model', 'type'))
+        self.psm = pcls(cfg, self.vm, Ne=Ne, eps=self.sm.prefac, device=self.device)
+        self._ux = self.torch.zeros(K * Ne * Nv, dtype=self.tdtype, device=self.device)
+        kind = cfg.lookup('time-integrator', 'scheme')
+        odescls = subclass_where(DGFSIntegratorAstd, intg_kind=kind)
+        return odescls(self.explicit, self.psm, shape, self.tdtype, device=self.device,
+                       t=self.time, dt=self.dt, f0=self.u, explicitQ=self.explicitQ)
+
+    def explicit(self, t, f, out):
+        if self.invmass.is_identity:
+            self.transport(t, f, out)
+        else:
+            self.transport(t, f, self._ux)
+            self.invmass(self._ux, out)
+
+    def explicitQ(self, t, f, out):
+        out.zero_()
+        self.sm.collide(f, out)
+
+    def _advance(self):
+        self.odes.integrate(self.time, self.dt, self.nacptsteps, self.u)
+
+
+def main(argv=None):
+    logging.basicConfig(level=logging.INFO, format='%(asctime)s %(message)s')
+    cfg, args = initialize(argv)
+    restart = args.dist[0][0].name if hasattr(args, 'process_restart') else None
+    solver = DGFSAstd1D(cfg, restart=restart)
+    solver.run()
+
+
+if __name__ == '__main__':
+    sys.exit(main())

@@ -67,6 +67,58 @@ k_axnpby(long n, int nv, T* __restrict__ x0, const T* __restrict__ x1, const T* 
     x0[i] = acc;
 }
 
+// ------------------------------------------- compensated FP32 axnpby
+// Double-float (hi + lo) arithmetic from FP32 error-free transformations:
+// two_sum (Knuth) and the fma residual of a product.  HIP contracts a*b + c
+// into one fma across statements, which would fuse the rounded product into
+// the following two_sum and break it; an empty asm statement makes the
+// rounded product opaque, so it stays a separate v_mul_f32.
+struct df32 { float hi, lo; };
+
+DGFS_DEV df32 two_sum(float a, float b) {
+    const float s = __fadd_rn(a, b);
+    const float bb = __fsub_rn(s, a);
+    const float e = __fadd_rn(__fsub_rn(a, __fsub_rn(s, bb)), __fsub_rn(b, bb));
+    return {s, e};
+}
+
+// (ahi + alo) * (xh + xl) + acc, |result error| ~ 2^-44 |result|
+DGFS_DEV df32 df_fma(float ahi, float alo, float xh, float xl, df32 acc) {
+    float ph = __fmul_rn(ahi, xh);
+    asm volatile("" : "+v"(ph));
+    const float pe = __fmaf_rn(ahi, xh, -ph);                   // exact low part of ahi*xh
+    const float pl = __fadd_rn(pe, __fadd_rn(__fmul_rn(ahi, xl), __fmul_rn(alo, xh)));
+    const df32 s = two_sum(acc.hi, ph);
+    const float lo = __fadd_rn(__fadd_rn(s.lo, pl), acc.lo);
+    const float hi = __fadd_rn(s.hi, lo);                       // fast two-sum renormalisation
+    return {hi, __fsub_rn(lo, __fsub_rn(hi, s.hi))};
+}
+
+struct Coef2 { float hi, lo; };
+inline Coef2 split_coef(double a) {
+    const float h = float(a);
+    return {h, float(a - double(h))};
+}
+
+// x0 = a0 x0 + sum a_i x_i with every x_i = hi_i + lo_i (lo_i may be null);
+// x0_lo receives the low word (or null: x0 keeps the rounded value).
+__global__ void __launch_bounds__(kBlk)
+k_axnpby_df(long n, int nv, float* __restrict__ x0, float* __restrict__ x0l,
+            const float* __restrict__ x1, const float* __restrict__ x1l,
+            const float* __restrict__ x2, const float* __restrict__ x2l,
+            const float* __restrict__ x3, const float* __restrict__ x3l, Coef2 a0, Coef2 a1,
+            Coef2 a2, Coef2 a3) {
+    const long i = long(blockIdx.x) * kBlk + threadIdx.x;
+    if (i >= n) return;
+    df32 acc{0.f, 0.f};
+    if (a0.hi != 0.f) acc = df_fma(a0.hi, a0.lo, x0[i], x0l ? x0l[i] : 0.f, acc);
+    if (nv > 1) acc = df_fma(a1.hi, a1.lo, x1[i], x1l ? x1l[i] : 0.f, acc);
+    if (nv > 2) acc = df_fma(a2.hi, a2.lo, x2[i], x2l ? x2l[i] : 0.f, acc);
+    if (nv > 3) acc = df_fma(a3.hi, a3.lo, x3[i], x3l ? x3l[i] : 0.f, acc);
+    x0[i] = acc.hi;
+    if (x0l) x0l[i] = acc.lo;
+}
+
 // ------------------------------------------------------------- wall BCs
 struct WallDev {
     int kind;         // DGFS_BC_*
@@ -390,6 +442,22 @@ extern "C" int dgfs_axnpby(int dtype, long n, int nv, void* x0, const void* x1, 
         hipLaunchKernelGGL((k_axnpby<float>), g, dim3(kBlk), 0, stream, n, nv, (float*)x0,
                            (const float*)x1, (const float*)x2, (const float*)x3, float(a0),
                            float(a1), float(a2), float(a3));
+    DGFS_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int dgfs_axnpby_compensated(long n, int nv, float* x0, float* x0_lo, const float* x1,
+                                       const float* x1_lo, const float* x2, const float* x2_lo,
+                                       const float* x3, const float* x3_lo, double a0, double a1,
+                                       double a2, double a3, hipStream_t stream) {
+    DGFS_REQUIRE(nv >= 1 && nv <= 4 && n >= 0 && x0, "axnpby_compensated: bad arguments");
+    DGFS_REQUIRE((nv < 2 || x1) && (nv < 3 || x2) && (nv < 4 || x3),
+                 "axnpby_compensated: null operand");
+    if (n == 0) return 0;
+    const dim3 g((n + kBlk - 1) / kBlk);
+    hipLaunchKernelGGL(k_axnpby_df, g, dim3(kBlk), 0, stream, n, nv, x0, x0_lo, x1, x1_lo, x2,
+                       x2_lo, x3, x3_lo, split_coef(a0), split_coef(a1), split_coef(a2),
+                       split_coef(a3));
     DGFS_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -25,6 +25,9 @@ class Dictionary:
             cp.read_string(inistr)
         if defaults:
             cp.read_dict(defaults)
+        self._derive()
+
+    def _derive(self):
         self._dtypename = self.lookupordefault('config', 'precision', 'double')
         if self._dtypename not in _NP_MAP:
             raise ValueError('precision must be double or float')
@@ -48,6 +51,8 @@ class Dictionary:
         if not self._cp.has_section(section):
             self._cp.add_section(section)
         self._cp.set(section, option, str(value))
+        if section == 'config':
+            self._derive()
 
     def lookup(self, section, option, vars=None):
         return os.path.expandvars(self._cp.get(section, option, vars=vars))

@@ -99,9 +99,7 @@ class DGFSStd1D:
         self.uprev = self.u.clone()
         self.nacptsteps = 0
 
-        odescls = subclass_where(DGFSIntegratorStd,
-                                 intg_kind=cfg.lookup('time-integrator', 'scheme'))
-        self.odes = odescls(self.rhs, (K, Ne, Nv), self.tdtype, device=self.device)
+        self.odes = self._make_integrator(cfg, (K, Ne, Nv))
 
         self.residual = None
         self.moments = None
@@ -126,9 +124,26 @@ class DGFSStd1D:
             self.sm.collide(u_in, self.ux)
             self.invmass(self.ux, u_out)
 
+    def _make_integrator(self, cfg, shape):
+        odescls = subclass_where(DGFSIntegratorStd,
+                                 intg_kind=cfg.lookup('time-integrator', 'scheme'))
+        comp = cfg.lookupordefault('time-integrator', 'compensated', 'false').lower()
+        if comp not in ('true', 'false', '1', '0', 'yes', 'no'):
+            raise ValueError('time-integrator::compensated must be true or false')
+        return odescls(self.rhs, shape, self.tdtype, device=self.device,
+                       compensated=comp in ('true', '1', 'yes'))
+
+    def _advance(self):
+        self.odes.integrate(self.time, self.dt, self.u)
+
+    def state_fp64(self):
+        """The state in FP64: u, plus the low word of a compensated FP32 state."""
+        lo = self.odes.lo_of(self.u) if hasattr(self.odes, 'lo_of') else None
+        return self.u.double() if lo is None else self.u.double() + lo.double()
+
     # ----------------------------------------------------------- time loop
     def step(self):
-        self.odes.integrate(self.time, self.dt, self.u)
+        self._advance()
         self.time += self.dt
         self.nacptsteps += 1
         if self.residual is not None:

@@ -46,6 +46,28 @@ def axnpby(x0, terms, a0, stream=None):
         float(a0), ks[0], ks[1], ks[2], _lib.stream_ptr(stream)))
 
 
+def axnpby_compensated(x0, x0_lo, terms, a0, stream=None):
+    """(x0, x0_lo) = a0 (x0 + x0_lo) + sum a_i (x_i + lo_i) for (a_i, x_i, lo_i)
+    in terms (<= 3 terms, FP32, lo_i / x0_lo may be None)."""
+    import torch
+    if len(terms) > 3:
+        raise ValueError('axnpby takes at most 3 extra operands')
+    for t in [x0, x0_lo] + [x for _, x, _ in terms] + [lo for _, _, lo in terms]:
+        if t is not None and (t.dtype != torch.float32 or t.numel() != x0.numel()):
+            raise ValueError('axnpby_compensated: FP32 operands of equal size required')
+    null = _lib.C.c_void_p(None)
+    xs, ls = [], []
+    for _, x, lo in terms:
+        xs.append(_lib.ptr(x))
+        ls.append(_lib.ptr(lo))
+    xs += [null] * (3 - len(terms))
+    ls += [null] * (3 - len(terms))
+    ks = [float(a) for a, _, _ in terms] + [0.0] * (3 - len(terms))
+    _lib.check(_lib.lib().dgfs_axnpby_compensated(
+        x0.numel(), 1 + len(terms), _lib.ptr(x0), _lib.ptr(x0_lo), xs[0], ls[0], xs[1], ls[1],
+        xs[2], ls[2], float(a0), ks[0], ks[1], ks[2], _lib.stream_ptr(stream)))
+
+
 class Transport:
     """Fused DG transport operator (std.py:283-348) bound to one basis/mesh."""
 
@@ -126,6 +148,9 @@ class Moments:
 
     def compute(self, u, stream=None):
         vm = self.vm
+        if u.dtype != vm.d_cvx().dtype:
+            raise TypeError('moments: state dtype %s differs from the velocity grid dtype %s'
+                            % (u.dtype, vm.d_cvx().dtype))
         _lib.check(_lib.lib().dgfs_moments(
             _lib.dtype_code(u.dtype), self.K, self.Ne, self.Nqr, self.Nv, _lib.ptr(self.im),
             _lib.ptr(u), _lib.ptr(vm.d_cvx()), _lib.ptr(vm.d_cvy()), _lib.ptr(vm.d_cvz()),

@@ -154,6 +154,19 @@ int dgfs_matop(int dtype, int Kout, int Kin, long ncol, const double* A, const v
 int dgfs_axnpby(int dtype, long n, int nv, void* x0, const void* x1, const void* x2,
                 const void* x3, double a0, double a1, double a2, double a3, hipStream_t stream);
 
+/* Compensated FP32 stage update (no reference counterpart; extends
+ * axnpby.py:13-43 for FP32 states carried as an unevaluated sum hi + lo):
+ *     (x0, x0_lo) = a0 (x0 + x0_lo) + sum_{i<nv} a_i (x_i + x_i_lo)
+ * in double-float arithmetic built from FP32 error-free transformations
+ * (two_sum, fma product residual); relative error ~2^-44.  Any *_lo may be
+ * NULL (zero low word; x0_lo NULL stores only the rounded sum).  a0 == 0 does
+ * not read x0.  Integrators use it so that dt*L increments are not lost to
+ * the FP32 rounding of the state. */
+int dgfs_axnpby_compensated(long n, int nv, float* x0, float* x0_lo, const float* x1,
+                            const float* x1_lo, const float* x2, const float* x2_lo,
+                            const float* x3, const float* x3_lo, double a0, double a1, double a2,
+                            double a3, hipStream_t stream);
+
 /* FP64 moments at Nqr reconstruction points per element
  * (dgfs1D/std/moments.py:11-62, non-dimensional, before the dimensional
  * scaling of :63-70).  im: device K*Nqr (interpMat); out: device
@@ -205,6 +218,77 @@ int dgfs_shakov(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nte
                 const int* group_start, const int* group_slice, const void* u, const void* cx,
                 const void* cy, const void* cz, double cw, double prefac, double omega,
                 double Pr, void* Q, double* work, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * IMEX penalisation with the 14-moment Shakov model
+ * (DGFSShakovDirectGLLScatteringModelAstd, dgfs1D/astd/scattering.py:355-397,
+ * kernels astd/kernels/scattering/shakov-direct-gll.mako:193-691).
+ * Moment vectors are FP64 [nslice][14] = {rho, rho u_x, rho u_y, rho u_z, E,
+ * P_xx, P_yy, P_zz, P_xy, P_yz, P_zx, Q_x, Q_y, Q_z}: raw moments
+ * sum_v f {1, c, |c|^2, c_i c_j, |c|^2 c} cw of each (mode, element) slice.
+ *
+ * dgfs_mom14         U = f . mat                     (moment(), scattering.py:155-162)
+ * dgfs_shakov_target M = Shakov target of U          (momentNorm + cmaxwellian, mako:193-264)
+ * dgfs_relax_collide Q = nu (M - f), nu = prefac rho T^{1-omega}   (collide, mako:266-305)
+ * work of dgfs_mom14: dgfs_mom14_work_bytes(nslice, Nv) bytes. */
+size_t dgfs_mom14_work_bytes(int nslice, int Nv);
+int dgfs_mom14(int dtype, int nslice, int Nv, const void* f, const void* cx, const void* cy,
+               const void* cz, double cw, double* work, double* U, hipStream_t stream);
+int dgfs_shakov_target(int dtype, int nslice, int Nv, const double* U, const void* cx,
+                       const void* cy, const void* cz, double Pr, void* M, hipStream_t stream);
+int dgfs_relax_collide(int dtype, int nslice, int Nv, const double* U, const void* M,
+                       const void* f, double prefac, double omega, void* Q, hipStream_t stream);
+
+#define DGFS_IMEX_MAXS 8
+
+/* s-step IMEX linear multistep update (updateMomentLM / updateDistLM,
+ * astd/scattering.py:262-279; kernels updateMom{s}_LM and
+ * updateDistribution{s}_LM, mako:548-691).  History index i = 0..s-1 is
+ * oldest first; term i carries a[s-1-i], b[s-1-i], c[s-i]:
+ *   what = 0:  Unew[k] = sum_i (-a U_i + dt b LU_i + dt c LM_i)[k] for k < 5;
+ *              stress / heat flux relax implicitly with c[0] dt nu(Unew)
+ *              (heat flux at rate Pr nu) towards their Shakov values.
+ *   what = 1:  fnew = (sum_i (-a F_i + dt b L_i + dt c Pn_i) + c[0] dt nu Mnew)
+ *                     / (1 + c[0] dt nu),  nu = prefac rho T^{1-omega} of Unew. */
+typedef struct {
+    int nstages;
+    double dt, prefac, omega, Pr;
+    double a[DGFS_IMEX_MAXS], b[DGFS_IMEX_MAXS], c[DGFS_IMEX_MAXS + 1];
+    const double* U[DGFS_IMEX_MAXS];
+    const double* LU[DGFS_IMEX_MAXS];
+    const double* LM[DGFS_IMEX_MAXS];
+    double* Unew;
+    const void* F[DGFS_IMEX_MAXS];
+    const void* L[DGFS_IMEX_MAXS];
+    const void* Pn[DGFS_IMEX_MAXS];
+    const void* Mnew;
+    void* fnew;
+} dgfs_lm_t;
+int dgfs_imex_lm_update(int dtype, int nslice, int Nv, const dgfs_lm_t* args, int what,
+                        hipStream_t stream);
+
+/* Stage s (>= 1) of an ARS IMEX Runge-Kutta tableau (updateMomentARS /
+ * updateDistARS, astd/scattering.py:220-240; kernels updateMom{s}_ARS and
+ * updateDistribution{s}_ARS, mako:326-484): explicit weights ah[0..s-1] on
+ * the explicit parts L_0..L_{s-1}, implicit weights ai[1..s] on the
+ * penalisations of stages 1..s (stage s implicit):
+ *   what = 0:  U[s] from U[0], LU[0..s-1] and U[1..s-1]
+ *   what = 1:  F[s] = (F[0] + sum dt ah_i L_i + sum_{i<s} dt ai_i nu_i (M_i - F_i)
+ *                      + dt ai_s nu_s M_s) / (1 + dt ai_s nu_s)
+ * M[i] is the Shakov target of U[i]; F[s] may alias F[0]. */
+typedef struct {
+    int stage;
+    double dt, prefac, omega, Pr;
+    double ah[DGFS_IMEX_MAXS];
+    double ai[DGFS_IMEX_MAXS + 1];
+    double* U[DGFS_IMEX_MAXS + 1];
+    const double* LU[DGFS_IMEX_MAXS];
+    void* F[DGFS_IMEX_MAXS + 1];
+    const void* L[DGFS_IMEX_MAXS];
+    const void* M[DGFS_IMEX_MAXS + 1];
+} dgfs_ars_t;
+int dgfs_imex_ars_update(int dtype, int nslice, int Nv, const dgfs_ars_t* args, int what,
+                         hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,264 @@
+"""Oracle: IMEX penalisation (14-moment Shakov) and IMEX integrators (FP64).
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+Restates, per slice of a (nslice, n) array:
+  moment()             dgfs1D/astd/scattering.py:155-162, mat rows :380-397
+  momentNorm           astd/kernels/scattering/shakov-direct-gll.mako:193-230
+  cmaxwellian          :232-264
+  collide              :266-305
+  updateMom{s}_ARS     :326-440      updateDistribution{s}_ARS   :443-484
+  updateMom{s}_LM      :548-641      updateDistribution{s}_LM    :644-691
+and the integrators of dgfs1D/astd/integrator.py:
+  ARS (a,b,c) stage loop             :48-95 with tables :100-178
+  IMEX linear multistep              :623-736 with tables :739-849
+Documented deviation (see the product's integrator): the newest explicit
+part of an s >= 2 multistep scheme subtracts the newest penalisation.
+"""
+import numpy as np
+
+NALPH = 14
+
+
+def mat14(cv, cw):
+    c2 = (cv * cv).sum(0)
+    return np.vstack([np.ones(cv.shape[1]), cv, c2,
+                      cv[0] * cv[0], cv[1] * cv[1], cv[2] * cv[2],
+                      cv[0] * cv[1], cv[1] * cv[2], cv[2] * cv[0],
+                      c2 * cv[0], c2 * cv[1], c2 * cv[2]]) * cw
+
+
+def moment(f, cv, cw):
+    """U = f . mat^T for f of shape (nslice, n)."""
+    return f @ mat14(cv, cw).T
+
+
+def temperature(U):
+    return (U[:, 4] - (U[:, 1] ** 2 + U[:, 2] ** 2 + U[:, 3] ** 2) / U[:, 0]) / (1.5 * U[:, 0])
+
+
+def momentnorm(U):
+    rho = U[:, 0]
+    u = U[:, 1:4] / rho[:, None]
+    u2 = (u * u).sum(1)
+    T = U[:, 4] / (1.5 * rho) - u2 / 1.5
+    P = np.stack([np.stack([U[:, 5], U[:, 8], U[:, 10]], 1),
+                  np.stack([U[:, 8], U[:, 6], U[:, 9]], 1),
+                  np.stack([U[:, 10], U[:, 9], U[:, 7]], 1)], 1)        # (ns, 3, 3)
+    Pu = np.einsum('sij,sj->si', P, u)
+    q = U[:, 11:14] - 2.0 * Pu + (rho * u2)[:, None] * u - 1.5 * u * (rho * T)[:, None]
+    return rho, u, T, q
+
+
+def target(U, cv, Pr):
+    rho, u, T, q = momentnorm(U)
+    C = cv[None, :, :] - u[:, :, None]
+    C2 = (C * C).sum(1)
+    fe = (rho / (np.pi * T) ** 1.5)[:, None] * np.exp(-C2 / T[:, None])
+    qc = (q[:, :, None] * C).sum(1)
+    return fe * (1.0 + 2.0 * (1.0 - Pr) / 5.0 * qc / (rho * T * T)[:, None]
+                 * (2.0 * C2 / T[:, None] - 5.0))
+
+
+def nu_of(U, prefac, omega):
+    return prefac * U[:, 0] * temperature(U) ** (1.0 - omega)
+
+
+def collide(U, M, f, prefac, omega):
+    return nu_of(U, prefac, omega)[:, None] * (M - f)
+
+
+def _targets(U, T, Pr):
+    r = U[:, 0]
+    P = np.stack([0.5 * r * T + U[:, 1] ** 2 / r, 0.5 * r * T + U[:, 2] ** 2 / r,
+                  0.5 * r * T + U[:, 3] ** 2 / r, U[:, 1] * U[:, 2] / r,
+                  U[:, 2] * U[:, 3] / r, U[:, 3] * U[:, 1] / r], 1)
+    m2 = (U[:, 1] ** 2 + U[:, 2] ** 2 + U[:, 3] ** 2) / r ** 2
+    rows = ((0, 3, 5), (3, 1, 4), (5, 4, 2))
+    Q = []
+    for i, (q1, q2, q3) in enumerate(rows):
+        Pu = U[:, 5 + q1] * U[:, 1] + U[:, 5 + q2] * U[:, 2] + U[:, 5 + q3] * U[:, 3]
+        ui = U[:, 1 + i]
+        Q.append((1.0 - Pr) * (-2.0 * Pu / r + ui * m2 - 1.5 * ui * T) + (ui * m2 + 2.5 * ui * T))
+    return P, np.stack(Q, 1)
+
+
+def lm_moments(dt, A, B, C, LU, LM, U, s, prefac, omega, Pr):
+    E = sum(-A[s - 1 - i] * U[i] + dt * B[s - 1 - i] * LU[i] + dt * C[s - i] * LM[i]
+            for i in range(s))
+    T = temperature(E)
+    w = C[0] * dt * prefac * E[:, 0] * T ** (1.0 - omega)
+    Un = E.copy()
+    P, _ = _targets(E, T, Pr)
+    Un[:, 5:11] = (E[:, 5:11] + w[:, None] * P) / (1.0 + w[:, None])
+    _, Q = _targets(Un, T, Pr)
+    Un[:, 11:14] = (E[:, 11:14] + w[:, None] * Q) / (1.0 + Pr * w[:, None])
+    return Un
+
+
+def lm_dist(dt, A, B, C, L, Pn, F, Mnew, Unew, s, prefac, omega):
+    acc = sum(-A[s - 1 - i] * F[i] + dt * B[s - 1 - i] * L[i] + dt * C[s - i] * Pn[i]
+              for i in range(s))
+    cdn = (C[0] * dt * nu_of(Unew, prefac, omega))[:, None]
+    return (acc + cdn * Mnew) / (1.0 + cdn)
+
+
+def ars_moments(dt, ah, ai, LU, U, s, prefac, omega, Pr):
+    """U has s entries U_0..U_{s-1}; returns U_s (ai indexed 1..s)."""
+    E = U[0] + sum(dt * ah[i] * LU[i] for i in range(s))
+    for i in range(1, s):
+        Ti = temperature(U[i])
+        w = ai[i] * dt * prefac * U[i][:, 0] * Ti ** (1.0 - omega)
+        P, Q = _targets(U[i], Ti, Pr)
+        E[:, 5:11] += w[:, None] * (P - U[i][:, 5:11])
+        E[:, 11:14] += w[:, None] * (Q - Pr * U[i][:, 11:14])
+    Ts = temperature(E)
+    w = ai[s] * dt * prefac * E[:, 0] * Ts ** (1.0 - omega)
+    Un = E.copy()
+    P, _ = _targets(E, Ts, Pr)
+    Un[:, 5:11] = (E[:, 5:11] + w[:, None] * P) / (1.0 + w[:, None])
+    _, Q = _targets(Un, Ts, Pr)
+    Un[:, 11:14] = (E[:, 11:14] + w[:, None] * Q) / (1.0 + Pr * w[:, None])
+    return Un
+
+
+def ars_dist(dt, ah, ai, L, U, M, F, s, prefac, omega):
+    """U: U_0..U_s, M: targets indexed 1..s (M[0] unused), F: f_0..f_{s-1}."""
+    acc = F[0] + sum(ah[i] * dt * L[i] for i in range(s))
+    for i in range(1, s):
+        acc = acc + ai[i] * dt * nu_of(U[i], prefac, omega)[:, None] * (M[i] - F[i])
+    w = (ai[s] * dt * nu_of(U[s], prefac, omega))[:, None]
+    return (acc + w * M[s]) / (1.0 + w)
+
+
+# ----------------------------------------------------------------- tableaux
+ARS = {
+    'ars-111': ([[0., 0.], [1., 0.]], [[0., 0.], [0., 1.]]),
+    'ars-222': None,
+    'ars-443': ([[0., 0., 0., 0., 0.], [1. / 2., 0., 0., 0., 0.],
+                 [11. / 18., 1. / 18., 0., 0., 0.], [5. / 6., -5. / 6., 1. / 2., 0., 0.],
+                 [1. / 4., 7. / 4., 3. / 4., -7. / 4., 0.]],
+                [[0., 0., 0., 0., 0.], [0., 1. / 2., 0., 0., 0.], [0., 1. / 6., 1. / 2., 0., 0.],
+                 [0., -1. / 2., 1. / 2., 1. / 2., 0.], [0., 3. / 2., -3. / 2., 1. / 2., 1. / 2.]]),
+}
+_g = 1.0 - 2.0 ** 0.5 / 2.0
+_d = 1.0 - 1.0 / (2.0 * _g)
+ARS['ars-222'] = ([[0., 0., 0.], [_g, 0., 0.], [_d, 1 - _d, 0.]],
+                  [[0., 0., 0.], [0., _g, 0.], [0., 1 - _g, _g]])
+
+LM = {
+    'imex-lm-bdf1': ([-1.], [1.], [1., 0.]),
+    'imex-lm-cn2': ([-1., 0.], [1.5, -0.5], [0.5, 0.5, 0.]),
+    'imex-lm-ad2': ([-1., 0.], [1.5, -0.5], [9. / 16., 3. / 8., 1. / 16.]),
+    'imex-lm-bdf2': ([-4. / 3., 1. / 3.], [4. / 3., -2. / 3.], [2. / 3., 0., 0.]),
+    'imex-lm-sg2': ([-3. / 4., 0., -1. / 4.], [3. / 2., 0., 0.], [1., 0., 0., 1. / 2.]),
+    'imex-lm-bdf3': ([-18. / 11., 9. / 11., -2. / 11.], [18. / 11., -18. / 11., 6. / 11.],
+                     [6. / 11., 0., 0., 0.]),
+    'imex-lm-ad3': ([-1., 0., 0.], [23. / 12., -4. / 3., 5. / 12.],
+                    [4661. / 10000., 15551. / 30000., 1949. / 30000., -1483. / 30000.]),
+    'imex-lm-tvb3': ([-3909. / 2048., 1367. / 1024., -873. / 2048.],
+                     [18463. / 12288., -1271. / 768., 8233. / 12288.],
+                     [1089. / 2048., -1139. / 12288., -367. / 6144., 1699. / 12288.]),
+    'imex-lm-bdf4': ([-48. / 25., 36. / 25., -16. / 25., 3. / 25.],
+                     [48. / 25., -72. / 25., 48. / 25., -12. / 25.],
+                     [12. / 25., 0., 0., 0., 0.]),
+}
+
+
+class Penal:
+    """The penalisation model bound to a grid: prefac, omega, Pr, cv, cw."""
+
+    def __init__(self, cv, cw, prefac, omega, Pr):
+        self.cv, self.cw, self.prefac, self.omega, self.Pr = cv, cw, prefac, omega, Pr
+
+    def moment(self, f):
+        return moment(f, self.cv, self.cw)
+
+    def target(self, U):
+        return target(U, self.cv, self.Pr)
+
+    def penal(self, U, f):
+        return collide(U, self.target(U), f, self.prefac, self.omega)
+
+
+def ars_step(kind, pm, explicit, t, dt, f, penalize=True):
+    """One ARS step (astd/integrator.py:48-95); explicit(t, f) -> L.  With the
+    Boltzmann part present the reference overwrites it by the Maxwellian
+    before use (:68-77), so the explicit part is explicit(f) - P(f)
+    (penalize=True); penalize=False is the plain IMEX splitting
+    explicit + implicit P (used by the order tests)."""
+    _A, A = ARS[kind]
+    ns = len(_A) - 1
+    F = [f]
+    U = [pm.moment(f)]
+    L, LU, M = [], [], [None]
+    for i in range(ns):
+        Li = explicit(t, F[i]) - pm.penal(U[i], F[i]) if penalize else explicit(t, F[i])
+        L.append(Li)
+        LU.append(pm.moment(Li))
+        ah, ai = _A[i + 1][0:i + 1], [0.0] + A[i + 1][1:i + 2]
+        U.append(ars_moments(dt, ah, ai, LU, U, i + 1, pm.prefac, pm.omega, pm.Pr))
+        M.append(pm.target(U[i + 1]))
+        F.append(ars_dist(dt, ah, ai, L, U, M, F, i + 1, pm.prefac, pm.omega))
+    return F[-1]
+
+
+class LMIntegrator:
+    """IMEX linear multistep (astd/integrator.py:623-736): history from
+    ARS-443 substeps (10 per dt), then the s-step update."""
+    nsub = 10
+
+    def __init__(self, kind, pm, explicit, boltz, t, dt, f0, penalize_init=True):
+        self.A, self.B, self.C = LM[kind]
+        self.s = s = len(self.A)
+        self.pm, self.explicit, self.boltz = pm, explicit, boltz
+        fn, f, tl = [], f0.copy(), t
+        for n in range(s):
+            fn.append(f.copy())
+            for _ in range(self.nsub):
+                f = ars_step('ars-443', pm, self._ex_only, tl, dt / self.nsub, f,
+                             penalize=penalize_init)
+                tl += dt / self.nsub
+        self.F, self.U, self.P, self.LM, self.L, self.LU = [], [], [], [], [], []
+        for n in range(s):
+            self._push(fn[n], t + n * dt)
+
+    def _ex_only(self, t, f):
+        return self.explicit(t, f)
+
+    def _push(self, f, t):
+        pm = self.pm
+        U = pm.moment(f)
+        P = pm.penal(U, f)
+        L = self.explicit(t, f)
+        if self.boltz is not None:
+            L = L + self.boltz(t, f) - P
+        self.F.append(f.copy())
+        self.U.append(U)
+        self.P.append(P)
+        self.LM.append(pm.moment(P))
+        self.L.append(L)
+        self.LU.append(pm.moment(L))
+
+    def step(self, t, dt, nacptsteps, f):
+        s, pm = self.s, self.pm
+        if nacptsteps < s - 1:
+            return self.F[nacptsteps + 1].copy()
+        Un = lm_moments(dt, self.A, self.B, self.C, self.LU, self.LM, self.U, s,
+                        pm.prefac, pm.omega, pm.Pr)
+        Mn = pm.target(Un)
+        fn = lm_dist(dt, self.A, self.B, self.C, self.L, self.P, self.F, Mn, Un, s,
+                     pm.prefac, pm.omega)
+        for lst in (self.F, self.U, self.P, self.LM, self.L, self.LU):
+            lst.pop(0)
+        Pn = collide(Un, Mn, fn, pm.prefac, pm.omega)
+        L = self.explicit(t, fn)
+        if self.boltz is not None:
+            L = L + self.boltz(t, fn) - Pn
+        self.F.append(fn.copy())
+        self.U.append(Un)
+        self.P.append(Pn)
+        self.LM.append(pm.moment(Pn))
+        self.L.append(L)
+        self.LU.append(pm.moment(L))
+        return fn

@@ -110,3 +110,37 @@ def test_projection_vs_oracle(cuda, dtype):
         m = np.abs(phi @ qc.T) / scale
         print('projected moments', m.max())
         assert m.max() < 1e-12
+
+
+def test_axnpby_compensated(cuda):
+    """Double-float stage update: matches FP64 arithmetic on hi + lo to ~1e-13
+    and keeps increments far below FP32 resolution that plain FP32 loses."""
+    import torch
+    from dgfs1d_gpu_amd.transport import axnpby, axnpby_compensated
+    rng = np.random.default_rng(1)
+    n = 100003
+    hi = [torch.tensor(rng.uniform(0.5, 2, n), dtype=torch.float32, device=cuda) for _ in range(3)]
+    lo = [torch.tensor(rng.uniform(-1, 1, n) * 1e-8, dtype=torch.float32, device=cuda)
+          for _ in range(3)]
+    d = rng.standard_normal(n)
+    x0 = torch.tensor(d, dtype=torch.float32, device=cuda)
+    x0l = torch.zeros_like(x0)
+    a = (0.7, 1.0 / 3.0, -2.5e-4, 0.123456789)
+    ref = a[0] * x0.double().cpu().numpy() + sum(
+        a[i + 1] * (hi[i].double().cpu().numpy() + lo[i].double().cpu().numpy()) for i in range(3))
+    axnpby_compensated(x0, x0l, [(a[i + 1], hi[i], lo[i]) for i in range(3)], a[0])
+    got = x0.double().cpu().numpy() + x0l.double().cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print('compensated axnpby', err)
+    assert err < 1e-12
+    # 4000 increments of 1e-9 on 1.0: plain FP32 stays at 1.0, compensated keeps them
+    u = torch.ones(1024, dtype=torch.float32, device=cuda)
+    ul = torch.zeros_like(u)
+    up = torch.ones_like(u)
+    inc = torch.full_like(u, 1e-9)
+    for _ in range(4000):
+        axnpby_compensated(u, ul, [(1.0, inc, None)], 1.0)
+        axnpby(up, [(1.0, inc)], 1.0)
+    tot = (u.double() + ul.double()).cpu().numpy()
+    assert np.abs(tot - (1.0 + 4000 * float(np.float32(1e-9)))).max() < 1e-10
+    assert np.all(up.cpu().numpy() == 1.0)

@@ -101,3 +101,11 @@ def test_mesh_types():
     assert np.allclose(m.xmesh, [0, 0.25, 0.5, 0.75, 1.0]) and np.allclose(m.invjac, 8.0)
     m = Mesh(Dictionary(base + 'type=bump\nmu=1.0\n'))
     assert m.Ne == 4 and np.isclose(m.xmesh[0], 0) and np.isclose(m.xmesh[-1], 1)
+
+
+def test_dictionary_set_precision():
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    d = Dictionary('[config]\nprecision = float\n')
+    assert d.dtype == np.float32
+    d.set('config', 'precision', 'double')
+    assert d.dtype == np.float64 and d.dtypename == 'double'

@@ -1,0 +1,125 @@
+"""Pins of the IMEX oracle (CPU): closed-form moments of Maxwellians, the
+central heat flux of the 14-moment normalisation, and the convergence order
+of every ARS and linear-multistep tableau on a relaxation problem with an
+exact solution (explicit part -mu (f - M0), implicit Shakov/BGK part
+nu (M0 - f) with Pr = 1, so that f(t) = M0 + (f0 - M0) exp(-(mu + nu) t))."""
+import numpy as np
+import pytest
+
+from oracle import imex as OI
+
+
+def grid(N=12, L=6.0):
+    c0 = np.linspace(-L + L / N, L - L / N, N)
+    I, J, K = np.meshgrid(c0, c0, c0, indexing='ij')
+    return np.stack([I.ravel(), J.ravel(), K.ravel()]), (2 * L / N) ** 3
+
+
+def maxw(cv, rho, u, T):
+    u = np.asarray(u, float).reshape(3, 1)
+    return rho / (np.pi * T) ** 1.5 * np.exp(-((cv - u) ** 2).sum(0) / T)
+
+
+def test_moments_of_maxwellian_closed_form():
+    cv, cw = grid(24, 7.0)
+    rho, u, T = 1.3, np.array([0.3, -0.2, 0.1]), 0.9
+    U = OI.moment(maxw(cv, rho, u, T)[None, :], cv, cw)[0]
+    u2 = (u * u).sum()
+    exp = np.concatenate([[rho], rho * u, [rho * (u2 + 1.5 * T)],
+                          rho * (u * u + T / 2), rho * np.array([u[0] * u[1], u[1] * u[2],
+                                                                 u[2] * u[0]]),
+                          rho * u * (u2 + 2.5 * T)])
+    assert np.abs(U - exp).max() < 1e-8 * np.abs(exp).max()      # grid quadrature of the tails
+    r, uu, TT, q = OI.momentnorm(U[None, :])
+    assert abs(r[0] - rho) < 1e-8 and np.abs(uu[0] - u).max() < 1e-8
+    assert abs(TT[0] - T) < 1e-8 and np.abs(q[0]).max() < 1e-8
+    # the target of a Maxwellian's moments is that Maxwellian, for any Pr
+    for Pr in (2.0 / 3.0, 1.0):
+        M = OI.target(U[None, :], cv, Pr)[0]
+        assert np.abs(M - maxw(cv, rho, u, T)).max() < 1e-7 * M.max()
+
+
+def test_heat_flux_normalisation_is_central():
+    cv, cw = grid(24, 7.0)
+    f = 0.6 * maxw(cv, 1.0, (0.3, 0.1, 0.0), 0.8) + 0.5 * maxw(cv, 0.9, (-0.2, 0.25, 0.1), 1.3)
+    U = OI.moment(f[None, :], cv, cw)
+    rho, u, T, q = OI.momentnorm(U)
+    C = cv - u[0][:, None]
+    C2 = (C * C).sum(0)
+    assert abs(rho[0] - f.sum() * cw) < 1e-13
+    assert abs(T[0] - 2.0 / 3.0 * (f * C2).sum() * cw / rho[0]) < 1e-12
+    qd = (f[None, :] * C2[None, :] * C).sum(1) * cw
+    assert np.abs(q[0] - qd).max() < 1e-12
+
+
+def _relax_problem():
+    # fine enough that the discrete Maxwellian's moments equal its parameters
+    # to ~1e-10, so the conserved moments and nu stay fixed along the solution
+    cv, cw = grid(24, 7.0)
+    f0 = 0.6 * maxw(cv, 1.0, (0.3, 0.1, 0.0), 0.8) + 0.5 * maxw(cv, 0.9, (-0.2, 0.25, 0.1), 1.3)
+    U0 = OI.moment(f0[None, :], cv, cw)
+    M0 = OI.target(U0, cv, 1.0)                       # Maxwellian of f0's conserved moments
+    pm = OI.Penal(cv, cw, prefac=1.0, omega=0.5, Pr=1.0)
+    nu = OI.nu_of(U0, 1.0, 0.5)[0]
+    mu = 0.7
+    explicit = lambda t, f: -mu * (f - M0)
+    exact = lambda t: M0 + (f0[None, :] - M0) * np.exp(-(mu + nu) * t)
+    return pm, explicit, exact, f0[None, :], nu
+
+
+@pytest.mark.parametrize('kind,order', [('ars-111', 1), ('ars-222', 2), ('ars-443', 3)])
+def test_ars_order(kind, order):
+    pm, explicit, exact, f0, nu = _relax_problem()
+    tend = 0.4 / nu
+    errs = []
+    for nsteps in (8, 16):
+        dt, f = tend / nsteps, f0.copy()
+        for k in range(nsteps):
+            f = OI.ars_step(kind, pm, explicit, k * dt, dt, f, penalize=False)
+        errs.append(np.abs(f - exact(tend)).max())
+    rate = np.log2(errs[0] / errs[1])
+    print(kind, errs, rate)
+    assert rate > order - 0.3
+
+
+@pytest.mark.parametrize('kind,order', [('imex-lm-bdf1', 1), ('imex-lm-cn2', 2),
+                                        ('imex-lm-ad2', 2), ('imex-lm-bdf2', 2),
+                                        ('imex-lm-sg2', 2), ('imex-lm-bdf3', 3),
+                                        ('imex-lm-ad3', 3), ('imex-lm-tvb3', 3),
+                                        ('imex-lm-bdf4', 4)])
+def test_lm_order(kind, order):
+    pm, explicit, exact, f0, nu = _relax_problem()
+    tend = 0.5 / nu
+    errs = []
+    for nsteps in (16, 32):
+        dt = tend / nsteps
+        it = OI.LMIntegrator(kind, pm, explicit, None, 0.0, dt, f0.copy(), penalize_init=False)
+        f = f0.copy()
+        for k in range(nsteps):
+            f = it.step(k * dt, dt, k, f)
+        errs.append(np.abs(f - exact(tend)).max())
+    rate = np.log2(errs[0] / errs[1])
+    print(kind, errs, rate)
+    assert rate > order - 0.3
+
+
+def test_planted_lm_coefficient_fails():
+    """A wrong BDF2 coefficient (4/3 -> 5/4) destroys the order (mutant
+    planted in a copy of the oracle's table)."""
+    pm, explicit, exact, f0, nu = _relax_problem()
+    saved = OI.LM['imex-lm-bdf2']
+    OI.LM['imex-lm-bdf2'] = ([-5. / 4., 1. / 3.], [4. / 3., -2. / 3.], [2. / 3., 0., 0.])
+    try:
+        tend = 0.5 / nu
+        errs = []
+        for nsteps in (16, 32):
+            dt = tend / nsteps
+            it = OI.LMIntegrator('imex-lm-bdf2', pm, explicit, None, 0.0, dt, f0.copy(),
+                                 penalize_init=False)
+            f = f0.copy()
+            for k in range(nsteps):
+                f = it.step(k * dt, dt, k, f)
+            errs.append(np.abs(f - exact(tend)).max())
+        assert not np.log2(errs[0] / errs[1]) > 1.7
+    finally:
+        OI.LM['imex-lm-bdf2'] = saved
