@@ -64,12 +64,23 @@ class ARS(C.Structure):
                 ('M', C.c_void_p * (IMEX_MAXS + 1))]
 
 
+class ClosureNet(C.Structure):
+    _fields_ = [('W1', C.c_void_p), ('b1', C.c_void_p), ('W2f', C.c_void_p), ('W2t', C.c_void_p),
+                ('b2', C.c_void_p), ('W3f', C.c_void_p), ('W3t', C.c_void_p), ('b3', C.c_void_p),
+                ('W4', C.c_void_p), ('b4', C.c_void_p)]
+
+
+class ClosureGrad(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ('W1', 'b1', 'W2', 'b2', 'W3', 'b3', 'W4', 'b4')]
+
+
 SIGNATURES = {
     'dgfs_last_error': (C.c_char_p, []),
     'dgfs_collision_plan_create': (_i, [_i, _i, _i, _d, _dp, _dp, _dp, _d, _d, _d, _i, _i, _i,
                                         C.POINTER(_vp)]),
     'dgfs_collision_plan_destroy': (_i, [_vp]),
     'dgfs_collision_plan_query': (_i, [_vp, _ip, _ip, _ip, _szp]),
+    'dgfs_collision_plan_set_qgroups': (_i, [_vp, _i]),
     'dgfs_collide': (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _i,
                           _vp]),
     'dgfs_collision_timing': (_i, [_vp, _i]),
@@ -93,6 +104,15 @@ SIGNATURES = {
     'dgfs_relax_collide': (_i, [_i, _i, _i, _vp, _vp, _vp, _d, _d, _vp, _vp]),
     'dgfs_imex_lm_update': (_i, [_i, _i, _i, C.POINTER(LM), _i, _vp]),
     'dgfs_imex_ars_update': (_i, [_i, _i, _i, C.POINTER(ARS), _i, _vp]),
+    'dgfs_closure_work_bytes': (_sz, [_i, _i, _i]),
+    'dgfs_closure_forward': (_i, [C.POINTER(ClosureNet), _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp,
+                                  _vp, _vp, _vp, _vp]),
+    'dgfs_closure_backward': (_i, [C.POINTER(ClosureNet), C.POINTER(ClosureGrad), _i, _i, _i,
+                                   _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp]),
+    'dgfs_closure_pack': (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'dgfs_closure_loss_work_bytes': (_sz, [_i, _i]),
+    'dgfs_closure_loss': (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'dgfs_adam': (_i, [_i, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _vp]),
     'dgfs_shakov': (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                          _d, _d, _d, _d, _vp, _vp, _vp]),
 }

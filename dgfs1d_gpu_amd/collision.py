@@ -90,6 +90,10 @@ class CollisionPlan:
             self._h, _lib.C.byref(s), _lib.C.byref(qg), _lib.C.byref(pg), _lib.C.byref(b)))
         return dict(split=s.value, qgroups=qg.value, pgroups=pg.value, scratch_bytes=b.value)
 
+    def set_qgroups(self, qg):
+        """Split the M sphere directions of the gain kernel into qg groups."""
+        _lib.check(_lib.lib().dgfs_collision_plan_set_qgroups(self._h, int(qg)))
+
     def set_tasks(self, K, Ne, tasks):
         """tasks: list of (out_slice, f_terms, g_terms), terms = [(slice, coef), ...].
         Output slices are u-layout slice indices mode*Ne + elem."""

@@ -139,6 +139,7 @@ extern "C" int dgfs_collision_plan_create(int N, int M, int Nrho, double L, cons
     const size_t cs = 2 * es;
     hipError_t e1 = hipMalloc(&pl->Fhat, size_t(ntask_max) * 2 * n * cs);
     hipError_t e2 = hipMalloc(&pl->acc, size_t(ntask_max) * Nrho * pl->QG * n * cs);
+    pl->QG_cap = pl->QG;
     hipError_t e3 = hipMalloc(&pl->Iw, size_t(ntask_max) * 2 * n * cs);
     hipError_t e4 = hipMalloc(&pl->H, size_t(ntask_max) * 2 * n * cs);
     hipError_t e5 = hipMalloc(&pl->nyq, size_t(ntask_max) * 2 * Nrho * M * 6 * N * N * es);
@@ -212,6 +213,22 @@ extern "C" int dgfs_collision_plan_query(dgfs_collision_plan_t plan, int* split,
     if (pgroups) *pgroups = pl->SD;
     if (scratch_bytes)
         *scratch_bytes = size_t(pl->ntask_max) * pl->n * cs * (6 + pl->Nrho * pl->QG);
+    return 0;
+}
+
+extern "C" int dgfs_collision_plan_set_qgroups(dgfs_collision_plan_t plan, int qgroups) {
+    auto* pl = static_cast<CollisionPlan*>(plan);
+    DGFS_REQUIRE(pl, "collision_plan_set_qgroups: null plan");
+    DGFS_REQUIRE(qgroups >= 1 && qgroups <= pl->M, "collision_plan_set_qgroups: need 1 <= qg <= M");
+    if (qgroups > pl->QG_cap) {
+        const size_t cs = pl->dtype == DGFS_F64 ? 16 : 8;
+        DGFS_HIP_CHECK(hipDeviceSynchronize());
+        DGFS_HIP_CHECK(hipFree(pl->acc));
+        pl->acc = nullptr;
+        DGFS_HIP_CHECK(hipMalloc(&pl->acc, size_t(pl->ntask_max) * pl->Nrho * qgroups * pl->n * cs));
+        pl->QG_cap = qgroups;
+    }
+    pl->QG = qgroups;
     return 0;
 }
 

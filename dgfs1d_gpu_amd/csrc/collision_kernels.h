@@ -168,55 +168,60 @@ k_fwd_line(long nlines, int nfg, const cplx<T>* __restrict__ Iw, cplx<T>* __rest
 // k_nyq writes Im of A+-, B+-, C+- for every (task, which, p, q):
 // nyq[t][w][p][q][6][N*N] with planes A+ A- [y][z], B+ B- [z][x], C+ C- [y][x].
 template <typename T, int N>
-__global__ void __launch_bounds__(planes_per_wg(N) * N)
+__global__ void __launch_bounds__(6 * N)
 k_nyq(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase, int M, int Nrho,
       int NW, T* __restrict__ nyq) {
-    constexpr int PL = planes_per_wg(N), NN = N * N, n = N * NN, PS = N * (N + 1);
-    constexpr int H = N / 2;
-    __shared__ cplx<T> L[PL * PS];
+    // one 2D plane per N-thread slot: slot = 2 type + sign; thread c walks k
+    constexpr int NN = N * N, n = N * NN, PS = N * (N + 1), H = N / 2;
+    __shared__ cplx<T> L[6 * PS];
     const int bid = blockIdx.x;
     const int q = bid % M, p = (bid / M) % Nrho, w = (bid / (M * Nrho)) % NW;
     const int t = bid / (M * Nrho * NW);
     const cplx<T>* F = Fhat + (size_t(t) * 2 + w) * n;
     const cplx<T>* ph = phase + (size_t(p) * M + q) * 3 * N;
     const int slot = threadIdx.x / N, c = threadIdx.x - slot * N;
+    const int type = slot >> 1;
+    const bool minus = (slot & 1) != 0;
     cplx<T>* P = L + slot * PS;
-    T* out = nyq + ((((size_t(t) * NW + w) * Nrho + p) * M + q) * 6) * NN;
-    constexpr int NIT = (6 + PL - 1) / PL;
-    for (int it = 0; it < NIT; ++it) {
-        const int pl = it * PL + slot;
-        const bool live = pl < 6;
-        const int type = pl / 2;
-        const bool minus = (pl & 1) != 0;
-        __syncthreads();
-        if (live) {
-            sfor<0, N>([&](auto A) {
-                constexpr int a = A;
-                int l1, l2, l3;
-                bool zero = false;
-                if (type == 0) { l1 = H; l2 = a; l3 = c; }
-                else if (type == 1) { l1 = a; l2 = H; l3 = c; zero = (a == H); }
-                else { l1 = a; l2 = c; l3 = H; zero = (a == H) || (c == H); }
-                const cplx<T> e = ph[l1] * ph[N + l2] * ph[2 * N + l3];
-                const cplx<T> v = cmul_conj_if(F[(size_t(l3) * N + l1) * N + l2], e, minus);
-                P[a * (N + 1) + c] = zero ? mk(T(0), T(0)) : v;
-            });
-        }
-        __syncthreads();
-        if (live) lds_fft_row<T, N, +1>(P, c);      // over b = column index
-        __syncthreads();
-        if (live) lds_fft_col<T, N, +1>(P, c);      // over a = row index
-        __syncthreads();
-        if (live) {
-            T* dst = out + size_t(pl) * NN;
-            // thread c writes row c of the stored plane
-            sfor<0, N>([&](auto K) {
-                constexpr int k = K;
-                if (type == 0) dst[c * N + k] = P[c * (N + 1) + k].y;        // A[y=c][z=k]
-                else dst[c * N + k] = P[k * (N + 1) + c].y;                  // B[z=c][x=k], C[y=c][x=k]
-            });
-        }
+    // loads: rows a, columns b of the plane (A: a = l2, b = l3; B: a = l1,
+    // b = l3; C: a = l1, b = l2); consecutive threads read consecutive l2
+    // where the plane has an l2 axis (A, C)
+    if (type == 0) {                       // thread c = l2, k = l3
+        const cplx<T> e0 = ph[H] * ph[N + c];
+        sfor<0, N>([&](auto K) {
+            constexpr int k = K;
+            const cplx<T> e = e0 * ph[2 * N + k];
+            P[c * (N + 1) + k] = cmul_conj_if(F[(size_t(k) * N + H) * N + c], e, minus);
+        });
+    } else if (type == 1) {                // thread c = l1, k = l3
+        const cplx<T> e0 = ph[c] * ph[N + H];
+        const bool zero = c == H;
+        sfor<0, N>([&](auto K) {
+            constexpr int k = K;
+            const cplx<T> e = e0 * ph[2 * N + k];
+            const cplx<T> v = cmul_conj_if(F[(size_t(k) * N + c) * N + H], e, minus);
+            P[c * (N + 1) + k] = zero ? mk(T(0), T(0)) : v;
+        });
+    } else {                               // thread c = l2, k = l1
+        const cplx<T> e0 = ph[N + c] * ph[2 * N + H];
+        sfor<0, N>([&](auto K) {
+            constexpr int k = K;
+            const cplx<T> e = ph[k] * e0;
+            const cplx<T> v = cmul_conj_if(F[(size_t(H) * N + k) * N + c], e, minus);
+            P[k * (N + 1) + c] = (k == H || c == H) ? mk(T(0), T(0)) : v;
+        });
     }
+    __syncthreads();
+    lds_fft_row<T, N, +1>(P, c);          // over b for row c
+    __syncthreads();
+    lds_fft_col<T, N, +1>(P, c);          // over a for column c
+    __syncthreads();
+    // stores (coalesced): A[y][z] = P[y][z]; B[z][x] = P[x][z]; C[y][x] = P[x][y]
+    T* dst = nyq + ((((size_t(t) * NW + w) * Nrho + p) * M + q) * 6 + slot) * NN;
+    sfor<0, N>([&](auto K) {
+        constexpr int k = K;
+        dst[k * N + c] = (type == 0) ? P[k * (N + 1) + c].y : P[c * (N + 1) + k].y;
+    });
 }
 
 // LDS image of one (which)'s Nyquist data for residue r, row pitch BP
@@ -482,6 +487,7 @@ k_line_output(const T* __restrict__ u, TaskView tv, const cplx<T>* __restrict__ 
 struct CollisionPlan {
     int N, M, Nrho, dtype, variant, ntask_max, ngroup_max;
     int SA, SB, SC, SD, QG;
+    int QG_cap = 0;         // q groups the acc scratch holds
     double prefac;
     size_t n;
     void* phase = nullptr;  // [Nrho][M][3][N] cplx<T>
@@ -529,7 +535,7 @@ struct Impl {
                            nfg, (const cplx<T>*)pl->Iw, (cplx<T>*)pl->Fhat);
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 1, st);
-        hipLaunchKernelGGL((k_nyq<T, N>), dim3(ntask * nfg * pl->Nrho * pl->M), dim3(PL * N), 0,
+        hipLaunchKernelGGL((k_nyq<T, N>), dim3(ntask * nfg * pl->Nrho * pl->M), dim3(6 * N), 0,
                            st, (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->phase, pl->M,
                            pl->Nrho, nfg, (T*)pl->nyq);
         DGFS_HIP_CHECK(hipGetLastError());

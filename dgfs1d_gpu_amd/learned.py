@@ -1,0 +1,291 @@
+"""Learned collision closure: network, training and the runtime fallback policy.
+
+No reference counterpart.  The network (see ``ClosureNet``) predicts the
+residual R(f) = Q(f, f) - Q_Shakov(f) of the spectral operator over the
+Shakov model (dgfs1D/std/scattering.py:1636-1833) on the velocity grid.
+Scaling: the input is f / rho (unit density, rho = cw sum f) and the target
+R / (prefac rho^2), both exact consequences of Q and Q_Shakov being
+quadratic in f and proportional to their prefactors (whose ratio depends
+only on the gas model).  All arithmetic runs in the HIP kernels of
+libdgfs_hip.so (closure.hip): bf16 MFMA for the 32-channel layers, FP32
+accumulation, FP32 Adam on master weights.
+
+``LearnedClosure`` is the runtime: Q = Q_Shakov + R_net for every node,
+except nodes handed back to the spectral operator.  Every ``check_every``
+calls the spectral operator evaluates a sample of the learned nodes; a node
+whose relative L2 error exceeds ``tol`` is switched to the spectral operator
+for the rest of the run.
+"""
+import numpy as np
+
+from . import _lib
+
+C = 32
+
+
+class ClosureNet:
+    """FP32 master weights on the device plus bf16 packed copies."""
+
+    SHAPES = dict(W1=(27, C), b1=(C,), W2=(27, C, C), b2=(C,), W3=(27, C, C), b3=(C,),
+                  W4=(27, C), b4=(1,))
+
+    def __init__(self, device, seed=0, params=None):
+        import torch
+        self.torch = torch
+        self.device = device
+        if params is None:
+            params = self.init_params(seed)
+        self.p = {k: torch.tensor(np.asarray(params[k], np.float32).reshape(self.SHAPES[k]),
+                                  device=device) for k in self.SHAPES}
+        self.packed = {k: torch.empty(27 * C * C, dtype=torch.bfloat16, device=device)
+                       for k in ('W2f', 'W2t', 'W3f', 'W3t')}
+        self.m = {k: torch.zeros_like(v) for k, v in self.p.items()}
+        self.v = {k: torch.zeros_like(v) for k, v in self.p.items()}
+        self.g = {k: torch.zeros_like(v) for k, v in self.p.items()}
+        self.step_count = 0
+        self.pack()
+
+    @classmethod
+    def init_params(cls, seed):
+        """He-uniform fan-in initialisation (fan-in 27 for layer 1, 864 after)."""
+        rng = np.random.default_rng(seed)
+        out = {}
+        for k, shp in cls.SHAPES.items():
+            if k.startswith('b'):
+                out[k] = np.zeros(shp)
+            else:
+                fan = 27 if k == 'W1' else 27 * C
+                lim = np.sqrt(6.0 / fan)
+                out[k] = rng.uniform(-lim, lim, shp)
+        out['W4'] *= 0.1
+        return out
+
+    def params_numpy(self):
+        return {k: v.cpu().numpy().astype(np.float64) for k, v in self.p.items()}
+
+    def save(self, path):
+        np.savez(path, **{k: v.cpu().numpy() for k, v in self.p.items()})
+
+    @classmethod
+    def load(cls, path, device):
+        with np.load(path) as z:
+            return cls(device, params={k: z[k] for k in cls.SHAPES})
+
+    def pack(self, stream=None):
+        P, K = self.p, self.packed
+        _lib.check(_lib.lib().dgfs_closure_pack(
+            _lib.ptr(P['W2']), _lib.ptr(P['W3']), _lib.ptr(K['W2f']), _lib.ptr(K['W2t']),
+            _lib.ptr(K['W3f']), _lib.ptr(K['W3t']), _lib.stream_ptr(stream)))
+
+    def cstruct(self):
+        P, K = self.p, self.packed
+        s = _lib.ClosureNet()
+        s.W1, s.b1, s.b2, s.b3 = (P['W1'].data_ptr(), P['b1'].data_ptr(), P['b2'].data_ptr(),
+                                  P['b3'].data_ptr())
+        s.W4, s.b4 = P['W4'].data_ptr(), P['b4'].data_ptr()
+        s.W2f, s.W2t, s.W3f, s.W3t = (K['W2f'].data_ptr(), K['W2t'].data_ptr(),
+                                      K['W3f'].data_ptr(), K['W3t'].data_ptr())
+        return s
+
+    def gstruct(self):
+        s = _lib.ClosureGrad()
+        for k in self.SHAPES:
+            setattr(s, k, self.g[k].data_ptr())
+        return s
+
+
+class ClosureEngine:
+    """Buffers for one batch shape (nb slices of G^3) and the kernel calls."""
+
+    def __init__(self, net, G, nb, chunk=4096):
+        import torch
+        self.torch, self.net, self.G, self.nb = torch, net, int(G), int(nb)
+        dev = net.device
+        nvox = self.nb * self.G ** 3
+        self.chunk = max(int(chunk), (nvox + 65534) // 65535)
+        self.act = torch.empty(6 * nvox * C, dtype=torch.bfloat16, device=dev)
+        self.gact = None
+        self.fin = None
+        self.work = None
+        self.pred = torch.empty(nvox, dtype=torch.float32, device=dev)
+        self.dout = torch.empty(nvox, dtype=torch.float32, device=dev)
+        self.loss = torch.empty(self.nb, dtype=torch.float64, device=dev)
+        nlw = _lib.lib().dgfs_closure_loss_work_bytes(self.G, self.nb)
+        self.lwork = torch.empty(nlw // 8 + 1, dtype=torch.float64, device=dev)
+
+    def _train_buffers(self):
+        t = self.torch
+        if self.gact is None:
+            dev = self.net.device
+            nvox = self.nb * self.G ** 3
+            self.gact = t.empty(2 * nvox * C, dtype=t.bfloat16, device=dev)
+            self.fin = t.empty(nvox, dtype=t.float32, device=dev)
+            nw = _lib.lib().dgfs_closure_work_bytes(self.G, self.nb, self.chunk)
+            self.work = t.empty(nw // 4 + 1, dtype=t.float32, device=dev)
+
+    def forward(self, u, src, scale, mode=0, dst=None, oscale=None, Q=None, stream=None):
+        """mode 0: self.pred = net(scale * u[src]); mode 1: Q[dst] += oscale * net(...)."""
+        net = self.net.cstruct()
+        _lib.check(_lib.lib().dgfs_closure_forward(
+            _lib.C.byref(net), _lib.dtype_code(u.dtype), self.G, self.nb, _lib.ptr(u),
+            _lib.ptr(src), _lib.ptr(scale), _lib.ptr(self.act), int(mode),
+            _lib.ptr(self.pred if mode == 0 else None), _lib.ptr(dst), _lib.ptr(oscale),
+            _lib.ptr(Q), _lib.stream_ptr(stream)))
+        return self.pred
+
+    def backward(self, u, src, scale, dout, stream=None):
+        self._train_buffers()
+        net, gr = self.net.cstruct(), self.net.gstruct()
+        _lib.check(_lib.lib().dgfs_closure_backward(
+            _lib.C.byref(net), _lib.C.byref(gr), _lib.dtype_code(u.dtype), self.G, self.nb,
+            _lib.ptr(u), _lib.ptr(src), _lib.ptr(scale), _lib.ptr(self.act), _lib.ptr(dout),
+            _lib.ptr(self.gact), _lib.ptr(self.fin), self.chunk, _lib.ptr(self.work),
+            _lib.stream_ptr(stream)))
+        return self.net.g
+
+    def loss_grad(self, target, stream=None):
+        _lib.check(_lib.lib().dgfs_closure_loss(
+            self.G, self.nb, _lib.ptr(self.pred), _lib.ptr(target), _lib.ptr(self.dout),
+            _lib.ptr(self.loss), _lib.ptr(self.lwork), _lib.stream_ptr(stream)))
+        return self.loss
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, stream=None):
+        net = self.net
+        net.step_count += 1
+        t = net.step_count
+        lr_t = lr * np.sqrt(1.0 - beta2 ** t) / (1.0 - beta1 ** t)
+        for k in net.SHAPES:
+            _lib.check(_lib.lib().dgfs_adam(
+                net.p[k].numel(), _lib.ptr(net.p[k]), _lib.ptr(net.g[k]), _lib.ptr(net.m[k]),
+                _lib.ptr(net.v[k]), float(lr_t), float(beta1), float(beta2), float(eps),
+                _lib.stream_ptr(stream)))
+        net.pack(stream)
+
+
+def train(net, f, target, G, batch=16, epochs=10, lr=1e-3, seed=0, log=None):
+    """Adam on the relative-L2 loss over (f, target) pairs: f (S, G^3) FP32
+    already scaled to unit density, target (S, G^3) FP32 scaled residuals.
+    Returns the per-epoch mean training loss."""
+    import torch
+    S = f.shape[0]
+    nb = min(batch, S)
+    eng = ClosureEngine(net, G, nb)
+    dev = net.device
+    src = torch.arange(nb, dtype=torch.int32, device=dev)
+    ones = torch.ones(nb, dtype=torch.float32, device=dev)
+    rng = np.random.default_rng(seed)
+    fb = torch.empty((nb, G ** 3), dtype=torch.float32, device=dev)
+    tb = torch.empty((nb, G ** 3), dtype=torch.float32, device=dev)
+    hist = []
+    for ep in range(epochs):
+        perm = rng.permutation(S)
+        tot, cnt = 0.0, 0
+        for i in range(0, S - nb + 1, nb):
+            idx = torch.tensor(perm[i:i + nb], device=dev)
+            fb.copy_(f.index_select(0, idx))
+            tb.copy_(target.index_select(0, idx))
+            eng.forward(fb, src, ones, mode=0)
+            loss = eng.loss_grad(tb)
+            eng.backward(fb, src, ones, eng.dout)
+            eng.adam(lr)
+            tot += float(loss.sum())
+            cnt += nb
+        hist.append(tot / max(cnt, 1))
+        if log:
+            log('epoch %d loss %.4e' % (ep, hist[-1]))
+    return hist
+
+
+class LearnedClosure:
+    """Q += Q_Shakov + R_net per node, with the spectral operator as fallback.
+
+    nodes: velocity slices s with tasks (s -> s) of a nodal basis.  shakov:
+    relax.Shakov bound to the grid; spectral: collision.CollisionPlan;
+    prefac: the spectral operator's prefactor (scales R)."""
+
+    def __init__(self, net, G, nslice, shakov, spectral, prefac, cw, device, check_every=10,
+                 sample=0.25, tol=0.05, seed=0):
+        import torch
+        from .collision import task_table
+        self.torch, self.net, self.G, self.ns = torch, net, int(G), int(nslice)
+        self.shakov, self.spectral, self.prefac, self.cw = shakov, spectral, float(prefac), cw
+        self.device, self.check_every, self.sample, self.tol = device, check_every, sample, tol
+        self.rng = np.random.default_rng(seed)
+        self.task_table = task_table
+        self.learned = np.ones(self.ns, bool)
+        self.calls = 0
+        self.checks = []                     # (call, node errors) history
+        self._eng = {}
+        self._rebuild()
+
+    def _rebuild(self):
+        t, dev = self.torch, self.device
+        idx = np.nonzero(self.learned)[0].astype(np.int32)
+        spec = np.nonzero(~self.learned)[0].astype(np.int32)
+        self.idx = t.tensor(idx, device=dev)
+        tasks = [(int(s), [(int(s), 1.0)], [(int(s), 1.0)]) for s in idx]
+        self.sh_tab = self.task_table(1, self.ns, tasks, self.shakov.tdtype, dev) if len(idx) else None
+        stasks = [(int(s), [(int(s), 1.0)], [(int(s), 1.0)]) for s in spec]
+        self.sp_tab = self.task_table(1, self.ns, stasks, self.spectral.tdtype, dev) \
+            if len(spec) else None
+        self.nl = len(idx)
+
+    def _engine(self, nb):
+        if nb not in self._eng:
+            self._eng[nb] = ClosureEngine(self.net, self.G, nb)
+        return self._eng[nb]
+
+    def _scales(self, u, idx):
+        """1/rho and prefac rho^2 per listed slice (rho = cw sum f)."""
+        n = self.G ** 3
+        rho = u.view(self.ns, n).index_select(0, idx).double().sum(1) * self.cw
+        return (1.0 / rho).float(), (self.prefac * rho * rho).float()
+
+    def learned_into(self, u, Q, idx):
+        if len(idx) == 0:
+            return
+        t = self.torch
+        nb = len(idx)
+        tab = self.task_table(1, self.ns, [(int(s), [(int(s), 1.0)], [(int(s), 1.0)])
+                                           for s in idx.cpu().numpy()], self.shakov.tdtype,
+                              self.device)
+        self.shakov(u, Q, tasks=tab)
+        sc, osc = self._scales(u, idx)
+        self._engine(nb).forward(u, idx, sc, mode=1, dst=idx, oscale=osc, Q=Q)
+
+    def __call__(self, u, Q):
+        """Q += collision term of every slice of u (shape (nslice * G^3,))."""
+        self.calls += 1
+        if self.nl:
+            self.shakov(u, Q, tasks=self.sh_tab)
+            sc, osc = self._scales(u, self.idx)
+            self._engine(self.nl).forward(u, self.idx, sc, mode=1, dst=self.idx, oscale=osc,
+                                          Q=Q)
+        if self.sp_tab is not None:
+            self.spectral(u, Q, tasks=self.sp_tab)
+        if self.check_every and self.calls % self.check_every == 0 and self.nl:
+            self.check(u)
+
+    def check(self, u):
+        """Spectral vs learned on a sample of learned nodes; switch failures."""
+        t = self.torch
+        cand = np.nonzero(self.learned)[0]
+        k = max(1, int(round(self.sample * len(cand))))
+        pick = np.sort(self.rng.choice(cand, size=min(k, len(cand)), replace=False))
+        idx = t.tensor(pick.astype(np.int32), device=self.device)
+        n = self.G ** 3
+        ql = t.zeros_like(u)
+        self.learned_into(u, ql, idx)
+        qs = t.zeros_like(u)
+        tab = self.task_table(1, self.ns, [(int(s), [(int(s), 1.0)], [(int(s), 1.0)])
+                                           for s in pick], self.spectral.tdtype, self.device)
+        self.spectral(u, qs, tasks=tab)
+        a = ql.view(self.ns, n).index_select(0, idx).double()
+        b = qs.view(self.ns, n).index_select(0, idx).double()
+        err = ((a - b).norm(dim=1) / b.norm(dim=1).clamp_min(1e-300)).cpu().numpy()
+        self.checks.append((self.calls, dict(zip(pick.tolist(), err.tolist()))))
+        bad = pick[err > self.tol]
+        if len(bad):
+            self.learned[bad] = False
+            self._rebuild()
+        return err

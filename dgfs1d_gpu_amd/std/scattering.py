@@ -11,6 +11,7 @@ one RHS in one set of launches with the U/V contraction fused (std.py:352-356).
 reference behaviour) routes the collision output through the FP64
 mass/momentum/energy projection before it is added to the RHS.
 """
+import os
 from math import gamma
 
 import numpy as np
@@ -129,9 +130,46 @@ class DGFSVHSGLLScatteringModelStd(DGFSScatteringModelStd):
             tasks = [(o, f, f) for o, f, _ in tasks]       # fs() uses d_arr_in only
         self.plan.set_tasks(K, Ne, _fold_same_slice(tasks))
         self._single = {}
+        self.closure = self._make_closure(K, Ne, tasks)
+
+    def _make_closure(self, K, Ne, tasks):
+        """[scattering-model] closure = learned: Q = Q_Shakov + R_net per node,
+        spectral fallback (see learned.LearnedClosure).  Needs a nodal basis
+        (one task per slice, f = g = that slice) and muRef for Q_Shakov."""
+        cfg = self.cfg
+        mode = cfg.lookupordefault('scattering-model', 'closure', 'none')
+        if mode == 'none':
+            return None
+        if mode != 'learned':
+            raise ValueError("scattering-model::closure must be 'none' or 'learned'")
+        for o, f, g in tasks:
+            if len(f) != 1 or f[0][0] != o or sorted(f) != sorted(g):
+                raise ValueError('the learned closure needs a nodal basis (slice -> slice tasks)')
+        from ..learned import ClosureNet, LearnedClosure
+        from ..relax import Shakov
+        vm = self.vm
+        Pr = float(cfg.lookupordefault('scattering-model', 'Pr', 2. / 3.))
+        muRef = float(cfg.lookupfloat('scattering-model', 'muRef'))
+        Tref = float(cfg.lookupfloat('scattering-model', 'Tref'))
+        t0 = vm.H0() / vm.u0()
+        visc = muRef * ((vm.T0() / Tref) ** self._omega)
+        p0 = vm.n0() * vm.R0 / vm.NA * vm.T0()
+        td = _torch_dtype(cfg)
+        sk = Shakov(vm.vsize(), vm.d_cvx(), vm.d_cvy(), vm.d_cvz(), vm.cw(), t0 * p0 / visc,
+                    self._omega, Pr, td, self._device)
+        net = ClosureNet.load(os.path.expanduser(cfg.lookup('scattering-model', 'closure-weights')),
+                              self._device)
+        return LearnedClosure(
+            net, vm.Nv(), K * Ne, sk, self.plan, self._prefactor, vm.cw(), self._device,
+            check_every=int(cfg.lookupordefault('scattering-model', 'closure-check-every', 10)),
+            sample=float(cfg.lookupordefault('scattering-model', 'closure-sample', 0.25)),
+            tol=float(cfg.lookupordefault('scattering-model', 'closure-tol', 0.05)))
 
     def _collide(self, u, Q, stream=None):
-        self.plan(u, Q, stream)
+        if self.closure is not None:
+            self.closure(u, Q)
+        else:
+            self.plan(u, Q, stream)
 
     # ---------------------------------------------------- single node fs()
     def fs(self, d_f, d_g, d_Q, elem, modein, modeout):

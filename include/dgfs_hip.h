@@ -57,6 +57,12 @@ int dgfs_collision_plan_destroy(dgfs_collision_plan_t plan);
  * split of the forward/weight/output kernels, and the scratch bytes */
 int dgfs_collision_plan_query(dgfs_collision_plan_t plan, int* split, int* qgroups,
                               int* pgroups, size_t* scratch_bytes);
+/* Tuning knob (no reference counterpart): the number of q groups the gain
+ * kernel splits the M sphere directions into (more groups = more
+ * work-groups, more partial sums).  1 <= qgroups <= M.  Grows the scratch
+ * when needed (synchronises the device): call between launches, not while
+ * capturing a graph. */
+int dgfs_collision_plan_set_qgroups(dgfs_collision_plan_t plan, int qgroups);
 
 /* Batched collision: replaces the per-node loop
  *     for m: U/V transform; for r in sigModes[m], e: sm.fs(d_f, d_g, d_ux, e, r, m)
@@ -289,6 +295,53 @@ typedef struct {
 } dgfs_ars_t;
 int dgfs_imex_ars_update(int dtype, int nslice, int Nv, const dgfs_ars_t* args, int what,
                          hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Learned collision closure (no reference counterpart; its training targets
+ * are dgfs_collide minus dgfs_shakov, std/scattering.py:1636-1833).
+ * Network on the G^3 velocity grid (G = 16, 32 or 48): conv3x3x3 1->32, SiLU,
+ * 32->32, SiLU, 32->32, SiLU, 32->1; zero padding; tap = (dz+1)*9+(dy+1)*3+(dx+1),
+ * out[v] = sum_tap W[tap] . in[v + d(tap)].  Device weights: W1 [27][32] and
+ * W4 [27][32] FP32; W2/W3 as bf16 copies W*f [27][32 out][32 in] and
+ * tap-flipped transposes W*t [27][32 in][32 out] (dgfs_closure_pack from FP32
+ * masters [27][32][32]); biases FP32.  The 32->32 layers use bf16 MFMA with
+ * FP32 accumulation; activations are bf16 channels-last.
+ *
+ * dgfs_closure_forward: input slice b is scale[b] * u[src[b]] (solver dtype);
+ *   act = 6 * nb*G^3*32 bf16 (Z1 Y1 Z2 Y2 Z3 Y3, kept for the backward pass);
+ *   mode 0: pred[b][v] = network output (FP32);
+ *   mode 1: Q[dst[b]][v] += oscale[b] * output (solver dtype).
+ * dgfs_closure_backward: gradients of sum_v dOut[b][v] * pred[b][v] with
+ *   respect to every weight and bias (FP32, overwritten); gact = 2 * nb*G^3*32
+ *   bf16, fin = nb*G^3 FP32, work = dgfs_closure_work_bytes(G, nb, chunk);
+ *   chunk = voxels per partial weight-gradient block (fixed-order reduction).
+ * dgfs_closure_loss: loss[b] = |pred_b - target_b|^2 / |target_b|^2 (FP64) and
+ *   dOut = d(mean_b loss_b)/d(pred); work = dgfs_closure_loss_work_bytes.
+ * dgfs_adam: w -= lr_t m / (sqrt(v) + eps) after the moment updates. */
+typedef struct {
+    const float* W1; const float* b1;
+    const void* W2f; const void* W2t; const float* b2;
+    const void* W3f; const void* W3t; const float* b3;
+    const float* W4; const float* b4;
+} dgfs_closure_t;
+typedef struct {
+    float* W1; float* b1; float* W2; float* b2; float* W3; float* b3; float* W4; float* b4;
+} dgfs_closure_grad_t;
+size_t dgfs_closure_work_bytes(int G, int nb, int chunk);
+int dgfs_closure_forward(const dgfs_closure_t* net, int dtype, int G, int nb, const void* u,
+                         const int* src, const float* scale, void* act, int mode, float* pred,
+                         const int* dst, const float* oscale, void* Q, hipStream_t stream);
+int dgfs_closure_backward(const dgfs_closure_t* net, const dgfs_closure_grad_t* grad, int dtype,
+                          int G, int nb, const void* u, const int* src, const float* scale,
+                          const void* act, const float* dOut, void* gact, float* fin, int chunk,
+                          float* work, hipStream_t stream);
+int dgfs_closure_pack(const float* W2, const float* W3, void* W2f, void* W2t, void* W3f,
+                      void* W3t, hipStream_t stream);
+size_t dgfs_closure_loss_work_bytes(int G, int nb);
+int dgfs_closure_loss(int G, int nb, const float* pred, const float* target, float* dOut,
+                      double* loss, double* work, hipStream_t stream);
+int dgfs_adam(int n, float* w, const float* g, float* m, float* v, double lr_t, double beta1,
+              double beta2, double eps, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,140 @@
+"""GPU parity of the learned-closure kernels (bf16 MFMA convolution network)
+against the closure oracle.
+
+The oracle rounds to bf16 exactly where the kernels store bf16 (hidden
+weights, pre-activations, activations, back-propagated gradients), so the
+remaining differences are FP32 accumulation order and the rare bf16 rounding
+flips it causes: tolerances are relative L2 per tensor, 5e-3 for the forward
+output and 2e-2 for each parameter gradient; against the exact FP64 network
+(no bf16) the forward stays within 3e-2.  Loss and Adam kernels: 1e-6.
+"""
+import numpy as np
+import pytest
+
+from oracle import closure as OCL
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(G, B, seed):
+    c0 = np.linspace(-5 + 5 / G, 5 - 5 / G, G)
+    Z, Y, X = np.meshgrid(c0, c0, c0, indexing='ij')
+    rng = np.random.default_rng(seed)
+    f = np.empty((B, G, G, G))
+    for b in range(B):
+        u = rng.uniform(-0.5, 0.5, 3)
+        T = rng.uniform(0.7, 1.5)
+        f[b] = np.exp(-((X - u[0]) ** 2 + (Y - u[1]) ** 2 + (Z - u[2]) ** 2) / T) / (np.pi * T) ** 1.5
+        f[b] *= 1.0 + 0.1 * rng.uniform(-1, 1, (G, G, G))
+    return f
+
+
+def _rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def _setup(cuda, G, B, seed=0):
+    import torch
+    from dgfs1d_gpu_amd.learned import ClosureNet, ClosureEngine
+    net = ClosureNet(cuda, seed=seed)
+    P = net.params_numpy()
+    onet = OCL.Net(P['W1'].reshape(27, 32, 1), P['b1'], P['W2'], P['b2'], P['W3'], P['b3'],
+                   P['W4'].reshape(27, 1, 32), P['b4'])
+    f = _inputs(G, B, seed + 1)
+    rho = f.reshape(B, -1).sum(1)
+    scale = (1.0 / rho).astype(np.float32)
+    u = torch.tensor(f.reshape(-1), dtype=torch.float32, device=cuda)
+    src = torch.arange(B, dtype=torch.int32, device=cuda)
+    sc = torch.tensor(scale, device=cuda)
+    fs = (scale[:, None, None, None] * f.astype(np.float32)).astype(np.float32).astype(np.float64)
+    eng = ClosureEngine(net, G, B, chunk=2048)
+    return net, onet, eng, u, src, sc, fs
+
+
+@pytest.mark.parametrize('G,B', [(16, 3), (32, 1)])
+def test_forward(cuda, G, B):
+    net, onet, eng, u, src, sc, fs = _setup(cuda, G, B)
+    pred = eng.forward(u, src, sc, mode=0).cpu().numpy().reshape(B, G, G, G)
+    ref_bf, _ = onet.forward(fs, bf=True)
+    ref_ex, _ = onet.forward(fs, bf=False)
+    e_bf, e_ex = _rel(pred, ref_bf), _rel(pred, ref_ex)
+    print('closure forward G=%d: vs bf16 oracle %.3e, vs FP64 network %.3e' % (G, e_bf, e_ex))
+    assert e_bf < 5e-3 and e_ex < 3e-2
+
+
+def test_forward_accumulate_mode(cuda):
+    import torch
+    G, B = 16, 3
+    net, onet, eng, u, src, sc, fs = _setup(cuda, G, B)
+    pred = eng.forward(u, src, sc, mode=0).clone()
+    Q = torch.ones(B * G ** 3, dtype=torch.float32, device=cuda)
+    dst = torch.tensor([2, 0, 1], dtype=torch.int32, device=cuda)
+    osc = torch.tensor([2.0, -1.0, 0.5], dtype=torch.float32, device=cuda)
+    eng.forward(u, src, sc, mode=1, dst=dst, oscale=osc, Q=Q)
+    got = Q.cpu().numpy().reshape(B, -1)
+    p = pred.cpu().numpy().reshape(B, -1)
+    ref = np.ones_like(got)
+    for b in range(B):
+        ref[int(dst[b])] += float(osc[b]) * p[b]
+    assert np.abs(got - ref).max() < 1e-5 * np.abs(ref).max()
+
+
+def test_backward(cuda):
+    import torch
+    G, B = 16, 2
+    net, onet, eng, u, src, sc, fs = _setup(cuda, G, B, seed=3)
+    eng.forward(u, src, sc, mode=0)
+    rng = np.random.default_rng(9)
+    dout = rng.standard_normal((B, G, G, G)).astype(np.float32)
+    g = eng.backward(u, src, sc, torch.tensor(dout.ravel(), device=cuda))
+    _, cache = onet.forward(fs, bf=True)
+    ref = onet.backward(dout.astype(np.float64), cache, bf=True)
+    for k in ('W1', 'b1', 'W2', 'b2', 'W3', 'b3', 'W4', 'b4'):
+        got = g[k].cpu().numpy().astype(np.float64).reshape(ref[k].shape)
+        e = _rel(got, ref[k])
+        print('grad', k, e)
+        assert e < 2e-2, k
+
+
+def test_loss_and_adam(cuda):
+    import torch
+    from dgfs1d_gpu_amd.learned import ClosureNet, ClosureEngine
+    G, B = 16, 3
+    net = ClosureNet(cuda, seed=1)
+    eng = ClosureEngine(net, G, B)
+    rng = np.random.default_rng(4)
+    p = rng.standard_normal((B, G ** 3)).astype(np.float32)
+    t = rng.standard_normal((B, G ** 3)).astype(np.float32)
+    eng.pred.copy_(torch.tensor(p.ravel(), device=cuda))
+    loss = eng.loss_grad(torch.tensor(t.ravel(), device=cuda)).cpu().numpy()
+    rl, rg = OCL.rel_loss(p.astype(np.float64), t.astype(np.float64))
+    assert np.abs(loss - rl).max() < 1e-10 * rl.max()
+    assert np.abs(eng.dout.cpu().numpy().reshape(B, -1) - rg).max() < 1e-6 * np.abs(rg).max()
+    # one Adam step against the formula
+    w0 = {k: v.cpu().numpy().copy() for k, v in net.p.items()}
+    for k in net.g:
+        net.g[k].copy_(torch.tensor(rng.standard_normal(net.g[k].shape), dtype=torch.float32))
+    gr = {k: v.cpu().numpy().copy() for k, v in net.g.items()}
+    eng.adam(1e-3)
+    lr_t = 1e-3 * np.sqrt(1 - 0.999) / (1 - 0.9)
+    for k in net.p:
+        m, v = 0.1 * gr[k], 0.001 * gr[k] ** 2
+        ref = w0[k] - lr_t * m / (np.sqrt(v) + 1e-8)
+        assert np.abs(net.p[k].cpu().numpy() - ref).max() < 1e-6 * max(1.0, np.abs(ref).max())
+
+
+def test_training_reduces_loss(cuda):
+    """A few Adam steps on a fixed batch reduce the relative-L2 loss (the
+    target is a fixed smooth field of each input)."""
+    import torch
+    from dgfs1d_gpu_amd.learned import ClosureNet, train
+    G, S = 16, 8
+    f = _inputs(G, S, 11)
+    f = f / f.reshape(S, -1).sum(1)[:, None, None, None]
+    tgt = np.tanh(5.0 * (f - f.mean(axis=(1, 2, 3), keepdims=True))) * 1e-1
+    net = ClosureNet(cuda, seed=2)
+    hist = train(net, torch.tensor(f.reshape(S, -1), dtype=torch.float32, device=cuda),
+                 torch.tensor(tgt.reshape(S, -1), dtype=torch.float32, device=cuda), G,
+                 batch=4, epochs=15, lr=3e-3)
+    print('training loss', hist[0], hist[-1])
+    assert hist[-1] < 0.5 * hist[0]

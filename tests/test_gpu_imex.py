@@ -1,0 +1,123 @@
+"""GPU parity of the IMEX path (dgfsAstd1D) against the FP64 oracle.
+
+The 14-moment Shakov penalisation kernels (moments, target, collide) and
+full IMEX steps -- ARS tableaux and linear-multistep schemes with their
+ARS-443 starting values -- on a small Couette deck (N = 8).  Tolerance: FP64
+relative 1e-11 of max|value|.
+"""
+import numpy as np
+import pytest
+
+from oracle import imex as OI
+from oracle.deck import Deck
+
+pytestmark = pytest.mark.gpu
+
+EXTRA = """
+[penalized-scattering-model]
+type = shakov-direct-gll
+"""
+
+
+def _text(scheme, model='vhs'):
+    from tests.test_gpu_solver import _ini
+    return _ini({('time-integrator', 'scheme'): scheme, ('scattering-model', 'type'): model,
+                 ('scattering-model', 'muRef'): 2.117e-5}) + EXTRA
+
+
+def _oracle(deck):
+    b, op = deck.basis, deck.op
+    K, Ne = deck.K, deck.Ne
+    n = deck.vm.vsize
+    Pr = 2.0 / 3.0
+    from oracle import relax as OR
+    cp = deck.cp
+    prefac = OR.shakov_prefactor(float(cp.get('mesh', 'H0')), deck.vm.u0, deck.vm.T0,
+                                 deck.vm.n0, float(cp.get('scattering-model', 'muRef')),
+                                 float(cp.get('scattering-model', 'Tref')),
+                                 float(cp.get('scattering-model', 'omega')))
+    pm = OI.Penal(deck.vm.cv, deck.vm.cw, prefac, float(cp.get('scattering-model', 'omega')), Pr)
+
+    def explicit(t, f):
+        u = f.reshape(K, Ne, n)
+        ux = op.transport(t, u)
+        return (b.invmass @ ux.reshape(K, -1)).reshape(K * Ne, n)
+
+    def boltz(t, f):
+        from oracle import collision as OC
+        out = np.zeros_like(f)
+        for s in range(K * Ne):
+            if deck.model == 'vhs':
+                out[s] = OC.q_vhs(deck.pre, f[s], f[s], deck.prefac)
+            else:
+                out[s] = OC.q_vhs_gll(deck.pre, f[s], deck.prefac)
+        return out
+    return pm, explicit, boltz
+
+
+def _solver(cuda, text):
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    from dgfs1d_gpu_amd.astd.abstd import DGFSAstd1D
+    return DGFSAstd1D(Dictionary(text), device=cuda, writers=False)
+
+
+def test_penalisation_kernels(cuda):
+    import torch
+    text = _text('imex-lm-bdf1')
+    deck = Deck(text)
+    s = _solver(cuda, text)
+    pm, _, _ = _oracle(deck)
+    rng = np.random.default_rng(2)
+    f = deck.u0.reshape(s.K * s.Ne, -1) * (1.0 + 0.1 * rng.uniform(-1, 1, (s.K * s.Ne, 1)))
+    f = f * (1.0 + 0.05 * rng.uniform(-1, 1, f.shape))
+    fd = torch.tensor(f.ravel(), dtype=torch.float64, device=cuda)
+    U = s.psm.new_moments(s.K * s.Ne)
+    s.psm.moment(0.0, fd, U)
+    Ur = pm.moment(f)
+    eU = np.abs(U.cpu().numpy().reshape(Ur.shape) - Ur).max() / np.abs(Ur).max()
+    M = torch.zeros_like(fd)
+    s.psm.constructMaxwellian(0.0, U, M)
+    Mr = pm.target(Ur)
+    eM = np.abs(M.cpu().numpy().reshape(Mr.shape) - Mr).max() / np.abs(Mr).max()
+    P = torch.zeros_like(fd)
+    s.psm.collide(0.0, U, M, fd, P)
+    Pr_ = pm.penal(Ur, f)
+    eP = np.abs(P.cpu().numpy().reshape(Pr_.shape) - Pr_).max() / np.abs(Pr_).max()
+    print('penalisation', eU, eM, eP)
+    assert max(eU, eM, eP) < 1e-12
+
+
+@pytest.mark.parametrize('scheme', ['ars-111', 'ars-222', 'ars-443'])
+def test_ars_steps(cuda, scheme):
+    text = _text(scheme)
+    deck = Deck(text)
+    s = _solver(cuda, text)
+    pm, explicit, _ = _oracle(deck)
+    f = deck.u0.reshape(s.K * s.Ne, -1).copy()
+    t = 0.0
+    for k in range(3):
+        f = OI.ars_step(scheme, pm, explicit, t, deck.dt, f)
+        s.step()
+        t += deck.dt
+    err = np.abs(s.u.cpu().numpy().reshape(f.shape) - f).max() / np.abs(f).max()
+    print(scheme, err)
+    assert err < 1e-11
+
+
+@pytest.mark.parametrize('scheme,model', [('imex-lm-bdf1', 'vhs'), ('imex-lm-bdf2', 'vhs'),
+                                          ('imex-lm-bdf3', 'vhs-gll'), ('imex-lm-cn2', 'vhs')])
+def test_lm_steps(cuda, scheme, model):
+    text = _text(scheme, model)
+    deck = Deck(text)
+    s = _solver(cuda, text)
+    pm, explicit, boltz = _oracle(deck)
+    f = deck.u0.reshape(s.K * s.Ne, -1).copy()
+    it = OI.LMIntegrator(scheme, pm, explicit, boltz, 0.0, deck.dt, f.copy())
+    t = 0.0
+    for k in range(4):
+        f = it.step(t, deck.dt, k, f)
+        s.step()
+        t += deck.dt
+        err = np.abs(s.u.cpu().numpy().reshape(f.shape) - f).max() / np.abs(f).max()
+        print(scheme, model, k, err)
+        assert err < 1e-11
