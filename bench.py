@@ -106,6 +106,33 @@ def fp64_accuracy(text, nsteps, s32, dev):
             'collision_invariant_moments_rel': cons}
 
 
+def batched_workload(seed, count, steps, warmup, dev):
+    """The seeded uneven batch (64 Couette/Fourier problems, Ne in {2,4,8},
+    K in {3,5,7}, N in {16,24,32}, M=12, N_rho=N/2, FP32): autotune the
+    collision plans, capture one step in a HIP graph, time graph replays."""
+    import torch
+    from dgfs1d_gpu_amd.batch import BatchedStd1D, make_problems
+    specs = make_problems(seed, count)
+    t0 = time.perf_counter()
+    b = BatchedStd1D(specs, dev, precision='float')
+    setup = time.perf_counter() - t0
+    tune = b.autotune()
+    b.capture(warmup=warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {'problems': count, 'seed': seed, 'points': b.points,
+            'groups': [{'N': g['N'], 'problems': len(g['solvers']), 'collisions_per_rhs':
+                        g['ntask'], 'qgroups': tune[g['N']][0]} for g in b.groups],
+            'steps': steps, 'ms_per_step': wall / steps * 1e3,
+            'value': b.points * steps / wall, 'unit': 'phase-space point-steps/s',
+            'graph': True, 'setup_s': setup,
+            'autotune_ms': {str(k): v[1] for k, v in tune.items()}}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -116,6 +143,9 @@ def main(argv=None):
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-accuracy', action='store_true',
                     help='skip the FP64 comparison run after the timed region')
+    ap.add_argument('--batch', action='store_true',
+                    help='also run the seeded 64-problem uneven batch (autotuned, graph-captured)')
+    ap.add_argument('--batch-seed', type=int, default=0)
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run 5 device-to-device copies of MIB MiB (the known-size '
                          'kernel that scales the FETCH_SIZE / WRITE_SIZE counters)')
@@ -204,6 +234,9 @@ def main(argv=None):
     }
     if not args.no_accuracy:
         out['accuracy'] = fp64_accuracy(text, args.warmup + args.steps, s, dev)
+    if args.batch:
+        del s
+        out['batched'] = batched_workload(args.batch_seed, 64, 20, args.warmup, dev)
     if not args.no_cpu_baseline:
         cores = min(16, os.cpu_count() or 1)
         out['cpu_baseline'] = cpu_baseline(text, cores)
