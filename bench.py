@@ -133,6 +133,36 @@ def batched_workload(seed, count, steps, warmup, dev):
             'autotune_ms': {str(k): v[1] for k, v in tune.items()}}
 
 
+def closure_forward(nodes, G, reps, dev):
+    """Learned-closure network forward over `nodes` velocity slices of G^3
+    (random weights and Maxwellian-mixture inputs), HIP-event timed."""
+    import numpy as np
+    import torch
+    from dgfs1d_gpu_amd.learned import ClosureEngine, ClosureNet
+    net = ClosureNet(dev, seed=0)
+    eng = ClosureEngine(net, G, nodes)
+    c0 = np.linspace(-5 + 5 / G, 5 - 5 / G, G)
+    Z, Y, X = np.meshgrid(c0, c0, c0, indexing='ij')
+    rng = np.random.default_rng(0)
+    f = np.stack([np.exp(-((X - a) ** 2 + Y ** 2 + Z ** 2) / T) / (np.pi * T) ** 1.5
+                  for a, T in zip(rng.uniform(-1, 1, nodes), rng.uniform(0.7, 1.5, nodes))])
+    u = torch.tensor(f.reshape(-1), dtype=torch.float32, device=dev)
+    src = torch.arange(nodes, dtype=torch.int32, device=dev)
+    sc = torch.ones(nodes, dtype=torch.float32, device=dev)
+    for _ in range(3):
+        eng.forward(u, src, sc, mode=0)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        eng.forward(u, src, sc, mode=0)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    flop = 2.0 * G ** 3 * 27 * (32 + 32 * 32 + 32 * 32 + 32) * nodes
+    return {'nodes': nodes, 'G': G, 'ms_per_call': ms, 'us_per_node': ms / nodes * 1e3,
+            'tflops': flop / (ms * 1e-3) / 1e12, 'peak_bf16_dense_tflops': 2500.0}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -146,6 +176,8 @@ def main(argv=None):
     ap.add_argument('--batch', action='store_true',
                     help='also run the seeded 64-problem uneven batch (autotuned, graph-captured)')
     ap.add_argument('--batch-seed', type=int, default=0)
+    ap.add_argument('--closure-forward', type=int, default=0, metavar='NODES',
+                    help='only time the learned-closure network forward on NODES slices of 32^3')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run 5 device-to-device copies of MIB MiB (the known-size '
                          'kernel that scales the FETCH_SIZE / WRITE_SIZE counters)')
@@ -156,6 +188,9 @@ def main(argv=None):
         return 2
 
     import torch
+    if args.closure_forward:
+        print(json.dumps(closure_forward(args.closure_forward, 32, 10, torch.device('cuda:0'))))
+        return 0
     if args.copy_calibration:
         nbytes = args.copy_calibration << 20
         src = torch.ones(nbytes // 4, dtype=torch.float32, device='cuda:0')

@@ -111,7 +111,7 @@ SIGNATURES = {
                                    _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp]),
     'dgfs_closure_pack': (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     'dgfs_closure_loss_work_bytes': (_sz, [_i, _i]),
-    'dgfs_closure_loss': (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'dgfs_closure_loss': (_i, [_i, _i, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp]),
     'dgfs_adam': (_i, [_i, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _vp]),
     'dgfs_shakov': (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                          _d, _d, _d, _d, _vp, _vp, _vp]),

@@ -30,7 +30,7 @@ def rankine_hugoniot(mach, gamma=5.0 / 3.0):
 
 
 def shock_deck(mach=2.0, T1=223.0, Ne=32, K=3, N=32, M=12, Nrho=16, precision='float',
-               dt=0.002, tend=5.0, rho1=1.91607e-5, dev=7.0, extra=''):
+               dt=5e-4, tend=5.0, rho1=1.91607e-5, dev=7.0, scheme='ssp-rk2', extra=''):
     """Hard-sphere (omega = 0.5) normal shock between Rankine-Hugoniot states,
     after the reference's examples/std/normalshock/dgfs1D.ini layout: x in
     [-15, 15] mm, linear initial profile, inlet boundaries."""
@@ -70,7 +70,7 @@ Tref = 273
 muRef = 1.865e-5
 {extra}
 [time-integrator]
-scheme = euler
+scheme = {scheme}
 controller = none
 tstart = 0.0
 tend = {tend}
@@ -107,7 +107,7 @@ def couette_deck(uw=0.5, Tw=1.0, N=32, Ne=4, K=3, M=12, Nrho=16, precision='floa
         .split('[soln-ics]')[0].replace('xlo = -15e-3', 'xlo = -0.5').replace(
             'xhi = 15e-3', 'xhi = 0.5').replace('H0 = 30e-3', 'H0 = 1.0').replace(
             'T0 = 223.0', 'T0 = 273').replace('Tmax = 223.0', 'Tmax = 273').replace(
-            'dt = 0.002', 'dt = %g' % dt) + f"""[soln-ics]
+            'dt = 0.0005', 'dt = %g' % dt) + f"""[soln-ics]
 type = maxwellian
 rho = 1.91607e-5
 T = 273
@@ -208,17 +208,26 @@ def main(argv=None):
     nval = max(1, S // 10)
     val, tr = perm[:nval], perm[nval:]
     log('%d pairs (G=%d), %d for validation; data %.1f s' % (S, G, nval, t_data))
-    net = ClosureNet(dev, seed=args.seed)
+    # output scale: the RMS per-voxel magnitude of the scaled collision
+    # integrals; |Q| floor: 10 % of their median norm (both in scaled units)
+    qn = qs.double().norm(dim=1)
+    sigma = float(qn.pow(2).mean().sqrt()) / math.sqrt(qs.shape[1])
+    qfloor = 0.1 * float(qn.median()) / sigma
+    net = ClosureNet(dev, seed=args.seed, meta=dict(out_scale=sigma, qfloor=qfloor))
+    tn = (tgt / sigma).contiguous()
+    qsn = (qs / sigma).contiguous()
     t1 = time.perf_counter()
-    hist = train(net, f.index_select(0, tr), tgt.index_select(0, tr), G, batch=args.batch,
-                 epochs=args.epochs, lr=args.lr, seed=args.seed, log=log)
+    hist = train(net, f.index_select(0, tr), tn.index_select(0, tr), G, batch=args.batch,
+                 epochs=args.epochs, lr=args.lr, seed=args.seed, log=log,
+                 ref=qsn.index_select(0, tr), floor2=qfloor ** 2)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
     net.save(args.out)
-    rep = evaluate(net, f.index_select(0, val), tgt.index_select(0, val),
-                   qs.index_select(0, val), G)
+    rep = evaluate(net, f.index_select(0, val), tn.index_select(0, val),
+                   qsn.index_select(0, val), G, qfloor)
     rep.update(pairs=S, grid=G, data_s=t_data, train_s=t_train, epochs=args.epochs,
-               loss_history=hist)
+               loss_history=hist, out_scale=sigma, qfloor=qfloor,
+               inference=time_inference(net, f, G))
     log(json.dumps(rep))
     if args.report:
         with open(args.report, 'w') as fh:
@@ -226,9 +235,34 @@ def main(argv=None):
     return 0
 
 
-def evaluate(net, f, tgt, qs, G, batch=16):
-    """Relative L2 error of Q on held-out pairs: Shakov alone (R = 0) versus
-    Shakov + network (all in the scaled units)."""
+def time_inference(net, f, G, nb=96, reps=10):
+    """Network forward over nb nodes (mode 0), HIP-event timed: ms per call,
+    ms per node and the algorithmic TFLOP/s (3.74 GFLOP per node at G = 32:
+    2 G^3 27 (32 + 32*32 + 32*32 + 32) multiply-adds)."""
+    import torch
+    from .learned import ClosureEngine
+    nb = min(nb, f.shape[0])
+    eng = ClosureEngine(net, G, nb)
+    src = torch.arange(nb, dtype=torch.int32, device=net.device)
+    ones = torch.ones(nb, dtype=torch.float32, device=net.device)
+    fb = f[:nb].contiguous()
+    for _ in range(3):
+        eng.forward(fb, src, ones, mode=0)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        eng.forward(fb, src, ones, mode=0)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    flop = 2.0 * G ** 3 * 27 * (32 + 32 * 32 + 32 * 32 + 32) * nb
+    return dict(nodes=nb, ms_per_call=ms, ms_per_node=ms / nb, tflops=flop / (ms * 1e-3) / 1e12)
+
+
+def evaluate(net, f, tgt, qs, G, qfloor=0.0, batch=16):
+    """Relative L2 error of Q on held-out pairs, Shakov alone (R = 0) versus
+    Shakov + network, relative to |Q| and to max(|Q|, qfloor) (scaled units,
+    targets already divided by the output scale)."""
     import torch
     from .learned import ClosureEngine
     S = f.shape[0]
@@ -236,21 +270,23 @@ def evaluate(net, f, tgt, qs, G, batch=16):
     eng = ClosureEngine(net, G, nb)
     src = torch.arange(nb, dtype=torch.int32, device=net.device)
     ones = torch.ones(nb, dtype=torch.float32, device=net.device)
-    e_sk, e_net = [], []
+    out = dict(sk=[], net=[], skf=[], netf=[])
     for i in range(0, S - nb + 1, nb):
         fb = f[i:i + nb].contiguous()
         pred = eng.forward(fb, src, ones, mode=0).view(nb, -1).double()
         t = tgt[i:i + nb].double()
-        q = qs[i:i + nb].double()
-        qn = q.norm(dim=1)
-        e_sk.append((t.norm(dim=1) / qn).cpu().numpy())
-        e_net.append(((t - pred).norm(dim=1) / qn).cpu().numpy())
-    e_sk, e_net = np.concatenate(e_sk), np.concatenate(e_net)
-    return dict(val_pairs=int(len(e_sk)),
-                q_rel_l2_shakov_median=float(np.median(e_sk)),
-                q_rel_l2_learned_median=float(np.median(e_net)),
-                q_rel_l2_shakov_p90=float(np.percentile(e_sk, 90)),
-                q_rel_l2_learned_p90=float(np.percentile(e_net, 90)))
+        qn = qs[i:i + nb].double().norm(dim=1)
+        qf = qn.clamp_min(qfloor)
+        for key, num in (('sk', t.norm(dim=1)), ('net', (t - pred).norm(dim=1))):
+            out[key].append((num / qn).cpu().numpy())
+            out[key + 'f'].append((num / qf).cpu().numpy())
+    r = {k: np.concatenate(v) for k, v in out.items()}
+    rep = dict(val_pairs=int(len(r['sk'])))
+    for k, name in (('sk', 'shakov'), ('net', 'learned'), ('skf', 'shakov_floored'),
+                    ('netf', 'learned_floored')):
+        rep['q_rel_l2_%s_median' % name] = float(np.median(r[k]))
+        rep['q_rel_l2_%s_p90' % name] = float(np.percentile(r[k], 90))
+    return rep
 
 
 if __name__ == '__main__':

@@ -134,6 +134,92 @@ k_conv_mid(int nb, const __bf16* __restrict__ X, const __bf16* __restrict__ W /*
     }
 }
 
+// Same layer with the input tile staged in LDS: the work-group's 3 x 10 x
+// (G + 2) halo of voxels (zero padded) is loaded once, so each activation is
+// read from HBM/L2 once per work-group instead of once per tap.  Voxel pitch
+// 40 bf16 (80 B) keeps the 16 lanes of a k-group on disjoint banks.
+// LDS: 27*32*40*2 + 3*10*(G+2)*40*2 bytes (150.7 KiB at G = 32).
+template <int G, int MODE>
+__global__ void __launch_bounds__(256)
+k_conv_mid_lds(int nb, const __bf16* __restrict__ X, const __bf16* __restrict__ W,
+               const float* __restrict__ bias, const __bf16* __restrict__ Zp,
+               __bf16* __restrict__ Zout, __bf16* __restrict__ Y) {
+    constexpr int NXT = G / 16, NT = 2 * NXT, YB = G / 8;
+    constexpr int XP = G + 2, XR = 10, VP = kWP;          // halo row length, rows, voxel pitch
+    __shared__ __attribute__((aligned(16))) __bf16 Ws[kTaps * kC * kWP];
+    __shared__ __attribute__((aligned(16))) __bf16 Xs[3 * XR * XP * VP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int bid = blockIdx.x;
+    const int yb = bid % YB, z = (bid / YB) % G, b = bid / (YB * G);
+    for (int r = tid; r < kTaps * kC * 4; r += 256) {
+        const int row = r >> 2, ch = r & 3;
+        *reinterpret_cast<uint4*>(&Ws[row * kWP + ch * 8]) =
+            *reinterpret_cast<const uint4*>(&W[row * kC + ch * 8]);
+    }
+    const size_t plane = size_t(G) * G;
+    const __bf16* Xb = X + size_t(b) * G * plane * kC;
+    for (int r = tid; r < 3 * XR * XP * 4; r += 256) {
+        const int ch = r & 3, vox = r >> 2;
+        const int xi = vox % XP, yi = (vox / XP) % XR, zi = vox / (XP * XR);
+        const int zz = z + zi - 1, yy = yb * 8 + yi - 1, xx = xi - 1;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (zz >= 0 && zz < G && yy >= 0 && yy < G && xx >= 0 && xx < G)
+            v = *reinterpret_cast<const uint4*>(&Xb[((size_t(zz) * G + yy) * G + xx) * kC + ch * 8]);
+        *reinterpret_cast<uint4*>(&Xs[vox * VP + ch * 8]) = v;
+    }
+    __syncthreads();
+    if (b >= nb) return;
+    const int kq = lane >> 4, li = lane & 15;
+    floatx4 acc[NT][2];
+    for (int t = 0; t < NT; ++t)
+        for (int j = 0; j < 2; ++j) acc[t][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int tap = 0; tap < kTaps; ++tap) {
+        int dz, dy, dx;
+        tap_offset(tap, dz, dy, dx);
+        bf16x8 a[2];
+        for (int j = 0; j < 2; ++j)
+            a[j] = *reinterpret_cast<const bf16x8*>(
+                &Ws[(tap * kC + 16 * j + li) * kWP + 8 * kq]);
+        for (int t = 0; t < NT; ++t) {
+            const int yl = 2 * wave + t / NXT;                 // row within the block
+            const int xi = (t % NXT) * 16 + li + dx + 1;       // halo column
+            const int vox = ((dz + 1) * XR + (yl + dy + 1)) * XP + xi;
+            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&Xs[vox * VP + 8 * kq]);
+            for (int j = 0; j < 2; ++j)
+                acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[j], bv, acc[t][j], 0, 0, 0);
+        }
+    }
+    for (int t = 0; t < NT; ++t) {
+        const int y = yb * 8 + 2 * wave + t / NXT;
+        const int x = (t % NXT) * 16 + li;
+        const size_t vo = ((size_t(b) * G + z) * plane + size_t(y) * G + x) * kC;
+        for (int j = 0; j < 2; ++j) {
+            const int c0 = 16 * j + 4 * kq;
+            __bf16 yo[4], zo[4];
+            for (int i = 0; i < 4; ++i) {
+                const float v = acc[t][j][i];
+                if (MODE == 0) {
+                    const float zv = v + bias[c0 + i];
+                    zo[i] = f2bf(zv);
+                    yo[i] = f2bf(silu(zv));
+                } else if (MODE == 1) {
+                    yo[i] = f2bf(v + bias[c0 + i]);
+                } else {
+                    yo[i] = f2bf(v * silu_d(bf2f(Zp[vo + c0 + i])));
+                }
+            }
+            if (MODE == 0) {
+                uint2 pz;
+                __builtin_memcpy(&pz, zo, 8);
+                *reinterpret_cast<uint2*>(&Zout[vo + c0]) = pz;
+            }
+            uint2 py;
+            __builtin_memcpy(&py, yo, 8);
+            *reinterpret_cast<uint2*>(&Y[vo + c0]) = py;
+        }
+    }
+}
+
 // ------------------------------------------------------- 1 -> 32, VALU
 // f (solver dtype) of slice src[b] scaled by scale[b]; stores Z1 and Y1.
 template <typename T>
@@ -363,18 +449,20 @@ k_pack_mid(const float* __restrict__ W, __bf16* __restrict__ Wf, __bf16* __restr
     Wt[((kTaps - 1 - tap) * kC + c) * kC + r] = f2bf(W[i]);
 }
 
-// relative squared error per slice and its gradient:
-// loss_b = |p - t|^2 / |t|^2, dOut = 2 (p - t) / (|t|^2 nb);  two passes.
+// relative squared error per slice and its gradient, relative to a
+// reference field r (the target itself, or the full collision integral):
+// loss_b = |p - t|^2 / max(|r|^2, floor2), dOut = 2 (p - t) / (max(..) nb);
+// the floor keeps near-equilibrium slices (|r| -> 0) from dominating.
 __global__ void __launch_bounds__(256)
 k_loss_partial(int n, const float* __restrict__ p, const float* __restrict__ t,
-               double* __restrict__ part /*[b][nblk][2]*/) {
+               const float* __restrict__ ref, double* __restrict__ part /*[b][nblk][2]*/) {
     __shared__ double sh[2 * 256];
     const int b = blockIdx.y, nblk = gridDim.x;
     const int v = blockIdx.x * 256 + threadIdx.x;
     double e = 0.0, s = 0.0;
     if (v < n) {
         const double d = double(p[size_t(b) * n + v]) - double(t[size_t(b) * n + v]);
-        const double tt = t[size_t(b) * n + v];
+        const double tt = ref[size_t(b) * n + v];
         e = d * d;
         s = tt * tt;
     }
@@ -395,14 +483,16 @@ k_loss_partial(int n, const float* __restrict__ p, const float* __restrict__ t,
 }
 
 __global__ void __launch_bounds__(256)
-k_loss_grad(int n, int nb, int nblk, const double* __restrict__ part, const float* __restrict__ p,
-            const float* __restrict__ t, float* __restrict__ dOut, double* __restrict__ loss) {
+k_loss_grad(int n, int nb, int nblk, double floor2, const double* __restrict__ part,
+            const float* __restrict__ p, const float* __restrict__ t, float* __restrict__ dOut,
+            double* __restrict__ loss) {
     const int b = blockIdx.y;
     double e = 0.0, s = 0.0;
     for (int k = 0; k < nblk; ++k) {
         e += part[(size_t(b) * nblk + k) * 2];
         s += part[(size_t(b) * nblk + k) * 2 + 1];
     }
+    s = s > floor2 ? s : floor2;
     const int v = blockIdx.x * 256 + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) loss[b] = e / s;
     if (v >= n) return;
@@ -439,23 +529,24 @@ int dispatch_mid(int G, int mode, int nb, const void* X, const void* W, const fl
     const __bf16* zp = (const __bf16*)Zp;
     __bf16* zo = (__bf16*)Zout;
     __bf16* y = (__bf16*)Y;
-#define DGFS_MID(GG)                                                                          \
+#define DGFS_MID(GG, KER)                                                                     \
     if (G == GG) {                                                                            \
         if (mode == 0)                                                                        \
-            hipLaunchKernelGGL((k_conv_mid<GG, 0>), grid, dim3(256), 0, st, nb, x, w, bias,   \
-                               zp, zo, y);                                                    \
+            hipLaunchKernelGGL((KER<GG, 0>), grid, dim3(256), 0, st, nb, x, w, bias, zp, zo,  \
+                               y);                                                            \
         else if (mode == 1)                                                                   \
-            hipLaunchKernelGGL((k_conv_mid<GG, 1>), grid, dim3(256), 0, st, nb, x, w, bias,   \
-                               zp, zo, y);                                                    \
+            hipLaunchKernelGGL((KER<GG, 1>), grid, dim3(256), 0, st, nb, x, w, bias, zp, zo,  \
+                               y);                                                            \
         else                                                                                  \
-            hipLaunchKernelGGL((k_conv_mid<GG, 2>), grid, dim3(256), 0, st, nb, x, w, bias,   \
-                               zp, zo, y);                                                    \
+            hipLaunchKernelGGL((KER<GG, 2>), grid, dim3(256), 0, st, nb, x, w, bias, zp, zo,  \
+                               y);                                                            \
         DGFS_HIP_CHECK(hipGetLastError());                                                    \
         return 0;                                                                             \
     }
-    DGFS_MID(16)
-    DGFS_MID(32)
-    DGFS_MID(48)
+    // LDS-staged input where the tile fits (G <= 32); direct loads at G = 48
+    DGFS_MID(16, k_conv_mid_lds)
+    DGFS_MID(32, k_conv_mid_lds)
+    DGFS_MID(48, k_conv_mid)
 #undef DGFS_MID
     DGFS_REQUIRE(false, "closure: grid size must be 16, 32 or 48");
 }
@@ -589,15 +680,17 @@ extern "C" size_t dgfs_closure_loss_work_bytes(int G, int nb) {
 }
 
 extern "C" int dgfs_closure_loss(int G, int nb, const float* pred, const float* target,
-                                 float* dOut, double* loss, double* work, hipStream_t st) {
+                                 const float* ref, double floor2, float* dOut, double* loss,
+                                 double* work, hipStream_t st) {
     DGFS_REQUIRE(pred && target && dOut && loss && work, "closure_loss: null pointer");
     DGFS_REQUIRE(nb >= 0 && nb <= 65535 && G > 0, "closure_loss: bad sizes");
     if (nb == 0) return 0;
     const int n = G * G * G, nblk = (n + 255) / 256;
-    hipLaunchKernelGGL(k_loss_partial, dim3(nblk, nb), dim3(256), 0, st, n, pred, target, work);
+    hipLaunchKernelGGL(k_loss_partial, dim3(nblk, nb), dim3(256), 0, st, n, pred, target,
+                       ref ? ref : target, work);
     DGFS_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_loss_grad, dim3(nblk, nb), dim3(256), 0, st, n, nb, nblk, work, pred,
-                       target, dOut, loss);
+    hipLaunchKernelGGL(k_loss_grad, dim3(nblk, nb), dim3(256), 0, st, n, nb, nblk, floor2, work,
+                       pred, target, dOut, loss);
     DGFS_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -13,8 +13,9 @@ accumulation, FP32 Adam on master weights.
 ``LearnedClosure`` is the runtime: Q = Q_Shakov + R_net for every node,
 except nodes handed back to the spectral operator.  Every ``check_every``
 calls the spectral operator evaluates a sample of the learned nodes; a node
-whose relative L2 error exceeds ``tol`` is switched to the spectral operator
-for the rest of the run.
+whose relative L2 error (relative to max(|Q|, the network's |Q| floor), so
+that near-equilibrium nodes with Q -> 0 are judged on an absolute scale)
+exceeds ``tol`` is switched to the spectral operator for the rest of the run.
 """
 import numpy as np
 
@@ -29,10 +30,14 @@ class ClosureNet:
     SHAPES = dict(W1=(27, C), b1=(C,), W2=(27, C, C), b2=(C,), W3=(27, C, C), b3=(C,),
                   W4=(27, C), b4=(1,))
 
-    def __init__(self, device, seed=0, params=None):
+    def __init__(self, device, seed=0, params=None, meta=None):
+        """meta: out_scale (the network predicts target / out_scale) and
+        qfloor (floor of |Q| in scaled units for relative errors)."""
         import torch
         self.torch = torch
         self.device = device
+        self.meta = dict(out_scale=1.0, qfloor=0.0)
+        self.meta.update(meta or {})
         if params is None:
             params = self.init_params(seed)
         self.p = {k: torch.tensor(np.asarray(params[k], np.float32).reshape(self.SHAPES[k]),
@@ -64,12 +69,15 @@ class ClosureNet:
         return {k: v.cpu().numpy().astype(np.float64) for k, v in self.p.items()}
 
     def save(self, path):
-        np.savez(path, **{k: v.cpu().numpy() for k, v in self.p.items()})
+        np.savez(path, meta=np.array([self.meta['out_scale'], self.meta['qfloor']]),
+                 **{k: v.cpu().numpy() for k, v in self.p.items()})
 
     @classmethod
     def load(cls, path, device):
         with np.load(path) as z:
-            return cls(device, params={k: z[k] for k in cls.SHAPES})
+            meta = dict(out_scale=float(z['meta'][0]), qfloor=float(z['meta'][1])) \
+                if 'meta' in z.files else None
+            return cls(device, params={k: z[k] for k in cls.SHAPES}, meta=meta)
 
     def pack(self, stream=None):
         P, K = self.p, self.packed
@@ -143,10 +151,12 @@ class ClosureEngine:
             _lib.stream_ptr(stream)))
         return self.net.g
 
-    def loss_grad(self, target, stream=None):
+    def loss_grad(self, target, ref=None, floor2=0.0, stream=None):
+        """loss_b = |pred_b - target_b|^2 / max(|ref_b|^2, floor2) (ref = target when None)."""
         _lib.check(_lib.lib().dgfs_closure_loss(
-            self.G, self.nb, _lib.ptr(self.pred), _lib.ptr(target), _lib.ptr(self.dout),
-            _lib.ptr(self.loss), _lib.ptr(self.lwork), _lib.stream_ptr(stream)))
+            self.G, self.nb, _lib.ptr(self.pred), _lib.ptr(target), _lib.ptr(ref),
+            float(floor2), _lib.ptr(self.dout), _lib.ptr(self.loss), _lib.ptr(self.lwork),
+            _lib.stream_ptr(stream)))
         return self.loss
 
     def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, stream=None):
@@ -162,10 +172,13 @@ class ClosureEngine:
         net.pack(stream)
 
 
-def train(net, f, target, G, batch=16, epochs=10, lr=1e-3, seed=0, log=None):
+def train(net, f, target, G, batch=16, epochs=10, lr=1e-3, seed=0, log=None, ref=None,
+          floor2=0.0):
     """Adam on the relative-L2 loss over (f, target) pairs: f (S, G^3) FP32
-    already scaled to unit density, target (S, G^3) FP32 scaled residuals.
-    Returns the per-epoch mean training loss."""
+    already scaled to unit density, target (S, G^3) FP32 scaled residuals;
+    the error of each pair is measured relative to ref (S, G^3) when given
+    (the scaled collision integral), else to the target.  Returns the
+    per-epoch mean training loss."""
     import torch
     S = f.shape[0]
     nb = min(batch, S)
@@ -176,58 +189,83 @@ def train(net, f, target, G, batch=16, epochs=10, lr=1e-3, seed=0, log=None):
     rng = np.random.default_rng(seed)
     fb = torch.empty((nb, G ** 3), dtype=torch.float32, device=dev)
     tb = torch.empty((nb, G ** 3), dtype=torch.float32, device=dev)
+    rb = torch.empty((nb, G ** 3), dtype=torch.float32, device=dev) if ref is not None else None
     hist = []
     for ep in range(epochs):
-        perm = rng.permutation(S)
-        tot, cnt = 0.0, 0
+        perm = torch.tensor(rng.permutation(S), device=dev)
+        tot = torch.zeros((), dtype=torch.float64, device=dev)
+        cnt = 0
         for i in range(0, S - nb + 1, nb):
-            idx = torch.tensor(perm[i:i + nb], device=dev)
+            idx = perm[i:i + nb]
             fb.copy_(f.index_select(0, idx))
             tb.copy_(target.index_select(0, idx))
+            if rb is not None:
+                rb.copy_(ref.index_select(0, idx))
             eng.forward(fb, src, ones, mode=0)
-            loss = eng.loss_grad(tb)
+            tot += eng.loss_grad(tb, rb, floor2).sum()
             eng.backward(fb, src, ones, eng.dout)
             eng.adam(lr)
-            tot += float(loss.sum())
             cnt += nb
-        hist.append(tot / max(cnt, 1))
+        hist.append(float(tot) / max(cnt, 1))
         if log:
             log('epoch %d loss %.4e' % (ep, hist[-1]))
     return hist
 
 
 class LearnedClosure:
-    """Q += Q_Shakov + R_net per node, with the spectral operator as fallback.
+    """Q = Q_Shakov + R_net + C per node, with the spectral operator as judge
+    and fallback.
+
+    C is each node's correction at its last spectral evaluation,
+    C = Q_spectral - (Q_Shakov + R_net) (zero for spectral nodes); it is set for
+    every node on the first call and refreshed, round robin, for a fraction
+    ``sample`` of the learned nodes every ``check_every`` calls.  Before a
+    node's correction is refreshed its prediction is judged against the
+    spectral result: relative L2 error above ``tol`` (relative to
+    max(|Q|, the network's |Q| floor)) hands the node to the spectral operator
+    for the rest of the run.  ``use_net=False`` keeps the same schedule without
+    the network (Q_Shakov + C), the method the network is measured against.
+    The first ``start`` calls use the spectral operator for every node (the
+    initial transient, where the residual changes fastest).
 
     nodes: velocity slices s with tasks (s -> s) of a nodal basis.  shakov:
     relax.Shakov bound to the grid; spectral: collision.CollisionPlan;
     prefac: the spectral operator's prefactor (scales R)."""
 
     def __init__(self, net, G, nslice, shakov, spectral, prefac, cw, device, check_every=10,
-                 sample=0.25, tol=0.05, seed=0):
+                 sample=0.25, tol=0.05, seed=0, use_net=True, start=0):
         import torch
         from .collision import task_table
         self.torch, self.net, self.G, self.ns = torch, net, int(G), int(nslice)
         self.shakov, self.spectral, self.prefac, self.cw = shakov, spectral, float(prefac), cw
         self.device, self.check_every, self.sample, self.tol = device, check_every, sample, tol
-        self.rng = np.random.default_rng(seed)
+        self.use_net = use_net
+        self.start = int(start)              # calls served by the spectral operator first
         self.task_table = task_table
         self.learned = np.ones(self.ns, bool)
         self.calls = 0
-        self.checks = []                     # (call, node errors) history
+        self.cursor = 0
+        self.checks = []                     # (call, {node: error}) history
         self._eng = {}
+        self.C = None
+        self._tabs = {}
         self._rebuild()
 
+    def _tab(self, nodes, dtype):
+        key = (tuple(int(s) for s in nodes), dtype)
+        if key not in self._tabs:
+            self._tabs[key] = self.task_table(
+                1, self.ns, [(int(s), [(int(s), 1.0)], [(int(s), 1.0)]) for s in nodes], dtype,
+                self.device)
+        return self._tabs[key]
+
     def _rebuild(self):
-        t, dev = self.torch, self.device
         idx = np.nonzero(self.learned)[0].astype(np.int32)
         spec = np.nonzero(~self.learned)[0].astype(np.int32)
-        self.idx = t.tensor(idx, device=dev)
-        tasks = [(int(s), [(int(s), 1.0)], [(int(s), 1.0)]) for s in idx]
-        self.sh_tab = self.task_table(1, self.ns, tasks, self.shakov.tdtype, dev) if len(idx) else None
-        stasks = [(int(s), [(int(s), 1.0)], [(int(s), 1.0)]) for s in spec]
-        self.sp_tab = self.task_table(1, self.ns, stasks, self.spectral.tdtype, dev) \
-            if len(spec) else None
+        self.idx_np = idx
+        self.idx = self.torch.tensor(idx, device=self.device)
+        self.sh_tab = self._tab(idx, self.shakov.tdtype) if len(idx) else None
+        self.sp_tab = self._tab(spec, self.spectral.tdtype) if len(spec) else None
         self.nl = len(idx)
 
     def _engine(self, nb):
@@ -236,56 +274,71 @@ class LearnedClosure:
         return self._eng[nb]
 
     def _scales(self, u, idx):
-        """1/rho and prefac rho^2 per listed slice (rho = cw sum f)."""
+        """1/rho and out_scale prefac rho^2 per listed slice (rho = cw sum f)."""
         n = self.G ** 3
         rho = u.view(self.ns, n).index_select(0, idx).double().sum(1) * self.cw
-        return (1.0 / rho).float(), (self.prefac * rho * rho).float()
+        osc = self.net.meta['out_scale'] * self.prefac * rho * rho
+        return (1.0 / rho).float(), osc.float()
 
-    def learned_into(self, u, Q, idx):
-        if len(idx) == 0:
-            return
-        t = self.torch
-        nb = len(idx)
-        tab = self.task_table(1, self.ns, [(int(s), [(int(s), 1.0)], [(int(s), 1.0)])
-                                           for s in idx.cpu().numpy()], self.shakov.tdtype,
-                              self.device)
+    def _model(self, u, Q, nodes, idx, tab):
+        """Q += Q_Shakov + R_net on the listed nodes (no correction)."""
         self.shakov(u, Q, tasks=tab)
-        sc, osc = self._scales(u, idx)
-        self._engine(nb).forward(u, idx, sc, mode=1, dst=idx, oscale=osc, Q=Q)
+        if self.use_net:
+            sc, osc = self._scales(u, idx)
+            self._engine(len(nodes)).forward(u, idx, sc, mode=1, dst=idx, oscale=osc, Q=Q)
+
+    def _spectral(self, u, Q, nodes):
+        self.spectral(u, Q, tasks=self._tab(nodes, self.spectral.tdtype))
 
     def __call__(self, u, Q):
         """Q += collision term of every slice of u (shape (nslice * G^3,))."""
+        t = self.torch
         self.calls += 1
+        if self.calls <= self.start:
+            self._spectral(u, Q, np.arange(self.ns, dtype=np.int32))
+            return
+        if self.C is None:                   # first call: corrections of every node
+            self.C = t.zeros_like(u)
+            allnodes = np.arange(self.ns, dtype=np.int32)
+            qs = t.zeros_like(u)
+            self._spectral(u, qs, allnodes)
+            self._model(u, self.C, allnodes, t.tensor(allnodes, device=self.device),
+                        self._tab(allnodes, self.shakov.tdtype))
+            self.C.neg_().add_(qs)
         if self.nl:
-            self.shakov(u, Q, tasks=self.sh_tab)
-            sc, osc = self._scales(u, self.idx)
-            self._engine(self.nl).forward(u, self.idx, sc, mode=1, dst=self.idx, oscale=osc,
-                                          Q=Q)
+            self._model(u, Q, self.idx_np, self.idx, self.sh_tab)
+        Q.add_(self.C)
         if self.sp_tab is not None:
             self.spectral(u, Q, tasks=self.sp_tab)
-        if self.check_every and self.calls % self.check_every == 0 and self.nl:
+        if self.check_every and (self.calls - self.start) % self.check_every == 0 and self.nl:
             self.check(u)
 
     def check(self, u):
-        """Spectral vs learned on a sample of learned nodes; switch failures."""
+        """Judge and refresh a round-robin sample of the learned nodes."""
         t = self.torch
-        cand = np.nonzero(self.learned)[0]
-        k = max(1, int(round(self.sample * len(cand))))
-        pick = np.sort(self.rng.choice(cand, size=min(k, len(cand)), replace=False))
-        idx = t.tensor(pick.astype(np.int32), device=self.device)
+        cand = self.idx_np
+        k = min(len(cand), max(1, int(round(self.sample * len(cand)))))
+        pick = np.sort(cand[(self.cursor + np.arange(k)) % len(cand)])
+        self.cursor = (self.cursor + k) % len(cand)
+        idx = t.tensor(pick, device=self.device)
         n = self.G ** 3
         ql = t.zeros_like(u)
-        self.learned_into(u, ql, idx)
+        self._model(u, ql, pick, idx, self._tab(pick, self.shakov.tdtype))
         qs = t.zeros_like(u)
-        tab = self.task_table(1, self.ns, [(int(s), [(int(s), 1.0)], [(int(s), 1.0)])
-                                           for s in pick], self.spectral.tdtype, self.device)
-        self.spectral(u, qs, tasks=tab)
+        self._spectral(u, qs, pick)
+        Cv = self.C.view(self.ns, n)
         a = ql.view(self.ns, n).index_select(0, idx).double()
         b = qs.view(self.ns, n).index_select(0, idx).double()
-        err = ((a - b).norm(dim=1) / b.norm(dim=1).clamp_min(1e-300)).cpu().numpy()
+        c = Cv.index_select(0, idx).double()
+        rho = u.view(self.ns, n).index_select(0, idx).double().sum(1) * self.cw
+        floor = self.net.meta['qfloor'] * self.net.meta['out_scale'] * self.prefac * rho * rho
+        den = t.maximum(b.norm(dim=1), floor).clamp_min(1e-300)
+        err = ((a + c - b).norm(dim=1) / den).cpu().numpy()
         self.checks.append((self.calls, dict(zip(pick.tolist(), err.tolist()))))
+        Cv.index_copy_(0, idx.long(), (b - a).to(self.C.dtype))  # refreshed corrections
         bad = pick[err > self.tol]
         if len(bad):
+            Cv.index_fill_(0, t.tensor(bad, device=self.device).long(), 0.0)
             self.learned[bad] = False
             self._rebuild()
         return err

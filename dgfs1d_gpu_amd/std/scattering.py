@@ -143,7 +143,8 @@ class DGFSVHSGLLScatteringModelStd(DGFSScatteringModelStd):
         if mode != 'learned':
             raise ValueError("scattering-model::closure must be 'none' or 'learned'")
         for o, f, g in tasks:
-            if len(f) != 1 or f[0][0] != o or sorted(f) != sorted(g):
+            if len(f) != 1 or len(g) != 1 or f[0][0] != o or g[0][0] != o or \
+                    abs(f[0][1] * g[0][1] - 1.0) > 1e-12:
                 raise ValueError('the learned closure needs a nodal basis (slice -> slice tasks)')
         from ..learned import ClosureNet, LearnedClosure
         from ..relax import Shakov
@@ -163,7 +164,10 @@ class DGFSVHSGLLScatteringModelStd(DGFSScatteringModelStd):
             net, vm.Nv(), K * Ne, sk, self.plan, self._prefactor, vm.cw(), self._device,
             check_every=int(cfg.lookupordefault('scattering-model', 'closure-check-every', 10)),
             sample=float(cfg.lookupordefault('scattering-model', 'closure-sample', 0.25)),
-            tol=float(cfg.lookupordefault('scattering-model', 'closure-tol', 0.05)))
+            tol=float(cfg.lookupordefault('scattering-model', 'closure-tol', 0.05)),
+            use_net=cfg.lookupordefault('scattering-model', 'closure-use-net', 'true').lower()
+            in ('1', 'true', 'yes'),
+            start=int(cfg.lookupordefault('scattering-model', 'closure-start', 0)))
 
     def _collide(self, u, Q, stream=None):
         if self.closure is not None:

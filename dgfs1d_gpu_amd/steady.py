@@ -3,7 +3,7 @@
     python -m dgfs1d_gpu_amd.steady --weights w.npz [--mach 2.0] [--tau 1e-5]
 
 Runs the hard-sphere normal shock (closure_train.shock_deck: Ne=32, K=3,
-N=32^3, M=12, N_rho=16, FP32, forward Euler dt=0.002) from the same initial
+N=32^3, M=12, N_rho=16, FP32, SSP-RK2 dt=5e-4) from the same initial
 profile twice -- spectral collision, then Q_Shakov + network with the
 spectral fallback -- until the reference residual
 sqrt(sum (f^{n+1} - f^n)^2 / sum (f^n)^2) (dgfs1D/std/residual.py:21-29),
@@ -64,6 +64,9 @@ def main(argv=None):
     ap.add_argument('--nres', type=int, default=10)
     ap.add_argument('--tol', type=float, default=0.05)
     ap.add_argument('--check-every', type=int, default=10)
+    ap.add_argument('--sample', type=float, default=0.25)
+    ap.add_argument('--start', type=int, default=400,
+                    help='RHS calls on the spectral operator before the closure takes over')
     ap.add_argument('--out', default=None)
     args = ap.parse_args(argv)
     import torch
@@ -74,17 +77,26 @@ def main(argv=None):
     base = shock_deck(mach=args.mach)
     _, spec, m_spec = run_to_steady(base, taus, args.max_steps, args.nres, dev, log)
     log('spectral: ' + json.dumps(spec))
-    extra = ('closure = learned\nclosure-weights = %s\nclosure-check-every = %d\n'
-             'closure-tol = %g\n' % (args.weights, args.check_every, args.tol))
-    learned_text = shock_deck(mach=args.mach, extra=extra)
-    s, lrn, m_lrn = run_to_steady(learned_text, taus, args.max_steps, args.nres, dev, log)
-    cl = s.sm.closure
-    lrn['nodes_learned_at_end'] = int(cl.learned.sum())
-    lrn['nodes_total'] = int(cl.ns)
-    lrn['checks'] = len(cl.checks)
-    errs = [e for _, d in cl.checks for e in d.values()]
-    lrn['check_err_median'] = float(np.median(errs)) if errs else None
-    log('learned: ' + json.dumps(lrn))
+    runs = {}
+    for tag, use_net in (('learned', True), ('shakov_corrected', False)):
+        extra = ('closure = learned\nclosure-weights = %s\nclosure-check-every = %d\n'
+                 'closure-tol = %g\nclosure-sample = %g\nclosure-use-net = %s\n'
+                 'closure-start = %d\n'
+                 % (args.weights, args.check_every, args.tol, args.sample,
+                    'true' if use_net else 'false', args.start))
+        s, rep, m = run_to_steady(shock_deck(mach=args.mach, extra=extra), taus, args.max_steps,
+                                  args.nres, dev, log)
+        cl = s.sm.closure
+        rep['nodes_learned_at_end'] = int(cl.learned.sum())
+        rep['nodes_total'] = int(cl.ns)
+        rep['checks'] = len(cl.checks)
+        errs = np.array([e for _, d in cl.checks for e in d.values()])
+        rep['check_err_median'] = float(np.median(errs)) if len(errs) else None
+        rep['check_err_p90'] = float(np.percentile(errs, 90)) if len(errs) else None
+        log('%s: %s' % (tag, json.dumps(rep)))
+        runs[tag] = (rep, m)
+        del s
+    lrn, m_lrn = runs['learned']
     names = ('rho', 'ux', 'uy', 'T', 'qx', 'qy', 'Pxx', 'Pyy', 'Pxy')
     dm = {nm: float(np.abs(m_lrn[:, i] - m_spec[:, i]).max() /
                     max(np.abs(m_spec[:, i]).max(), 1e-300)) for i, nm in enumerate(names)}
@@ -93,6 +105,7 @@ def main(argv=None):
         if k in spec['reached']:
             speed[k] = spec['reached'][k]['wall_s'] / lrn['reached'][k]['wall_s']
     out = dict(mach=args.mach, spectral=spec, learned=lrn,
+               shakov_corrected_without_network=runs['shakov_corrected'][0],
                moment_rel_diff_learned_vs_spectral=dm, speedup_to_tau=speed,
                per_step_speedup=spec['ms_per_step'] / lrn['ms_per_step'])
     print(json.dumps(out))

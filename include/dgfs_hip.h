@@ -315,8 +315,9 @@ int dgfs_imex_ars_update(int dtype, int nslice, int Nv, const dgfs_ars_t* args, 
  *   respect to every weight and bias (FP32, overwritten); gact = 2 * nb*G^3*32
  *   bf16, fin = nb*G^3 FP32, work = dgfs_closure_work_bytes(G, nb, chunk);
  *   chunk = voxels per partial weight-gradient block (fixed-order reduction).
- * dgfs_closure_loss: loss[b] = |pred_b - target_b|^2 / |target_b|^2 (FP64) and
- *   dOut = d(mean_b loss_b)/d(pred); work = dgfs_closure_loss_work_bytes.
+ * dgfs_closure_loss: loss[b] = |pred_b - target_b|^2 / max(|ref_b|^2, floor2) (FP64;
+ *   ref = NULL means ref = target) and dOut = d(mean_b loss_b)/d(pred);
+ *   work = dgfs_closure_loss_work_bytes.
  * dgfs_adam: w -= lr_t m / (sqrt(v) + eps) after the moment updates. */
 typedef struct {
     const float* W1; const float* b1;
@@ -338,8 +339,8 @@ int dgfs_closure_backward(const dgfs_closure_t* net, const dgfs_closure_grad_t* 
 int dgfs_closure_pack(const float* W2, const float* W3, void* W2f, void* W2t, void* W3f,
                       void* W3t, hipStream_t stream);
 size_t dgfs_closure_loss_work_bytes(int G, int nb);
-int dgfs_closure_loss(int G, int nb, const float* pred, const float* target, float* dOut,
-                      double* loss, double* work, hipStream_t stream);
+int dgfs_closure_loss(int G, int nb, const float* pred, const float* target, const float* ref,
+                      double floor2, float* dOut, double* loss, double* work, hipStream_t stream);
 int dgfs_adam(int n, float* w, const float* g, float* m, float* v, double lr_t, double beta1,
               double beta2, double eps, hipStream_t stream);
 

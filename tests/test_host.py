@@ -109,3 +109,33 @@ def test_dictionary_set_precision():
     assert d.dtype == np.float32
     d.set('config', 'precision', 'double')
     assert d.dtype == np.float64 and d.dtypename == 'double'
+
+
+def test_batch_problem_generator():
+    from dgfs1d_gpu_amd.batch import make_problems, problem_deck
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    a, b = make_problems(7), make_problems(7)
+    assert a == b and len(a) == 64
+    for p in a:
+        assert p['Ne'] in (2, 4, 8) and p['K'] in (3, 5, 7) and p['N'] in (16, 24, 32)
+        assert 0.1 <= abs(p['uw']) <= 1.0 and 0.8 <= p['Tl'] <= 1.5 and 0.8 <= p['Tr'] <= 1.5
+        if p['kind'] == 'couette':
+            assert p['Tl'] == p['Tr']
+    d = Dictionary(problem_deck(**{k: v for k, v in a[0].items()}))
+    assert d.lookupint('velocity-mesh', 'Nrho') == a[0]['N'] // 2
+    assert {p['kind'] for p in a} == {'couette', 'fourier'}
+
+
+def test_rankine_hugoniot_limits():
+    from dgfs1d_gpu_amd.closure_train import rankine_hugoniot
+    r, u, T = rankine_hugoniot(1.0)
+    assert abs(r - 1) < 1e-14 and abs(u - 1) < 1e-14 and abs(T - 1) < 1e-14
+    r, u, T = rankine_hugoniot(1e4)                    # strong-shock limit (g+1)/(g-1) = 4
+    assert abs(r - 4.0) < 1e-6
+    # mass flux and momentum conservation across the shock for M = 2
+    g = 5.0 / 3.0
+    r, u, T = rankine_hugoniot(2.0)
+    p = r * T                                          # p2/p1
+    assert abs(r * u - 1.0) < 1e-14
+    m2 = 4.0
+    assert abs((1 + g * m2) - (p + g * m2 * r * u * u)) < 1e-12
