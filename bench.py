@@ -149,18 +149,24 @@ def closure_forward(nodes, G, reps, dev):
     u = torch.tensor(f.reshape(-1), dtype=torch.float32, device=dev)
     src = torch.arange(nodes, dtype=torch.int32, device=dev)
     sc = torch.ones(nodes, dtype=torch.float32, device=dev)
-    for _ in range(3):
-        eng.forward(u, src, sc, mode=0)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        eng.forward(u, src, sc, mode=0)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
+    Q = torch.zeros_like(u)
+    osc = torch.ones(nodes, dtype=torch.float32, device=dev)
     flop = 2.0 * G ** 3 * 27 * (32 + 32 * 32 + 32 * 32 + 32) * nodes
-    return {'nodes': nodes, 'G': G, 'ms_per_call': ms, 'us_per_node': ms / nodes * 1e3,
-            'tflops': flop / (ms * 1e-3) / 1e12, 'peak_bf16_dense_tflops': 2500.0}
+    out = {'nodes': nodes, 'G': G, 'peak_bf16_dense_tflops': 2500.0}
+    # mode 0: training forward (stores every activation); 1: inference (Q += out)
+    for tag, kw in (('train', dict(mode=0)), ('infer', dict(mode=1, dst=src, oscale=osc, Q=Q))):
+        for _ in range(3):
+            eng.forward(u, src, sc, **kw)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            eng.forward(u, src, sc, **kw)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[tag] = {'ms_per_call': ms, 'us_per_node': ms / nodes * 1e3,
+                    'tflops': flop / (ms * 1e-3) / 1e12}
+    return out
 
 
 def main(argv=None):

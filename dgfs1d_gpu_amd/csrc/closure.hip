@@ -11,14 +11,16 @@
 //   W1 [27][32], W2, W3 [27][32 out][32 in], W4 [27][32], biases [32]/[1]
 // with tap = (dz+1)*9 + (dy+1)*3 + (dx+1) and out[v] = sum W[tap] in[v + d(tap)].
 //
-// The two 32 -> 32 layers (96 % of the arithmetic) run on the bf16 matrix
-// cores: v_mfma_f32_16x16x32_bf16 with A = weights (16 out channels x 32 in
-// channels, from LDS) and B = activations (32 in channels x 16 voxels along
-// x, one 16 B global load per lane), FP32 accumulation.  The same kernel
-// computes the input gradient of those layers with tap-flipped, transposed
-// weights.  The thin first/last layers and the weight gradients run on FP32
-// VALU with FP32 accumulation; partial weight gradients are reduced in a
-// fixed order (deterministic).
+// Forward: all four layers run on the bf16 matrix cores
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation) in three z-streaming kernels:
+// layers 1 + 2 fused (layer 1's K dimension is the 27 taps of the bf16-rounded
+// scaled input; its output never leaves LDS), layer 3, and layer 4 (the taps
+// as MFMA rows, then a shifted 27-term sum).  Weights are bf16 on the device
+// (W2/W3 packed copies; W1/W4 rounded from the FP32 masters in-kernel).
+// Backward: the 32 -> 32 input gradients reuse the layer-3 kernel with
+// tap-flipped, transposed weights; the thin layers' gradients and all weight
+// gradients run on FP32 VALU with FP32 accumulation, partial weight gradients
+// reduced in a fixed order (deterministic).
 #include "dgfs_common.h"
 #include "dgfs_hip.h"
 
@@ -30,11 +32,20 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 
 constexpr int kC = 32;          // hidden channels
 constexpr int kTaps = 27;
-constexpr int kWP = 40;         // LDS row pitch (bf16) of a weight tile row
+// LDS voxel pitch (bf16) of the activation planes: 96 B.  ds_read_b128 of 16
+// consecutive voxels (one k-group of a B fragment) is free of bank conflicts
+// at this pitch on gfx950, from any starting voxel; 64 B and 80 B pitches
+// conflict (measured with the SQ_LDS_BANK_CONFLICT counter).
+constexpr int kVP = 48;
 
-DGFS_DEV float silu(float z) { return z / (1.0f + __expf(-z)); }
+// sigmoid from the hardware 2^x and reciprocal (v_exp_f32, v_rcp_f32, about
+// 1 ulp each; the results are rounded to bf16 right after)
+DGFS_DEV float sigm(float z) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
+DGFS_DEV float silu(float z) { return z * sigm(z); }
 DGFS_DEV float silu_d(float z) {
-    const float s = 1.0f / (1.0f + __expf(-z));
+    const float s = sigm(z);
     return s * (1.0f + z * (1.0f - s));
 }
 DGFS_DEV float bf2f(__bf16 h) { return float(h); }
@@ -46,259 +57,552 @@ DGFS_DEV void tap_offset(int tap, int& dz, int& dy, int& dx) {
     dx = tap % 3 - 1;
 }
 
-// ------------------------------------------------- 32 -> 32 layer, MFMA
-// Work-group = 4 waves over 8 y-rows of one (b, z) plane; wave w owns rows
-// y0 + 2w, y0 + 2w + 1 and every 16-voxel x tile of them.
-// MODE 0: z = acc + bias -> store Z (bf16) and Y = silu(z) (bf16)
-// MODE 1: z = acc + bias -> store Y = z (bf16)
-// MODE 2: (input gradient) store Y = acc * silu'(Zp)  (Zp: pre-activation of
-//         the layer below, bf16), no bias
-template <int G, int MODE>
-__global__ void __launch_bounds__(256)
-k_conv_mid(int nb, const __bf16* __restrict__ X, const __bf16* __restrict__ W /*[27][32][32]*/,
-           const float* __restrict__ bias, const __bf16* __restrict__ Zp,
-           __bf16* __restrict__ Zout, __bf16* __restrict__ Y) {
-    constexpr int NXT = G / 16;             // x tiles per row
-    constexpr int NT = 2 * NXT;             // voxel tiles per wave
-    constexpr int YB = G / 8;               // y blocks per plane
-    __shared__ __attribute__((aligned(16))) __bf16 Ws[kTaps * kC * kWP];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int bid = blockIdx.x;
-    const int yb = bid % YB, z = (bid / YB) % G, b = bid / (YB * G);
-    // stage the weights (27*32 rows of 32 bf16 = 4 x 16 B chunks each)
-    for (int r = tid; r < kTaps * kC * 4; r += 256) {
-        const int row = r >> 2, ch = r & 3;
-        *reinterpret_cast<uint4*>(&Ws[row * kWP + ch * 8]) =
-            *reinterpret_cast<const uint4*>(&W[row * kC + ch * 8]);
-    }
-    __syncthreads();
-    if (b >= nb) return;
-    const int kq = lane >> 4, li = lane & 15;
-    floatx4 acc[NT][2];
-    for (int t = 0; t < NT; ++t)
-        for (int j = 0; j < 2; ++j) acc[t][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const size_t plane = size_t(G) * G;
-    const __bf16* Xb = X + size_t(b) * G * plane * kC;
-    for (int tap = 0; tap < kTaps; ++tap) {
-        int dz, dy, dx;
-        tap_offset(tap, dz, dy, dx);
-        const int zz = z + dz;
-        bf16x8 a[2];
-        for (int j = 0; j < 2; ++j)
-            a[j] = *reinterpret_cast<const bf16x8*>(
-                &Ws[(tap * kC + 16 * j + li) * kWP + 8 * kq]);
-        for (int t = 0; t < NT; ++t) {
-            const int y = yb * 8 + 2 * wave + t / NXT;
-            const int x = (t % NXT) * 16 + li + dx;
-            const int yy = y + dy;
-            bf16x8 bv;
-            if (zz >= 0 && zz < G && yy >= 0 && yy < G && x >= 0 && x < G) {
-                bv = *reinterpret_cast<const bf16x8*>(
-                    &Xb[((size_t(zz) * G + yy) * G + x) * kC + 8 * kq]);
-            } else {
-                for (int i = 0; i < 8; ++i) bv[i] = __bf16(0.0f);
-            }
-            for (int j = 0; j < 2; ++j)
-                acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[j], bv, acc[t][j], 0, 0, 0);
-        }
-    }
-    // epilogue: lane holds out channels 16j + 4kq + i of voxel (tile base + li)
-    for (int t = 0; t < NT; ++t) {
-        const int y = yb * 8 + 2 * wave + t / NXT;
-        const int x = (t % NXT) * 16 + li;
-        const size_t vo = ((size_t(b) * G + z) * plane + size_t(y) * G + x) * kC;
-        for (int j = 0; j < 2; ++j) {
-            const int c0 = 16 * j + 4 * kq;
-            __bf16 yo[4], zo[4];
-            for (int i = 0; i < 4; ++i) {
-                const float v = acc[t][j][i];
-                if (MODE == 0) {
-                    const float zv = v + bias[c0 + i];
-                    zo[i] = f2bf(zv);
-                    yo[i] = f2bf(silu(zv));
-                } else if (MODE == 1) {
-                    yo[i] = f2bf(v + bias[c0 + i]);
-                } else {
-                    yo[i] = f2bf(v * silu_d(bf2f(Zp[vo + c0 + i])));
-                }
-            }
-            if (MODE == 0) {
-                uint2 pz;
-                __builtin_memcpy(&pz, zo, 8);
-                *reinterpret_cast<uint2*>(&Zout[vo + c0]) = pz;
-            }
-            uint2 py;
-            __builtin_memcpy(&py, yo, 8);
-            *reinterpret_cast<uint2*>(&Y[vo + c0]) = py;
-        }
-    }
+// ================================================= z-streaming tile kernels
+// A task is RB rows (y) x all G columns (x) of ZC consecutive planes (z) of
+// one slice.  A work-group marches through the planes of its task keeping a
+// ring of three zero-padded input planes ((RB + 2) x (G + 2) voxels) in LDS:
+// each input plane is fetched from HBM once per task (plus one halo plane at
+// each end of the chunk and one halo row on each side), and the fetch of the
+// next plane is issued before the matrix-core work of the current one so that
+// its latency is hidden.  Its 4 waves own RB/4 rows each; a wave's MFMA tiles
+// are 16 consecutive x of one row, and the B fragments of each input row it
+// reads serve all three dy taps of the rows it owns (voxel pitch kVP).  The
+// 32 -> 32
+// weights stay in registers (27 taps x 2 A fragments = 216 VGPRs per lane,
+// one wave per SIMD), so LDS serves only the activations; work-groups are
+// persistent (one per CU) and load the weights once.
+template <int G_, int RB_>
+struct Zs {
+    static constexpr int G = G_;
+    static constexpr int RB = RB_;                        // rows per task
+    static constexpr int RW = RB / 4;                     // rows per wave
+    static constexpr int NXT = G / 16;                    // x tiles per row
+    static constexpr int NT = RW * NXT;                   // output tiles per wave
+    static constexpr int ZC = 8;                          // planes per task
+    static constexpr int HR = RB + 2, HX = G + 2;         // halo plane rows, columns
+    static constexpr int PS = HR * HX * kVP;              // halo plane, bf16 elements
+    static constexpr int NCH = HR * HX * 4;               // its 16 B chunks
+    static constexpr int NPF = (NCH + 255) / 256;         // chunks per thread
+    static constexpr int YB = G / RB, ZB = G / ZC;
+    static constexpr int NBLK = YB * ZB;                  // tasks per slice
+    static_assert(G % 16 == 0 && G % ZC == 0 && G % RB == 0 && RB % 4 == 0, "tile shape");
+};
+// 32 -> 32 layers (and layer 1 fused in front of layer 2)
+template <int G>
+using ZsMid = Zs<G, (G == 48 ? 4 : 8)>;
+// layer 4: one output voxel per thread
+template <int G>
+using ZsOut = Zs<G, (G == 48 ? 4 : 256 / G)>;
+
+struct ZsTile {
+    int b, y0, z0;
+};
+
+template <typename S>
+DGFS_DEV ZsTile zs_decode(int task) {
+    ZsTile t;
+    t.b = task / S::NBLK;
+    const int r = task - t.b * S::NBLK;
+    t.y0 = (r % S::YB) * S::RB;
+    t.z0 = (r / S::YB) * S::ZC;
+    return t;
 }
 
-// Same layer with the input tile staged in LDS: the work-group's 3 x 10 x
-// (G + 2) halo of voxels (zero padded) is loaded once, so each activation is
-// read from HBM/L2 once per work-group instead of once per tap.  Voxel pitch
-// 40 bf16 (80 B) keeps the 16 lanes of a k-group on disjoint banks.
-// LDS: 27*32*40*2 + 3*10*(G+2)*40*2 bytes (150.7 KiB at G = 32).
-template <int G, int MODE>
-__global__ void __launch_bounds__(256)
-k_conv_mid_lds(int nb, const __bf16* __restrict__ X, const __bf16* __restrict__ W,
-               const float* __restrict__ bias, const __bf16* __restrict__ Zp,
-               __bf16* __restrict__ Zout, __bf16* __restrict__ Y) {
-    constexpr int NXT = G / 16, NT = 2 * NXT, YB = G / 8;
-    constexpr int XP = G + 2, XR = 10, VP = kWP;          // halo row length, rows, voxel pitch
-    __shared__ __attribute__((aligned(16))) __bf16 Ws[kTaps * kC * kWP];
-    __shared__ __attribute__((aligned(16))) __bf16 Xs[3 * XR * XP * VP];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int bid = blockIdx.x;
-    const int yb = bid % YB, z = (bid / YB) % G, b = bid / (YB * G);
-    for (int r = tid; r < kTaps * kC * 4; r += 256) {
-        const int row = r >> 2, ch = r & 3;
-        *reinterpret_cast<uint4*>(&Ws[row * kWP + ch * 8]) =
-            *reinterpret_cast<const uint4*>(&W[row * kC + ch * 8]);
-    }
-    const size_t plane = size_t(G) * G;
-    const __bf16* Xb = X + size_t(b) * G * plane * kC;
-    for (int r = tid; r < 3 * XR * XP * 4; r += 256) {
-        const int ch = r & 3, vox = r >> 2;
-        const int xi = vox % XP, yi = (vox / XP) % XR, zi = vox / (XP * XR);
-        const int zz = z + zi - 1, yy = yb * 8 + yi - 1, xx = xi - 1;
+// one task per work-group
+template <typename S>
+DGFS_DEV ZsTile zs_tile(int nb) {
+    return zs_decode<S>(xcd_remap(blockIdx.x, nb * S::NBLK));
+}
+
+// persistent work-groups: round r runs task r * gridDim.x + (XCD-remapped
+// block id), so the tasks in flight on one XCD are neighbours in the slice
+DGFS_DEV int zs_task(int round) { return round * gridDim.x + xcd_remap(blockIdx.x, gridDim.x); }
+
+// per-lane A fragments of [27][32 out][32 in] bf16 weights:
+// wf[tap][j] = W[tap][16j + li][8kq .. 8kq + 7]
+DGFS_DEV void load_wfrag(bf16x8 (&wf)[kTaps][2], const __bf16* __restrict__ W, int li, int kq) {
+#pragma unroll
+    for (int tap = 0; tap < kTaps; ++tap)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            wf[tap][j] =
+                *reinterpret_cast<const bf16x8*>(&W[(tap * kC + 16 * j + li) * kC + 8 * kq]);
+}
+
+// halo plane p (rows y0-1 .. y0+RB, columns -1 .. G) of a 32-channel slice
+template <typename S>
+DGFS_DEV void plane_fetch(uint4 (&pf)[S::NPF], const __bf16* __restrict__ Xb, int p, int y0) {
+    constexpr int G = S::G;
+#pragma unroll
+    for (int i = 0; i < S::NPF; ++i) {
+        const int k = threadIdx.x + 256 * i;
+        const int ch = k & 3, vox = k >> 2;
+        const int xi = vox % S::HX, yi = vox / S::HX;
+        const int y = y0 + yi - 1, x = xi - 1;
         uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (zz >= 0 && zz < G && yy >= 0 && yy < G && xx >= 0 && xx < G)
-            v = *reinterpret_cast<const uint4*>(&Xb[((size_t(zz) * G + yy) * G + xx) * kC + ch * 8]);
-        *reinterpret_cast<uint4*>(&Xs[vox * VP + ch * 8]) = v;
+        if (k < S::NCH && p >= 0 && p < G && y >= 0 && y < G && x >= 0 && x < G)
+            v = *reinterpret_cast<const uint4*>(&Xb[((size_t(p) * G + y) * G + x) * kC + ch * 8]);
+        pf[i] = v;
     }
-    __syncthreads();
-    if (b >= nb) return;
-    const int kq = lane >> 4, li = lane & 15;
-    floatx4 acc[NT][2];
-    for (int t = 0; t < NT; ++t)
+}
+
+template <typename S>
+DGFS_DEV void plane_store(__bf16* slot, const uint4 (&pf)[S::NPF]) {
+#pragma unroll
+    for (int i = 0; i < S::NPF; ++i) {
+        const int k = threadIdx.x + 256 * i;
+        if (k < S::NCH) *reinterpret_cast<uint4*>(&slot[(k >> 2) * kVP + (k & 3) * 8]) = pf[i];
+    }
+}
+
+// 32 -> 32 convolution of output plane z from the ring (slots p mod 3), A
+// fragments from registers.  Nine steps (dz, dx); a step reads the B
+// fragments of the RW + 2 input rows under the wave's RW output rows once and
+// applies each to the (output row, dy) pairs it feeds.  Software-pipelined:
+// the fragments of step k + 1 are read from LDS while the MFMAs of step k run.
+// epi(step) is called in the region of step's MFMAs: the previous plane's
+// epilogue (VALU and stores) issues between them instead of after them.
+// (Deriving the dx = +-1 fragments from the dx = 0 ones with DPP row shifts
+// reads LDS three times less but measured slower: the shifts cost more than
+// the reads they save.)
+template <typename S, typename Epi>
+DGFS_DEV void mid_plane(floatx4 (&acc)[S::NT][2], const bf16x8 (&wf)[kTaps][2],
+                        const __bf16* ring, int z, int wave, int li, int kq, Epi&& epi) {
+    constexpr int NR = S::RW + 2;                          // input rows of a wave
+    constexpr int NB = NR * S::NXT;                        // B fragments per step
+    const __bf16* slot[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) slot[d] = ring + ((z + d + 2) % 3) * S::PS;   // planes z-1, z, z+1
+    const int base = (wave * S::RW * S::HX + li) * kVP + 8 * kq;
+    auto load = [&](int step, bf16x8(&b)[NB]) {
+        const int dz = step / 3, dx = step % 3;
+#pragma unroll
+        for (int ir = 0; ir < NR; ++ir)
+#pragma unroll
+            for (int xt = 0; xt < S::NXT; ++xt)
+                b[ir * S::NXT + xt] = *reinterpret_cast<const bf16x8*>(
+                    slot[dz] + base + (ir * S::HX + xt * 16 + dx) * kVP);
+    };
+    auto mma = [&](int step, const bf16x8(&b)[NB]) {
+        const int dz = step / 3, dx = step % 3;
+        // dy outermost: successive updates of one accumulator are RW*NXT*2 MFMAs apart
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int r = 0; r < S::RW; ++r)
+#pragma unroll
+                for (int xt = 0; xt < S::NXT; ++xt)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        const int tap = dz * 9 + dy * 3 + dx;
+                        floatx4& c = acc[r * S::NXT + xt][j];
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            wf[tap][j], b[(r + dy) * S::NXT + xt], c, 0, 0, 0);
+                    }
+    };
+#pragma unroll
+    for (int t = 0; t < S::NT; ++t)
+#pragma unroll
         for (int j = 0; j < 2; ++j) acc[t][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int tap = 0; tap < kTaps; ++tap) {
-        int dz, dy, dx;
-        tap_offset(tap, dz, dy, dx);
-        bf16x8 a[2];
-        for (int j = 0; j < 2; ++j)
-            a[j] = *reinterpret_cast<const bf16x8*>(
-                &Ws[(tap * kC + 16 * j + li) * kWP + 8 * kq]);
-        for (int t = 0; t < NT; ++t) {
-            const int yl = 2 * wave + t / NXT;                 // row within the block
-            const int xi = (t % NXT) * 16 + li + dx + 1;       // halo column
-            const int vox = ((dz + 1) * XR + (yl + dy + 1)) * XP + xi;
-            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&Xs[vox * VP + 8 * kq]);
-            for (int j = 0; j < 2; ++j)
-                acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[j], bv, acc[t][j], 0, 0, 0);
+    bf16x8 b0[NB], b1[NB];
+    load(0, b0);
+    sfor<0, 9>([&](auto K) {
+        constexpr int step = decltype(K)::value;
+        // the scheduling fences keep the compiler from sinking the reads of
+        // step k + 1 below the MFMAs of step k
+        if constexpr (step % 2 == 0) {
+            if constexpr (step + 1 < 9) load(step + 1, b1);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(step, b0);
+        } else {
+            if constexpr (step + 1 < 9) load(step + 1, b0);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(step, b1);
+        }
+        epi(K);
+        __builtin_amdgcn_sched_barrier(0);
+    });
+}
+
+// Epilogue of one output tile group GRP = (tile t, half j) of a finished plane.
+// MODE 0: z = acc + bias -> Z (bf16), Y = silu(z) (bf16)
+// MODE 1: Y = silu(acc + bias) only (inference)
+// MODE 2: input gradient Y = acc * silu'(Zp), no bias
+// lane holds output channels 16j + 4kq + i of voxel (tile base + li)
+template <typename S, int MODE, int GRP>
+DGFS_DEV void store_group(const floatx4 (&acc)[S::NT][2], size_t vrow0, int wave, int li, int kq,
+                          const float (&bias)[2][4], const __bf16* __restrict__ Zp,
+                          __bf16* __restrict__ Zout, __bf16* __restrict__ Y) {
+    constexpr int G = S::G, t = GRP / 2, j = GRP % 2;
+    const int yl = wave * S::RW + t / S::NXT;
+    const int x = (t % S::NXT) * 16 + li;
+    const size_t vo = (vrow0 + size_t(yl) * G + x) * kC;
+    const int c0 = 16 * j + 4 * kq;
+    __bf16 yo[4], zo[4];
+    if (MODE == 2) {
+        const uint2 zw = *reinterpret_cast<const uint2*>(&Zp[vo + c0]);
+        __bf16 zh[4];
+        __builtin_memcpy(zh, &zw, 8);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) yo[i] = f2bf(acc[t][j][i] * silu_d(bf2f(zh[i])));
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float zv = acc[t][j][i] + bias[j][i];
+            zo[i] = f2bf(zv);
+            yo[i] = f2bf(silu(zv));
         }
     }
-    for (int t = 0; t < NT; ++t) {
-        const int y = yb * 8 + 2 * wave + t / NXT;
-        const int x = (t % NXT) * 16 + li;
-        const size_t vo = ((size_t(b) * G + z) * plane + size_t(y) * G + x) * kC;
-        for (int j = 0; j < 2; ++j) {
-            const int c0 = 16 * j + 4 * kq;
-            __bf16 yo[4], zo[4];
-            for (int i = 0; i < 4; ++i) {
-                const float v = acc[t][j][i];
-                if (MODE == 0) {
-                    const float zv = v + bias[c0 + i];
-                    zo[i] = f2bf(zv);
-                    yo[i] = f2bf(silu(zv));
-                } else if (MODE == 1) {
-                    yo[i] = f2bf(v + bias[c0 + i]);
-                } else {
-                    yo[i] = f2bf(v * silu_d(bf2f(Zp[vo + c0 + i])));
+    if (MODE == 0) {
+        uint2 pz;
+        __builtin_memcpy(&pz, zo, 8);
+        *reinterpret_cast<uint2*>(&Zout[vo + c0]) = pz;
+    }
+    uint2 py;
+    __builtin_memcpy(&py, yo, 8);
+    *reinterpret_cast<uint2*>(&Y[vo + c0]) = py;
+}
+
+// The plane sequence of one task with the epilogue of plane z - 1 interleaved
+// into the MFMAs of plane z (two accumulator sets in turn).  advance(z) runs
+// before plane z's MFMAs (prefetch issue) and finish(z) after them (ring
+// update); both return nothing and see the same ring state as a plain loop.
+template <typename S, int MODE, typename Pre, typename Post>
+DGFS_DEV void mid_task(const bf16x8 (&wf)[kTaps][2], const __bf16* ring, int z0, size_t row0,
+                       int wave, int li, int kq, const float (&bias)[2][4],
+                       const __bf16* __restrict__ Zp, __bf16* __restrict__ Zout,
+                       __bf16* __restrict__ Y, Pre&& pre, Post&& post) {
+    static_assert(S::ZC % 2 == 0, "planes are processed in pairs");
+    constexpr int NG = 2 * S::NT;
+    constexpr size_t PL = size_t(S::G) * S::G;
+    floatx4 accA[S::NT][2], accB[S::NT][2];
+    auto pending = [&](const floatx4(&pa)[S::NT][2], size_t vrow0) {
+        return [&pa, vrow0, wave, li, kq, &bias, Zp, Zout, Y](auto K) {
+            constexpr int step = decltype(K)::value;
+            sfor<0, NG>([&](auto Gi) {
+                constexpr int g = decltype(Gi)::value;
+                if constexpr (g % 9 == step)
+                    store_group<S, MODE, g>(pa, vrow0, wave, li, kq, bias, Zp, Zout, Y);
+            });
+        };
+    };
+    auto none = [](auto) {};
+    pre(z0);
+    mid_plane<S>(accA, wf, ring, z0, wave, li, kq, none);
+    post(z0);
+    for (int k = 1; k < S::ZC; k += 2) {
+        const int z = z0 + k;
+        pre(z);
+        mid_plane<S>(accB, wf, ring, z, wave, li, kq, pending(accA, row0 + (z - 1) * PL));
+        post(z);
+        if (k + 1 < S::ZC) {
+            pre(z + 1);
+            mid_plane<S>(accA, wf, ring, z + 1, wave, li, kq, pending(accB, row0 + z * PL));
+            post(z + 1);
+        }
+    }
+    // the last plane (z0 + ZC - 1) is in accB
+    sfor<0, NG>([&](auto Gi) {
+        store_group<S, MODE, decltype(Gi)::value>(accB, row0 + (z0 + S::ZC - 1) * PL, wave, li,
+                                                  kq, bias, Zp, Zout, Y);
+    });
+}
+
+DGFS_DEV void load_bias(float (&bias)[2][4], const float* __restrict__ b, int kq) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) bias[j][i] = b ? b[16 * j + 4 * kq + i] : 0.f;
+}
+
+// ---------------------------------------------- 32 -> 32 layer (z-streaming)
+// Y = act(conv(X, W) + bias) over a [nb][G][G][G][32] bf16 tensor (MODE as in
+// mid_store; for MODE 2, W is the tap-flipped transposed weight set and bias
+// is unused).  Persistent: launch at most one work-group per CU.
+template <int G, int MODE>
+__global__ void __launch_bounds__(256)
+k_mid_zs(int nb, const __bf16* __restrict__ X, const __bf16* __restrict__ W,
+         const float* __restrict__ bias, const __bf16* __restrict__ Zp, __bf16* __restrict__ Zout,
+         __bf16* __restrict__ Y) {
+    using S = ZsMid<G>;
+    __shared__ __attribute__((aligned(16))) __bf16 ring[3 * S::PS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, li = lane & 15, kq = lane >> 4;
+    const size_t n = size_t(G) * G * G;
+    const int ntask = nb * S::NBLK;
+    bf16x8 wf[kTaps][2];
+    load_wfrag(wf, W, li, kq);
+    float bv[2][4];
+    load_bias(bv, MODE == 2 ? nullptr : bias, kq);
+    uint4 pf[S::NPF];
+    for (int round = 0;; ++round) {
+        const int task = zs_task(round);
+        if (task >= ntask) break;
+        const ZsTile tl = zs_decode<S>(task);
+        const __bf16* Xb = X + size_t(tl.b) * n * kC;
+        if (round > 0) __syncthreads();                    // last task's ring reads are done
+        for (int p = tl.z0 - 1; p <= tl.z0 + 1; ++p) {
+            plane_fetch<S>(pf, Xb, p, tl.y0);
+            plane_store<S>(ring + ((p + 3) % 3) * S::PS, pf);
+        }
+        __syncthreads();
+        const int zend = tl.z0 + S::ZC;
+        mid_task<S, MODE>(
+            wf, ring, tl.z0, size_t(tl.b) * n + size_t(tl.y0) * G, wave, li, kq, bv, Zp, Zout, Y,
+            [&](int z) {                                   // plane z + 2 feeds output z + 1
+                if (z + 1 < zend) plane_fetch<S>(pf, Xb, z + 2, tl.y0);
+            },
+            [&](int z) {
+                if (z + 1 < zend) {
+                    __syncthreads();                       // slot (z - 1) mod 3 is free
+                    plane_store<S>(ring + ((z + 2) % 3) * S::PS, pf);
+                    __syncthreads();
+                }
+            });
+    }
+}
+
+// ------------------------------------- layers 1 + 2 fused (z-streaming)
+// Layer 1 (1 -> 32) runs on the matrix cores as well: its K dimension is the
+// 27 taps (zero padded to 32) of the scaled input s f, rounded to bf16, times
+// the bf16-rounded W1; FP32 accumulation, bias and SiLU in FP32.  Its output
+// planes (with their halo rows, zero outside the grid) go straight into the
+// LDS ring that layer 2 consumes: Y1 never round-trips through HBM.  The
+// input f is staged in a 4-plane LDS ring of (RB + 4) x (G + 2) bf16 values.
+// MODE 0 also stores Z1, Y1, Z2 of the tile (training); MODE 1 stores Y2 only.
+// Persistent like k_mid_zs.
+template <int G, int MODE, typename T>
+__global__ void __launch_bounds__(256)
+k_l12_zs(int nb, const T* __restrict__ u, const int* __restrict__ src,
+         const float* __restrict__ scale, const float* __restrict__ W1,
+         const float* __restrict__ b1, const __bf16* __restrict__ W2,
+         const float* __restrict__ b2, __bf16* __restrict__ Z1, __bf16* __restrict__ Y1,
+         __bf16* __restrict__ Z2, __bf16* __restrict__ Y2) {
+    using S = ZsMid<G>;
+    constexpr int FR = S::HR + 2, FX = G + 2, FP = FR * FX;   // f tile rows, columns, size
+    constexpr int NFF = (FP + 255) / 256;
+    constexpr int NY1 = S::HR * S::NXT;                       // layer-1 tiles per plane
+    __shared__ __attribute__((aligned(16))) __bf16 ring[3 * S::PS];
+    __shared__ __attribute__((aligned(16))) __bf16 fr[4 * FP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, li = lane & 15, kq = lane >> 4;
+    const size_t n = size_t(G) * G * G;
+    const int ntask = nb * S::NBLK;
+    // the halo columns of the ring are never written again: zero once
+    for (int i = threadIdx.x; i < 3 * S::PS / 8; i += 256)
+        reinterpret_cast<uint4*>(ring)[i] = make_uint4(0u, 0u, 0u, 0u);
+    bf16x8 wf[kTaps][2];
+    load_wfrag(wf, W2, li, kq);
+    float bias2[2][4], bias1[2][4];
+    load_bias(bias2, b2, kq);
+    load_bias(bias1, b1, kq);
+    // layer-1 operands: A[j] row li = out channel 16j + li, k = tap 8kq + i
+    bf16x8 a1[2];
+    int toff[8], tdz[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int tap = 8 * kq + i;
+        int dz = 0, dy = 0, dx = 0;
+        if (tap < kTaps) tap_offset(tap, dz, dy, dx);
+        tdz[i] = dz;
+        toff[i] = (1 + dy) * FX + (1 + dx);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            a1[j][i] = tap < kTaps ? f2bf(W1[tap * kC + 16 * j + li]) : __bf16(0.0f);
+    }
+
+    for (int round = 0;; ++round) {
+        const int task = zs_task(round);
+        if (task >= ntask) break;
+        const ZsTile tl = zs_decode<S>(task);
+        const T* f = u + size_t(src[tl.b]) * n;
+        const float s = scale[tl.b];
+
+        // f plane p (rows y0-2 .. y0+RB+1, columns -1 .. G), scaled, bf16
+        auto f_fetch = [&](float(&fv)[NFF], int p) {
+#pragma unroll
+            for (int i = 0; i < NFF; ++i) {
+                const int k = threadIdx.x + 256 * i;
+                const int y = tl.y0 - 2 + k / FX, x = k % FX - 1;
+                float v = 0.f;
+                if (k < FP && p >= 0 && p < G && y >= 0 && y < G && x >= 0 && x < G)
+                    v = s * float(f[(size_t(p) * G + y) * G + x]);
+                fv[i] = v;
+            }
+        };
+        auto f_store = [&](const float(&fv)[NFF], int p) {
+            __bf16* slot = fr + ((p + 8) & 3) * FP;
+#pragma unroll
+            for (int i = 0; i < NFF; ++i) {
+                const int k = threadIdx.x + 256 * i;
+                if (k < FP) slot[k] = f2bf(fv[i]);
+            }
+        };
+        // layer-1 output plane p (halo rows, interior columns) -> ring slot p mod 3
+        auto y1_plane = [&](int p) {
+            __bf16* slot = ring + ((p + 3) % 3) * S::PS;
+            const bool pin = p >= 0 && p < G;
+            const bool own = MODE == 0 && p >= tl.z0 && p < tl.z0 + S::ZC;
+            for (int t = wave; t < NY1; t += 4) {
+                const int r = t / S::NXT, x = (t % S::NXT) * 16 + li;
+                const int y = tl.y0 - 1 + r;
+                // planes and rows outside the grid are zeros (the slot may hold an older plane)
+                const bool in = pin && y >= 0 && y < G;
+                bf16x8 bv;
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    bv[i] = in && 8 * kq + i < kTaps
+                                ? fr[((p + tdz[i] + 8) & 3) * FP + r * FX + x + toff[i]]
+                                : __bf16(0.0f);
+                const bool st = in && own && r >= 1 && r <= S::RB;
+                const size_t vo = ((size_t(tl.b) * G + p) * G + y) * G * kC + size_t(x) * kC;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        a1[j], bv, floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                    __bf16 zo[4], yo[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float zv = c[i] + bias1[j][i];
+                        zo[i] = f2bf(zv);
+                        yo[i] = in ? f2bf(silu(zv)) : __bf16(0.0f);
+                    }
+                    uint2 py;
+                    __builtin_memcpy(&py, yo, 8);
+                    *reinterpret_cast<uint2*>(&slot[(r * S::HX + x + 1) * kVP + 16 * j + 4 * kq]) =
+                        py;
+                    if (st) {
+                        uint2 pz;
+                        __builtin_memcpy(&pz, zo, 8);
+                        *reinterpret_cast<uint2*>(&Z1[vo + 16 * j + 4 * kq]) = pz;
+                        *reinterpret_cast<uint2*>(&Y1[vo + 16 * j + 4 * kq]) = py;
+                    }
                 }
             }
-            if (MODE == 0) {
-                uint2 pz;
-                __builtin_memcpy(&pz, zo, 8);
-                *reinterpret_cast<uint2*>(&Zout[vo + c0]) = pz;
+        };
+
+        float fv[NFF];
+        if (round > 0) __syncthreads();                    // last task's LDS reads are done
+        for (int p = tl.z0 - 2; p <= tl.z0 + 1; ++p) {
+            f_fetch(fv, p);
+            f_store(fv, p);
+        }
+        __syncthreads();
+        y1_plane(tl.z0 - 1);
+        y1_plane(tl.z0);
+        __syncthreads();                                   // f slot of z0 - 2 is free
+        f_fetch(fv, tl.z0 + 2);
+        f_store(fv, tl.z0 + 2);
+        __syncthreads();
+        y1_plane(tl.z0 + 1);
+        __syncthreads();
+        // invariant at the top of iteration z: the ring holds Y1 planes z-1,
+        // z, z+1; the f ring holds planes z-1 .. z+2
+        const int zend = tl.z0 + S::ZC;
+        mid_task<S, MODE>(
+            wf, ring, tl.z0, size_t(tl.b) * n + size_t(tl.y0) * G, wave, li, kq, bias2, nullptr, Z2,
+            Y2,
+            [&](int z) {
+                if (z + 1 < zend) f_fetch(fv, z + 3);
+            },
+            [&](int z) {
+                if (z + 1 < zend) {
+                    f_store(fv, z + 3);                    // slot of plane z - 1, unused since z
+                    __syncthreads();                       // ring slot (z - 1) mod 3 is free
+                    y1_plane(z + 2);
+                    __syncthreads();
+                }
+            });
+    }
+}
+
+// --------------------------------------------- layer 4 (z-streaming, MFMA)
+// out[v] = b4 + sum_tap P[tap][v + d(tap)] with P[tap][w] = W4[tap] . Y3[w]:
+// P of one input plane (its RB + 2 rows) comes from two MFMAs per 16 voxels
+// (A = W4 with the taps as rows, bf16, zero padded to 32; B = Y3 straight from
+// HBM), goes to LDS as FP32, and each thread adds the 27 shifted values into
+// three rolling output-plane accumulators (output plane q - 1 is complete
+// after input plane q).  mode 0: pred[b][v] = out (FP32);
+// mode 1: Q[dst[b]][v] += oscale[b] * out (solver dtype).
+template <int G, typename T>
+__global__ void __launch_bounds__(256)
+k_l4_zs(int nb, const __bf16* __restrict__ Y3, const float* __restrict__ W4,
+        const float* __restrict__ b4, int mode, float* __restrict__ pred,
+        const int* __restrict__ dst, const float* __restrict__ oscale, T* __restrict__ Q) {
+    using S = ZsOut<G>;
+    constexpr int TS = S::HR * G + 4;          // tap stride: kq groups land on distinct banks
+    constexpr int NY = S::HR * S::NXT;
+    __shared__ float Ps[kTaps * TS];
+    const ZsTile tl = zs_tile<S>(nb);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, li = lane & 15, kq = lane >> 4;
+    const size_t n = size_t(G) * G * G;
+    const __bf16* Yb = Y3 + size_t(tl.b) * n * kC;
+    bf16x8 a4[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int tap = 16 * j + li;
+            a4[j][i] = tap < kTaps ? f2bf(W4[tap * kC + 8 * kq + i]) : __bf16(0.0f);
+        }
+    const int oy = threadIdx.x / G, ox = threadIdx.x % G;       // output voxel of this thread
+    const bool owner = threadIdx.x < S::RB * G;
+    float o_prev = 0.f, o_cur = 0.f, o_next = 0.f;              // planes q - 1, q, q + 1
+    const float bias = b4[0];
+    for (int q = tl.z0 - 1; q <= tl.z0 + S::ZC; ++q) {
+        if (q >= 0 && q < G) {
+            for (int t = wave; t < NY; t += 4) {
+                const int r = t / S::NXT, x = (t % S::NXT) * 16 + li;
+                const int y = tl.y0 - 1 + r;
+                floatx4 c[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
+                if (y >= 0 && y < G) {
+                    const bf16x8 bv = *reinterpret_cast<const bf16x8*>(
+                        &Yb[((size_t(q) * G + y) * G + x) * kC + 8 * kq]);
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        c[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a4[j], bv, c[j], 0, 0, 0);
+                }
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int tap = 16 * j + 4 * kq + i;
+                        if (tap < kTaps) Ps[tap * TS + r * G + x] = c[j][i];
+                    }
             }
-            uint2 py;
-            __builtin_memcpy(&py, yo, 8);
-            *reinterpret_cast<uint2*>(&Y[vo + c0]) = py;
+            __syncthreads();
+            if (owner) {
+#pragma unroll
+                for (int dz = -1; dz <= 1; ++dz) {
+                    float sum = 0.f;
+#pragma unroll
+                    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+                            const int xx = ox + dx;
+                            if (xx >= 0 && xx < G) sum += Ps[tap * TS + (oy + 1 + dy) * G + xx];
+                        }
+                    // input plane q feeds output plane q - dz
+                    if (dz == 1) o_prev += sum;
+                    else if (dz == 0) o_cur += sum;
+                    else o_next += sum;
+                }
+            }
+            __syncthreads();
         }
-    }
-}
-
-// ------------------------------------------------------- 1 -> 32, VALU
-// f (solver dtype) of slice src[b] scaled by scale[b]; stores Z1 and Y1.
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_conv_in(int G, long nvox, const T* __restrict__ u, const int* __restrict__ src,
-          const float* __restrict__ scale, const float* __restrict__ W1 /*[27][32]*/,
-          const float* __restrict__ b1, __bf16* __restrict__ Z, __bf16* __restrict__ Y) {
-    __shared__ float Ws[kTaps * kC + kC];
-    for (int i = threadIdx.x; i < kTaps * kC; i += 256) Ws[i] = W1[i];
-    for (int i = threadIdx.x; i < kC; i += 256) Ws[kTaps * kC + i] = b1[i];
-    __syncthreads();
-    const long gid = long(blockIdx.x) * 256 + threadIdx.x;
-    if (gid >= nvox) return;
-    const int n = G * G * G;
-    const int b = int(gid / n), v = int(gid - long(b) * n);
-    const int x = v % G, y = (v / G) % G, z = v / (G * G);
-    const T* f = u + size_t(src[b]) * n;
-    const float s = scale[b];
-    float acc[kC];
-    for (int c = 0; c < kC; ++c) acc[c] = Ws[kTaps * kC + c];
-    for (int tap = 0; tap < kTaps; ++tap) {
-        int dz, dy, dx;
-        tap_offset(tap, dz, dy, dx);
-        const int zz = z + dz, yy = y + dy, xx = x + dx;
-        if (zz < 0 || zz >= G || yy < 0 || yy >= G || xx < 0 || xx >= G) continue;
-        const float fv = s * float(f[(zz * G + yy) * G + xx]);
-        for (int c = 0; c < kC; ++c) acc[c] += fv * Ws[tap * kC + c];
-    }
-    __bf16 zo[kC], yo[kC];
-    for (int c = 0; c < kC; ++c) {
-        zo[c] = f2bf(acc[c]);
-        yo[c] = f2bf(silu(acc[c]));
-    }
-    uint4* zp = reinterpret_cast<uint4*>(&Z[size_t(gid) * kC]);
-    uint4* yp = reinterpret_cast<uint4*>(&Y[size_t(gid) * kC]);
-    for (int q = 0; q < 4; ++q) {
-        uint4 t1, t2;
-        __builtin_memcpy(&t1, &zo[8 * q], 16);
-        __builtin_memcpy(&t2, &yo[8 * q], 16);
-        zp[q] = t1;
-        yp[q] = t2;
-    }
-}
-
-// ------------------------------------------------------- 32 -> 1, VALU
-// out[b][v] = b4 + sum W4 . Y3; mode 0: store FP32 prediction pred[b][v];
-// mode 1: Q[dst[b]][v] += oscale[b] * out (solver dtype)
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_conv_out(int G, long nvox, const __bf16* __restrict__ Y3, const float* __restrict__ W4,
-           const float* __restrict__ b4, int mode, float* __restrict__ pred,
-           const int* __restrict__ dst, const float* __restrict__ oscale, T* __restrict__ Q) {
-    __shared__ float Ws[kTaps * kC];
-    for (int i = threadIdx.x; i < kTaps * kC; i += 256) Ws[i] = W4[i];
-    __syncthreads();
-    const long gid = long(blockIdx.x) * 256 + threadIdx.x;
-    if (gid >= nvox) return;
-    const int n = G * G * G;
-    const int b = int(gid / n), v = int(gid - long(b) * n);
-    const int x = v % G, y = (v / G) % G, z = v / (G * G);
-    const __bf16* Yb = Y3 + size_t(b) * n * kC;
-    float acc = b4[0];
-    for (int tap = 0; tap < kTaps; ++tap) {
-        int dz, dy, dx;
-        tap_offset(tap, dz, dy, dx);
-        const int zz = z + dz, yy = y + dy, xx = x + dx;
-        if (zz < 0 || zz >= G || yy < 0 || yy >= G || xx < 0 || xx >= G) continue;
-        const uint4* p = reinterpret_cast<const uint4*>(&Yb[size_t((zz * G + yy) * G + xx) * kC]);
-        for (int q = 0; q < 4; ++q) {
-            const uint4 w = p[q];
-            __bf16 h[8];
-            __builtin_memcpy(h, &w, 16);
-            for (int i = 0; i < 8; ++i) acc += bf2f(h[i]) * Ws[tap * kC + 8 * q + i];
+        const int zo = q - 1;                                    // complete now
+        if (owner && zo >= tl.z0 && zo < tl.z0 + S::ZC) {
+            const size_t v = (size_t(zo) * G + tl.y0 + oy) * G + ox;
+            const float out = o_prev + bias;
+            if (mode == 0) {
+                pred[size_t(tl.b) * n + v] = out;
+            } else {
+                T* qp = Q + size_t(dst[tl.b]) * n + v;
+                *qp = T(float(*qp) + oscale[tl.b] * out);
+            }
         }
-    }
-    if (mode == 0) pred[gid] = acc;
-    else {
-        T* q = Q + size_t(dst[b]) * n + v;
-        *q = T(float(*q) + oscale[b] * acc);
+        o_prev = o_cur;
+        o_cur = o_next;
+        o_next = 0.f;
     }
 }
 
@@ -427,7 +731,8 @@ k_reduce_chunks(int nchunk, int len, const float* __restrict__ part, float* __re
     out[i] = acc;
 }
 
-// scaled input of layer 1 as FP32 (kept for its weight gradient)
+// scaled input of layer 1, rounded to bf16 as the forward uses it, stored as
+// FP32 (kept for the layer-1 weight gradient)
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_gather_scaled(int n, long nvox, const T* __restrict__ u, const int* __restrict__ src,
@@ -435,7 +740,7 @@ k_gather_scaled(int n, long nvox, const T* __restrict__ u, const int* __restrict
     const long gid = long(blockIdx.x) * 256 + threadIdx.x;
     if (gid >= nvox) return;
     const int b = int(gid / n), v = int(gid - long(b) * n);
-    out[gid] = scale[b] * float(u[size_t(src[b]) * n + v]);
+    out[gid] = bf2f(f2bf(scale[b] * float(u[size_t(src[b]) * n + v])));
 }
 
 // master FP32 [27][32 out][32 in] -> bf16 forward copy and the tap-flipped,
@@ -521,33 +826,90 @@ using namespace dgfs;
 
 namespace {
 
+// compute units of the current device (persistent grids: one work-group per CU)
+int cu_count() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cached[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n <= 0)
+            n = 256;
+        cached[dev] = n;
+    }
+    return cached[dev];
+}
+
+template <int G>
+dim3 persistent_grid(int nb) {
+    const int ntask = nb * ZsMid<G>::NBLK, ncu = cu_count();
+    return dim3(ntask < ncu ? ntask : ncu);
+}
+
+template <int G>
+int launch_mid(int mode, int nb, const __bf16* x, const __bf16* w, const float* bias,
+               const __bf16* zp, __bf16* zo, __bf16* y, hipStream_t st) {
+    const dim3 grid = persistent_grid<G>(nb);
+    if (mode == 0)
+        hipLaunchKernelGGL((k_mid_zs<G, 0>), grid, dim3(256), 0, st, nb, x, w, bias, zp, zo, y);
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_mid_zs<G, 1>), grid, dim3(256), 0, st, nb, x, w, bias, zp, zo, y);
+    else
+        hipLaunchKernelGGL((k_mid_zs<G, 2>), grid, dim3(256), 0, st, nb, x, w, bias, zp, zo, y);
+    DGFS_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// 32 -> 32 layer; mode 0: store Z and Y = silu(Z), 1: Y only, 2: input gradient
 int dispatch_mid(int G, int mode, int nb, const void* X, const void* W, const float* bias,
                  const void* Zp, void* Zout, void* Y, hipStream_t st) {
-    const dim3 grid(nb * G * (G / 8));
     const __bf16* x = (const __bf16*)X;
     const __bf16* w = (const __bf16*)W;
     const __bf16* zp = (const __bf16*)Zp;
     __bf16* zo = (__bf16*)Zout;
     __bf16* y = (__bf16*)Y;
-#define DGFS_MID(GG, KER)                                                                     \
-    if (G == GG) {                                                                            \
-        if (mode == 0)                                                                        \
-            hipLaunchKernelGGL((KER<GG, 0>), grid, dim3(256), 0, st, nb, x, w, bias, zp, zo,  \
-                               y);                                                            \
-        else if (mode == 1)                                                                   \
-            hipLaunchKernelGGL((KER<GG, 1>), grid, dim3(256), 0, st, nb, x, w, bias, zp, zo,  \
-                               y);                                                            \
-        else                                                                                  \
-            hipLaunchKernelGGL((KER<GG, 2>), grid, dim3(256), 0, st, nb, x, w, bias, zp, zo,  \
-                               y);                                                            \
-        DGFS_HIP_CHECK(hipGetLastError());                                                    \
-        return 0;                                                                             \
-    }
-    // LDS-staged input where the tile fits (G <= 32); direct loads at G = 48
-    DGFS_MID(16, k_conv_mid_lds)
-    DGFS_MID(32, k_conv_mid_lds)
-    DGFS_MID(48, k_conv_mid)
-#undef DGFS_MID
+    if (G == 16) return launch_mid<16>(mode, nb, x, w, bias, zp, zo, y, st);
+    if (G == 32) return launch_mid<32>(mode, nb, x, w, bias, zp, zo, y, st);
+    if (G == 48) return launch_mid<48>(mode, nb, x, w, bias, zp, zo, y, st);
+    DGFS_REQUIRE(false, "closure: grid size must be 16, 32 or 48");
+}
+
+// layers 1 + 2 (mode 0 stores Z1 Y1 Z2 Y2, 1 stores Y2), then layer 3 and 4
+template <int G, typename T>
+int launch_forward(const dgfs_closure_t* net, int nb, const T* u, const int* src,
+                   const float* scale, __bf16* A, size_t a, int mode, float* pred, const int* dst,
+                   const float* oscale, T* Q, hipStream_t st) {
+    const dim3 grid = persistent_grid<G>(nb);
+    if (mode == 0)
+        hipLaunchKernelGGL((k_l12_zs<G, 0, T>), grid, dim3(256), 0, st, nb, u, src, scale, net->W1,
+                           net->b1, (const __bf16*)net->W2f, net->b2, A, A + a, A + 2 * a,
+                           A + 3 * a);
+    else
+        hipLaunchKernelGGL((k_l12_zs<G, 1, T>), grid, dim3(256), 0, st, nb, u, src, scale, net->W1,
+                           net->b1, (const __bf16*)net->W2f, net->b2, A, A + a, A + 2 * a,
+                           A + 3 * a);
+    DGFS_HIP_CHECK(hipGetLastError());
+    const int rc = launch_mid<G>(mode, nb, A + 3 * a, (const __bf16*)net->W3f, net->b3, nullptr,
+                                 A + 4 * a, A + 5 * a, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_l4_zs<G, T>), dim3(nb * ZsOut<G>::NBLK), dim3(256), 0, st, nb, A + 5 * a,
+                       net->W4, net->b4,
+                       mode, pred, dst, oscale, Q);
+    DGFS_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+int dispatch_forward(int G, const dgfs_closure_t* net, int nb, const T* u, const int* src,
+                     const float* scale, __bf16* A, size_t a, int mode, float* pred,
+                     const int* dst, const float* oscale, T* Q, hipStream_t st) {
+    if (G == 16)
+        return launch_forward<16, T>(net, nb, u, src, scale, A, a, mode, pred, dst, oscale, Q, st);
+    if (G == 32)
+        return launch_forward<32, T>(net, nb, u, src, scale, A, a, mode, pred, dst, oscale, Q, st);
+    if (G == 48)
+        return launch_forward<48, T>(net, nb, u, src, scale, A, a, mode, pred, dst, oscale, Q, st);
     DGFS_REQUIRE(false, "closure: grid size must be 16, 32 or 48");
 }
 
@@ -570,29 +932,14 @@ extern "C" int dgfs_closure_forward(const dgfs_closure_t* net, int dtype, int G,
     DGFS_REQUIRE(mode == 0 || (dst && oscale && Q), "closure_forward: mode 1 needs dst, oscale, Q");
     if (nb == 0) return 0;
     const long nvox = long(nb) * G * G * G;
-    DGFS_REQUIRE(nvox / 256 < 2147483647L && long(nb) * G * (G / 8) < 2147483647L, "too large");
+    DGFS_REQUIRE(long(nb) * 72 < 2147483647L, "closure_forward: batch too large");
     const size_t a = size_t(nvox) * kC;         // one activation tensor (bf16 elements)
     __bf16* A = (__bf16*)act;                   // [Z1 Y1 Z2 Y2 Z3 Y3], 6 tensors
-    const dim3 gv((nvox + 255) / 256);
     if (dtype == DGFS_F64)
-        hipLaunchKernelGGL(k_conv_in<double>, gv, dim3(256), 0, st, G, nvox, (const double*)u, src,
-                           scale, net->W1, net->b1, A, A + a);
-    else
-        hipLaunchKernelGGL(k_conv_in<float>, gv, dim3(256), 0, st, G, nvox, (const float*)u, src,
-                           scale, net->W1, net->b1, A, A + a);
-    DGFS_HIP_CHECK(hipGetLastError());
-    int rc = dispatch_mid(G, 0, nb, A + a, net->W2f, net->b2, nullptr, A + 2 * a, A + 3 * a, st);
-    if (rc) return rc;
-    rc = dispatch_mid(G, 0, nb, A + 3 * a, net->W3f, net->b3, nullptr, A + 4 * a, A + 5 * a, st);
-    if (rc) return rc;
-    if (dtype == DGFS_F64)
-        hipLaunchKernelGGL(k_conv_out<double>, gv, dim3(256), 0, st, G, nvox, A + 5 * a, net->W4,
-                           net->b4, mode, pred, dst, oscale, (double*)Q);
-    else
-        hipLaunchKernelGGL(k_conv_out<float>, gv, dim3(256), 0, st, G, nvox, A + 5 * a, net->W4,
-                           net->b4, mode, pred, dst, oscale, (float*)Q);
-    DGFS_HIP_CHECK(hipGetLastError());
-    return 0;
+        return dispatch_forward<double>(G, net, nb, (const double*)u, src, scale, A, a, mode, pred,
+                                        dst, oscale, (double*)Q, st);
+    return dispatch_forward<float>(G, net, nb, (const float*)u, src, scale, A, a, mode, pred, dst,
+                                   oscale, (float*)Q, st);
 }
 
 extern "C" int dgfs_closure_backward(const dgfs_closure_t* net, const dgfs_closure_grad_t* gr,

@@ -66,14 +66,6 @@ DGFS_DEV T task_value(const TaskView& tv, const T* u, int t, int which, int v) {
     return acc;
 }
 
-// XCD-aware block remap: consecutive logical ids land on one XCD (8 XCDs,
-// round-robin dispatch), so the work-groups of one task share an L2.
-DGFS_DEV int xcd_remap(int bid, int nblk) {
-    const int per = nblk / 8;
-    if (bid < per * 8) return (bid % 8) * per + bid / 8;
-    return bid;
-}
-
 // ============================================================ plane helpers
 // P points at PL planes of N x (N+1) complex in LDS; thread (pl, i) owns row
 // or column i of plane pl.

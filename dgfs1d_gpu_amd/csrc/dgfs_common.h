@@ -106,6 +106,14 @@ constexpr int smallest_radix(int L) {
     return L;
 }
 
+// XCD-aware block remap: consecutive logical ids land on one XCD (8 XCDs,
+// round-robin dispatch), so neighbouring work-groups share an L2.
+DGFS_DEV int xcd_remap(int bid, int nblk) {
+    const int per = nblk / 8;
+    if (bid < per * 8) return (bid % 8) * per + bid / 8;
+    return bid;
+}
+
 }  // namespace dgfs
 
 // ---------------------------------------------------------------- C-ABI errors

@@ -9,8 +9,10 @@ tap = (dz+1)*9 + (dy+1)*3 + (dx+1), out[v] = sum_tap W[tap] . in[v + d(tap)].
 Arrays are channels-last: act[b, z, y, x, c].
 
 ``bf16=True`` rounds to bfloat16 (round-to-nearest-even) exactly where the
-HIP kernels store bf16: the hidden weights, every pre-activation Z and
-activation Y, and the back-propagated dZ; arithmetic is FP64 in between.
+HIP kernels round to bf16: the scaled input f and every forward weight (the
+matrix cores take bf16 operands), every pre-activation Z and activation Y,
+and the back-propagated dZ; arithmetic is FP64 in between.  The backward
+uses the FP32 master W4 (the kernels' VALU path), as here.
 ``bf16=False`` is the exact FP64 network (pinned by finite differences).
 """
 import numpy as np
@@ -85,16 +87,16 @@ class Net:
     def forward(self, f, bf=False):
         """f (B, G, G, G) already scaled -> prediction (B, G, G, G), cache."""
         r = bf16 if bf else (lambda a: a)
-        x0 = np.asarray(f, float)[..., None]
+        x0 = r(np.asarray(f, float))[..., None]
         W2 = r(self.W2)
         W3 = r(self.W3)
-        z1 = conv(x0, self.W1) + self.b1
+        z1 = conv(x0, r(self.W1)) + self.b1
         Z1, Y1 = r(z1), r(silu(z1))
         z2 = conv(Y1, W2) + self.b2
         Z2, Y2 = r(z2), r(silu(z2))
         z3 = conv(Y2, W3) + self.b3
         Z3, Y3 = r(z3), r(silu(z3))
-        out = conv(Y3, self.W4)[..., 0] + self.b4[0]
+        out = conv(Y3, r(self.W4))[..., 0] + self.b4[0]
         return out, dict(x0=x0, Z1=Z1, Y1=Y1, Z2=Z2, Y2=Y2, Z3=Z3, Y3=Y3, W2=W2, W3=W3)
 
     def backward(self, dout, cache, bf=False):
