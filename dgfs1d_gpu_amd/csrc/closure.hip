@@ -71,14 +71,14 @@ DGFS_DEV void tap_offset(int tap, int& dz, int& dy, int& dx) {
 // weights stay in registers (27 taps x 2 A fragments = 216 VGPRs per lane,
 // one wave per SIMD), so LDS serves only the activations; work-groups are
 // persistent (one per CU) and load the weights once.
-template <int G_, int RB_>
+template <int G_, int RB_, int ZC_ = 8>
 struct Zs {
     static constexpr int G = G_;
     static constexpr int RB = RB_;                        // rows per task
     static constexpr int RW = RB / 4;                     // rows per wave
     static constexpr int NXT = G / 16;                    // x tiles per row
     static constexpr int NT = RW * NXT;                   // output tiles per wave
-    static constexpr int ZC = 8;                          // planes per task
+    static constexpr int ZC = ZC_;                        // planes per task
     static constexpr int HR = RB + 2, HX = G + 2;         // halo plane rows, columns
     static constexpr int PS = HR * HX * kVP;              // halo plane, bf16 elements
     static constexpr int NCH = HR * HX * 4;               // its 16 B chunks
@@ -90,9 +90,9 @@ struct Zs {
 // 32 -> 32 layers (and layer 1 fused in front of layer 2)
 template <int G>
 using ZsMid = Zs<G, (G == 48 ? 4 : 8)>;
-// layer 4: one output voxel per thread
+// layer 4: one output voxel per thread, 16 planes per task
 template <int G>
-using ZsOut = Zs<G, (G == 48 ? 4 : 256 / G)>;
+using ZsOut = Zs<G, (G == 48 ? 4 : 256 / G), 16>;
 
 struct ZsTile {
     int b, y0, z0;
@@ -531,6 +531,7 @@ k_l4_zs(int nb, const __bf16* __restrict__ Y3, const float* __restrict__ W4,
     constexpr int TS = S::HR * G + 4;          // tap stride: kq groups land on distinct banks
     constexpr int NY = S::HR * S::NXT;
     __shared__ float Ps[kTaps * TS];
+    constexpr int NYW = (NY + 3) / 4;          // P tiles per wave and plane
     const ZsTile tl = zs_tile<S>(nb);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, li = lane & 15, kq = lane >> 4;
     const size_t n = size_t(G) * G * G;
@@ -547,19 +548,35 @@ k_l4_zs(int nb, const __bf16* __restrict__ Y3, const float* __restrict__ W4,
     const bool owner = threadIdx.x < S::RB * G;
     float o_prev = 0.f, o_cur = 0.f, o_next = 0.f;              // planes q - 1, q, q + 1
     const float bias = b4[0];
-    for (int q = tl.z0 - 1; q <= tl.z0 + S::ZC; ++q) {
-        if (q >= 0 && q < G) {
-            for (int t = wave; t < NY; t += 4) {
-                const int r = t / S::NXT, x = (t % S::NXT) * 16 + li;
-                const int y = tl.y0 - 1 + r;
-                floatx4 c[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
-                if (y >= 0 && y < G) {
-                    const bf16x8 bv = *reinterpret_cast<const bf16x8*>(
-                        &Yb[((size_t(q) * G + y) * G + x) * kC + 8 * kq]);
+    // B fragments of input plane q (zero outside the grid), one plane ahead
+    auto fetch = [&](int q, bf16x8(&b)[NYW]) __attribute__((always_inline)) {
 #pragma unroll
-                    for (int j = 0; j < 2; ++j)
-                        c[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a4[j], bv, c[j], 0, 0, 0);
-                }
+        for (int k = 0; k < NYW; ++k) {
+            const int t = wave + 4 * k;
+            const int r = t / S::NXT, x = (t % S::NXT) * 16 + li;
+            const int y = tl.y0 - 1 + r;
+            bf16x8 v;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = __bf16(0.0f);
+            if (t < NY && q >= 0 && q < G && y >= 0 && y < G)
+                v = *reinterpret_cast<const bf16x8*>(&Yb[((size_t(q) * G + y) * G + x) * kC + 8 * kq]);
+            b[k] = v;
+        }
+    };
+    bf16x8 bc[NYW], bn[NYW];
+    fetch(tl.z0 - 1, bc);
+    for (int q = tl.z0 - 1; q <= tl.z0 + S::ZC; ++q) {
+        if (q < tl.z0 + S::ZC) fetch(q + 1, bn);
+        if (q >= 0 && q < G) {
+#pragma unroll
+            for (int k = 0; k < NYW; ++k) {
+                const int t = wave + 4 * k;
+                if (t >= NY) break;
+                const int r = t / S::NXT, x = (t % S::NXT) * 16 + li;
+                floatx4 c[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    c[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a4[j], bc[k], c[j], 0, 0, 0);
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
 #pragma unroll
@@ -600,6 +617,8 @@ k_l4_zs(int nb, const __bf16* __restrict__ Y3, const float* __restrict__ W4,
                 *qp = T(float(*qp) + oscale[tl.b] * out);
             }
         }
+#pragma unroll
+        for (int k = 0; k < NYW; ++k) bc[k] = bn[k];
         o_prev = o_cur;
         o_cur = o_next;
         o_next = 0.f;

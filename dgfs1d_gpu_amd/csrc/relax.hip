@@ -9,10 +9,12 @@
 //                                       std/scattering.py:647-695)
 //
 // Both operators work on whole velocity slices (n = N^3 points of one
-// (mode, element) pair).  Reductions are two-kernel and deterministic: a
-// partial kernel writes one FP64 vector per (slice, 256-point block), and the
-// apply kernel re-reduces those partials in a fixed order inside every block
-// that needs them (no atomics, no host synchronisation).
+// (mode, element) pair).  Reductions are deterministic and three-kernel: a
+// partial kernel writes one FP64 vector per (slice, 256-point block), a
+// finalize kernel (one work-group per slice) sums those partials in a fixed
+// order and turns them into the slice's constants (Lagrange multipliers,
+// Shakov state), and the apply kernel reads the constants (no atomics, no
+// host synchronisation).
 #include "dgfs_common.h"
 #include "dgfs_hip.h"
 
@@ -21,18 +23,21 @@ namespace {
 
 constexpr int kRB = 256;
 
-// block-wide sum of NV doubles; every thread receives the totals
+// block-wide sum of NV doubles in a fixed order (wave shuffles, then the four
+// wave totals); every thread receives the totals.  sh holds NV * 4 doubles.
 template <int NV>
-DGFS_DEV void block_sum_all(double (&v)[NV], double* sh /* [NV][kRB] */) {
-    const int tid = threadIdx.x;
-    for (int i = 0; i < NV; ++i) sh[i * kRB + tid] = v[i];
+DGFS_DEV void block_sum_all(double (&v)[NV], double* sh) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_xor(v[i], o);
+    if (l == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) sh[i * 4 + w] = v[i];
     __syncthreads();
-    for (int s = kRB / 2; s > 0; s >>= 1) {
-        if (tid < s)
-            for (int i = 0; i < NV; ++i) sh[i * kRB + tid] += sh[i * kRB + tid + s];
-        __syncthreads();
-    }
-    for (int i = 0; i < NV; ++i) v[i] = sh[i * kRB];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = (sh[i * 4] + sh[i * 4 + 1]) + (sh[i * 4 + 2] + sh[i * 4 + 3]);
     __syncthreads();
 }
 
@@ -57,7 +62,7 @@ __global__ void __launch_bounds__(kRB)
 k_proj_partial(int n, const T* __restrict__ Q, const T* __restrict__ cx,
                const T* __restrict__ cy, const T* __restrict__ cz,
                double* __restrict__ part /*[slice][nb][5]*/) {
-    __shared__ double sh[5 * kRB];
+    __shared__ double sh[5 * 4];
     const int nb = gridDim.x, s = blockIdx.y;
     const int v = blockIdx.x * kRB + threadIdx.x;
     double acc[5] = {0, 0, 0, 0, 0};
@@ -75,28 +80,33 @@ k_proj_partial(int n, const T* __restrict__ Q, const T* __restrict__ cx,
         for (int i = 0; i < 5; ++i) part[(size_t(s) * nb + blockIdx.x) * 5 + i] = acc[i];
 }
 
-template <typename T>
+// multipliers lam = G (Phi Q) of slice s -> lam[s][8] (first 5 used)
 __global__ void __launch_bounds__(kRB)
-k_proj_apply(int n, Gram5 G, const T* __restrict__ Q, const T* __restrict__ cx,
-             const T* __restrict__ cy, const T* __restrict__ cz,
-             const double* __restrict__ part, T* __restrict__ out, int accumulate) {
-    __shared__ double sh[5 * kRB];
-    const int nb = gridDim.x, s = blockIdx.y;
+k_proj_final(int nb, Gram5 G, const double* __restrict__ part, double* __restrict__ lam) {
+    __shared__ double sh[5 * 4];
+    const int s = blockIdx.x;
     double m[5];
     reduce_partials<5>(part + size_t(s) * nb * 5, nb, m, sh);
-    double lam[5];
-    for (int i = 0; i < 5; ++i) {
+    if (threadIdx.x < 5) {
         double a = 0.0;
-        for (int j = 0; j < 5; ++j) a += G.g[i * 5 + j] * m[j];
-        lam[i] = a;
+        for (int j = 0; j < 5; ++j) a += G.g[threadIdx.x * 5 + j] * m[j];
+        lam[size_t(s) * 8 + threadIdx.x] = a;
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRB)
+k_proj_apply(int n, const T* __restrict__ Q, const T* __restrict__ cx,
+             const T* __restrict__ cy, const T* __restrict__ cz,
+             const double* __restrict__ lam, T* __restrict__ out, int accumulate) {
+    const int s = blockIdx.y;
     const int v = blockIdx.x * kRB + threadIdx.x;
     if (v >= n) return;
+    const double* L = lam + size_t(s) * 8;
     const double x = cx[v], y = cy[v], z = cz[v];
     const size_t o = size_t(s) * n + v;
     const double qc = double(Q[o]) -
-                      (lam[0] + lam[1] * x + lam[2] * y + lam[3] * z +
-                       lam[4] * (x * x + y * y + z * z));
+                      (L[0] + L[1] * x + L[2] * y + L[3] * z + L[4] * (x * x + y * y + z * z));
     if (accumulate) out[o] = T(double(out[o]) + qc);
     else out[o] = T(qc);
 }
@@ -127,7 +137,7 @@ k_shk_partial(int K, int n, const int* __restrict__ nterm, const int* __restrict
               const T* __restrict__ coef, const T* __restrict__ u, const T* __restrict__ cx,
               const T* __restrict__ cy, const T* __restrict__ cz,
               double* __restrict__ part /*[task][nb][13]*/) {
-    __shared__ double sh[kShkMom * kRB];
+    __shared__ double sh[kShkMom * 4];
     const int nb = gridDim.x, t = blockIdx.y;
     const int v = blockIdx.x * kRB + threadIdx.x;
     double a[kShkMom];
@@ -179,6 +189,32 @@ DGFS_DEV ShkState shk_state(const double (&m)[kShkMom], double cw) {
     return s;
 }
 
+// per-task constants of f_S and the relaxation rate, from the summed raw
+// moments: cst[t][16] = {ux, uy, uz, 1/T, rho/(pi T)^1.5,
+// (1-Pr)/5 * 4/(rho T^2), qx, qy, qz, prefac rho T^(1-omega)}  (mako:50-141)
+constexpr int kShkCst = 16;
+
+__global__ void __launch_bounds__(kRB)
+k_shk_final(int nb, ShkParams P, const double* __restrict__ part, double* __restrict__ cst) {
+    __shared__ double sh[kShkMom * 4];
+    const int t = blockIdx.x;
+    double m[kShkMom];
+    reduce_partials<kShkMom>(part + size_t(t) * nb * kShkMom, nb, m, sh);
+    if (threadIdx.x != 0) return;
+    const ShkState st = shk_state(m, P.cw);
+    double* c = cst + size_t(t) * kShkCst;
+    c[0] = st.ux;
+    c[1] = st.uy;
+    c[2] = st.uz;
+    c[3] = 1.0 / st.T;
+    c[4] = st.rho / pow(kPi * st.T, 1.5);
+    c[5] = (1.0 - P.Pr) / 5.0 * 4.0 / (st.rho * st.T * st.T);
+    c[6] = st.qx;
+    c[7] = st.qy;
+    c[8] = st.qz;
+    c[9] = P.prefac * (st.rho / pow(st.T, P.omega - 1.0));
+}
+
 // Q[gslice[g]] += sum_{t in group g} prefac rho T^{1-omega} (f_S(t) - f_t)
 template <typename T>
 __global__ void __launch_bounds__(kRB)
@@ -186,30 +222,23 @@ k_shk_apply(int K, int n, const int* __restrict__ nterm, const int* __restrict__
             const T* __restrict__ coef, const int* __restrict__ gstart,
             const int* __restrict__ gslice, const T* __restrict__ u, const T* __restrict__ cx,
             const T* __restrict__ cy, const T* __restrict__ cz,
-            const double* __restrict__ part, ShkParams P, T* __restrict__ Q) {
-    __shared__ double sh[kShkMom * kRB];
-    const int nb = gridDim.x, g = blockIdx.y;
+            const double* __restrict__ cst, T* __restrict__ Q) {
+    const int g = blockIdx.y;
     const int v = blockIdx.x * kRB + threadIdx.x;
-    const bool in = v < n;
-    const double x = in ? double(cx[v]) : 0.0, y = in ? double(cy[v]) : 0.0,
-                 z = in ? double(cz[v]) : 0.0;
+    if (v >= n) return;
+    const double x = cx[v], y = cy[v], z = cz[v];
     double acc = 0.0;
     for (int t = gstart[g]; t < gstart[g + 1]; ++t) {
-        double m[kShkMom];
-        reduce_partials<kShkMom>(part + size_t(t) * nb * kShkMom, nb, m, sh);
-        if (!in) continue;
-        const ShkState s = shk_state(m, P.cw);
-        const double Cx = x - s.ux, Cy = y - s.uy, Cz = z - s.uz;
+        const double* c = cst + size_t(t) * kShkCst;
+        const double Cx = x - c[0], Cy = y - c[1], Cz = z - c[2];
         const double C2 = Cx * Cx + Cy * Cy + Cz * Cz;
-        const double fe = s.rho / pow(kPi * s.T, 1.5) * exp(-C2 / s.T);
-        // (1-Pr)/5 * 4 (q.C)/(rho T^2) (C^2/T - 5/2)   (mako:123-141)
-        const double corr = (1.0 - P.Pr) / 5.0 * 4.0 * (s.qx * Cx + s.qy * Cy + s.qz * Cz) /
-                            (s.rho * s.T * s.T) * (C2 / s.T - 2.5);
-        const double fS = fe * (1.0 + corr);
+        const double C2T = C2 * c[3];
+        // f_S = f_M (1 + (1-Pr)/5 * 4 (q.C)/(rho T^2) (C^2/T - 5/2))   (mako:123-141)
+        const double fS = c[4] * exp(-C2T) *
+                          (1.0 + c[5] * (c[6] * Cx + c[7] * Cy + c[8] * Cz) * (C2T - 2.5));
         const double f = shk_task_value(nterm, slice, coef, K, n, u, t, v);
-        acc += P.prefac * (s.rho / pow(s.T, P.omega - 1.0)) * (fS - f);
+        acc += c[9] * (fS - f);
     }
-    if (!in) return;
     const size_t o = size_t(gslice[g]) * n + v;
     Q[o] = T(double(Q[o]) + acc);
 }
@@ -221,7 +250,7 @@ using namespace dgfs;
 
 extern "C" size_t dgfs_projection_work_bytes(int nslice, int Nv) {
     const int nb = (Nv + kRB - 1) / kRB;
-    return size_t(nslice) * nb * 5 * sizeof(double);
+    return size_t(nslice) * (nb * 5 + 8) * sizeof(double);
 }
 
 extern "C" int dgfs_project_conservative(int dtype, int nslice, int Nv, const void* cx,
@@ -238,23 +267,27 @@ extern "C" int dgfs_project_conservative(int dtype, int nslice, int Nv, const vo
         for (int i = 0; i < 25; ++i) G.g[i] = gram_inv[i];
         const int nb = (Nv + kRB - 1) / kRB;
         const dim3 grid(nb, nslice);
-        if (dtype == DGFS_F64) {
+        double* lam = work + size_t(nslice) * nb * 5;
+        if (dtype == DGFS_F64)
             hipLaunchKernelGGL(k_proj_partial<double>, grid, dim3(kRB), 0, stream, Nv,
                                (const double*)Q, (const double*)cx, (const double*)cy,
                                (const double*)cz, work);
-            DGFS_HIP_CHECK(hipGetLastError());
-            hipLaunchKernelGGL(k_proj_apply<double>, grid, dim3(kRB), 0, stream, Nv, G,
-                               (const double*)Q, (const double*)cx, (const double*)cy,
-                               (const double*)cz, (const double*)work, (double*)out, accumulate);
-        } else {
+        else
             hipLaunchKernelGGL(k_proj_partial<float>, grid, dim3(kRB), 0, stream, Nv,
                                (const float*)Q, (const float*)cx, (const float*)cy,
                                (const float*)cz, work);
-            DGFS_HIP_CHECK(hipGetLastError());
-            hipLaunchKernelGGL(k_proj_apply<float>, grid, dim3(kRB), 0, stream, Nv, G,
+        DGFS_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_proj_final, dim3(nslice), dim3(kRB), 0, stream, nb, G,
+                           (const double*)work, lam);
+        DGFS_HIP_CHECK(hipGetLastError());
+        if (dtype == DGFS_F64)
+            hipLaunchKernelGGL(k_proj_apply<double>, grid, dim3(kRB), 0, stream, Nv,
+                               (const double*)Q, (const double*)cx, (const double*)cy,
+                               (const double*)cz, (const double*)lam, (double*)out, accumulate);
+        else
+            hipLaunchKernelGGL(k_proj_apply<float>, grid, dim3(kRB), 0, stream, Nv,
                                (const float*)Q, (const float*)cx, (const float*)cy,
-                               (const float*)cz, (const double*)work, (float*)out, accumulate);
-        }
+                               (const float*)cz, (const double*)lam, (float*)out, accumulate);
         DGFS_HIP_CHECK(hipGetLastError());
         return 0;
     }
@@ -262,7 +295,7 @@ extern "C" int dgfs_project_conservative(int dtype, int nslice, int Nv, const vo
 
 extern "C" size_t dgfs_shakov_work_bytes(int ntask, int Nv) {
     const int nb = (Nv + kRB - 1) / kRB;
-    return size_t(ntask) * nb * kShkMom * sizeof(double);
+    return size_t(ntask) * (nb * kShkMom + kShkCst) * sizeof(double);
 }
 
 extern "C" int dgfs_shakov(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nterm,
@@ -281,29 +314,33 @@ extern "C" int dgfs_shakov(int dtype, int K, int Ne, int Nv, int ntask, const in
                      "null pointer");
         const int nb = (Nv + kRB - 1) / kRB;
         const ShkParams P{prefac, omega, Pr, cw};
-        if (dtype == DGFS_F64) {
+        double* cst = work + size_t(ntask) * nb * kShkMom;
+        if (dtype == DGFS_F64)
             hipLaunchKernelGGL(k_shk_partial<double>, dim3(nb, ntask), dim3(kRB), 0, stream, K,
                                Nv, task_nterm, task_slice, (const double*)task_coef,
                                (const double*)u, (const double*)cx, (const double*)cy,
                                (const double*)cz, work);
-            DGFS_HIP_CHECK(hipGetLastError());
-            hipLaunchKernelGGL(k_shk_apply<double>, dim3(nb, ngroup), dim3(kRB), 0, stream, K,
-                               Nv, task_nterm, task_slice, (const double*)task_coef, group_start,
-                               group_slice, (const double*)u, (const double*)cx,
-                               (const double*)cy, (const double*)cz, (const double*)work, P,
-                               (double*)Q);
-        } else {
+        else
             hipLaunchKernelGGL(k_shk_partial<float>, dim3(nb, ntask), dim3(kRB), 0, stream, K,
                                Nv, task_nterm, task_slice, (const float*)task_coef,
                                (const float*)u, (const float*)cx, (const float*)cy,
                                (const float*)cz, work);
-            DGFS_HIP_CHECK(hipGetLastError());
+        DGFS_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_shk_final, dim3(ntask), dim3(kRB), 0, stream, nb, P,
+                           (const double*)work, cst);
+        DGFS_HIP_CHECK(hipGetLastError());
+        if (dtype == DGFS_F64)
+            hipLaunchKernelGGL(k_shk_apply<double>, dim3(nb, ngroup), dim3(kRB), 0, stream, K,
+                               Nv, task_nterm, task_slice, (const double*)task_coef, group_start,
+                               group_slice, (const double*)u, (const double*)cx,
+                               (const double*)cy, (const double*)cz, (const double*)cst,
+                               (double*)Q);
+        else
             hipLaunchKernelGGL(k_shk_apply<float>, dim3(nb, ngroup), dim3(kRB), 0, stream, K,
                                Nv, task_nterm, task_slice, (const float*)task_coef, group_start,
                                group_slice, (const float*)u, (const float*)cx,
-                               (const float*)cy, (const float*)cz, (const double*)work, P,
+                               (const float*)cy, (const float*)cz, (const double*)cst,
                                (float*)Q);
-        }
         DGFS_HIP_CHECK(hipGetLastError());
         (void)Ne;
         return 0;

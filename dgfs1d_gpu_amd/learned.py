@@ -222,18 +222,21 @@ class LearnedClosure:
     ``sample`` of the learned nodes every ``check_every`` calls.  Before a
     node's correction is refreshed its prediction is judged against the
     spectral result: relative L2 error above ``tol`` (relative to
-    max(|Q|, the network's |Q| floor)) hands the node to the spectral operator
-    for the rest of the run.  ``use_net=False`` keeps the same schedule without
-    the network (Q_Shakov + C), the method the network is measured against.
-    The first ``start`` calls use the spectral operator for every node (the
-    initial transient, where the residual changes fastest).
+    max(|Q|, the network's |Q| floor)) hands the node to the spectral operator.
+    The spectral nodes are sampled the same way (their own round robin) and
+    keep a candidate correction from their previous sample: a node whose
+    model plus candidate correction is within ``tol * readmit`` of the
+    spectral result returns to the closure with a fresh correction.  ``use_net=False`` keeps the same
+    schedule without the network (Q_Shakov + C), the method the network is
+    measured against.  The first ``start`` calls use the spectral operator for
+    every node (the initial transient, where the residual changes fastest).
 
     nodes: velocity slices s with tasks (s -> s) of a nodal basis.  shakov:
     relax.Shakov bound to the grid; spectral: collision.CollisionPlan;
     prefac: the spectral operator's prefactor (scales R)."""
 
     def __init__(self, net, G, nslice, shakov, spectral, prefac, cw, device, check_every=10,
-                 sample=0.25, tol=0.05, seed=0, use_net=True, start=0):
+                 sample=0.25, tol=0.05, seed=0, use_net=True, start=0, readmit=0.5):
         import torch
         from .collision import task_table
         self.torch, self.net, self.G, self.ns = torch, net, int(G), int(nslice)
@@ -242,12 +245,18 @@ class LearnedClosure:
         self.use_net = use_net
         self.start = int(start)              # calls served by the spectral operator first
         self.task_table = task_table
+        self.readmit = float(readmit)
         self.learned = np.ones(self.ns, bool)
         self.calls = 0
         self.cursor = 0
+        self.cursor_s = 0
+        self.fallbacks = 0
+        self.readmissions = 0
+        self._qbuf = None
         self.checks = []                     # (call, {node: error}) history
         self._eng = {}
         self.C = None
+        self.Cc = None                       # candidate corrections of spectral nodes
         self._tabs = {}
         self._rebuild()
 
@@ -276,7 +285,7 @@ class LearnedClosure:
     def _scales(self, u, idx):
         """1/rho and out_scale prefac rho^2 per listed slice (rho = cw sum f)."""
         n = self.G ** 3
-        rho = u.view(self.ns, n).index_select(0, idx).double().sum(1) * self.cw
+        rho = u.view(self.ns, n).sum(1, dtype=self.torch.float64).index_select(0, idx) * self.cw
         osc = self.net.meta['out_scale'] * self.prefac * rho * rho
         return (1.0 / rho).float(), osc.float()
 
@@ -313,32 +322,62 @@ class LearnedClosure:
         if self.check_every and (self.calls - self.start) % self.check_every == 0 and self.nl:
             self.check(u)
 
-    def check(self, u):
-        """Judge and refresh a round-robin sample of the learned nodes."""
-        t = self.torch
-        cand = self.idx_np
+    def _buffers(self, u):
+        if self._qbuf is None or self._qbuf[0].numel() != u.numel():
+            self._qbuf = (self.torch.empty_like(u), self.torch.empty_like(u))
+        ql, qs = self._qbuf
+        ql.zero_()
+        qs.zero_()
+        return ql, qs
+
+    def _sample(self, cand, cursor):
         k = min(len(cand), max(1, int(round(self.sample * len(cand)))))
-        pick = np.sort(cand[(self.cursor + np.arange(k)) % len(cand)])
-        self.cursor = (self.cursor + k) % len(cand)
+        return cand[(cursor + np.arange(k)) % len(cand)], (cursor + k) % max(1, len(cand))
+
+    def check(self, u):
+        """Judge and refresh a round-robin sample of the learned nodes; offer
+        a round-robin sample of the spectral nodes their way back."""
+        t = self.torch
+        pl, ps = np.zeros(0, np.int64), np.zeros(0, np.int64)
+        if self.nl:
+            pl, self.cursor = self._sample(self.idx_np, self.cursor)
+        spec = np.nonzero(~self.learned)[0]
+        if len(spec) and self.readmit > 0:
+            ps, self.cursor_s = self._sample(spec, self.cursor_s)
+        pick = np.sort(np.concatenate([pl, ps])).astype(np.int32)
+        if not len(pick):
+            return np.zeros(0)
         idx = t.tensor(pick, device=self.device)
         n = self.G ** 3
-        ql = t.zeros_like(u)
+        ql, qs = self._buffers(u)
         self._model(u, ql, pick, idx, self._tab(pick, self.shakov.tdtype))
-        qs = t.zeros_like(u)
         self._spectral(u, qs, pick)
-        Cv = self.C.view(self.ns, n)
+        if self.Cc is None:
+            self.Cc = t.zeros_like(self.C)
+        Cv, Ccv = self.C.view(self.ns, n), self.Cc.view(self.ns, n)
+        was = self.learned[pick]
         a = ql.view(self.ns, n).index_select(0, idx).double()
         b = qs.view(self.ns, n).index_select(0, idx).double()
-        c = Cv.index_select(0, idx).double()
-        rho = u.view(self.ns, n).index_select(0, idx).double().sum(1) * self.cw
+        w = t.tensor(was, device=self.device).view(-1, 1)
+        c = t.where(w, Cv.index_select(0, idx), Ccv.index_select(0, idx)).double()
+        rho = u.view(self.ns, n).sum(1, dtype=t.float64).index_select(0, idx) * self.cw
         floor = self.net.meta['qfloor'] * self.net.meta['out_scale'] * self.prefac * rho * rho
         den = t.maximum(b.norm(dim=1), floor).clamp_min(1e-300)
         err = ((a + c - b).norm(dim=1) / den).cpu().numpy()
         self.checks.append((self.calls, dict(zip(pick.tolist(), err.tolist()))))
-        Cv.index_copy_(0, idx.long(), (b - a).to(self.C.dtype))  # refreshed corrections
-        bad = pick[err > self.tol]
-        if len(bad):
-            Cv.index_fill_(0, t.tensor(bad, device=self.device).long(), 0.0)
+        bad = pick[was & (err > self.tol)]
+        back = pick[~was & (err < self.tol * self.readmit)]
+        fresh = (b - a).to(self.C.dtype)
+        Cv.index_copy_(0, idx.long(), fresh)               # refreshed corrections
+        Ccv.index_copy_(0, idx.long(), fresh)              # candidates for the next sample
+        off = pick[~was & ~(err < self.tol * self.readmit)]
+        off = np.concatenate([bad, off])
+        if len(off):
+            Cv.index_fill_(0, t.tensor(off, device=self.device).long(), 0.0)
+        if len(bad) or len(back):
             self.learned[bad] = False
+            self.learned[back] = True
+            self.fallbacks += len(bad)
+            self.readmissions += len(back)
             self._rebuild()
         return err

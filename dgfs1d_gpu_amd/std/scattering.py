@@ -167,7 +167,8 @@ class DGFSVHSGLLScatteringModelStd(DGFSScatteringModelStd):
             tol=float(cfg.lookupordefault('scattering-model', 'closure-tol', 0.05)),
             use_net=cfg.lookupordefault('scattering-model', 'closure-use-net', 'true').lower()
             in ('1', 'true', 'yes'),
-            start=int(cfg.lookupordefault('scattering-model', 'closure-start', 0)))
+            start=int(cfg.lookupordefault('scattering-model', 'closure-start', 0)),
+            readmit=float(cfg.lookupordefault('scattering-model', 'closure-readmit', 0.5)))
 
     def _collide(self, u, Q, stream=None):
         if self.closure is not None:

@@ -65,8 +65,12 @@ def main(argv=None):
     ap.add_argument('--tol', type=float, default=0.05)
     ap.add_argument('--check-every', type=int, default=10)
     ap.add_argument('--sample', type=float, default=0.25)
-    ap.add_argument('--start', type=int, default=400,
-                    help='RHS calls on the spectral operator before the closure takes over')
+    ap.add_argument('--start', default='100',
+                    help='RHS calls on the spectral operator before the closure takes over '
+                         '(comma-separated: one learned run per value)')
+    ap.add_argument('--sparse', default=None, metavar='CHECK_EVERY:SAMPLE',
+                    help='also run the closure with and without the network at this sparser '
+                         'check schedule (first --start value)')
     ap.add_argument('--out', default=None)
     args = ap.parse_args(argv)
     import torch
@@ -78,21 +82,35 @@ def main(argv=None):
     _, spec, m_spec = run_to_steady(base, taus, args.max_steps, args.nres, dev, log)
     log('spectral: ' + json.dumps(spec))
     runs = {}
-    for tag, use_net in (('learned', True), ('shakov_corrected', False)):
+    starts = [int(x) for x in str(args.start).split(',')]
+    ce, sa = args.check_every, args.sample
+    plan = [('learned' if i == 0 else 'learned_start%d' % st, True, st, ce, sa)
+            for i, st in enumerate(starts)] + [('shakov_corrected', False, starts[0], ce, sa)]
+    if args.sparse:
+        ce2, sa2 = args.sparse.split(':')
+        plan += [('learned_sparse', True, starts[0], int(ce2), float(sa2)),
+                 ('shakov_corrected_sparse', False, starts[0], int(ce2), float(sa2))]
+    for tag, use_net, start, ce_, sa_ in plan:
         extra = ('closure = learned\nclosure-weights = %s\nclosure-check-every = %d\n'
                  'closure-tol = %g\nclosure-sample = %g\nclosure-use-net = %s\n'
                  'closure-start = %d\n'
-                 % (args.weights, args.check_every, args.tol, args.sample,
-                    'true' if use_net else 'false', args.start))
+                 % (args.weights, ce_, args.tol, sa_, 'true' if use_net else 'false', start))
         s, rep, m = run_to_steady(shock_deck(mach=args.mach, extra=extra), taus, args.max_steps,
                                   args.nres, dev, log)
         cl = s.sm.closure
         rep['nodes_learned_at_end'] = int(cl.learned.sum())
+        rep['fallbacks'] = int(cl.fallbacks)
+        rep['readmissions'] = int(cl.readmissions)
         rep['nodes_total'] = int(cl.ns)
         rep['checks'] = len(cl.checks)
         errs = np.array([e for _, d in cl.checks for e in d.values()])
         rep['check_err_median'] = float(np.median(errs)) if len(errs) else None
         rep['check_err_p90'] = float(np.percentile(errs, 90)) if len(errs) else None
+        rep['start'] = start
+        rep['check_every'] = ce_
+        rep['sample'] = sa_
+        rep['speedup_to_tau'] = {k: spec['reached'][k]['wall_s'] / v['wall_s']
+                                 for k, v in rep['reached'].items() if k in spec['reached']}
         log('%s: %s' % (tag, json.dumps(rep)))
         runs[tag] = (rep, m)
         del s
@@ -104,7 +122,22 @@ def main(argv=None):
     for k in lrn['reached']:
         if k in spec['reached']:
             speed[k] = spec['reached'][k]['wall_s'] / lrn['reached'][k]['wall_s']
+    scale = np.abs(m_spec[:, :4]).max(axis=0)          # rho, ux, uy, T scales
+    ref = {'rho': scale[0], 'ux': max(scale[1], scale[2]), 'uy': max(scale[1], scale[2]),
+           'T': scale[3]}
+    # q and P are compared relative to rho T^1.5 and rho T (their natural sizes)
+    rT = float(np.abs(m_spec[:, 0] * m_spec[:, 3]).max())
+    rT15 = float(np.abs(m_spec[:, 0] * np.abs(m_spec[:, 3]) ** 1.5).max())
+    for nm in ('qx', 'qy'):
+        ref[nm] = rT15
+    for nm in ('Pxx', 'Pyy', 'Pxy'):
+        ref[nm] = rT
+    dm_scaled = {nm: float(np.abs(m_lrn[:, i] - m_spec[:, i]).max() / ref[nm])
+                 for i, nm in enumerate(names)}
     out = dict(mach=args.mach, spectral=spec, learned=lrn,
+               other_runs={k: v[0] for k, v in runs.items()
+                           if k not in ('learned', 'shakov_corrected')},
+               moment_diff_scaled_learned_vs_spectral=dm_scaled,
                shakov_corrected_without_network=runs['shakov_corrected'][0],
                moment_rel_diff_learned_vs_spectral=dm, speedup_to_tau=speed,
                per_step_speedup=spec['ms_per_step'] / lrn['ms_per_step'])

@@ -138,3 +138,71 @@ def test_training_reduces_loss(cuda):
                  batch=4, epochs=15, lr=3e-3)
     print('training loss', hist[0], hist[-1])
     assert hist[-1] < 0.5 * hist[0]
+
+
+def _closure_solvers(cuda, tmp_path, **opts):
+    """Couette 2 x 3 nodes on 16^3 in FP32: one solver with the learned closure
+    (random network), one with the plain spectral operator."""
+    import os
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    from dgfs1d_gpu_amd.learned import ClosureNet
+    from dgfs1d_gpu_amd.std.std import DGFSStd1D
+    here = os.path.dirname(os.path.abspath(__file__))
+    base = open(os.path.join(here, '..', 'dgfs1d_gpu_amd', 'configs', 'couette_small.ini')).read()
+    base = base.replace('precision = double', 'precision = single')
+    w = str(tmp_path / 'net16.npz')
+    ClosureNet(cuda, seed=5).save(w)
+    o = dict(check_every=1000, sample=1.0, tol=0.05, readmit=0.5, start=0)
+    o.update(opts)
+    extra = ('closure = learned\nclosure-weights = %s\nclosure-check-every = %d\n'
+             'closure-sample = %g\nclosure-tol = %g\nclosure-readmit = %g\nclosure-start = %d\n'
+             % (w, o['check_every'], o['sample'], o['tol'], o['readmit'], o['start']))
+    text = base.replace('[scattering-model]\n', '[scattering-model]\n' + extra)
+    return (DGFSStd1D(Dictionary(text), device=cuda, writers=False),
+            DGFSStd1D(Dictionary(base), device=cuda, writers=False))
+
+
+def _perturbed(solver, seed=2):
+    import torch
+    g = torch.Generator(device='cpu').manual_seed(seed)
+    noise = 1.0 + 0.1 * (2 * torch.rand(solver.u.numel(), generator=g) - 1)
+    return (solver.u * noise.to(solver.u.device, solver.u.dtype)).contiguous()
+
+
+def test_closure_first_call_reproduces_spectral(cuda, tmp_path):
+    """The first call sets C = Q_spectral - (Q_Shakov + R_net) for every node,
+    so Q_Shakov + R_net + C equals the spectral collision term there."""
+    import torch
+    sc, ss = _closure_solvers(cuda, tmp_path)
+    u = _perturbed(sc)
+    qc, qs = torch.zeros_like(u), torch.zeros_like(u)
+    sc.sm.collide(u, qc)
+    ss.sm.collide(u, qs)
+    err = float((qc - qs).abs().max() / qs.abs().max())
+    print('closure first call vs spectral: %.3e' % err)
+    assert err < 1e-5
+
+
+def test_closure_fallback_and_readmission(cuda, tmp_path):
+    """A failed check hands every sampled node to the spectral operator (C
+    zeroed: the output is the spectral term); a check within tol * readmit
+    brings them back with a fresh correction."""
+    import torch
+    sc, ss = _closure_solvers(cuda, tmp_path, check_every=1, tol=-1.0)
+    cl = sc.sm.closure
+    u = _perturbed(sc)
+    qs = torch.zeros_like(u)
+    ss.sm.collide(u, qs)
+    q = torch.zeros_like(u)
+    sc.sm.collide(u, q)                  # first call, then a check that fails every node
+    assert cl.learned.sum() == 0 and cl.fallbacks == cl.ns
+    q.zero_()
+    sc.sm.collide(u, q)                  # all spectral now
+    assert float((q - qs).abs().max() / qs.abs().max()) < 1e-6
+    cl.tol = 1e9                         # every spectral node is within tol * readmit
+    cl.check(u)
+    assert cl.learned.sum() == cl.ns and cl.readmissions == cl.ns
+    cl.check_every = 0
+    q.zero_()
+    sc.sm.collide(u, q)                  # learned again, corrections fresh at this u
+    assert float((q - qs).abs().max() / qs.abs().max()) < 1e-5
