@@ -660,83 +660,245 @@ k_out_bwd(int G, long nvox, const float* __restrict__ dOut, const float* __restr
     }
 }
 
-// ------------------------------------------------ weight gradients (VALU)
-// part[chunk][tap][co][ci] = sum_{v in chunk} G[v][co] X[v + d(tap)][ci]
-// (G, X channels-last bf16 with 32 channels); block (chunk, tap), 64 threads,
-// each a 4x4 block of (co, ci).  Tap 13 (centre) also sums G for the bias.
-__global__ void __launch_bounds__(64)
-k_wgrad_mid(int G, int nb, int chunk, const __bf16* __restrict__ Gr, const __bf16* __restrict__ X,
+// ------------------------------------------------ weight gradients
+// ------------------------------------------- 32 -> 32 weight gradient, MFMA
+// dW[tap][co][ci] = sum_v Gr[v][co] X[v + d(tap)][ci] and db[co] = sum_v Gr[v][co]
+// over [nb][G^3][32] bf16 tensors.  A task is RB rows x G columns of one
+// plane z of one slice; persistent work-groups stage, per task, the task's
+// Gr tile and X planes z-1..z+1 (rows y0-1..y0+RB, zero halo) channel-major
+// in LDS: GT[co][voxel] and XT[ci][plane][row][column].
+// v_mfma_f32_16x16x32_bf16 with rows = co (16 jc + ..), columns = ci
+// (16 ic + ..) and K = 32 voxels of the tile: A lane (li, kq) =
+// GT[16 jc + li][k0 + 8kq .. +7] (one 16 B read, shared by every tap), B lane
+// (li, kq) = X at the same 8 voxels shifted by d(tap), channel 16 ic + li
+// (eight 2 B reads).  Wave w accumulates taps w, w+4, ... in FP32 over all of
+// its tasks; each work-group writes one partial dW and db (reduced afterwards
+// in a fixed order).
+template <int G>
+struct WgTile {
+    static constexpr int RB = G == 16 ? 8 : 4;             // rows per task
+    static constexpr int TV = RB * G;                     // tile voxels (multiple of 32)
+    static constexpr int NK = TV / 32;                    // K chunks
+    static constexpr int HR = RB + 2, HX = G + 2;
+    static constexpr int GTP = TV + 8;                    // GT row pitch (bf16)
+    static constexpr int XTP = 3 * HR * HX + 8;           // XT channel pitch (bf16)
+    static constexpr int NTASK = G * (G / RB);            // tasks per slice
+    static_assert(TV % 32 == 0 && G % RB == 0, "tile");
+};
+
+template <int G>
+__global__ void __launch_bounds__(256)
+k_wgrad_mma(int nb, const __bf16* __restrict__ Gr, const __bf16* __restrict__ X,
             float* __restrict__ part, float* __restrict__ bpart) {
-    const int c = blockIdx.x, tap = blockIdx.y, t = threadIdx.x;
-    const int co0 = (t >> 3) * 4, ci0 = (t & 7) * 4;
-    int dz, dy, dx;
-    tap_offset(tap, dz, dy, dx);
-    const int n = G * G * G;
-    const long total = long(nb) * n;
-    const long v0 = long(c) * chunk;
-    const long v1 = v0 + chunk < total ? v0 + chunk : total;
-    float acc[4][4] = {};
-    float bacc[4] = {0.f, 0.f, 0.f, 0.f};
-    for (long gv = v0; gv < v1; ++gv) {
-        const int b = int(gv / n), v = int(gv - long(b) * n);
-        const int x = v % G, y = (v / G) % G, z = v / (G * G);
-        const int zz = z + dz, yy = y + dy, xx = x + dx;
-        __bf16 gh[4];
-        const uint2 gw = *reinterpret_cast<const uint2*>(&Gr[size_t(gv) * kC + co0]);
-        __builtin_memcpy(gh, &gw, 8);
-        if (tap == 13)
-            for (int i = 0; i < 4; ++i) bacc[i] += bf2f(gh[i]);
-        if (zz < 0 || zz >= G || yy < 0 || yy >= G || xx < 0 || xx >= G) continue;
-        __bf16 xh[4];
-        const uint2 xw = *reinterpret_cast<const uint2*>(
-            &X[(size_t(b) * n + (zz * G + yy) * G + xx) * kC + ci0]);
-        __builtin_memcpy(xh, &xw, 8);
-        for (int i = 0; i < 4; ++i)
-            for (int k = 0; k < 4; ++k) acc[i][k] += bf2f(gh[i]) * bf2f(xh[k]);
+    using W = WgTile<G>;
+    constexpr int MT = (kTaps + 3) / 4;                   // taps per wave
+    __shared__ __attribute__((aligned(16))) __bf16 GT[kC * W::GTP];
+    __shared__ __attribute__((aligned(16))) __bf16 XT[kC * W::XTP];
+    __shared__ float sb[256 * 8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, li = lane & 15, kq = lane >> 4;
+    const size_t n = size_t(G) * G * G;
+    const int ntask = nb * W::NTASK;
+    const int cg = tid & 3;                               // channel group of this thread's loads
+    float bsum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    floatx4 acc[MT][2][2];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int jc = 0; jc < 2; ++jc)
+#pragma unroll
+            for (int ic = 0; ic < 2; ++ic) acc[m][jc][ic] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int b = task / W::NTASK, rr = task - b * W::NTASK;
+        const int z = rr / (G / W::RB), y0 = (rr % (G / W::RB)) * W::RB;
+        const __bf16* Gb = Gr + size_t(b) * n * kC;
+        const __bf16* Xb = X + size_t(b) * n * kC;
+        __syncthreads();                                  // previous task's reads are done
+        // Gr tile, transposed; bias partial sums (this thread's 8 channels)
+        for (int k = tid; k < W::TV * 4; k += 256) {
+            const int tv = k >> 2;
+            const int y = y0 + tv / G, x = tv % G;
+            const uint4 w = *reinterpret_cast<const uint4*>(
+                &Gb[((size_t(z) * G + y) * G + x) * kC + 8 * cg]);
+            __bf16 h[8];
+            __builtin_memcpy(h, &w, 16);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                GT[(8 * cg + e) * W::GTP + tv] = h[e];
+                bsum[e] += bf2f(h[e]);
+            }
+        }
+        // X planes z-1..z+1 with zero halo, transposed
+        for (int k = tid; k < 3 * W::HR * W::HX * 4; k += 256) {
+            const int vox = k >> 2;
+            const int pz = vox / (W::HR * W::HX), rem = vox - pz * (W::HR * W::HX);
+            const int r = rem / W::HX, c = rem - r * W::HX;
+            const int zz = z + pz - 1, y = y0 + r - 1, x = c - 1;
+            uint4 w = make_uint4(0u, 0u, 0u, 0u);
+            if (zz >= 0 && zz < G && y >= 0 && y < G && x >= 0 && x < G)
+                w = *reinterpret_cast<const uint4*>(
+                    &Xb[((size_t(zz) * G + y) * G + x) * kC + 8 * cg]);
+            __bf16 h[8];
+            __builtin_memcpy(h, &w, 16);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) XT[(8 * cg + e) * W::XTP + vox] = h[e];
+        }
+        __syncthreads();
+        bf16x8 a[W::NK][2];
+#pragma unroll
+        for (int kc = 0; kc < W::NK; ++kc)
+#pragma unroll
+            for (int jc = 0; jc < 2; ++jc)
+                a[kc][jc] = *reinterpret_cast<const bf16x8*>(
+                    &GT[(16 * jc + li) * W::GTP + kc * 32 + 8 * kq]);
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int tap = wave + 4 * m;
+            if (tap >= kTaps) break;
+            int dz, dy, dx;
+            tap_offset(tap, dz, dy, dx);
+#pragma unroll
+            for (int kc = 0; kc < W::NK; ++kc) {
+                // the lane's 8 voxels lie in one row: tile voxel kc*32 + 8kq + e
+                const int tv = kc * 32 + 8 * kq;
+                const int r = tv / G, x = tv % G;
+                const int off = ((dz + 1) * W::HR + r + dy + 1) * W::HX + x + dx + 1;
+#pragma unroll
+                for (int ic = 0; ic < 2; ++ic) {
+                    const __bf16* src = &XT[(16 * ic + li) * W::XTP + off];
+                    bf16x8 bv;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) bv[e] = src[e];
+#pragma unroll
+                    for (int jc = 0; jc < 2; ++jc)
+                        acc[m][jc][ic] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            a[kc][jc], bv, acc[m][jc][ic], 0, 0, 0);
+                }
+            }
+        }
     }
-    float* p = part + (size_t(c) * kTaps + tap) * kC * kC;
-    for (int i = 0; i < 4; ++i)
-        for (int k = 0; k < 4; ++k) p[(co0 + i) * kC + ci0 + k] = acc[i][k];
-    if (tap == 13 && ci0 == 0)
-        for (int i = 0; i < 4; ++i) bpart[size_t(c) * kC + co0 + i] = bacc[i];
+    // partial dW of this work-group: lane holds co = 16 jc + 4 kq + i, ci = 16 ic + li
+    float* p = part + size_t(blockIdx.x) * kTaps * kC * kC;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+        const int tap = wave + 4 * m;
+        if (tap >= kTaps) break;
+#pragma unroll
+        for (int jc = 0; jc < 2; ++jc)
+#pragma unroll
+            for (int ic = 0; ic < 2; ++ic)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    p[(size_t(tap) * kC + 16 * jc + 4 * kq + i) * kC + 16 * ic + li] =
+                        acc[m][jc][ic][i];
+    }
+    // bias partial: threads with the same channel group, in a fixed order
+#pragma unroll
+    for (int e = 0; e < 8; ++e) sb[tid * 8 + e] = bsum[e];
+    __syncthreads();
+    if (tid < kC) {
+        const int g = tid >> 3, e = tid & 7;
+        float s = 0.f;
+        for (int t = g; t < 256; t += 4) s += sb[t * 8 + e];
+        bpart[size_t(blockIdx.x) * kC + tid] = s;
+    }
 }
 
-// thin layers: part[chunk][tap][c] = sum_v g[v] x[v + d][c] where exactly one
-// operand has one channel.  GC1 = true: g is FP32 with one channel (layer 4:
-// g = dOut, x = Y3 bf16); false: g = dZ1 bf16 32 channels, x = the scaled
-// input f (FP32 one channel, layer 1).  Threads own one channel each.
-template <bool GC1>
-__global__ void __launch_bounds__(64)
-k_wgrad_thin(int G, int nb, int chunk, const float* __restrict__ g1, const __bf16* __restrict__ g32,
-             const float* __restrict__ x1, const __bf16* __restrict__ x32,
+// Thin layers (one side has one channel), tiled like k_wgrad_mma:
+//   GC1 = true  (layer 4): out[tap][c] = sum_v s[v] T[v + d][c], s = dOut
+//                          (FP32, the tile), T = Y3 (bf16, planes z-1..z+1)
+//   GC1 = false (layer 1): out[tap][c] = sum_v T[v][c] s[v + d], T = dZ1
+//                          (bf16, the tile), s = scaled input (FP32, halo)
+// and the bias partials (sum of dOut, or of dZ1 per channel).  Thread
+// (c = tid & 31, tq = tid >> 5) owns channel c and taps tq, tq+8, tq+16,
+// tq+24; partials part[wg][tap][c], bpart[wg][32] (GC1: bpart[wg]).
+template <int G, bool GC1>
+__global__ void __launch_bounds__(256)
+k_wgrad_thin(int nb, const float* __restrict__ s1, const __bf16* __restrict__ t32,
              float* __restrict__ part, float* __restrict__ bpart) {
-    const int c = blockIdx.x, tap = blockIdx.y, t = threadIdx.x;
-    if (t >= kC) return;
-    int dz, dy, dx;
-    tap_offset(tap, dz, dy, dx);
-    const int n = G * G * G;
-    const long total = long(nb) * n;
-    const long v0 = long(c) * chunk;
-    const long v1 = v0 + chunk < total ? v0 + chunk : total;
-    float acc = 0.f, bacc = 0.f;
-    for (long gv = v0; gv < v1; ++gv) {
-        const int b = int(gv / n), v = int(gv - long(b) * n);
-        const int x = v % G, y = (v / G) % G, z = v / (G * G);
-        const float gval = GC1 ? g1[gv] : bf2f(g32[size_t(gv) * kC + t]);
-        if (tap == 13) bacc += gval;
-        const int zz = z + dz, yy = y + dy, xx = x + dx;
-        if (zz < 0 || zz >= G || yy < 0 || yy >= G || xx < 0 || xx >= G) continue;
-        const size_t o = size_t(b) * n + (zz * G + yy) * G + xx;
-        const float xv = GC1 ? bf2f(x32[o * kC + t]) : x1[o];
-        acc += gval * xv;
+    using W = WgTile<G>;
+    constexpr int HV = 3 * W::HR * W::HX;                   // halo voxels
+    constexpr int TP = GC1 ? HV + 8 : W::TV + 8;            // channel pitch of T (bf16)
+    constexpr int SN = GC1 ? W::TV : HV;                    // scalar field size
+    __shared__ __attribute__((aligned(16))) __bf16 Ts[kC * TP];
+    __shared__ float Ss[SN];
+    const int tid = threadIdx.x, c = tid & 31, tq = tid >> 5;
+    const size_t n = size_t(G) * G * G;
+    const int ntask = nb * W::NTASK;
+    int toff[4];
+    bool tin[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int tap = tq + 8 * k;
+        tin[k] = tap < kTaps;
+        int dz = 0, dy = 0, dx = 0;
+        if (tin[k]) tap_offset(tap, dz, dy, dx);
+        toff[k] = ((dz + 1) * W::HR + dy + 1) * W::HX + dx + 1;   // halo offset of (0, 0, 0) + d
     }
-    part[(size_t(c) * kTaps + tap) * kC + t] = acc;
-    if (tap == 13) {
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    float bacc = 0.f;
+    for (int task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int b = task / W::NTASK, rr = task - b * W::NTASK;
+        const int z = rr / (G / W::RB), y0 = (rr % (G / W::RB)) * W::RB;
+        __syncthreads();
         if (GC1) {
-            if (t == 0) bpart[c] = bacc;               // one bias: [chunk]
+            for (int k = tid; k < W::TV; k += 256) {
+                const int y = y0 + k / G, x = k % G;
+                Ss[k] = s1[size_t(b) * n + (size_t(z) * G + y) * G + x];
+            }
+            for (int k = tid; k < HV * 4; k += 256) {
+                const int vox = k >> 2, cg = k & 3;
+                const int pz = vox / (W::HR * W::HX), rem = vox - pz * (W::HR * W::HX);
+                const int r = rem / W::HX, cc = rem - r * W::HX;
+                const int zz = z + pz - 1, y = y0 + r - 1, x = cc - 1;
+                uint4 w = make_uint4(0u, 0u, 0u, 0u);
+                if (zz >= 0 && zz < G && y >= 0 && y < G && x >= 0 && x < G)
+                    w = *reinterpret_cast<const uint4*>(
+                        &t32[(size_t(b) * n + (size_t(zz) * G + y) * G + x) * kC + 8 * cg]);
+                __bf16 h[8];
+                __builtin_memcpy(h, &w, 16);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) Ts[(8 * cg + e) * TP + vox] = h[e];
+            }
         } else {
-            bpart[size_t(c) * kC + t] = bacc;          // [chunk][32]
+            for (int k = tid; k < HV; k += 256) {
+                const int pz = k / (W::HR * W::HX), rem = k - pz * (W::HR * W::HX);
+                const int r = rem / W::HX, cc = rem - r * W::HX;
+                const int zz = z + pz - 1, y = y0 + r - 1, x = cc - 1;
+                float v = 0.f;
+                if (zz >= 0 && zz < G && y >= 0 && y < G && x >= 0 && x < G)
+                    v = s1[size_t(b) * n + (size_t(zz) * G + y) * G + x];
+                Ss[k] = v;
+            }
+            for (int k = tid; k < W::TV * 4; k += 256) {
+                const int tv = k >> 2, cg = k & 3;
+                const int y = y0 + tv / G, x = tv % G;
+                const uint4 w = *reinterpret_cast<const uint4*>(
+                    &t32[(size_t(b) * n + (size_t(z) * G + y) * G + x) * kC + 8 * cg]);
+                __bf16 h[8];
+                __builtin_memcpy(h, &w, 16);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) Ts[(8 * cg + e) * TP + tv] = h[e];
+            }
         }
+        __syncthreads();
+        for (int tv = 0; tv < W::TV; ++tv) {
+            const int r = tv / G, x = tv % G;
+            const int h0 = r * W::HX + x;                 // halo index of (r, x) minus the d offset
+            const float a = GC1 ? Ss[tv] : bf2f(Ts[c * TP + tv]);
+            if (tq == 0) bacc += a;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (tin[k]) acc[k] += a * (GC1 ? bf2f(Ts[c * TP + h0 + toff[k]]) : Ss[h0 + toff[k]]);
+        }
+    }
+    float* p = part + size_t(blockIdx.x) * kTaps * kC;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (tin[k]) p[(tq + 8 * k) * kC + c] = acc[k];
+    if (GC1) {
+        if (tid == 0) bpart[blockIdx.x] = bacc;           // the same sum in every channel lane
+    } else if (tq == 0) {
+        bpart[size_t(blockIdx.x) * kC + c] = bacc;
     }
 }
 
@@ -845,6 +1007,8 @@ using namespace dgfs;
 
 namespace {
 
+constexpr int kWgradMaxGridDev = 512;   // == kWgradMaxGrid (work buffer sizing)
+
 // compute units of the current device (persistent grids: one work-group per CU)
 int cu_count() {
     static int cached[64] = {0};
@@ -919,6 +1083,55 @@ int launch_forward(const dgfs_closure_t* net, int nb, const T* u, const int* src
     return 0;
 }
 
+// MFMA weight gradient of a 32 -> 32 layer into per-work-group partials at
+// work (dW partials, then db partials); ng receives the number of partials
+template <int G>
+int launch_wgrad(int nb, const __bf16* g, const __bf16* x, float* work, int& ng, hipStream_t st) {
+    const int ntask = nb * WgTile<G>::NTASK;
+    int cap = 2 * cu_count();
+    if (cap > kWgradMaxGridDev) cap = kWgradMaxGridDev;
+    ng = ntask < cap ? ntask : cap;
+    hipLaunchKernelGGL((k_wgrad_mma<G>), dim3(ng), dim3(256), 0, st, nb, g, x, work,
+                       work + size_t(ng) * kTaps * kC * kC);
+    DGFS_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <int G, bool GC1>
+int launch_wthin(int nb, const float* s1, const __bf16* t32, float* work, int& ng,
+                 hipStream_t st) {
+    const int ntask = nb * WgTile<G>::NTASK;
+    int cap = 2 * cu_count();
+    if (cap > kWgradMaxGridDev) cap = kWgradMaxGridDev;
+    ng = ntask < cap ? ntask : cap;
+    hipLaunchKernelGGL((k_wgrad_thin<G, GC1>), dim3(ng), dim3(256), 0, st, nb, s1, t32, work,
+                       work + size_t(ng) * kTaps * kC);
+    DGFS_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// thin-layer weight gradient partials (dW then db) at work; ng partials
+int wgrad_thin(int G, bool gc1, int nb, const float* s1, const void* t32, float* work, int& ng,
+               hipStream_t st) {
+    const __bf16* t = (const __bf16*)t32;
+#define DGFS_WT(GG)                                                             \
+    if (G == GG)                                                                \
+        return gc1 ? launch_wthin<GG, true>(nb, s1, t, work, ng, st)            \
+                   : launch_wthin<GG, false>(nb, s1, t, work, ng, st);
+    DGFS_WT(16)
+    DGFS_WT(32)
+    DGFS_WT(48)
+#undef DGFS_WT
+    DGFS_REQUIRE(false, "closure: grid size must be 16, 32 or 48");
+}
+
+int wgrad_mid(int G, int nb, const void* g, const void* x, float* work, int& ng, hipStream_t st) {
+    if (G == 16) return launch_wgrad<16>(nb, (const __bf16*)g, (const __bf16*)x, work, ng, st);
+    if (G == 32) return launch_wgrad<32>(nb, (const __bf16*)g, (const __bf16*)x, work, ng, st);
+    if (G == 48) return launch_wgrad<48>(nb, (const __bf16*)g, (const __bf16*)x, work, ng, st);
+    DGFS_REQUIRE(false, "closure: grid size must be 16, 32 or 48");
+}
+
 template <typename T>
 int dispatch_forward(int G, const dgfs_closure_t* net, int nb, const T* u, const int* src,
                      const float* scale, __bf16* A, size_t a, int mode, float* pred,
@@ -934,9 +1147,13 @@ int dispatch_forward(int G, const dgfs_closure_t* net, int nb, const T* u, const
 
 }  // namespace
 
+// most work-groups of the MFMA weight-gradient kernel (one partial each)
+constexpr int kWgradMaxGrid = kWgradMaxGridDev;
+
 extern "C" size_t dgfs_closure_work_bytes(int G, int nb, int chunk) {
     const long total = long(nb) * G * G * G;
-    const long nchunk = (total + chunk - 1) / chunk;
+    long nchunk = (total + chunk - 1) / chunk;
+    if (nchunk < kWgradMaxGrid) nchunk = kWgradMaxGrid;
     return size_t(nchunk) * (kTaps * kC * kC + kC) * sizeof(float);
 }
 
@@ -973,42 +1190,39 @@ extern "C" int dgfs_closure_backward(const dgfs_closure_t* net, const dgfs_closu
     if (nb == 0) return 0;
     const int n = G * G * G;
     const long nvox = long(nb) * n;
-    const long nch = (nvox + chunk - 1) / chunk;
-    DGFS_REQUIRE(nch <= 65535, "closure_backward: chunk too small for this batch");
+
     const size_t a = size_t(nvox) * kC;
     const __bf16* A = (const __bf16*)act;       // Z1 Y1 Z2 Y2 Z3 Y3
     __bf16* GA = (__bf16*)gact;                 // two gradient tensors (ping-pong)
     const dim3 gv((nvox + 255) / 256);
-    const int nc = int(nch);
-    float* part = work;
-    float* bpart = work + size_t(nc) * kTaps * kC * kC;
     // layer 4: dW4, db4 and dZ3
-    hipLaunchKernelGGL(k_wgrad_thin<true>, dim3(nc, kTaps), dim3(64), 0, st, G, nb, chunk, dOut,
-                       (const __bf16*)nullptr, (const float*)nullptr, A + 5 * a, part, bpart);
-    DGFS_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_chunks, dim3((kTaps * kC + 255) / 256), dim3(256), 0, st, nc,
-                       kTaps * kC, part, gr->W4);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(256), 0, st, nc, 1, bpart, gr->b4);
+    int ng = 0;
+    int rc0 = wgrad_thin(G, true, nb, dOut, A + 5 * a, work, ng, st);
+    if (rc0) return rc0;
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((kTaps * kC + 255) / 256), dim3(256), 0, st, ng,
+                       kTaps * kC, work, gr->W4);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(256), 0, st, ng, 1,
+                       work + size_t(ng) * kTaps * kC, gr->b4);
     DGFS_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_out_bwd, gv, dim3(256), 0, st, G, nvox, dOut, net->W4, A + 4 * a, GA);
     DGFS_HIP_CHECK(hipGetLastError());
     // layer 3: dW3 = dZ3 x Y2, dZ2 = silu'(Z2) convT(dZ3, W3)
-    hipLaunchKernelGGL(k_wgrad_mid, dim3(nc, kTaps), dim3(64), 0, st, G, nb, chunk, GA, A + 3 * a,
-                       part, bpart);
-    DGFS_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_chunks, dim3((kTaps * kC * kC + 255) / 256), dim3(256), 0, st, nc,
-                       kTaps * kC * kC, part, gr->W3);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(256), 0, st, nc, kC, bpart, gr->b3);
+    rc0 = wgrad_mid(G, nb, GA, A + 3 * a, work, ng, st);
+    if (rc0) return rc0;
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((kTaps * kC * kC + 255) / 256), dim3(256), 0, st, ng,
+                       kTaps * kC * kC, work, gr->W3);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(256), 0, st, ng, kC,
+                       work + size_t(ng) * kTaps * kC * kC, gr->b3);
     DGFS_HIP_CHECK(hipGetLastError());
     int rc = dispatch_mid(G, 2, nb, GA, net->W3t, nullptr, A + 2 * a, nullptr, GA + a, st);
     if (rc) return rc;
     // layer 2: dW2 = dZ2 x Y1, dZ1 = silu'(Z1) convT(dZ2, W2)
-    hipLaunchKernelGGL(k_wgrad_mid, dim3(nc, kTaps), dim3(64), 0, st, G, nb, chunk, GA + a, A + a,
-                       part, bpart);
-    DGFS_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_chunks, dim3((kTaps * kC * kC + 255) / 256), dim3(256), 0, st, nc,
-                       kTaps * kC * kC, part, gr->W2);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(256), 0, st, nc, kC, bpart, gr->b2);
+    rc0 = wgrad_mid(G, nb, GA + a, A + a, work, ng, st);
+    if (rc0) return rc0;
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((kTaps * kC * kC + 255) / 256), dim3(256), 0, st, ng,
+                       kTaps * kC * kC, work, gr->W2);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(256), 0, st, ng, kC,
+                       work + size_t(ng) * kTaps * kC * kC, gr->b2);
     DGFS_HIP_CHECK(hipGetLastError());
     rc = dispatch_mid(G, 2, nb, GA + a, net->W2t, nullptr, A, nullptr, GA, st);
     if (rc) return rc;
@@ -1020,12 +1234,12 @@ extern "C" int dgfs_closure_backward(const dgfs_closure_t* net, const dgfs_closu
         hipLaunchKernelGGL(k_gather_scaled<float>, gv, dim3(256), 0, st, n, nvox,
                            (const float*)u, src, scale, fin);
     DGFS_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_wgrad_thin<false>, dim3(nc, kTaps), dim3(64), 0, st, G, nb, chunk,
-                       (const float*)nullptr, GA, fin, (const __bf16*)nullptr, part, bpart);
-    DGFS_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_chunks, dim3((kTaps * kC + 255) / 256), dim3(256), 0, st, nc,
-                       kTaps * kC, part, gr->W1);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(256), 0, st, nc, kC, bpart, gr->b1);
+    rc0 = wgrad_thin(G, false, nb, fin, GA, work, ng, st);
+    if (rc0) return rc0;
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((kTaps * kC + 255) / 256), dim3(256), 0, st, ng,
+                       kTaps * kC, work, gr->W1);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(256), 0, st, ng, kC,
+                       work + size_t(ng) * kTaps * kC, gr->b1);
     DGFS_HIP_CHECK(hipGetLastError());
     return 0;
 }

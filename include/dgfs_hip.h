@@ -304,8 +304,10 @@ int dgfs_imex_ars_update(int dtype, int nslice, int Nv, const dgfs_ars_t* args, 
  * out[v] = sum_tap W[tap] . in[v + d(tap)].  Device weights: W1 [27][32] and
  * W4 [27][32] FP32; W2/W3 as bf16 copies W*f [27][32 out][32 in] and
  * tap-flipped transposes W*t [27][32 in][32 out] (dgfs_closure_pack from FP32
- * masters [27][32][32]); biases FP32.  The 32->32 layers use bf16 MFMA with
- * FP32 accumulation; activations are bf16 channels-last.
+ * masters [27][32][32]); biases FP32.  Every forward layer and the 32->32
+ * input and weight gradients use bf16 MFMA with FP32 accumulation (the scaled
+ * input, W1 and W4 are rounded to bf16 for the forward); activations are bf16
+ * channels-last.
  *
  * dgfs_closure_forward: input slice b is scale[b] * u[src[b]] (solver dtype);
  *   act = 6 * nb*G^3*32 bf16 (Z1 Y1 Z2 Y2 Z3 Y3, kept for the backward pass);
@@ -313,8 +315,9 @@ int dgfs_imex_ars_update(int dtype, int nslice, int Nv, const dgfs_ars_t* args, 
  *   mode 1: Q[dst[b]][v] += oscale[b] * output (solver dtype).
  * dgfs_closure_backward: gradients of sum_v dOut[b][v] * pred[b][v] with
  *   respect to every weight and bias (FP32, overwritten); gact = 2 * nb*G^3*32
- *   bf16, fin = nb*G^3 FP32, work = dgfs_closure_work_bytes(G, nb, chunk);
- *   chunk = voxels per partial weight-gradient block (fixed-order reduction).
+ *   bf16, fin = nb*G^3 FP32, work = dgfs_closure_work_bytes(G, nb, chunk)
+ *   (room for the per-work-group weight-gradient partials, reduced in a fixed
+ *   order; chunk >= 1 only sizes the buffer).
  * dgfs_closure_loss: loss[b] = |pred_b - target_b|^2 / max(|ref_b|^2, floor2) (FP64;
  *   ref = NULL means ref = target) and dOut = d(mean_b loss_b)/d(pred);
  *   work = dgfs_closure_loss_work_bytes.
