@@ -1,13 +1,16 @@
 """Training data and training for the learned collision closure.
 
-    python -m dgfs1d_gpu_amd.closure_train --out weights.npz [--machs 1.2,2,3]
-        [--temps 0.8,1.4,2.0] [--steps 120] [--every 10] [--epochs 8]
+    python -m dgfs1d_gpu_amd.closure_train --out weights.npz [--shocks 32]
+        [--couette 12] [--fourier 12] [--epochs 10] [--report r.json]
 
 Data: (f, R) pairs with R = Q_spectral(f) - Q_Shakov(f) taken from the FP32
-spectral solver on hard-sphere normal-shock decks (Rankine-Hugoniot states at
-the given upstream Mach numbers and temperatures, ``shock_deck``) and on
-Couette decks with random wall speeds; every ``every`` steps every node of the
-run is one pair.  Scaling: f / rho and R / (prefac rho^2) (see learned.py).
+spectral solver (hard-sphere gas, 32^3 velocity grid) on normal-shock decks
+with Rankine-Hugoniot states at random upstream Mach numbers in [1.2, 3] and
+temperatures in [0.8, 2] T0 (``shock_deck``), and on Couette (random wall
+speeds, equal wall temperatures) and Fourier (resting walls at two random
+temperatures) decks (``wall_deck``); every ``every`` steps every node of the
+run is one pair.  The pairs live in preallocated device buffers.  Scaling:
+f / rho and R / (prefac rho^2) (see learned.py).
 """
 import argparse
 import json
@@ -99,15 +102,17 @@ uz = 0
 """
 
 
-def couette_deck(uw=0.5, Tw=1.0, N=32, Ne=4, K=3, M=12, Nrho=16, precision='float', dt=5e-4):
-    """Couette deck of the production case with wall speed uw u0 and wall
-    temperature Tw T0 (hard-sphere gas so the closure sees one gas model)."""
+def wall_deck(uy_lo=-0.5, uy_hi=0.5, T_lo=1.0, T_hi=1.0, N=32, Ne=4, K=3, M=12, Nrho=16,
+              precision='float', dt=5e-4):
+    """Diffuse walls at x = -/+ 0.5 (H0 = 1 m) with wall speeds uy u0 and
+    wall temperatures T T0 (T0 = 273 K), gas initially at rest at T0
+    (hard-sphere gas, the closure's gas model)."""
     u0 = math.sqrt(2.0 * R0 / HELIUM * 273.0)
     return shock_deck(N=N, Ne=Ne, K=K, M=M, Nrho=Nrho, precision=precision)\
         .split('[soln-ics]')[0].replace('xlo = -15e-3', 'xlo = -0.5').replace(
             'xhi = 15e-3', 'xhi = 0.5').replace('H0 = 30e-3', 'H0 = 1.0').replace(
-            'T0 = 223.0', 'T0 = 273').replace('Tmax = 223.0', 'Tmax = 273').replace(
-            'dt = 0.0005', 'dt = %g' % dt) + f"""[soln-ics]
+            'T0 = 223.0', 'T0 = 273').replace('Tmax = 223.0', 'Tmax = %g' % (
+                273 * max(1.0, T_lo, T_hi))).replace('dt = 0.0005', 'dt = %g' % dt) + f"""[soln-ics]
 type = maxwellian
 rho = 1.91607e-5
 T = 273
@@ -116,22 +121,41 @@ uy = 0
 uz = 0
 [soln-bcs-xlo]
 type = dgfs-wall-diffuse
-T = {273 * Tw}
+T = {273 * T_lo}
 ux = 0
-uy = {-uw * u0}
+uy = {uy_lo * u0}
 uz = 0
 [soln-bcs-xhi]
 type = dgfs-wall-diffuse
-T = {273 * Tw}
+T = {273 * T_hi}
 ux = 0
-uy = {uw * u0}
+uy = {uy_hi * u0}
 uz = 0
 """
 
 
-def collect(text, steps, every, device, store, log=None):
-    """Run the spectral FP32 solver; every ``every`` steps append (f, R)
-    of all nodes, scaled, to store (lists of device tensors)."""
+def couette_deck(uw=0.5, Tw=1.0, **kw):
+    """Couette deck: walls moving at -/+ uw u0, both at Tw T0."""
+    return wall_deck(-uw, uw, Tw, Tw, **kw)
+
+
+def fourier_deck(T_lo=0.9, T_hi=1.1, **kw):
+    """Fourier heat-transfer deck: resting walls at T_lo T0 and T_hi T0."""
+    return wall_deck(0.0, 0.0, T_lo, T_hi, **kw)
+
+
+def deck_nodes(text):
+    """K * Ne of a deck."""
+    import configparser
+    cp = configparser.ConfigParser(inline_comment_prefixes=[';'])
+    cp.read_string(text)
+    return int(cp['basis']['K']) * int(cp['mesh']['Ne'])
+
+
+def collect(text, steps, every, device, buf, off, log=None):
+    """Run the spectral FP32 solver; every ``every`` steps write (f, R, Q)
+    of all nodes, scaled, into the preallocated buffers buf['f'], buf['t'],
+    buf['qs'] at rows off, off + 1, ...; returns the next free row."""
     import torch
     from .dictionary import Dictionary
     from .relax import Shakov
@@ -157,28 +181,33 @@ def collect(text, steps, every, device, store, log=None):
             Qk.zero_()
             sm.plan(s.u, Qs, tasks=tab)
             sk(s.u, Qk)
-            f = s.u.view(ns, n).float()
-            rho = f.double().sum(1) * vm.cw()
-            store['f'].append((f.double() / rho[:, None]).float())
-            store['t'].append(((Qs - Qk).view(ns, n).double() /
-                               (sm.prefac * rho * rho)[:, None]).float())
-            store['qs'].append((Qs.view(ns, n).double() / (sm.prefac * rho * rho)[:, None]).float())
+            f = s.u.view(ns, n).double()
+            rho = f.sum(1) * vm.cw()
+            den = (sm.prefac * rho * rho)[:, None]
+            buf['f'][off:off + ns].copy_(f / rho[:, None])
+            buf['t'][off:off + ns].copy_((Qs - Qk).view(ns, n).double() / den)
+            buf['qs'][off:off + ns].copy_(Qs.view(ns, n).double() / den)
+            off += ns
         if k < steps:
             s.step()
     if log:
-        log('collected %d snapshots x %d nodes' % (steps // every + 1, ns))
-    return s
+        log('collected %d snapshots x %d nodes (%d pairs so far)' % (steps // every + 1, ns, off))
+    return off
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--out', required=True)
-    ap.add_argument('--machs', default='1.2,1.6,2.0,2.5,3.0')
-    ap.add_argument('--temps', default='0.8,1.4,2.0')
-    ap.add_argument('--couette', type=int, default=4, help='random Couette decks')
-    ap.add_argument('--steps', type=int, default=120)
-    ap.add_argument('--every', type=int, default=10)
-    ap.add_argument('--epochs', type=int, default=8)
+    ap.add_argument('--shocks', type=int, default=32,
+                    help='normal-shock runs at random Mach in [1.2, 3] and T in [0.8, 2] T0')
+    ap.add_argument('--couette', type=int, default=12, help='random Couette runs')
+    ap.add_argument('--fourier', type=int, default=12, help='random Fourier runs')
+    ap.add_argument('--steps', type=int, default=1200, help='steps per shock run')
+    ap.add_argument('--every', type=int, default=24, help='snapshot interval (shock runs)')
+    ap.add_argument('--wall-ne', type=int, default=16, help='elements of the wall runs')
+    ap.add_argument('--wall-steps', type=int, default=780)
+    ap.add_argument('--wall-every', type=int, default=20)
+    ap.add_argument('--epochs', type=int, default=10)
     ap.add_argument('--batch', type=int, default=16)
     ap.add_argument('--lr', type=float, default=1e-3)
     ap.add_argument('--seed', type=int, default=0)
@@ -188,47 +217,61 @@ def main(argv=None):
     from .learned import ClosureNet, train
     dev = torch.device('cuda:0')
     log = lambda m: print(m, flush=True)
-    store = dict(f=[], t=[], qs=[])
-    t0 = time.perf_counter()
     rng = np.random.default_rng(args.seed)
-    for mach in [float(x) for x in args.machs.split(',')]:
-        for temp in [float(x) for x in args.temps.split(',')]:
-            collect(shock_deck(mach=mach, T1=223.0 * temp), args.steps, args.every, dev, store,
-                    log)
+    runs = []
+    for _ in range(args.shocks):
+        mach, temp = float(rng.uniform(1.2, 3.0)), float(rng.uniform(0.8, 2.0))
+        runs.append(('shock M=%.2f T=%.2f' % (mach, temp), shock_deck(mach=mach, T1=223.0 * temp),
+                     args.steps, args.every))
     for _ in range(args.couette):
-        collect(couette_deck(uw=float(rng.uniform(0.1, 1.0)), Tw=float(rng.uniform(0.8, 1.5))),
-                args.steps, args.every, dev, store, log)
-    f = torch.cat(store['f'])
-    tgt = torch.cat(store['t'])
-    qs = torch.cat(store['qs'])
+        uw, tw = float(rng.uniform(0.1, 1.0)), float(rng.uniform(0.8, 2.0))
+        runs.append(('couette uw=%.2f Tw=%.2f' % (uw, tw),
+                     couette_deck(uw=uw, Tw=tw, Ne=args.wall_ne), args.wall_steps,
+                     args.wall_every))
+    for _ in range(args.fourier):
+        tl, th = sorted(float(x) for x in rng.uniform(0.8, 2.0, 2))
+        runs.append(('fourier Tl=%.2f Th=%.2f' % (tl, th),
+                     fourier_deck(T_lo=tl, T_hi=th, Ne=args.wall_ne), args.wall_steps,
+                     args.wall_every))
+    S = sum((st // ev + 1) * deck_nodes(text) for _, text, st, ev in runs)
+    G = 32
+    n = G ** 3
+    log('%d runs, %d pairs, %.1f GB of device buffers' % (len(runs), S, 3 * S * n * 4 / 1e9))
+    buf = {k: torch.empty((S, n), dtype=torch.float32, device=dev) for k in ('f', 't', 'qs')}
+    t0 = time.perf_counter()
+    off = 0
+    for name, text, st, ev in runs:
+        log(name)
+        off = collect(text, st, ev, dev, buf, off, log)
+    assert off == S
+    torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
-    S = f.shape[0]
-    G = round(f.shape[1] ** (1.0 / 3.0))
-    perm = torch.tensor(rng.permutation(S), device=dev)
+    f, tgt, qs = buf['f'], buf['t'], buf['qs']
+    perm = rng.permutation(S)
     nval = max(1, S // 10)
-    val, tr = perm[:nval], perm[nval:]
+    val, tr = np.sort(perm[:nval]), perm[nval:]
     log('%d pairs (G=%d), %d for validation; data %.1f s' % (S, G, nval, t_data))
     # output scale: the RMS per-voxel magnitude of the scaled collision
     # integrals; |Q| floor: 10 % of their median norm (both in scaled units)
-    qn = qs.double().norm(dim=1)
-    sigma = float(qn.pow(2).mean().sqrt()) / math.sqrt(qs.shape[1])
+    qn = torch.cat([qs[i:i + 4096].double().norm(dim=1) for i in range(0, S, 4096)])
+    sigma = float(qn.pow(2).mean().sqrt()) / math.sqrt(n)
     qfloor = 0.1 * float(qn.median()) / sigma
     net = ClosureNet(dev, seed=args.seed, meta=dict(out_scale=sigma, qfloor=qfloor))
-    tn = (tgt / sigma).contiguous()
-    qsn = (qs / sigma).contiguous()
+    tgt.div_(sigma)
+    qs.div_(sigma)
     t1 = time.perf_counter()
-    hist = train(net, f.index_select(0, tr), tn.index_select(0, tr), G, batch=args.batch,
-                 epochs=args.epochs, lr=args.lr, seed=args.seed, log=log,
-                 ref=qsn.index_select(0, tr), floor2=qfloor ** 2)
+    hist = train(net, f, tgt, G, batch=args.batch, epochs=args.epochs, lr=args.lr,
+                 seed=args.seed, log=log, ref=qs, floor2=qfloor ** 2, rows=tr)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
     net.save(args.out)
-    rep = evaluate(net, f.index_select(0, val), tn.index_select(0, val),
-                   qsn.index_select(0, val), G, qfloor)
-    rep.update(pairs=S, grid=G, data_s=t_data, train_s=t_train, epochs=args.epochs,
-               loss_history=hist, out_scale=sigma, qfloor=qfloor,
-               inference=time_inference(net, f, G))
-    log(json.dumps(rep))
+    vi = torch.tensor(val, device=dev)
+    rep = evaluate(net, f.index_select(0, vi), tgt.index_select(0, vi), qs.index_select(0, vi),
+                   G, qfloor)
+    rep.update(pairs=S, train_pairs=int(len(tr)), grid=G, runs=[r[0] for r in runs],
+               data_s=t_data, train_s=t_train, epochs=args.epochs, loss_history=hist,
+               out_scale=sigma, qfloor=qfloor, inference=time_inference(net, f, G))
+    log(json.dumps({k: v for k, v in rep.items() if k != 'runs'}))
     if args.report:
         with open(args.report, 'w') as fh:
             json.dump(rep, fh)

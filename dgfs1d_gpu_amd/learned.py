@@ -173,14 +173,16 @@ class ClosureEngine:
 
 
 def train(net, f, target, G, batch=16, epochs=10, lr=1e-3, seed=0, log=None, ref=None,
-          floor2=0.0):
+          floor2=0.0, rows=None):
     """Adam on the relative-L2 loss over (f, target) pairs: f (S, G^3) FP32
     already scaled to unit density, target (S, G^3) FP32 scaled residuals;
     the error of each pair is measured relative to ref (S, G^3) when given
-    (the scaled collision integral), else to the target.  Returns the
-    per-epoch mean training loss."""
+    (the scaled collision integral), else to the target.  rows: the pair
+    indices to train on (default all; no copy of the data is made).  Returns
+    the per-epoch mean training loss."""
     import torch
-    S = f.shape[0]
+    rows = np.arange(f.shape[0]) if rows is None else np.asarray(rows)
+    S = len(rows)
     nb = min(batch, S)
     eng = ClosureEngine(net, G, nb)
     dev = net.device
@@ -192,7 +194,7 @@ def train(net, f, target, G, batch=16, epochs=10, lr=1e-3, seed=0, log=None, ref
     rb = torch.empty((nb, G ** 3), dtype=torch.float32, device=dev) if ref is not None else None
     hist = []
     for ep in range(epochs):
-        perm = torch.tensor(rng.permutation(S), device=dev)
+        perm = torch.tensor(rows[rng.permutation(S)], device=dev)
         tot = torch.zeros((), dtype=torch.float64, device=dev)
         cnt = 0
         for i in range(0, S - nb + 1, nb):
@@ -226,10 +228,11 @@ class LearnedClosure:
     The spectral nodes are sampled the same way (their own round robin) and
     keep a candidate correction from their previous sample: a node whose
     model plus candidate correction is within ``tol * readmit`` of the
-    spectral result returns to the closure with a fresh correction.  ``use_net=False`` keeps the same
-    schedule without the network (Q_Shakov + C), the method the network is
-    measured against.  The first ``start`` calls use the spectral operator for
-    every node (the initial transient, where the residual changes fastest).
+    spectral result returns to the closure with a fresh correction.
+    ``use_net=False`` keeps the same schedule without the network
+    (Q_Shakov + C), the method the network is measured against.  The first
+    ``start`` calls use the spectral operator for every node (the initial
+    transient, where the residual changes fastest).
 
     nodes: velocity slices s with tasks (s -> s) of a nodal basis.  shakov:
     relax.Shakov bound to the grid; spectral: collision.CollisionPlan;
