@@ -169,6 +169,36 @@ def closure_forward(nodes, G, reps, dev):
     return out
 
 
+def steady_state(dev, tau=1e-5, max_steps=4000):
+    """Second part of the metric: wall seconds until the reference residual
+    (checked every 10 steps) falls below tau on the Mach-2 hard-sphere normal
+    shock (Ne=32, K=3, N=32^3, M=12, N_rho=16, FP32, SSP-RK2 dt=5e-4), plain
+    spectral versus the learned closure (100-call spectral warm start, checks
+    every 10 calls on 25 % of the nodes, tolerance 0.05)."""
+    from dgfs1d_gpu_amd.closure_train import shock_deck
+    from dgfs1d_gpu_amd.steady import run_to_steady
+    weights = os.path.join(ROOT, 'dgfs1d_gpu_amd', 'data', 'closure_hs_n32.npz')
+    extra = ('closure = learned\nclosure-weights = %s\nclosure-check-every = 10\n'
+             'closure-tol = 0.05\nclosure-sample = 0.25\nclosure-start = 100\n' % weights)
+    out = {'workload': 'Mach-2 hard-sphere normal shock Ne=32 K=3 N=32^3 M=12 Nrho=16 FP32 '
+                       'SSP-RK2 dt=5e-4', 'tau': tau}
+    for tag, text in (('spectral', shock_deck(mach=2.0)),
+                      ('learned', shock_deck(mach=2.0, extra=extra))):
+        s, rep, _ = run_to_steady(text, [tau], max_steps, 10, dev)
+        hit = rep['reached'].get('%g' % tau)
+        out[tag] = {'wall_s': hit['wall_s'] if hit else None,
+                    'steps': hit['steps'] if hit else None,
+                    'ms_per_step': rep['ms_per_step']}
+        if tag == 'learned':
+            cl = s.sm.closure
+            out[tag].update(nodes_learned=int(cl.learned.sum()), nodes=int(cl.ns),
+                            fallbacks=int(cl.fallbacks), readmissions=int(cl.readmissions))
+        del s
+    if out['spectral']['wall_s'] and out['learned']['wall_s']:
+        out['speedup'] = out['spectral']['wall_s'] / out['learned']['wall_s']
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -182,6 +212,9 @@ def main(argv=None):
     ap.add_argument('--batch', action='store_true',
                     help='also run the seeded 64-problem uneven batch (autotuned, graph-captured)')
     ap.add_argument('--batch-seed', type=int, default=0)
+    ap.add_argument('--no-steady', action='store_true',
+                    help='skip the time-to-steady-state comparison (learned closure vs '
+                         'spectral, Mach-2 hard-sphere shock, residual 1e-5)')
     ap.add_argument('--closure-forward', type=int, default=0, metavar='NODES',
                     help='only time the learned-closure network forward on NODES slices of 32^3')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
@@ -275,9 +308,12 @@ def main(argv=None):
     }
     if not args.no_accuracy:
         out['accuracy'] = fp64_accuracy(text, args.warmup + args.steps, s, dev)
-    if args.batch:
+    if args.batch or not args.no_steady:
         del s
+    if args.batch:
         out['batched'] = batched_workload(args.batch_seed, 64, 20, args.warmup, dev)
+    if not args.no_steady:
+        out['time_to_steady_state'] = steady_state(dev)
     if not args.no_cpu_baseline:
         cores = min(16, os.cpu_count() or 1)
         out['cpu_baseline'] = cpu_baseline(text, cores)
