@@ -20,6 +20,19 @@ updates).
   part (:727), which holds P(f^{n+1}) only for s = 1; here the newest
   explicit part always subtracts P(f^{n+1}).  The reference also stops after
   its second step (:731); this implementation continues.
+* SSPL IMEX Runge-Kutta (sspl-222, sspl-443; :182-278): stage 0 implicit
+  only; explicit part = transport; the step ends with the weighted
+  combination f^{n+1} = f^n + dt sum_k _w_k L_k + dt sum_k w_k P_k
+  (updateDistributionWeight_SSPL), evaluated as collide + axnpby.
+* BDF-k (bdf-111, bdf-222, bdf-333; :281-488): explicit part = transport.
+  The update is the linear-multistep one with a = A/A_q, b = G/A_q (newest
+  first) and c = (B/A_q, 0, ...) (no penalisation terms), so the same
+  kernels evaluate it.  bdf-222/333 start with ARS-222 substeps (nstepsInit
+  per dt); the reference's start calls ARS-222's stage loop without its
+  tableau arguments (:361-363, :431-433) and cannot run, so the evident intent
+  is implemented.  bdf-333 advances two steps at its first call and none at
+  its second, its second substep loop at a fixed time (:436-447), as the
+  reference.
 """
 import numpy as np
 
@@ -103,6 +116,147 @@ class ARS443IntegratorAstd(ARSabcIntegratorAstd):
           [5. / 6., -5. / 6., 1. / 2., 0., 0.], [1. / 4., 7. / 4., 3. / 4., -7. / 4., 0.]]
     A = [[0., 0., 0., 0., 0.], [0., 1. / 2., 0., 0., 0.], [0., 1. / 6., 1. / 2., 0., 0.],
          [0., -1. / 2., 1. / 2., 1. / 2., 0.], [0., 3. / 2., -3. / 2., 1. / 2., 1. / 2.]]
+
+
+class SSPLabcIntegratorAstd(DGFSIntegratorAstd):
+    _A = A = _w = w = None
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        ns = self._ns = len(self._A)
+        self.L = [self._dist() for _ in range(ns)]
+        self.M = [self._dist() for _ in range(ns)]
+        self.F = [self._dist() for _ in range(ns)]
+        self.U = [self._mom() for _ in range(ns + 1)]
+        self.LU = [self._mom() for _ in range(ns)]
+        self.P = self._dist()
+        self.dL, self.dLU = self._dist(), self._mom()      # weight-0 stand-ins
+
+    def integrate(self, t, dt, nacptsteps, d_ucoeff):
+        sm, ns = self._sm, self._ns
+        L, M, U, LU = self.L, self.M, self.U, self.LU
+        F = [d_ucoeff] + self.F
+        sm.moment(t, F[0], U[0])
+        sm.updateMomentARS(dt, [0.0], [self.A[0][0]], [self.dLU], U[:2])
+        sm.constructMaxwellian(t, U[1], M[0])
+        sm.updateDistARS(dt, [0.0], [self.A[0][0]], [self.dL], U[:2], M[:1], F[:2])
+        for i in range(1, ns):
+            self._explicit(t, F[i], L[i - 1])
+            sm.moment(t, L[i - 1], LU[i - 1])
+            ah, ai = self._A[i][:i + 1], self.A[i][:i + 1]
+            sm.updateMomentARS(dt, ah, ai, LU[:i] + [self.dLU], U[:i + 2])
+            sm.constructMaxwellian(t, U[i + 1], M[i])
+            sm.updateDistARS(dt, ah, ai, L[:i] + [self.dL], U[:i + 2], M[:i + 1], F[:i + 2])
+        self._explicit(t, F[ns], L[ns - 1])
+        # f^{n+1} = f^n + dt sum_k (_w_k L_k + w_k nu(U_{k+1}) (M_k - F_{k+1}))
+        terms = [(self._w[k] * dt, L[k]) for k in range(ns) if self._w[k] != 0.0]
+        for i in range(0, len(terms), 3):
+            axnpby(d_ucoeff, terms[i:i + 3], 1.0)
+        for k in range(ns):
+            if self.w[k] != 0.0:
+                sm.collide(t, U[k + 1], M[k], F[k + 1], self.P)
+                axnpby(d_ucoeff, [(self.w[k] * dt, self.P)], 1.0)
+
+
+class SSPL222IntegratorAstd(SSPLabcIntegratorAstd):
+    intg_kind = 'sspl-222'
+    gamma = 1. - (2. ** 0.5) / 2.
+    _A = [[0., 0.], [1., 0.]]
+    A = [[gamma, 0.], [1 - 2 * gamma, gamma]]
+    _w = [0.5, 0.5]
+    w = [0.5, 0.5]
+
+
+class SSPL443IntegratorAstd(SSPLabcIntegratorAstd):
+    intg_kind = 'sspl-443'
+    alpha, beta, eta = 0.24169426078821, 0.06042356519705, 0.12915286960590
+    _A = [[0., 0., 0., 0.], [0., 0., 0., 0.], [0., 1., 0., 0.], [0., 1. / 4., 1. / 4., 0.]]
+    A = [[alpha, 0., 0., 0.], [-alpha, alpha, 0., 0.], [0., 1. - alpha, alpha, 0.],
+         [beta, eta, 1. / 2. - beta - eta - alpha, alpha]]
+    _w = [0., 1. / 6., 1. / 6., 2. / 3.]
+    w = [0., 1. / 6., 1. / 6., 2. / 3.]
+
+
+class BDFabcIntegratorAstd(DGFSIntegratorAstd):
+    A = G = B = None
+    nstepsInit = 100
+
+    def __init__(self, explicit, sm, shape, dtype, device=None, **kwargs):
+        super().__init__(explicit, sm, shape, dtype, device=device, **kwargs)
+        q = self._q = len(self.G)
+        Aq = self.A[q]
+        self._a = [self.A[q - 1 - k] / Aq for k in range(q)]
+        self._b = [self.G[q - 1 - k] / Aq for k in range(q)]
+        self._c = [self.B / Aq] + [0.0] * q
+        # history, oldest first; U[q] is the new moments, M the new target
+        self.F = [self._dist() for _ in range(q)]
+        self.L = [self._dist() for _ in range(q)]
+        self.U = [self._mom() for _ in range(q + 1)]
+        self.LU = [self._mom() for _ in range(q)]
+        self.Mn = self._dist()
+        self._ars = None
+        if q > 1:
+            kw = dict(kwargs)
+            kw['explicitQ'] = None
+            self._ars = ARS222IntegratorAstd(explicit, sm, shape, dtype, device=device, **kw)
+
+    def _record(self, t, f, k):
+        self._sm.moment(t, f, self.U[k])
+        self._explicit(t, f, self.L[k])
+        self._sm.moment(t, self.L[k], self.LU[k])
+        self.F[k].copy_(f)
+
+    def _substeps(self, t, dt, f, advance=True):
+        tloc, dtloc = t, dt / self.nstepsInit
+        for step in range(self.nstepsInit):
+            self._ars.integrate(tloc, dtloc, step, f)
+            if advance:
+                tloc += dtloc
+        return tloc
+
+    def _roll(self):
+        q = self._q
+        for v in (self.F, self.L, self.LU):
+            v.append(v.pop(0))
+        self.U[:q] = self.U[1:q] + self.U[:1]
+
+    def integrate(self, t, dt, nacptsteps, d_ucoeff):
+        sm, q = self._sm, self._q
+        if q >= 2 and nacptsteps == 0:
+            fn = d_ucoeff.clone()
+            tloc = self._substeps(t, dt, d_ucoeff)
+            self._record(t, fn, q - 1)
+            if q == 3:
+                fn.copy_(d_ucoeff)
+                self._substeps(tloc, dt, d_ucoeff, advance=False)
+                self._roll()
+                self._record(t, fn, q - 1)
+            return
+        if q == 3 and nacptsteps == 1:
+            return
+        self._roll()
+        self._record(t, d_ucoeff, q - 1)
+        # LU stands in for the (zero-weight) penalisation moments, F for the
+        # (zero-weight) penalisation registers
+        sm.updateMomentLM(dt, self._a, self._b, self._c, self.LU, self.LU, self.U, q)
+        sm.constructMaxwellian(t, self.U[q], self.Mn)
+        sm.updateDistLM(dt, self._a, self._b, self._c, self.L, self.F + [self.Mn], self.F,
+                        d_ucoeff, self.U, q)
+
+
+class Bdf111IntegratorAstd(BDFabcIntegratorAstd):
+    intg_kind = 'bdf-111'
+    A, G, B = [-1., 1.], [1.], 1.
+
+
+class Bdf222IntegratorAstd(BDFabcIntegratorAstd):
+    intg_kind = 'bdf-222'
+    A, G, B = [1. / 3., -4. / 3., 1.], [-2. / 3., 4. / 3.], 2. / 3.
+
+
+class Bdf333IntegratorAstd(BDFabcIntegratorAstd):
+    intg_kind = 'bdf-333'
+    A, G, B = [-2. / 11., 9. / 11., -18. / 11., 1.], [6. / 11., -18. / 11., 18. / 11.], 6. / 11.
 
 
 class IMEXLMsIntegratorAstd(DGFSIntegratorAstd):

@@ -9,8 +9,12 @@ Restates, per slice of a (nslice, n) array:
   collide              :266-305
   updateMom{s}_ARS     :326-440      updateDistribution{s}_ARS   :443-484
   updateMom{s}_LM      :548-641      updateDistribution{s}_LM    :644-691
+  updateMom{q}_BDF     :17-120       updateDistribution{q}_BDF   :123-159
+  updateDistributionWeight{s}_SSPL   :505-541
 and the integrators of dgfs1D/astd/integrator.py:
   ARS (a,b,c) stage loop             :48-95 with tables :100-178
+  SSPL stage loop                    :182-237 with tables :240-278
+  BDF-111/222/333                    :281-488
   IMEX linear multistep              :623-736 with tables :739-849
 Documented deviation (see the product's integrator): the newest explicit
 part of an s >= 2 multistep scheme subtracts the newest penalisation.
@@ -165,6 +169,54 @@ LM = {
 }
 
 
+# BDF-k in the penalisation form: A (q + 1), G (q), B (astd/integrator.py:285-
+# 287, :334-336, :405-407); the reference passes g_i = -G_i to the kernels.
+BDF = {
+    'bdf-111': ([-1., 1.], [1.], 1.),
+    'bdf-222': ([1. / 3., -4. / 3., 1.], [-2. / 3., 4. / 3.], 2. / 3.),
+    'bdf-333': ([-2. / 11., 9. / 11., -18. / 11., 1.], [6. / 11., -18. / 11., 18. / 11.],
+                6. / 11.),
+}
+
+
+def bdf_moments(dt, A, G, B, U, LU, prefac, omega, Pr):
+    """updateMom{q}_BDF: the conserved moments from the history, then the
+    stresses and (with the new stresses) the heat flux relaxed implicitly
+    with weight B dt nu(U_new)."""
+    q = len(G)
+    N = -sum(A[i] * U[i] - G[i] * dt * LU[i] for i in range(q))
+    aq = A[q]
+    Un = N / aq
+    T = temperature(Un)
+    w = (B * dt * prefac * Un[:, 0] * T ** (1.0 - omega))[:, None]
+    P, _ = _targets(Un, T, Pr)
+    Un[:, 5:11] = (N[:, 5:11] + w * P) / (aq + w)
+    _, Q = _targets(Un, T, Pr)
+    Un[:, 11:14] = (N[:, 11:14] + w * Q) / (aq + Pr * w)
+    return Un
+
+
+def bdf_dist(dt, A, G, B, F, L, Mn, Un, prefac, omega):
+    """updateDistribution{q}_BDF."""
+    q = len(G)
+    N = -sum(A[i] * F[i] - G[i] * dt * L[i] for i in range(q))
+    w = (B * dt * nu_of(Un, prefac, omega))[:, None]
+    return (N + w * Mn) / (A[q] + w)
+
+
+# SSPL tableaux (_A explicit, A implicit, _w, w) (astd/integrator.py:240-278)
+_gs = 1. - (2. ** 0.5) / 2.
+_al, _be, _et = 0.24169426078821, 0.06042356519705, 0.12915286960590
+SSPL = {
+    'sspl-222': ([[0., 0.], [1., 0.]], [[_gs, 0.], [1 - 2 * _gs, _gs]], [0.5, 0.5], [0.5, 0.5]),
+    'sspl-443': ([[0., 0., 0., 0.], [0., 0., 0., 0.], [0., 1., 0., 0.],
+                  [0., 1. / 4., 1. / 4., 0.]],
+                 [[_al, 0., 0., 0.], [-_al, _al, 0., 0.], [0., 1. - _al, _al, 0.],
+                  [_be, _et, 1. / 2. - _be - _et - _al, _al]],
+                 [0., 1. / 6., 1. / 6., 2. / 3.], [0., 1. / 6., 1. / 6., 2. / 3.]),
+}
+
+
 class Penal:
     """The penalisation model bound to a grid: prefac, omega, Pr, cv, cw."""
 
@@ -262,3 +314,82 @@ class LMIntegrator:
         self.L.append(L)
         self.LU.append(pm.moment(L))
         return fn
+
+
+def sspl_step(kind, pm, explicit, t, dt, f):
+    """One SSPL step (astd/integrator.py:182-237): stage 0 implicit only, the
+    explicit part of the stages is explicit(t, F_i), and
+    f^{n+1} = f^n + dt sum_k _w_k L_k + dt sum_k w_k nu(U_{k+1}) (M_{k+1} - F_{k+1})
+    (updateDistributionWeight_SSPL); M_k is the target of U_k."""
+    _A, A, _w, w = SSPL[kind]
+    ns = len(_A)
+    zero, zeroU = np.zeros_like(f), np.zeros((f.shape[0], NALPH))
+    F, U, M = [f], [pm.moment(f)], [None]
+    L, LU = [], []
+    U.append(ars_moments(dt, [0.0], [0.0, A[0][0]], [zeroU], U, 1, pm.prefac, pm.omega, pm.Pr))
+    M.append(pm.target(U[1]))
+    F.append(ars_dist(dt, [0.0], [0.0, A[0][0]], [zero], U, M, F, 1, pm.prefac, pm.omega))
+    for i in range(1, ns):
+        L.append(explicit(t, F[i]))
+        LU.append(pm.moment(L[-1]))
+        ah, ai = _A[i][:i + 1], [0.0] + A[i][:i + 1]
+        U.append(ars_moments(dt, ah, ai, LU + [zeroU], U, i + 1, pm.prefac, pm.omega, pm.Pr))
+        M.append(pm.target(U[i + 1]))
+        F.append(ars_dist(dt, ah, ai, L + [zero], U, M, F, i + 1, pm.prefac, pm.omega))
+    L.append(explicit(t, F[ns]))
+    out = F[0].copy()
+    for k in range(ns):
+        out = out + _w[k] * dt * L[k] + w[k] * dt * nu_of(U[k + 1], pm.prefac, pm.omega)[:, None] \
+            * (M[k + 1] - F[k + 1])
+    return out
+
+
+class BDFIntegrator:
+    """bdf-111/222/333 (astd/integrator.py:281-488); explicit part
+    explicit(t, f).  bdf-222/333 start with nsub ARS-222 substeps per dt (the
+    reference's start calls ARS-222 without its tableau, :361-363, :431-433,
+    and fails; this is the evident intent); bdf-333 advances two steps at its
+    first call and none at its second, with the second substep loop at a
+    fixed time (:436-447), as the reference."""
+    nsub = 100
+
+    def __init__(self, kind, pm, explicit):
+        self.A, self.G, self.B = BDF[kind]
+        self.q = len(self.G)
+        self.pm, self.explicit = pm, explicit
+        self.hist = []
+
+    def _rec(self, t, f):
+        U = self.pm.moment(f)
+        L = self.explicit(t, f)
+        return f.copy(), L, U, self.pm.moment(L)
+
+    def _ars(self, t, dt, f, advance=True):
+        tl = t
+        for _ in range(self.nsub):
+            f = ars_step('ars-222', self.pm, self.explicit, tl, dt / self.nsub, f, penalize=False)
+            if advance:
+                tl += dt / self.nsub
+        return f, tl
+
+    def step(self, t, dt, nacptsteps, f):
+        q, pm = self.q, self.pm
+        if q >= 2 and nacptsteps == 0:
+            fn = f.copy()
+            f, tl = self._ars(t, dt, f)
+            self.hist = [self._rec(t, fn)]
+            if q == 3:
+                fn = f.copy()
+                f, _ = self._ars(tl, dt, f, advance=False)
+                self.hist.append(self._rec(t, fn))
+            return f
+        if q == 3 and nacptsteps == 1:
+            return f
+        self.hist = (self.hist + [self._rec(t, f)])[-q:]
+        F = [h[0] for h in self.hist]
+        L = [h[1] for h in self.hist]
+        U = [h[2] for h in self.hist]
+        LU = [h[3] for h in self.hist]
+        Un = bdf_moments(dt, self.A, self.G, self.B, U, LU, pm.prefac, pm.omega, pm.Pr)
+        Mn = pm.target(Un)
+        return bdf_dist(dt, self.A, self.G, self.B, F, L, Mn, Un, pm.prefac, pm.omega)

@@ -121,3 +121,40 @@ def test_lm_steps(cuda, scheme, model):
         err = np.abs(s.u.cpu().numpy().reshape(f.shape) - f).max() / np.abs(f).max()
         print(scheme, model, k, err)
         assert err < 1e-11
+
+
+@pytest.mark.parametrize('scheme', ['sspl-222', 'sspl-443'])
+def test_sspl_steps(cuda, scheme):
+    text = _text(scheme)
+    deck = Deck(text)
+    s = _solver(cuda, text)
+    pm, explicit, _ = _oracle(deck)
+    f = deck.u0.reshape(s.K * s.Ne, -1).copy()
+    t = 0.0
+    for k in range(3):
+        f = OI.sspl_step(scheme, pm, explicit, t, deck.dt, f)
+        s.step()
+        t += deck.dt
+    err = np.abs(s.u.cpu().numpy().reshape(f.shape) - f).max() / np.abs(f).max()
+    moved = np.abs(f - deck.u0.reshape(f.shape)).max() / np.abs(f).max()
+    print(scheme, err, moved)
+    assert err < 1e-11 and moved > 1e-6
+
+
+@pytest.mark.parametrize('scheme', ['bdf-111', 'bdf-222', 'bdf-333'])
+def test_bdf_steps(cuda, scheme):
+    text = _text(scheme)
+    deck = Deck(text)
+    s = _solver(cuda, text)
+    pm, explicit, _ = _oracle(deck)
+    it = OI.BDFIntegrator(scheme, pm, explicit)
+    f = deck.u0.reshape(s.K * s.Ne, -1).copy()
+    t = 0.0
+    for k in range(4):
+        f = it.step(t, deck.dt, k, f)
+        s.step()
+        t += deck.dt
+        err = np.abs(s.u.cpu().numpy().reshape(f.shape) - f).max() / np.abs(f).max()
+        moved = np.abs(f - deck.u0.reshape(f.shape)).max() / np.abs(f).max()
+        print(scheme, k, err, moved)
+        assert err < 1e-11 and moved > 1e-6

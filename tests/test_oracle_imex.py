@@ -123,3 +123,66 @@ def test_planted_lm_coefficient_fails():
         assert not np.log2(errs[0] / errs[1]) > 1.7
     finally:
         OI.LM['imex-lm-bdf2'] = saved
+
+
+@pytest.mark.parametrize('kind,order', [('sspl-222', 2), ('sspl-443', 3)])
+def test_sspl_order(kind, order):
+    pm, explicit, exact, f0, nu = _relax_problem()
+    tend = 0.4 / nu
+    errs = []
+    for nsteps in (8, 16):
+        dt, f = tend / nsteps, f0.copy()
+        for k in range(nsteps):
+            f = OI.sspl_step(kind, pm, explicit, k * dt, dt, f)
+        errs.append(np.abs(f - exact(tend)).max())
+    rate = np.log2(errs[0] / errs[1])
+    print(kind, errs, rate)
+    assert rate > order - 0.3
+
+
+@pytest.mark.parametrize('kind,order', [('bdf-111', 1), ('bdf-222', 2), ('bdf-333', 3)])
+def test_bdf_order(kind, order):
+    pm, explicit, exact, f0, nu = _relax_problem()
+    tend = 0.5 / nu
+    errs = []
+    for nsteps in (16, 32):
+        dt = tend / nsteps
+        it = OI.BDFIntegrator(kind, pm, explicit)
+        it.nsub = 20
+        f = f0.copy()
+        for k in range(nsteps):
+            f = it.step(k * dt, dt, k, f)
+        errs.append(np.abs(f - exact(tend)).max())
+    rate = np.log2(errs[0] / errs[1])
+    print(kind, errs, rate)
+    assert rate > order - 0.3
+
+
+@pytest.mark.parametrize('kind', ['bdf-111', 'bdf-222', 'bdf-333'])
+def test_bdf_is_a_linear_multistep_update(kind):
+    """The BDF-k update equals the linear-multistep update with
+    a = A/A_q, b = G/A_q (newest first) and c = (B/A_q, 0, ...) and no
+    penalisation terms: the mapping the product's bdf integrators use."""
+    rng = np.random.default_rng(3)
+    cv, cw = grid(8, 5.0)
+    pm = OI.Penal(cv, cw, prefac=0.8, omega=0.7, Pr=2. / 3.)
+    A, G, B = OI.BDF[kind]
+    q = len(G)
+    f = [maxw(cv, 1.0 + 0.1 * k, (0.1 * k, -0.05, 0.0), 0.9 + 0.05 * k)[None, :] *
+         (1 + 0.05 * rng.uniform(-1, 1, cv.shape[1])) for k in range(q)]
+    L = [0.1 * rng.uniform(-1, 1, f[0].shape) * f[0] for _ in range(q)]
+    U = [pm.moment(x) for x in f]
+    LU = [pm.moment(x) for x in L]
+    dt = 0.05
+    a = [A[q - 1 - k] / A[q] for k in range(q)]
+    b = [G[q - 1 - k] / A[q] for k in range(q)]
+    c = [B / A[q]] + [0.0] * q
+    zeroU = [np.zeros_like(x) for x in U]
+    Ub = OI.bdf_moments(dt, A, G, B, U, LU, pm.prefac, pm.omega, pm.Pr)
+    Ul = OI.lm_moments(dt, a, b, c, LU, zeroU, U, q, pm.prefac, pm.omega, pm.Pr)
+    assert np.abs(Ub - Ul).max() < 1e-13 * np.abs(Ub).max()
+    Mn = pm.target(Ub)
+    fb = OI.bdf_dist(dt, A, G, B, f, L, Mn, Ub, pm.prefac, pm.omega)
+    fl = OI.lm_dist(dt, a, b, c, L, [np.zeros_like(x) for x in f], f, Mn, Ub, q, pm.prefac,
+                    pm.omega)
+    assert np.abs(fb - fl).max() < 1e-13 * np.abs(fb).max()
