@@ -24,6 +24,8 @@ updates).
   only; explicit part = transport; the step ends with the weighted
   combination f^{n+1} = f^n + dt sum_k _w_k L_k + dt sum_k w_k P_k
   (updateDistributionWeight_SSPL), evaluated as collide + axnpby.
+* boltz-ars-111 (:491-550): BDF-111 with the explicit part transport + Q - P
+  for the distribution (the moments use the transport part alone).
 * BDF-k (bdf-111, bdf-222, bdf-333; :281-488): explicit part = transport.
   The update is the linear-multistep one with a = A/A_q, b = G/A_q (newest
   first) and c = (B/A_q, 0, ...) (no penalisation terms), so the same
@@ -257,6 +259,28 @@ class Bdf222IntegratorAstd(BDFabcIntegratorAstd):
 class Bdf333IntegratorAstd(BDFabcIntegratorAstd):
     intg_kind = 'bdf-333'
     A, G, B = [-2. / 11., 9. / 11., -18. / 11., 1.], [6. / 11., -18. / 11., 18. / 11.], 6. / 11.
+
+
+class BoltzARS111IntegratorAstd(Bdf111IntegratorAstd):
+    """boltz-ars-111 (astd/integrator.py:491-550): BDF-111 whose explicit
+    part for the distribution is transport + Q(f) - P(f); the moment update
+    uses the moments of the transport part alone, taken before Q - P is
+    added (:531-537), as the reference."""
+    intg_kind = 'boltz-ars-111'
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        if self._explicitQ is None:
+            raise ValueError('boltz-ars-111 needs the Boltzmann operator (explicitQ)')
+        self._Q, self._M0 = self._dist(), self._dist()
+
+    def _record(self, t, f, k):
+        super()._record(t, f, k)
+        sm = self._sm
+        sm.constructMaxwellian(t, self.U[k], self._M0)
+        sm.collide(t, self.U[k], self._M0, f, self._M0)          # P(f)
+        self._explicitQ(t, f, self._Q)
+        axnpby(self.L[k], [(1.0, self._Q), (-1.0, self._M0)], 1.0)
 
 
 class IMEXLMsIntegratorAstd(DGFSIntegratorAstd):

@@ -353,16 +353,22 @@ class BDFIntegrator:
     fixed time (:436-447), as the reference."""
     nsub = 100
 
-    def __init__(self, kind, pm, explicit):
+    def __init__(self, kind, pm, explicit, boltz=None):
+        """boltz: with kind 'bdf-111', the boltz-ars-111 scheme
+        (astd/integrator.py:491-550): the distribution's explicit part is
+        explicit + boltz - P(f), its moments those of explicit alone."""
         self.A, self.G, self.B = BDF[kind]
         self.q = len(self.G)
-        self.pm, self.explicit = pm, explicit
+        self.pm, self.explicit, self.boltz = pm, explicit, boltz
         self.hist = []
 
     def _rec(self, t, f):
         U = self.pm.moment(f)
         L = self.explicit(t, f)
-        return f.copy(), L, U, self.pm.moment(L)
+        LU = self.pm.moment(L)
+        if self.boltz is not None:
+            L = L + self.boltz(t, f) - self.pm.penal(U, f)
+        return f.copy(), L, U, LU
 
     def _ars(self, t, dt, f, advance=True):
         tl = t

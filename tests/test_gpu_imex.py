@@ -158,3 +158,21 @@ def test_bdf_steps(cuda, scheme):
         moved = np.abs(f - deck.u0.reshape(f.shape)).max() / np.abs(f).max()
         print(scheme, k, err, moved)
         assert err < 1e-11 and moved > 1e-6
+
+
+def test_boltz_ars111_steps(cuda):
+    text = _text('boltz-ars-111')
+    deck = Deck(text)
+    s = _solver(cuda, text)
+    pm, explicit, boltz = _oracle(deck)
+    it = OI.BDFIntegrator('bdf-111', pm, explicit, boltz)
+    f = deck.u0.reshape(s.K * s.Ne, -1).copy()
+    t = 0.0
+    for k in range(3):
+        f = it.step(t, deck.dt, k, f)
+        s.step()
+        t += deck.dt
+        err = np.abs(s.u.cpu().numpy().reshape(f.shape) - f).max() / np.abs(f).max()
+        moved = np.abs(f - deck.u0.reshape(f.shape)).max() / np.abs(f).max()
+        print('boltz-ars-111', k, err, moved)
+        assert err < 1e-11 and moved > 1e-6
