@@ -139,3 +139,50 @@ def test_rankine_hugoniot_limits():
     assert abs(r * u - 1.0) < 1e-14
     m2 = 4.0
     assert abs((1 + g * m2) - (p + g * m2 * r * u * u)) < 1e-12
+
+
+def _dg_file(path, fun, K, Ne, Nv=3, L=1.0):
+    """Nodal DG coefficients (GLL nodes) of fun(x, v) on Ne equal elements of [0, L]."""
+    from dgfs1d_gpu_amd.quadratures import zwglj
+    r, _ = zwglj(K, 0.0, 0.0)
+    h = L / Ne
+    x = (np.arange(Ne)[None, :] + 0.5 * (r[:, None] + 1.0)) * h          # (K, Ne)
+    coeff = np.stack([fun(x, v) for v in range(Nv)], axis=-1)           # (K, Ne, Nv)
+    np.savez(path, coeff=coeff.ravel(), time=0.0, K=K, Ne=Ne, Nv=Nv)
+
+
+def test_error_tool_polynomial_is_exact(tmp_path):
+    """A polynomial of degree < K is represented exactly on both meshes."""
+    from dgfs1d_gpu_amd.error import compute_error
+    fun = lambda x, v: 1.0 + (v + 1) * x - 0.7 * x ** 2
+    _dg_file(tmp_path / 'c.npz', fun, 3, 4)
+    _dg_file(tmp_path / 'f.npz', fun, 3, 8)
+    assert compute_error(tmp_path / 'c.npz', tmp_path / 'f.npz') < 1e-14
+
+
+def test_error_tool_converges_and_measures_l1(tmp_path):
+    """For a smooth function the coarse/fine difference falls like h^K; for
+    fine data shifted by a constant it equals that constant (mean L1 over
+    the reference element's weights, which sum to 2, times 0.5)."""
+    from dgfs1d_gpu_amd.error import compute_error
+    fun = lambda x, v: np.sin(2 * np.pi * x + v)
+    errs = []
+    for Ne in (4, 8):
+        _dg_file(tmp_path / 'c.npz', fun, 3, Ne)
+        _dg_file(tmp_path / 'f.npz', fun, 3, 2 * Ne)
+        errs.append(compute_error(tmp_path / 'c.npz', tmp_path / 'f.npz'))
+    rate = np.log2(errs[0] / errs[1])
+    assert rate > 2.5, (errs, rate)
+    _dg_file(tmp_path / 'c.npz', fun, 3, 4)
+    _dg_file(tmp_path / 'f.npz', lambda x, v: fun(x, v) + 0.25, 3, 8)
+    e = compute_error(tmp_path / 'c.npz', tmp_path / 'f.npz')
+    assert abs(e - 0.25) < 1e-3
+
+
+def test_error_tool_rejects_mismatched_meshes(tmp_path):
+    from dgfs1d_gpu_amd.error import compute_error
+    fun = lambda x, v: x
+    _dg_file(tmp_path / 'c.npz', fun, 3, 4)
+    _dg_file(tmp_path / 'f.npz', fun, 3, 6)
+    with pytest.raises(ValueError):
+        compute_error(tmp_path / 'c.npz', tmp_path / 'f.npz')
