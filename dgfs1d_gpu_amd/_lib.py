@@ -84,6 +84,11 @@ SIGNATURES = {
     'dgfs_collide': (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _i,
                           _vp]),
     'dgfs_collision_timing': (_i, [_vp, _i]),
+    'dgfs_bi_collision_plan_create': (_i, [_i, _i, _i, _d, _dp, _dp, _dp, _d, _i, _dp, _d, _d,
+                                           _d, _d, _d, _i, _i, C.POINTER(_vp)]),
+    'dgfs_bi_collision_plan_destroy': (_i, [_vp]),
+    'dgfs_bi_collision_plan_query': (_i, [_vp, _ip, _ip, _szp]),
+    'dgfs_bi_collide': (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
     'dgfs_collision_timing_read': (_i, [_vp, _dp, _ip]),
     'dgfs_transport_work_bytes': (_sz, [_i]),
     'dgfs_transport_rhs': (_i, [C.POINTER(Transport), _vp, _vp, _d, _vp]),

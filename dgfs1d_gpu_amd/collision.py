@@ -148,3 +148,78 @@ class CollisionPlan:
             self.close()
         except Exception:
             pass
+
+
+class BiCollisionPlan:
+    """One species pair (p, q) of the binary-mixture VSS operator
+    (reference DGFSBiVSSScatteringModel, bi/scattering.py:47-321) on a
+    ``dgfs_bi_collision_plan_t``: the complex kernels b_pq, c_pq are built on
+    the device at creation.  ``__call__(up, uq, Q)`` adds Q_pq(f_p, f_q) of
+    every task to Q; the task table is that of CollisionPlan (f terms
+    contract up, g terms contract uq)."""
+
+    def __init__(self, N, M, Nrho, L, qz, qw, sz, sw, szpre, mp, mq, alpha, omega, prefac,
+                 dtype, ntask_max, device='cuda'):
+        import torch
+        self.N, self.M, self.Nrho = int(N), int(M), int(Nrho)
+        self.n = self.N ** 3
+        self.dtype = dtype
+        self.tdtype = torch.float64 if dtype == _lib.F64 else torch.float32
+        self.prefac = float(prefac)
+        self.device = device
+        qz_a, qz_p = _lib.dbl_array(qz)
+        qw_a, qw_p = _lib.dbl_array(qw)
+        sz_a, sz_p = _lib.dbl_array(np.asarray(sz).reshape(-1, 3))
+        sp_a, sp_p = _lib.dbl_array(np.asarray(szpre).reshape(-1, 3))
+        if sz_a.shape[0] != self.M or len(qz_a) != self.Nrho or len(qw_a) != self.Nrho:
+            raise ValueError('inconsistent quadrature table sizes')
+        h = _lib.C.c_void_p()
+        _lib.check(_lib.lib().dgfs_bi_collision_plan_create(
+            self.N, self.M, self.Nrho, float(L), qz_p, qw_p, sz_p, float(sw), sp_a.shape[0], sp_p,
+            float(mp), float(mq), float(alpha), float(omega), self.prefac, dtype, int(ntask_max),
+            _lib.C.byref(h)))
+        self._h = h
+        self.ntask_max = int(ntask_max)
+        self.tasks = None
+
+    def query(self):
+        s, ch = _lib.C.c_int(), _lib.C.c_int()
+        b = _lib.C.c_size_t()
+        _lib.check(_lib.lib().dgfs_bi_collision_plan_query(
+            self._h, _lib.C.byref(s), _lib.C.byref(ch), _lib.C.byref(b)))
+        return dict(split=s.value, chunk=ch.value, scratch_bytes=b.value)
+
+    def set_tasks(self, K, Ne, tasks):
+        if len(tasks) > self.ntask_max:
+            raise ValueError('too many collision tasks (%d > %d)' % (len(tasks), self.ntask_max))
+        self.tasks = task_table(K, Ne, tasks, self.tdtype, self.device)
+        return self.tasks
+
+    def __call__(self, up, uq, Q, stream=None, tasks=None):
+        """Q += Q_pq(up, uq) for every task (stream-ordered, no host sync)."""
+        t = self.tasks if tasks is None else tasks
+        if t is None:
+            raise RuntimeError('set_tasks() must be called first')
+        if t['ntask'] > self.ntask_max:
+            raise ValueError('too many collision tasks (%d > %d)' % (t['ntask'], self.ntask_max))
+        for x in (up, uq, Q):
+            if x.dtype != self.tdtype:
+                raise TypeError('collision operands must be %s' % self.tdtype)
+        n = t['K'] * t['Ne'] * self.n
+        if min(up.numel(), uq.numel(), Q.numel()) < n:
+            raise ValueError('collision operands smaller than K*Ne*Nv')
+        _lib.check(_lib.lib().dgfs_bi_collide(
+            self._h, _lib.ptr(up), _lib.ptr(uq), _lib.ptr(Q), t['K'], t['Ne'], t['ntask'],
+            _lib.ptr(t['nterm']), _lib.ptr(t['slice']), _lib.ptr(t['coef']), t['ngroup'],
+            _lib.ptr(t['gstart']), _lib.ptr(t['gslice']), _lib.stream_ptr(stream)))
+
+    def close(self):
+        if getattr(self, '_h', None) is not None and self._h.value:
+            _lib.lib().dgfs_bi_collision_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,5 @@
+// double N=20 instance of the binary-mixture VSS kernels (see bivss_kernels.h).
+#include "bivss_kernels.h"
+namespace dgfs {
+DGFS_BI_INST(double, 20)
+}

@@ -97,6 +97,40 @@ int dgfs_collide(dgfs_collision_plan_t plan, const void* u, const void* ug, void
 int dgfs_collision_timing(dgfs_collision_plan_t plan, int max_calls);
 int dgfs_collision_timing_read(dgfs_collision_plan_t plan, double* ms /*[5]*/, int* ncalls);
 
+/* ------------------------------------------------- binary-mixture collision
+ * Replaces DGFSBiVSSScatteringModel.perform_precomputation for one species
+ * pair (p, q) (dgfs1D/bi/scattering.py:95-206, kernels precompute_a /
+ * precompute_bc_pq of bi/kernels/scattering/vss.mako:29-93): the phase
+ * tables and the complex VSS kernels b_pq[x][y][k], c_pq[k] are built in
+ * FP64 on the device (pre-rule sums), then held in the dtype.
+ *   N        velocity points per axis (4, 6, 8, 12, 16, 20, 24, 32)
+ *   qz,qw    N_rho radial nodes/weights on [0, R]  (bi/velocitymesh.py:101-103)
+ *   sz       M x 3 full-sphere directions, sw = 4 pi / M (bi/velocitymesh.py:111-118)
+ *   szpre    Mpre x 3 full-sphere pre-rule (scattering.py:55-61), swpre = 4 pi / Mpre
+ *   mp, mq   masses of the species pair (molar mass / molarMass0)
+ *   alpha, omega  VSS parameters of the pair; prefac as scattering.py:86-90
+ * Tasks are processed in chunks whose gain products fit a 2 GiB scratch. */
+typedef void* dgfs_bi_collision_plan_t;
+int dgfs_bi_collision_plan_create(int N, int M, int Nrho, double L, const double* qz,
+                                  const double* qw, const double* sz, double sw, int Mpre,
+                                  const double* szpre, double mp, double mq, double alpha,
+                                  double omega, double prefac, int dtype, int ntask_max,
+                                  dgfs_bi_collision_plan_t* plan);
+int dgfs_bi_collision_plan_destroy(dgfs_bi_collision_plan_t plan);
+/* split S of the gain kernel, tasks per chunk, scratch bytes */
+int dgfs_bi_collision_plan_query(dgfs_bi_collision_plan_t plan, int* split, int* chunk,
+                                 size_t* scratch_bytes);
+/* Q_p[out] += prefac (gain - f_p loss) of the pair for every task: replaces
+ * the per-node fs(cases[q], d_f, d_g, d_ux, e, r, m) loop of rhs()
+ * (dgfs1D/bi/bi.py:329-337, scattering.py:261-321).  The task table is that
+ * of dgfs_collide: f terms contract up (species p), g terms contract uq
+ * (species q), the loss uses the f value.  Call once per partner q into the
+ * same Q to form Q_p = sum_q Q_pq. */
+int dgfs_bi_collide(dgfs_bi_collision_plan_t plan, const void* up, const void* uq, void* Q,
+                    int K, int Ne, int ntask, const int* task_nterm, const int* task_slice,
+                    const void* task_coef, int ngroup, const int* group_start,
+                    const int* group_slice, hipStream_t stream);
+
 /* ---------------------------------------------------------------- transport
  * Boundary kinds (dgfs1D/std/bc.py):
  *   COPY          'dgfs-periodic' / 'dgfs-cyclic' on one rank: ghost = interior
