@@ -55,11 +55,13 @@ k_bi_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase, i
         constexpr int side = Si;
         const cplx<T>* F = Fhat + (size_t(t) * 2 + side) * n;     // [l3][l1][l2]
         const cplx<T>* e = ph + side * 3 * N;
+        // the separable phase exp(i th_1 l1) exp(i th_2 l2) exp(i th_3 l3) is
+        // applied per axis just before that axis' transform: l3 in the load,
+        // l2 in pass 2 and l1 in pass 3 (these touch only this residue's data)
         E::run(V, sp, map1,
-               [&](int i1, int i2, int i3) {
-                   return (e[i1] * e[N + i2] * e[2 * N + i3]) * F[(size_t(i3) * N + i1) * N + i2];
-               },
-               IdentityPre{}, map3, IdentityPre{},
+               [&](int i1, int i2, int i3) { return e[2 * N + i3] * F[(size_t(i3) * N + i1) * N + i2]; },
+               [&](auto I, cplx<T> v) { return e[N + int(I)] * v; }, map3,
+               [&](auto I, cplx<T> v) { return e[int(I)] * v; },
                [&](int o2, int j, const cplx<T>* o) {
                    if constexpr (side == 0) {
                        sfor<0, N>([&](auto I) { R1[I] = o[I]; });

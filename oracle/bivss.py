@@ -141,3 +141,65 @@ def maxwellian(cv, m, nden, u, T):
     """n (m/(pi T))^1.5 exp(-m |c - u|^2 / T) (bi/initcond.py:60-64)."""
     u = np.asarray(u, dtype=np.float64).reshape(3, 1)
     return nden * (m / np.pi / T) ** 1.5 * np.exp(-m * ((cv - u) ** 2).sum(0) / T)
+
+
+class BiOracle:
+    """FP64 dgfsBi1D operator (bi/bi.py:265-338): species p's RHS
+    invMass(transport_p(u_p) + sum_q sum_{m, r, e} Q_pq(U u_p, V u_q)).
+    ``transports[p]`` is an oracle.std.StdOracle carrying species p's walls
+    (its transport only is used); ``pres[p, q]`` / ``prefac[p, q]`` the
+    pair tables.  us is a list of (K, Ne, n) arrays."""
+
+    def __init__(self, transports, pres, prefac):
+        self.tr, self.pres, self.prefac = transports, pres, prefac
+        self.b = transports[0].b
+        self.K, self.Ne = self.b.K, transports[0].Ne
+
+    def rhs(self, p, t, us):
+        b, K, Ne = self.b, self.K, self.Ne
+        ux = self.tr[p].transport(t, us[p])
+        for m in range(K):
+            f = np.einsum('ra,aen->ren', b.U[m].T, us[p])                 # trans_U
+            for q in range(len(us)):
+                g = np.einsum('rb,ben->ren', b.V[m], us[q])               # trans_V
+                for r in b.sig[m]:
+                    for e in range(Ne):
+                        ux[m, e] += q_pq(self.pres[p, q], f[r, e], g[r, e], self.prefac[p, q])
+        return (b.invmass @ ux.reshape(K, -1)).reshape(ux.shape)
+
+
+def integrate_bi(op, scheme, t, dt, us):
+    """One step of the mixture schemes (bi/integrator.py:48-176); ssp-rk3 with
+    the second stage r1 = 0.75 u + 0.25 r0 + 0.25 dt L(r0)."""
+    ns = len(us)
+    L = lambda tt, state: [op.rhs(p, tt, state) for p in range(ns)]
+    if scheme == 'euler':
+        k = L(t, us)
+        return [us[p] + dt * k[p] for p in range(ns)]
+    if scheme == 'ssp-rk2':
+        k = L(t, us)
+        r0 = [us[p] + dt * k[p] for p in range(ns)]
+        k = L(t + dt, r0)
+        return [0.5 * us[p] + 0.5 * r0[p] + 0.5 * dt * k[p] for p in range(ns)]
+    if scheme == 'ssp-rk3':
+        k = L(t, us)
+        r0 = [us[p] + dt * k[p] for p in range(ns)]
+        k = L(t + dt, r0)
+        r1 = [0.75 * us[p] + 0.25 * r0[p] + 0.25 * dt * k[p] for p in range(ns)]
+        k = L(t + 0.5 * dt, r1)
+        return [us[p] / 3.0 + 2.0 / 3.0 * r1[p] + 2.0 / 3.0 * dt * k[p] for p in range(ns)]
+    if scheme == 'lse-rk45':
+        a = [0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+             -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0]
+        b = [1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+             1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+             2277821191437.0 / 14882151754819.0]
+        c = [0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+             2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0]
+        r1 = [np.zeros_like(u) for u in us]
+        for i in range(5):
+            k = L(t + c[i] * dt, us)
+            r1 = [a[i] * r1[p] + dt * k[p] for p in range(ns)]
+            us = [us[p] + b[i] * r1[p] for p in range(ns)]
+        return us
+    raise ValueError(scheme)

@@ -186,3 +186,80 @@ def test_error_tool_rejects_mismatched_meshes(tmp_path):
     _dg_file(tmp_path / 'f.npz', fun, 3, 6)
     with pytest.raises(ValueError):
         compute_error(tmp_path / 'c.npz', tmp_path / 'f.npz')
+
+
+def test_bi_mesh_and_initial_state_match_oracle():
+    """dgfsBi1D host set-up (velocity mesh, spherical rules, Maxwellian IC and
+    diffuse-wall f0 per species) against oracle/bideck.py."""
+    from dgfs1d_gpu_amd.bi.initcond import DGFSMaxwellianInitConditionBi
+    from dgfs1d_gpu_amd.bi.velocitymesh import DGFSVelocityMeshBi
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    from oracle.bideck import BiDeck
+    from tests.test_gpu_bi_solver import _ini
+    text = _ini({})
+    deck = BiDeck(text)
+    cfg = Dictionary(text)
+    vm = DGFSVelocityMeshBi(cfg)
+    ov = deck.vm
+    assert vm.L() == ov.L and vm.masses() == ov.masses
+    assert np.allclose(vm.qz(), ov.qz, rtol=1e-15) and np.allclose(vm.qw(), ov.qw, rtol=1e-15)
+    assert np.array_equal(vm.sz(), ov.sz) and vm.sw() == ov.sw
+    ic = DGFSMaxwellianInitConditionBi(cfg, vm, 'soln-ics')
+    for p in range(2):
+        fb = deck.basis.fwd
+        u0 = np.einsum('kq,qen->ken', fb, np.broadcast_to(ic.get_init_vals()[p],
+                                                            (fb.shape[1], deck.Ne, vm.vsize())))
+        assert np.allclose(u0, deck.u0[p], rtol=1e-14, atol=0)
+    wall = DGFSMaxwellianInitConditionBi(cfg, vm, 'soln-bcs-xlo', wall=True)
+    for p in range(2):
+        f0 = deck.op.tr[p].wl.f0
+        assert np.allclose(wall.get_init_vals()[p], f0, rtol=1e-14, atol=0)
+
+
+def test_bi_registries():
+    from dgfs1d_gpu_amd.bi.bc import DGFSBCBi
+    from dgfs1d_gpu_amd.bi.initcond import DGFSInitConditionBi
+    from dgfs1d_gpu_amd.bi.integrator import DGFSIntegratorBi
+    from dgfs1d_gpu_amd.bi.scattering import DGFSScatteringModelBi
+    for k in ('euler', 'ssp-rk2', 'ssp-rk3', 'lse-rk45'):
+        subclass_where(DGFSIntegratorBi, intg_kind=k)
+    for k in ('dgfs-wall-diffuse', 'dgfs-inlet', 'dgfs-periodic', 'dgfs-wall-expr-diffuse'):
+        subclass_where(DGFSBCBi, type=k)
+    for k in ('maxwellian', 'maxwellian-expr'):
+        subclass_where(DGFSInitConditionBi, model=k)
+    subclass_where(DGFSScatteringModelBi, scattering_model='vss')
+
+
+def test_bi_moments_of_a_maxwellian_state():
+    """The mixture moment writer returns the deck's number densities,
+    temperature and zero velocity / heat flux for the Maxwellian initial
+    state (well resolved: Nv = 24, dev = 5)."""
+    import torch
+    from dgfs1d_gpu_amd.bi.initcond import DGFSMaxwellianInitConditionBi
+    from dgfs1d_gpu_amd.bi.moments import bi_moments
+    from dgfs1d_gpu_amd.bi.velocitymesh import DGFSVelocityMeshBi
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    from tests.test_gpu_bi_solver import _ini
+    cfg = Dictionary(_ini({('velocity-mesh', 'Nv'): 24}))
+    vm = DGFSVelocityMeshBi(cfg)
+    ic = DGFSMaxwellianInitConditionBi(cfg, vm, 'soln-ics')
+    K, Ne, Nqr = 2, 3, 2
+    im = np.array([[1.0, 1.0], [0.0, 0.0]])          # reconstruction reads mode 0
+    coeffs = []
+    for p in range(2):
+        u = np.zeros((K, Ne, vm.vsize()))
+        u[0] = ic.get_init_vals()[p]
+        coeffs.append(torch.tensor(u.ravel()))
+    out = bi_moments(coeffs, im, vm)
+    nden = [1.6802782428845444e+21, 8.008802839916987e+20]
+    for p in range(2):
+        o = 7 * p
+        assert np.allclose(out[..., o + 6], nden[p], rtol=1e-8)          # nden
+        assert np.allclose(out[..., o + 3], 273.0, rtol=1e-6)            # T
+        assert np.abs(out[..., o + 1]).max() < 1e-9                      # U:x
+        assert np.abs(out[..., o + 4]).max() < 1e-9 * 0.5 * vm.rho0() * vm.u0() ** 3  # Q:x
+        # the reference writes p_p = m_p R0/molarMass0 rho_p T_p (bi/moments.py:106-109),
+        # which is m_p^2 times n_p k T (k = R0/NA); the writer keeps that column
+        m = vm.masses()[p]
+        assert np.allclose(out[..., o + 5], m * m * nden[p] * 8.3144598 / 6.0221409e+23 * 273.0,
+                           rtol=1e-6)
