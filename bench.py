@@ -199,6 +199,56 @@ def steady_state(dev, tau=1e-5, max_steps=4000):
     return out
 
 
+# the reference's published single-GPU times (README.md:104-115): binary VSS
+# Couette, FP64, M=12, N_rho=8, first 52 steps on one Tesla P100
+BI_PUBLISHED = {(72, 3, 20): 47.58, (72, 3, 32): 126.601, (72, 3, 48): 391.943,
+                (72, 6, 20): 94.682, (72, 6, 32): 253.016, (72, 6, 48): 782.343,
+                (216, 3, 20): 141.754, (216, 3, 32): 378.956, (216, 3, 48): 1172.907,
+                (216, 6, 20): 283.091, (216, 6, 32): 759.149, (216, 6, 48): 2347.099}
+
+
+def bi_couette_published(dev, cases=((72, 3, 20),), steps=52):
+    """The reference's only published GPU benchmark: the binary-mixture VSS
+    Couette case (walls at -50/+50 m/s, 273 K; Ar-Kr mass ratio, Mpre=192,
+    nodal-sem-gll, SSP-RK2 dt=0.01, FP64, M=12, N_rho=8), wall time of the
+    first 52 steps after 2 untimed warm-up steps, against the published P100
+    time of the same phase space Ne/K/N^3."""
+    from configparser import ConfigParser
+    import io
+    from dgfs1d_gpu_amd.bi.bi import DGFSBi1D
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    import torch
+    base = open(os.path.join(ROOT, 'dgfs1d_gpu_amd', 'configs', 'bi_fourier.ini')).read()
+    out = []
+    for Ne, K, N in cases:
+        cp = ConfigParser(inline_comment_prefixes=[';'])
+        cp.optionxform = str
+        cp.read_string(base)
+        for sect in ('dgfsresidual', 'dgfsdistwriter', 'dgfsmomwriter'):
+            cp.remove_section(sect)
+        for (sect, key), v in {('mesh', 'Ne'): Ne, ('basis', 'K'): K, ('basis', 'Nq'): K + 1,
+                               ('velocity-mesh', 'Nv'): N, ('velocity-mesh', 'Nrho'): 8,
+                               ('time-integrator', 'scheme'): 'ssp-rk2',
+                               ('soln-bcs-xlo', 'T'): 273, ('soln-bcs-xhi', 'T'): 273,
+                               ('soln-bcs-xlo', 'uy'): -50, ('soln-bcs-xhi', 'uy'): 50}.items():
+            cp.set(sect, key, str(v))
+        buf = io.StringIO()
+        cp.write(buf)
+        s = DGFSBi1D(Dictionary(buf.getvalue()), device=dev, writers=False)
+        s.run(2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.run(steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        pub = BI_PUBLISHED.get((Ne, K, N))
+        out.append({'phase_space': '%d/%d/%d^3' % (Ne, K, N), 'steps': steps, 'wall_s': wall,
+                    'published_p100_s': pub, 'speedup_vs_published': pub / wall if pub else None,
+                    'point_steps_per_s': Ne * K * N ** 3 * steps / wall})
+        del s
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -215,6 +265,10 @@ def main(argv=None):
     ap.add_argument('--no-steady', action='store_true',
                     help='skip the time-to-steady-state comparison (learned closure vs '
                          'spectral, Mach-2 hard-sphere shock, residual 1e-5)')
+    ap.add_argument('--no-bi', action='store_true',
+                    help='skip the binary-mixture Couette run against the published P100 time')
+    ap.add_argument('--bi-only', default=None, metavar='Ne/K/N[,Ne/K/N...]',
+                    help='only run the binary-mixture Couette benchmark at these phase spaces')
     ap.add_argument('--closure-forward', type=int, default=0, metavar='NODES',
                     help='only time the learned-closure network forward on NODES slices of 32^3')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
@@ -227,6 +281,10 @@ def main(argv=None):
         return 2
 
     import torch
+    if args.bi_only:
+        cases = [tuple(int(x) for x in c.split('/')) for c in args.bi_only.split(',')]
+        print(json.dumps(bi_couette_published(torch.device('cuda:0'), cases)))
+        return 0
     if args.closure_forward:
         print(json.dumps(closure_forward(args.closure_forward, 32, 10, torch.device('cuda:0'))))
         return 0
@@ -308,12 +366,14 @@ def main(argv=None):
     }
     if not args.no_accuracy:
         out['accuracy'] = fp64_accuracy(text, args.warmup + args.steps, s, dev)
-    if args.batch or not args.no_steady:
+    if args.batch or not args.no_steady or not args.no_bi:
         del s
     if args.batch:
         out['batched'] = batched_workload(args.batch_seed, 64, 20, args.warmup, dev)
     if not args.no_steady:
         out['time_to_steady_state'] = steady_state(dev)
+    if not args.no_bi:
+        out['bi_couette_published'] = bi_couette_published(dev)
     if not args.no_cpu_baseline:
         cores = min(16, os.cpu_count() or 1)
         out['cpu_baseline'] = cpu_baseline(text, cores)
