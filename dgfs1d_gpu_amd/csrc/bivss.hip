@@ -4,6 +4,7 @@
 #include "bivss_kernels.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 namespace dgfs {
@@ -85,9 +86,23 @@ static int bi_dispatch(BiPlan* pl, const void* up, const void* uq, void* Q, Task
 }
 
 static void bi_free(BiPlan* pl) {
-    for (void* p : {pl->phase, pl->wtab, pl->ctab, pl->Fhat, pl->Iw, pl->acc, pl->H})
+    for (void* p : {pl->phase, pl->wtab, pl->ctab, pl->Fhat, pl->Iw, pl->acc, pl->H, pl->Gp})
         if (p) (void)hipFree(p);
     delete pl;
+}
+
+// largest number of xy groups of the plane stage (bi_xgroups_max<N>)
+static int bi_gp_groups(int N, int NXY) {
+    switch (N) {
+        case 4: return bi_xgroups_max<4>(NXY);
+        case 6: return bi_xgroups_max<6>(NXY);
+        case 8: return bi_xgroups_max<8>(NXY);
+        case 12: return bi_xgroups_max<12>(NXY);
+        case 16: return bi_xgroups_max<16>(NXY);
+        case 20: return bi_xgroups_max<20>(NXY);
+        case 24: return bi_xgroups_max<24>(NXY);
+        default: return bi_xgroups_max<32>(NXY);
+    }
 }
 
 // scratch budget of the gain products: tasks are processed in chunks whose
@@ -154,8 +169,15 @@ extern "C" int dgfs_bi_collision_plan_create(int N, int M, int Nrho, double L, c
     auto* pl = new BiPlan();
     pl->N = N; pl->M = M; pl->Nrho = Nrho; pl->Mpre = Mpre; pl->dtype = dtype;
     pl->ntask_max = ntask_max; pl->prefac = prefac; pl->mp = mp; pl->mq = mq; pl->n = n;
-    const size_t per_task = size_t(NXY) * n * cs;
-    size_t chunk = dgfs::kBiAccBudget / per_task;
+    const bool slabs = (dtype == DGFS_F64 ? dgfs::pick_split<double>(N)
+                                          : dgfs::pick_split<float>(N)) > 1;
+    const size_t per_task = size_t(NXY) * n * cs * (slabs ? 2 : 1);
+    size_t budget = dgfs::kBiAccBudget;
+    if (const char* mb = std::getenv("DGFS_BI_SCRATCH_MB")) {
+        const long v = std::atol(mb);
+        if (v > 0) budget = size_t(v) << 20;
+    }
+    size_t chunk = budget / per_task;
     if (chunk < 1) chunk = 1;
     if (chunk > size_t(ntask_max)) chunk = size_t(ntask_max);
     pl->chunk = int(chunk);
@@ -184,7 +206,8 @@ extern "C" int dgfs_bi_collision_plan_create(int N, int M, int Nrho, double L, c
         hipMalloc(&pl->Fhat, chunk * 2 * n * cs) != hipSuccess ||
         hipMalloc(&pl->Iw, chunk * 2 * n * cs) != hipSuccess ||
         hipMalloc(&pl->acc, chunk * per_task) != hipSuccess ||
-        hipMalloc(&pl->H, size_t(ntask_max) * 2 * n * cs) != hipSuccess) {
+        hipMalloc(&pl->H, size_t(ntask_max) * 2 * n * cs) != hipSuccess ||
+        hipMalloc(&pl->Gp, chunk * size_t(dgfs::bi_gp_groups(N, NXY)) * n * cs) != hipSuccess) {
         dgfs::set_error("bi_collision_plan_create: allocation failed");
         return fail(1);
     }
@@ -233,9 +256,12 @@ extern "C" int dgfs_bi_collision_plan_query(dgfs_bi_collision_plan_t plan, int* 
     if (split) *split = pl->dtype == DGFS_F64 ? dgfs::pick_split<double>(pl->N)
                                               : dgfs::pick_split<float>(pl->N);
     if (chunk) *chunk = pl->chunk;
-    if (scratch_bytes)
-        *scratch_bytes = (size_t(pl->chunk) * (4 + size_t(pl->Nrho) * pl->M) +
-                          size_t(pl->ntask_max) * 2) * pl->n * cs;
+    if (scratch_bytes) {
+        const int S = pl->dtype == DGFS_F64 ? dgfs::pick_split<double>(pl->N)
+                                            : dgfs::pick_split<float>(pl->N);
+        const size_t acc = size_t(pl->Nrho) * pl->M * (S > 1 ? 2 : 1);
+        *scratch_bytes = (size_t(pl->chunk) * (4 + acc) + size_t(pl->ntask_max) * 2) * pl->n * cs;
+    }
     return 0;
 }
 

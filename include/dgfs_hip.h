@@ -109,7 +109,9 @@ int dgfs_collision_timing_read(dgfs_collision_plan_t plan, double* ms /*[5]*/, i
  *   szpre    Mpre x 3 full-sphere pre-rule (scattering.py:55-61), swpre = 4 pi / Mpre
  *   mp, mq   masses of the species pair (molar mass / molarMass0)
  *   alpha, omega  VSS parameters of the pair; prefac as scattering.py:86-90
- * Tasks are processed in chunks whose gain products fit a 2 GiB scratch. */
+ * Tasks are processed in chunks whose gain products fit a 2 GiB scratch
+ * (environment variable DGFS_BI_SCRATCH_MB, read at plan creation, sets
+ * another budget; at least one task per chunk). */
 typedef void* dgfs_bi_collision_plan_t;
 int dgfs_bi_collision_plan_create(int N, int M, int Nrho, double L, const double* qz,
                                   const double* qw, const double* sz, double sw, int Mpre,

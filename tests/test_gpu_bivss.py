@@ -85,7 +85,19 @@ def test_bi_collision_fp64(cuda, N, M, Nrho, Mpre, pq):
     assert err < 1e-11
 
 
-@pytest.mark.parametrize('N,M,Nrho,Mpre', [(8, 6, 4, 32), (24, 6, 6, 32)])
+@pytest.mark.parametrize('N,M,Nrho,Mpre', [(24, 6, 4, 32), (32, 6, 4, 32)])
+@pytest.mark.parametrize('pq', [(0, 1), (1, 1)])
+def test_bi_collision_fp64_slab_path(cuda, N, M, Nrho, Mpre, pq):
+    """Grids whose gain cube needs a residue split run the two-kernel slab
+    path (k_bi_yz, k_bi_xprod)."""
+    from dgfs1d_gpu_amd import _lib
+    err, info = _run(cuda, N, M, Nrho, Mpre, _lib.F64, *pq)
+    print('bi fp64 slab', N, pq, err, info, flush=True)
+    assert info['split'] > 1
+    assert err < 1e-11
+
+
+@pytest.mark.parametrize('N,M,Nrho,Mpre', [(8, 6, 4, 32), (24, 6, 6, 32), (32, 6, 4, 32)])
 def test_bi_collision_fp32(cuda, N, M, Nrho, Mpre):
     from dgfs1d_gpu_amd import _lib
     err, info = _run(cuda, N, M, Nrho, Mpre, _lib.F32, 0, 1)
@@ -102,3 +114,15 @@ def test_bi_collision_fp64_n24_fourier_example(cuda):
         err, info = _run(cuda, 24, 12, 6, 192, _lib.F64, *pq, tasks=tasks, K=1, Ne=1)
         print('bi fp64 n24', pq, err, info, flush=True)
         assert err < 1e-11
+
+
+def test_bi_collision_chunked(cuda, monkeypatch):
+    """A scratch budget below one chunk of all tasks splits the launch into
+    several chunks (task-table offsets per chunk); results are unchanged."""
+    from dgfs1d_gpu_amd import _lib
+    monkeypatch.setenv('DGFS_BI_SCRATCH_MB', '3')      # 2 tasks per chunk at N=16 / 24
+    for N, dt in ((16, _lib.F64), (24, _lib.F64), (16, _lib.F32)):
+        err, info = _run(cuda, N, 6, 4, 32, dt, 0, 1)
+        print('bi chunked', N, dt, err, info, flush=True)
+        assert info['chunk'] < len(TASKS)
+        assert err < (1e-11 if dt == _lib.F64 else 3e-5)
