@@ -21,7 +21,8 @@ from ..basis import Basis
 from ..initialize import initialize
 from ..mesh import Mesh
 from ..nputil import subclass_where
-from ..transport import MatOp, Transport
+from ..relax import MixtureProjection
+from ..transport import MatOp, Transport, axnpby
 from .bc import DGFSBCBi
 from .distribution import DGFSDistributionBi, load_distribution
 from .initcond import DGFSInitConditionBi
@@ -79,6 +80,20 @@ class DGFSBi1D:
         self.fwd = MatOp(basis.mat_fwd, self.device)
 
         n = K * Ne * Nv
+        # [scattering-model] projection = conservative: the two species'
+        # collision terms of every node are projected together (each species'
+        # mass, the mixture's momentum and energy), FP64 moments and solve
+        mode = cfg.lookupordefault('scattering-model', 'projection', 'none')
+        if mode not in ('none', 'conservative'):
+            raise ValueError("scattering-model::projection must be 'none' or 'conservative'")
+        self.projection = None
+        if mode == 'conservative':
+            self.projection = MixtureProjection(vm.cv(), vm.d_cvx(), vm.d_cvy(), vm.d_cvz(),
+                                                vm.masses()[0], vm.masses()[1], self.tdtype,
+                                                self.device)
+            self._qc = [torch.zeros(n, dtype=self.tdtype, device=self.device)
+                        for _ in range(ns)]
+            self._pending = None
         self.us = [torch.zeros(n, dtype=self.tdtype, device=self.device) for _ in range(ns)]
         self.ux = torch.zeros(n, dtype=self.tdtype, device=self.device)
         if restart is None:
@@ -111,14 +126,32 @@ class DGFSBi1D:
             if cfg.has_section('dgfsdistwriter'):
                 self.distribution = DGFSDistributionBi(ti, (K, Ne, Nv), cfg, 'dgfsdistwriter')
 
+    def _collide(self, p, us_in, Q):
+        if self.projection is None:
+            self.sm.collide(p, us_in, Q)
+            return
+        # both species' terms of the same state are projected together; the
+        # integrators evaluate species 0 then species 1 on one state, so the
+        # pair computed for p = 0 serves the following p = 1 call
+        if p == 0 or self._pending is not us_in:
+            for q, buf in enumerate(self._qc):
+                buf.zero_()
+                self.sm.collide(q, us_in, buf)
+            self.projection(self._qc[0], self._qc[1], self._qc[0], self._qc[1],
+                            self.K * self.Ne, accumulate=False)
+            self._pending = us_in if p == 0 else None
+        else:
+            self._pending = None
+        axnpby(Q, [(1.0, self._qc[p])], 1.0)
+
     def rhs(self, p, t, us_in, u_out):
         """u_out = invMass(transport_p(u_p) + sum_q Q_pq)  (bi.py:265-338)."""
         if self.invmass.is_identity:
             self.transport[p](t, us_in[p], u_out)
-            self.sm.collide(p, us_in, u_out)
+            self._collide(p, us_in, u_out)
         else:
             self.transport[p](t, us_in[p], self.ux)
-            self.sm.collide(p, us_in, self.ux)
+            self._collide(p, us_in, self.ux)
             self.invmass(self.ux, u_out)
 
     def step(self):

@@ -94,6 +94,42 @@ k_proj_final(int nb, Gram5 G, const double* __restrict__ part, double* __restric
     }
 }
 
+// mixture: per slice the two species' moments m0, m1 -> residuals of the
+// six constraints r = (m0[0], m1[0], M0 m0[1..3] + M1 m1[1..3],
+// M0 m0[4] + M1 m1[4]), multipliers l = G6 r, and each species' correction
+// coefficients in the basis {1, c, |c|^2}: lam_p = (l[p], M_p l[2..5]).
+struct Gram6 { double g[36]; };
+
+__global__ void __launch_bounds__(kRB)
+k_proj_final_mix(int nb, Gram6 G, double M0, double M1, const double* __restrict__ part0,
+                 const double* __restrict__ part1, double* __restrict__ lam0,
+                 double* __restrict__ lam1) {
+    __shared__ double sh[5 * 4];
+    const int s = blockIdx.x;
+    double a[5], b[5];
+    reduce_partials<5>(part0 + size_t(s) * nb * 5, nb, a, sh);
+    __syncthreads();
+    reduce_partials<5>(part1 + size_t(s) * nb * 5, nb, b, sh);
+    if (threadIdx.x == 0) {
+        const double r[6] = {a[0], b[0], M0 * a[1] + M1 * b[1], M0 * a[2] + M1 * b[2],
+                             M0 * a[3] + M1 * b[3], M0 * a[4] + M1 * b[4]};
+        double l[6];
+        for (int i = 0; i < 6; ++i) {
+            double acc = 0.0;
+            for (int j = 0; j < 6; ++j) acc += G.g[i * 6 + j] * r[j];
+            l[i] = acc;
+        }
+        double* L0 = lam0 + size_t(s) * 8;
+        double* L1 = lam1 + size_t(s) * 8;
+        L0[0] = l[0];
+        L1[0] = l[1];
+        for (int i = 1; i < 5; ++i) {
+            L0[i] = M0 * l[i + 1];
+            L1[i] = M1 * l[i + 1];
+        }
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kRB)
 k_proj_apply(int n, const T* __restrict__ Q, const T* __restrict__ cx,
@@ -291,6 +327,62 @@ extern "C" int dgfs_project_conservative(int dtype, int nslice, int Nv, const vo
         DGFS_HIP_CHECK(hipGetLastError());
         return 0;
     }
+}
+
+extern "C" size_t dgfs_projection_mixture_work_bytes(int nslice, int Nv) {
+    return 2 * dgfs_projection_work_bytes(nslice, Nv);
+}
+
+extern "C" int dgfs_project_conservative_mixture(int dtype, int nslice, int Nv, const void* cx,
+                                                 const void* cy, const void* cz, double m0,
+                                                 double m1, const double* gram_inv,
+                                                 const void* Q0, const void* Q1, void* out0,
+                                                 void* out1, int accumulate, double* work,
+                                                 hipStream_t stream) {
+    DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "dtype must be DGFS_F64 or DGFS_F32");
+    DGFS_REQUIRE(nslice >= 0 && Nv > 0, "bad sizes");
+    DGFS_REQUIRE(m0 > 0 && m1 > 0, "masses must be positive");
+    DGFS_REQUIRE(gram_inv && Q0 && Q1 && out0 && out1 && work && cx && cy && cz, "null pointer");
+    if (nslice == 0) return 0;
+    DGFS_REQUIRE(nslice <= 65535, "too many slices");
+    Gram6 G;
+    for (int i = 0; i < 36; ++i) G.g[i] = gram_inv[i];
+    const int nb = (Nv + kRB - 1) / kRB;
+    const dim3 grid(nb, nslice);
+    double* part0 = work;
+    double* part1 = work + size_t(nslice) * nb * 5;
+    double* lam0 = part1 + size_t(nslice) * nb * 5;
+    double* lam1 = lam0 + size_t(nslice) * 8;
+    const void* Qs[2] = {Q0, Q1};
+    double* parts[2] = {part0, part1};
+    for (int p = 0; p < 2; ++p) {
+        if (dtype == DGFS_F64)
+            hipLaunchKernelGGL(k_proj_partial<double>, grid, dim3(kRB), 0, stream, Nv,
+                               (const double*)Qs[p], (const double*)cx, (const double*)cy,
+                               (const double*)cz, parts[p]);
+        else
+            hipLaunchKernelGGL(k_proj_partial<float>, grid, dim3(kRB), 0, stream, Nv,
+                               (const float*)Qs[p], (const float*)cx, (const float*)cy,
+                               (const float*)cz, parts[p]);
+        DGFS_HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_proj_final_mix, dim3(nslice), dim3(kRB), 0, stream, nb, G, m0, m1,
+                       (const double*)part0, (const double*)part1, lam0, lam1);
+    DGFS_HIP_CHECK(hipGetLastError());
+    void* outs[2] = {out0, out1};
+    const double* lams[2] = {lam0, lam1};
+    for (int p = 0; p < 2; ++p) {
+        if (dtype == DGFS_F64)
+            hipLaunchKernelGGL(k_proj_apply<double>, grid, dim3(kRB), 0, stream, Nv,
+                               (const double*)Qs[p], (const double*)cx, (const double*)cy,
+                               (const double*)cz, lams[p], (double*)outs[p], accumulate);
+        else
+            hipLaunchKernelGGL(k_proj_apply<float>, grid, dim3(kRB), 0, stream, Nv,
+                               (const float*)Qs[p], (const float*)cx, (const float*)cy,
+                               (const float*)cz, lams[p], (float*)outs[p], accumulate);
+        DGFS_HIP_CHECK(hipGetLastError());
+    }
+    return 0;
 }
 
 extern "C" size_t dgfs_shakov_work_bytes(int ntask, int Nv) {

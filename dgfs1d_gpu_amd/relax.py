@@ -68,6 +68,56 @@ class ConservativeProjection:
             int(bool(accumulate)), _lib.ptr(w), _lib.stream_ptr(stream)))
 
 
+def mixture_constraints(cv, m0, m1):
+    """(Phi0, Phi1): the six constraint rows of the binary mixture on each
+    species' grid values, (mass 0, mass 1, m_p c, m_p |c|^2), each (6, n)."""
+    phi = collision_invariants(cv)
+    n = phi.shape[1]
+    out = []
+    for p, m in enumerate((m0, m1)):
+        P = np.zeros((6, n))
+        P[p] = 1.0
+        P[2:] = m * phi[1:]
+        out.append(P)
+    return out
+
+
+class MixtureProjection:
+    """Binary-mixture conservation projection (dgfs_project_conservative_mixture):
+    the Euclidean-nearest pair (Q0, Q1) with each species' mass and the
+    mixture's momentum and energy unchanged by collisions (FP64 moments and
+    solve)."""
+
+    def __init__(self, cv, cx, cy, cz, m0, m1, tdtype, device):
+        import torch
+        self.torch = torch
+        self.tdtype = tdtype
+        self.device = device
+        self.n = int(np.asarray(cv).shape[1])
+        self.m0, self.m1 = float(m0), float(m1)
+        P0, P1 = mixture_constraints(_grid_as_device(cv, tdtype), m0, m1)
+        self.gram_inv = np.ascontiguousarray(np.linalg.inv(P0 @ P0.T + P1 @ P1.T))
+        self._gp = self.gram_inv.ctypes.data_as(_lib.C.POINTER(_lib.C.c_double))
+        self.cx, self.cy, self.cz = cx, cy, cz
+        self._work = None
+
+    def __call__(self, Q0, Q1, out0, out1, nslice, accumulate=False, stream=None):
+        for x in (Q0, Q1, out0, out1):
+            if x.dtype != self.tdtype:
+                raise TypeError('projection operands must be %s' % self.tdtype)
+            if x.numel() < nslice * self.n:
+                raise ValueError('projection operands smaller than nslice*Nv')
+        nb = _lib.lib().dgfs_projection_mixture_work_bytes(int(nslice), self.n)
+        if self._work is None or self._work.numel() * 8 < nb:
+            self._work = self.torch.empty(max(1, nb // 8), dtype=self.torch.float64,
+                                          device=self.device)
+        _lib.check(_lib.lib().dgfs_project_conservative_mixture(
+            _lib.dtype_code(self.tdtype), int(nslice), self.n, _lib.ptr(self.cx),
+            _lib.ptr(self.cy), _lib.ptr(self.cz), self.m0, self.m1, self._gp, _lib.ptr(Q0),
+            _lib.ptr(Q1), _lib.ptr(out0), _lib.ptr(out1), int(bool(accumulate)),
+            _lib.ptr(self._work), _lib.stream_ptr(stream)))
+
+
 class Shakov:
     """Batched Shakov relaxation term Q += prefac rho T^{1-omega} (f_S - f)."""
 

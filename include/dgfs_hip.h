@@ -240,6 +240,25 @@ int dgfs_project_conservative(int dtype, int nslice, int Nv, const void* cx, con
                               const void* cz, const double* gram_inv, const void* Q, void* out,
                               int accumulate, double* work, hipStream_t stream);
 
+/* Mixture conservation projection (FP64; no reference counterpart).  Each
+ * Q_pq of the binary VSS operator keeps species p's mass, but momentum and
+ * energy are kept only by the pair sum, so for each slice s of the two
+ * species' collision terms Q0_s, Q1_s (same node) the Euclidean-nearest pair
+ * with
+ *     sum Q0 = 0,  sum Q1 = 0,
+ *     m0 sum c Q0 + m1 sum c Q1 = 0,  m0 sum |c|^2 Q0 + m1 sum |c|^2 Q1 = 0
+ * is out0_s (+)= Q0_s - Phi0^T l, out1_s (+)= Q1_s - Phi1^T l with the six
+ * constraint rows Phi_p = (delta_p0, delta_p1, m_p c, m_p |c|^2) and
+ * l = gram_inv (Phi0 Q0 + Phi1 Q1), gram_inv = (Phi0 Phi0^T + Phi1 Phi1^T)^{-1}
+ * (6x6 row-major, host memory, on the grid as the device holds it).
+ * work: dgfs_projection_mixture_work_bytes(nslice, Nv) bytes. */
+size_t dgfs_projection_mixture_work_bytes(int nslice, int Nv);
+int dgfs_project_conservative_mixture(int dtype, int nslice, int Nv, const void* cx,
+                                      const void* cy, const void* cz, double m0, double m1,
+                                      const double* gram_inv, const void* Q0, const void* Q1,
+                                      void* out0, void* out1, int accumulate, double* work,
+                                      hipStream_t stream);
+
 /* ---------------------------------------------------------------------------
  * Shakov relaxation model 'shakov-direct-gll' (std/scattering.py:1636-1833,
  * std/kernels/scattering/shakov-direct-gll.mako:15-167): its fs(d_f, d_g, d_Q,

@@ -62,7 +62,11 @@ class BiDeck:
         for p in range(ns):
             wl, wr = self._wall('soln-bcs-xlo', -1.0, p), self._wall('soln-bcs-xhi', 1.0, p)
             trs.append(OS.StdOracle(b, self.invjac, vm, wl, wr, None, 0.0))
-        self.op = OB.BiOracle(trs, self.pres, self.prefac)
+        project = None
+        if cp.has_option('scattering-model', 'projection') and \
+                cp.get('scattering-model', 'projection') == 'conservative':
+            project = (vm.cv, m[0], m[1])
+        self.op = OB.BiOracle(trs, self.pres, self.prefac, project=project)
         ti, dt, tf = g('time-integrator', 'tstart'), g('time-integrator', 'dt'), \
             g('time-integrator', 'tend')
         nsteps = np.ceil((tf - ti) / dt)

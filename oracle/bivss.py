@@ -150,21 +150,39 @@ class BiOracle:
     (its transport only is used); ``pres[p, q]`` / ``prefac[p, q]`` the
     pair tables.  us is a list of (K, Ne, n) arrays."""
 
-    def __init__(self, transports, pres, prefac):
+    def __init__(self, transports, pres, prefac, project=None):
+        """project = (cv, m0, m1): the collision terms of each node are replaced
+        by their mixture conservation projection (oracle.relax.project_mixture)."""
         self.tr, self.pres, self.prefac = transports, pres, prefac
         self.b = transports[0].b
         self.K, self.Ne = self.b.K, transports[0].Ne
+        self.project = project
 
-    def rhs(self, p, t, us):
+    def collision(self, p, us):
         b, K, Ne = self.b, self.K, self.Ne
-        ux = self.tr[p].transport(t, us[p])
+        out = np.zeros_like(us[p])
         for m in range(K):
             f = np.einsum('ra,aen->ren', b.U[m].T, us[p])                 # trans_U
             for q in range(len(us)):
                 g = np.einsum('rb,ben->ren', b.V[m], us[q])               # trans_V
                 for r in b.sig[m]:
                     for e in range(Ne):
-                        ux[m, e] += q_pq(self.pres[p, q], f[r, e], g[r, e], self.prefac[p, q])
+                        out[m, e] += q_pq(self.pres[p, q], f[r, e], g[r, e], self.prefac[p, q])
+        return out
+
+    def rhs(self, p, t, us):
+        b, K = self.b, self.K
+        ux = self.tr[p].transport(t, us[p])
+        if self.project is None:
+            ux += self.collision(p, us)
+        else:
+            from .relax import project_mixture
+            cv, m0, m1 = self.project
+            c0, c1 = self.collision(0, us), self.collision(1, us)
+            for m in range(K):
+                for e in range(self.Ne):
+                    c0[m, e], c1[m, e] = project_mixture(c0[m, e], c1[m, e], cv, m0, m1)
+            ux += (c0, c1)[p]
         return (b.invmass @ ux.reshape(K, -1)).reshape(ux.shape)
 
 

@@ -65,3 +65,20 @@ def project_conservative(Q, cv):
     phi = invariants(np.asarray(cv, dtype=np.float64))
     lam, *_ = np.linalg.lstsq(phi.T, np.asarray(Q, dtype=np.float64), rcond=None)
     return Q - phi.T @ lam
+
+
+def project_mixture(Q0, Q1, cv, m0, m1):
+    """Binary mixture: the least-norm (dQ0, dQ1) making sum Q0, sum Q1,
+    m0 sum c Q0 + m1 sum c Q1 and m0 sum |c|^2 Q0 + m1 sum |c|^2 Q1 vanish,
+    by a least-norm solve of the stacked 6 x 2n constraint system
+    (numpy.linalg.lstsq), independent of the product's Gram inverse."""
+    phi = invariants(np.asarray(cv, dtype=np.float64))
+    n = phi.shape[1]
+    A = np.zeros((6, 2 * n))
+    A[0, :n] = 1.0
+    A[1, n:] = 1.0
+    A[2:, :n] = m0 * phi[1:]
+    A[2:, n:] = m1 * phi[1:]
+    q = np.concatenate([np.asarray(Q0, dtype=np.float64), np.asarray(Q1, dtype=np.float64)])
+    d, *_ = np.linalg.lstsq(A, -(A @ q), rcond=None)     # least-norm correction
+    return Q0 + d[:n], Q1 + d[n:]

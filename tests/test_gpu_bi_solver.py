@@ -202,3 +202,55 @@ basename = dist-{t:.2f}
     s2.run(2)
     for p in range(2):
         assert torch.equal(r.us[p], s2.us[p])
+
+
+def test_bi_mixture_projection_entry(cuda):
+    """dgfs_project_conservative_mixture against oracle.relax.project_mixture on
+    random slices (FP64 and FP32 data, FP64 moments and solve)."""
+    import torch
+    from dgfs1d_gpu_amd.bi.velocitymesh import DGFSVelocityMeshBi
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    from dgfs1d_gpu_amd.relax import MixtureProjection
+    from oracle.relax import project_mixture
+    for prec, tol in (('double', 1e-12), ('single', 2e-5)):
+        cfg = Dictionary(_ini({('config', 'precision'): prec}))
+        vm = DGFSVelocityMeshBi(cfg, device=cuda)
+        td = torch.float64 if prec == 'double' else torch.float32
+        m0, m1 = vm.masses()
+        rng = np.random.default_rng(11)
+        ns, n = 5, vm.vsize()
+        Q0 = rng.standard_normal((ns, n))
+        Q1 = rng.standard_normal((ns, n))
+        Q0 = Q0.astype(np.float32).astype(np.float64) if prec == 'single' else Q0
+        Q1 = Q1.astype(np.float32).astype(np.float64) if prec == 'single' else Q1
+        P = MixtureProjection(vm.cv(), vm.d_cvx(), vm.d_cvy(), vm.d_cvz(), m0, m1, td, cuda)
+        d0 = torch.tensor(Q0, dtype=td, device=cuda)
+        d1 = torch.tensor(Q1, dtype=td, device=cuda)
+        o0, o1 = torch.zeros_like(d0), torch.zeros_like(d1)
+        P(d0, d1, o0, o1, ns, accumulate=False)
+        torch.cuda.synchronize()
+        cvd = np.asarray(vm.cv(), dtype=np.float64 if prec == 'double' else np.float32)
+        for s in range(ns):
+            r0, r1 = project_mixture(Q0[s], Q1[s], cvd.astype(np.float64), m0, m1)
+            e = max(np.abs(o0[s].double().cpu().numpy() - r0).max(),
+                    np.abs(o1[s].double().cpu().numpy() - r1).max()) / np.abs(Q0[s]).max()
+            print('mixture projection', prec, s, e, flush=True)
+            assert e < tol
+
+
+@pytest.mark.parametrize('scheme', ['euler', 'ssp-rk2'])
+def test_bi_steps_projected(cuda, scheme):
+    """The solver with [scattering-model] projection = conservative against the
+    oracle with the mixture projection of every node's collision terms."""
+    over = {('time-integrator', 'scheme'): scheme, ('scattering-model', 'projection'):
+            'conservative'}
+    deck, s, us = _pair(cuda, over)
+    t = 0.0
+    for k in range(2):
+        s.step()
+        us = deck.step(t, us)
+        t += deck.dt
+        for p in range(2):
+            err = _rel(s.us[p].cpu().numpy().reshape(us[p].shape), us[p])
+            print('bi projected step', scheme, k, p, err, flush=True)
+            assert err < 1e-11

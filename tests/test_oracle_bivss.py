@@ -118,3 +118,43 @@ def test_pair_conserves_momentum_energy_and_species_mass(mixture):
     assert abs(en01 + en10) < 5e-3 * m[0] * np.abs(c2 * q01).sum()
     for qq in (q01, q10):
         assert abs(qq.sum()) < 1e-3 * np.abs(qq).sum()
+
+
+def test_mixture_projection_pins(mixture):
+    """oracle.relax.project_mixture: the six mixture constraints vanish,
+    projecting twice changes nothing, the correction lies in the span of the
+    constraint rows (least norm), and on real collision terms it is a small
+    change.  A planted slip (species 0's mass used in both momentum rows)
+    leaves the mixture momentum unbalanced."""
+    from oracle.relax import project_mixture
+    vm, pres = mixture
+    m, cv = vm.masses, vm.cv
+    f = [maxwellian(cv, m[0], 1.0, [0.3, 0, 0], 0.8), maxwellian(cv, m[1], 0.5, [-0.2, 0.1, 0], 1.3)]
+    Q0 = q_pq(pres[0, 0], f[0], f[0], 1.0) + q_pq(pres[0, 1], f[0], f[1], 1.0)
+    Q1 = q_pq(pres[1, 0], f[1], f[0], 1.0) + q_pq(pres[1, 1], f[1], f[1], 1.0)
+    P0, P1 = project_mixture(Q0, Q1, cv, m[0], m[1])
+    c2 = (cv ** 2).sum(0)
+
+    def resid(a, b, ma=m[0], mb=m[1]):
+        return np.array([a.sum(), b.sum(), *(ma * (cv * a).sum(1) + mb * (cv * b).sum(1)),
+                         ma * (c2 * a).sum() + mb * (c2 * b).sum()])
+
+    scale = np.abs(resid(np.abs(Q0), np.abs(Q1))).max()
+    assert np.abs(resid(P0, P1)).max() < 1e-13 * scale
+    assert np.abs(resid(Q0, Q1)).max() > 1e-6 * scale           # the raw terms do not conserve
+    R0, R1 = project_mixture(P0, P1, cv, m[0], m[1])
+    assert np.abs(R0 - P0).max() < 1e-13 * np.abs(P0).max()
+    assert np.abs(R1 - P1).max() < 1e-13 * np.abs(P1).max()
+    # least norm: the correction is a combination of the constraint rows
+    n = cv.shape[1]
+    A = np.zeros((6, 2 * n))
+    phi = np.vstack([np.ones(n), cv, c2])
+    A[0, :n], A[1, n:] = 1.0, 1.0
+    A[2:, :n], A[2:, n:] = m[0] * phi[1:], m[1] * phi[1:]
+    d = np.concatenate([P0 - Q0, P1 - Q1])
+    y, *_ = np.linalg.lstsq(A.T, d, rcond=None)
+    assert np.abs(A.T @ y - d).max() < 1e-10 * np.abs(d).max()
+    assert np.abs(d).max() < 1e-2 * max(np.abs(Q0).max(), np.abs(Q1).max())
+    # planted slip: species 0's mass on both momentum rows
+    S0, S1 = project_mixture(Q0, Q1, cv, m[0], m[0])
+    assert np.abs(resid(S0, S1)[2:5]).max() > 1e3 * np.abs(resid(P0, P1)[2:5]).max()
