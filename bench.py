@@ -249,6 +249,35 @@ def bi_couette_published(dev, cases=((72, 3, 20),), steps=52):
     return out
 
 
+def bi_fourier_n48(dev, steps=3, warmup=1):
+    """The largest mixture grid (configs/bi_fourier_n48.ini: Ne=8, modal K=5,
+    N=48^3, M=24, N_rho=24, Mpre=192, FP32 collisions with the FP64 mixture
+    projection, SSP-RK2): set-up time and ms per step."""
+    from dgfs1d_gpu_amd.bi.bi import DGFSBi1D
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    import torch
+    text = open(os.path.join(ROOT, 'dgfs1d_gpu_amd', 'configs', 'bi_fourier_n48.ini')).read()
+    t0 = time.perf_counter()
+    s = DGFSBi1D(Dictionary(text), device=dev, writers=False)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    s.run(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.run(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    K, Ne, n = s.K, s.Ne, s.Nv
+    out = {'workload': 'binary VSS Fourier Ne=8 modal K=5 N=48^3 M=24 Nrho=24 Mpre=192 FP32 '
+                       '+ FP64 mixture projection, SSP-RK2',
+           'tasks_per_pair': s.sm.tasks['ntask'], 'setup_s': setup, 'steps': steps,
+           'ms_per_step': wall / steps * 1e3,
+           'point_steps_per_s': Ne * K * n * steps / wall,
+           'chunk': s.sm.plans[0, 0].query()['chunk']}
+    del s
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -269,6 +298,8 @@ def main(argv=None):
                     help='skip the binary-mixture Couette run against the published P100 time')
     ap.add_argument('--bi-only', default=None, metavar='Ne/K/N[,Ne/K/N...]',
                     help='only run the binary-mixture Couette benchmark at these phase spaces')
+    ap.add_argument('--bi-n48', type=int, default=0, metavar='STEPS',
+                    help='only time STEPS steps of the N=48 mixture deck (configs/bi_fourier_n48.ini)')
     ap.add_argument('--closure-forward', type=int, default=0, metavar='NODES',
                     help='only time the learned-closure network forward on NODES slices of 32^3')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
@@ -281,6 +312,9 @@ def main(argv=None):
         return 2
 
     import torch
+    if args.bi_n48:
+        print(json.dumps(bi_fourier_n48(torch.device('cuda:0'), args.bi_n48)))
+        return 0
     if args.bi_only:
         cases = [tuple(int(x) for x in c.split('/')) for c in args.bi_only.split(',')]
         print(json.dumps(bi_couette_published(torch.device('cuda:0'), cases)))

@@ -79,6 +79,10 @@ static int bi_dispatch(BiPlan* pl, const void* up, const void* uq, void* Q, Task
         case 20: return bi_launch<T, 20>(pl, up, uq, Q, tv, ntask, ngroup, gs, gsl, st);
         case 24: return bi_launch<T, 24>(pl, up, uq, Q, tv, ntask, ngroup, gs, gsl, st);
         case 32: return bi_launch<T, 32>(pl, up, uq, Q, tv, ntask, ngroup, gs, gsl, st);
+        case 48:
+            if constexpr (sizeof(T) == 4)
+                return bi_launch<T, 48>(pl, up, uq, Q, tv, ntask, ngroup, gs, gsl, st);
+            break;
         default: break;
     }
     set_error("bi_collide: unsupported velocity grid N=%d", pl->N);
@@ -101,7 +105,8 @@ static int bi_gp_groups(int N, int NXY) {
         case 16: return bi_xgroups_max<16>(NXY);
         case 20: return bi_xgroups_max<20>(NXY);
         case 24: return bi_xgroups_max<24>(NXY);
-        default: return bi_xgroups_max<32>(NXY);
+        case 32: return bi_xgroups_max<32>(NXY);
+        default: return bi_xgroups_max<48>(NXY);
     }
 }
 
@@ -122,8 +127,11 @@ extern "C" int dgfs_bi_collision_plan_create(int N, int M, int Nrho, double L, c
     DGFS_REQUIRE(out != nullptr, "bi_collision_plan_create: null output handle");
     *out = nullptr;
     DGFS_REQUIRE(N == 4 || N == 6 || N == 8 || N == 12 || N == 16 || N == 20 || N == 24 ||
-                     N == 32,
+                     N == 32 || N == 48,
                  "bi_collision_plan_create: unsupported N=%d", N);
+    DGFS_REQUIRE(N != 48 || dtype == DGFS_F32,
+                 "bi_collision_plan_create: N=48 is built for DGFS_F32 only (its FP64 plane "
+                 "buffers exceed the 160 KiB LDS of a work-group)");
     DGFS_REQUIRE(M > 0 && Nrho > 0 && Mpre > 0 && ntask_max > 0,
                  "bi_collision_plan_create: bad sizes");
     DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "bi_collision_plan_create: bad dtype");

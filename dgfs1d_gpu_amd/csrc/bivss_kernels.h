@@ -346,6 +346,7 @@ int bi_launch(BiPlan* pl, const void* up, const void* uq, void* Q, TaskView tv, 
                                  const int*, const int*, hipStream_t);
 
 DGFS_COLLIDE_SIZES(DGFS_BI_EXTERN, float)
+DGFS_BI_EXTERN(float, 48)
 DGFS_COLLIDE_SIZES(DGFS_BI_EXTERN, double)
 
 }  // namespace dgfs

@@ -103,7 +103,7 @@ int dgfs_collision_timing_read(dgfs_collision_plan_t plan, double* ms /*[5]*/, i
  * precompute_bc_pq of bi/kernels/scattering/vss.mako:29-93): the phase
  * tables and the complex VSS kernels b_pq[x][y][k], c_pq[k] are built in
  * FP64 on the device (pre-rule sums), then held in the dtype.
- *   N        velocity points per axis (4, 6, 8, 12, 16, 20, 24, 32)
+ *   N        velocity points per axis (4, 6, 8, 12, 16, 20, 24, 32; 48 with DGFS_F32)
  *   qz,qw    N_rho radial nodes/weights on [0, R]  (bi/velocitymesh.py:101-103)
  *   sz       M x 3 full-sphere directions, sw = 4 pi / M (bi/velocitymesh.py:111-118)
  *   szpre    Mpre x 3 full-sphere pre-rule (scattering.py:55-61), swpre = 4 pi / Mpre

@@ -126,3 +126,15 @@ def test_bi_collision_chunked(cuda, monkeypatch):
         print('bi chunked', N, dt, err, info, flush=True)
         assert info['chunk'] < len(TASKS)
         assert err < (1e-11 if dt == _lib.F64 else 3e-5)
+
+
+def test_bi_collision_fp32_n48(cuda):
+    """N = 48 (the mixture grid of the reference's largest published runs),
+    FP32 on the slab path; FP64 at N = 48 is refused with an error."""
+    from dgfs1d_gpu_amd import _lib
+    err, info = _run(cuda, 48, 6, 2, 32, _lib.F32, 0, 1, tasks=TASKS[:2])
+    print('bi fp32 n48', err, info, flush=True)
+    assert info['split'] > 1
+    assert err < 3e-5
+    with pytest.raises(_lib.DGFSError, match='N=48 is built for DGFS_F32 only'):
+        _run(cuda, 48, 6, 2, 32, _lib.F64, 0, 1, tasks=TASKS[:1])
