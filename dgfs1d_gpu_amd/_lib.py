@@ -90,6 +90,7 @@ SIGNATURES = {
     'dgfs_bi_collision_plan_query': (_i, [_vp, _ip, _ip, _szp]),
     'dgfs_bi_collide': (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
     'dgfs_collision_timing_read': (_i, [_vp, _dp, _ip]),
+    'dgfs_limit_linear': (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     'dgfs_transport_work_bytes': (_sz, [_i]),
     'dgfs_transport_rhs': (_i, [C.POINTER(Transport), _vp, _vp, _d, _vp]),
     'dgfs_matop': (_i, [_i, _i, _i, _l, _vp, _vp, _vp, _i, _vp]),

@@ -4,7 +4,9 @@ Same INI schema as the reference dgfs1D/astd/abstd.py:37-480: the mesh,
 basis, velocity mesh, [scattering-model] (the Boltzmann operator Q, any std
 plugin), [penalized-scattering-model] (type shakov-direct-gll, parameters
 from [scattering-model]) and [time-integrator] scheme = one of the IMEX kinds
-(imex-lm-bdf1..4, -cn2, -ad2, -sg2, -ad3, -tvb3, ars-111, ars-222, ars-443).
+(imex-lm-bdf1..4, -cn2, -ad2, -sg2, -ad3, -tvb3, ars-111, ars-222, ars-443,
+sspl-222/443, bdf-111/222/333, boltz-ars-111); [time-integrator] limiter = 1
+applies the linear minmod limiter after every step (nodal bases).
 
     explicit(t, f, out)   = invMass(transport(f))               (abstd.py:285-381)
     explicitQ(t, f, out)  = sum_{r,e} Q(f[r,e]) -> out[r,e]     (abstd.py:395-413)
@@ -17,6 +19,7 @@ import sys
 from ..initialize import initialize
 from ..nputil import subclass_where
 from ..std.std import DGFSStd1D
+from ..transport import Limiter
 from .integrator import DGFSIntegratorAstd
 from .scattering import DGFSScatteringModelAstd
 
@@ -31,6 +34,10 @@ class DGFSAstd1D(DGFSStd1D):
                               scattering_model=cfg.lookup('penalized-scattering-model', 'type'))
         self.psm = pcls(cfg, self.vm, Ne=Ne, eps=self.sm.prefac, device=self.device)
         self._ux = self.torch.zeros(K * Ne * Nv, dtype=self.tdtype, device=self.device)
+        self.limiter = None
+        if int(cfg.lookupordefault('time-integrator', 'limiter', 0)):
+            self.limiter = Limiter(self.basis, self.xsol, self.mesh.invjac[:, 0], Nv,
+                                   self.tdtype, self.device)
         kind = cfg.lookup('time-integrator', 'scheme')
         odescls = subclass_where(DGFSIntegratorAstd, intg_kind=kind)
         return odescls(self.explicit, self.psm, shape, self.tdtype, device=self.device,
@@ -49,6 +56,8 @@ class DGFSAstd1D(DGFSStd1D):
 
     def _advance(self):
         self.odes.integrate(self.time, self.dt, self.nacptsteps, self.u)
+        if self.limiter is not None:                   # abstd.py:459, :475
+            self.limiter(self.u, self.u)
 
 
 def main(argv=None):

@@ -46,6 +46,21 @@ def svd_modes(T):
 class Basis:
     basis_kind = None
 
+    def limiter_ops(self):
+        """(cell-average operator, slope operator of the linear part) of the
+        nodal bases (reference basis.py:289-298, :408-417, :598-607):
+        V = orthonormal basis at the nodes, A = V[:, :1] inv(V)[:1],
+        Lin = V[:, :2] inv(V)[:2], DrLin = D^T Lin with D the nodal
+        derivative matrix (d/dr; multiply by invjac for d/dx)."""
+        Dn = getattr(self, 'Dn', None)
+        if Dn is None or self.Nq != self.K:
+            raise ValueError('the linear limiter needs a nodal basis (%s)' % self.basis_kind)
+        V = ortho_basis_at(self.K, self.z).T
+        invV = np.linalg.inv(V)
+        A = V[:, :1] @ invV[:1, :]
+        Lin = V[:, :2] @ invV[:2, :]
+        return A, Dn.T @ Lin
+
     def __init__(self, cfg):
         self.cfg = cfg
         self.K = cfg.lookupint(basissect, 'K')
@@ -164,6 +179,7 @@ class NodalGLL(Basis):
         H = np.einsum('mq,aq,bq->mab', I, I, I)     # mass-lumped (basis.py:368)
         U, V, sig = svd_modes(H)
         Sx = jac_nodal_basis_at(K, self.z, self.z)
+        self.Dn = Sx
         self.derivMat = Sx.T
         self.Nqf = K
         self.gLD, self.gRD = self.invM[0, :].copy(), self.invM[-1, :].copy()

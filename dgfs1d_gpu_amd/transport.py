@@ -114,6 +114,40 @@ class Transport:
         return self.keep['nw'].cpu().numpy()
 
 
+class Limiter:
+    """Linear minmod limiter (dgfs_limit_linear; reference abstd.py:402-416):
+    cell means and the linear slope by the basis' operators, then the
+    per-element limiting."""
+
+    def __init__(self, basis, xsol, invjac, Nv, dtype, device):
+        import torch
+        A, DrLin = basis.limiter_ops()
+        self.avg, self.drlin = MatOp(A, device), MatOp(DrLin, device)
+        K, Ne = basis.K, len(invjac)
+        self.K, self.Ne, self.Nv = K, Ne, int(Nv)
+        invjac = np.asarray(invjac, dtype=np.float64).ravel()
+        h = 2.0 / invjac                                    # limiters.mako:37
+        xsol = np.asarray(xsol, dtype=np.float64)           # (K, Ne) node coordinates
+        x0 = xsol[0] + h / 2                                # :38
+        # node offsets from the cell centre and element widths: FP64 device arrays
+        self.xrel = _dev_f64((xsol - x0[None, :]).ravel(), device)
+        self.h = _dev_f64(h, device)
+        n = K * Ne * self.Nv
+        self.uavg = torch.zeros(n, dtype=dtype, device=device)
+        self.ulx = torch.zeros(n, dtype=dtype, device=device)
+        self.invjac = torch.tensor(invjac, dtype=dtype, device=device).view(1, Ne, 1)
+
+    def __call__(self, u_in, u_out, stream=None):
+        self.avg(u_in, self.uavg, stream=stream)
+        self.drlin(u_in, self.ulx, stream=stream)
+        self.ulx.view(self.K, self.Ne, self.Nv).mul_(self.invjac)     # mul_by_invjac
+        _lib.check(_lib.lib().dgfs_limit_linear(
+            _lib.dtype_code(u_in.dtype), self.K, self.Ne, self.Nv, _lib.ptr(self.xrel),
+            _lib.ptr(self.h),
+            _lib.ptr(u_in), _lib.ptr(self.uavg), _lib.ptr(self.ulx), _lib.ptr(u_out),
+            _lib.stream_ptr(stream)))
+
+
 class Residual:
     def __init__(self, device):
         import torch

@@ -133,6 +133,20 @@ int dgfs_bi_collide(dgfs_bi_collision_plan_t plan, const void* up, const void* u
                     const void* task_coef, int ngroup, const int* group_start,
                     const int* group_slice, hipStream_t stream);
 
+/* Linear minmod slope limiter (reference limitLin,
+ * dgfs1D/astd/kernels/limiters.mako:17-76, applied by astd/abstd.py:402-416
+ * after each step when [time-integrator] limiter is set; nodal bases).
+ * u, uavg (cell means at every node: the basis' cell-average operator),
+ * ulx (slope of the linear part: derivative operator times invjac) and ulim
+ * are (K, Ne, Nv) arrays of the dtype; xrel[k*Ne + e] = x_k,e - (cell
+ * centre) and h[e] the element widths are FP64 DEVICE arrays.  Elements 0 and Ne-1
+ * are copied; an interior element whose end values leave the minmod band of
+ * its neighbours' means (threshold 1e-6) is replaced by its mean plus the
+ * minmod slope.  ulim may alias u. */
+int dgfs_limit_linear(int dtype, int K, int Ne, int Nv, const double* xrel, const double* h,
+                      const void* u, const void* uavg, const void* ulx, void* ulim,
+                      hipStream_t stream);
+
 /* ---------------------------------------------------------------- transport
  * Boundary kinds (dgfs1D/std/bc.py):
  *   COPY          'dgfs-periodic' / 'dgfs-cyclic' on one rank: ghost = interior
