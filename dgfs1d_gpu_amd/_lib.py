@@ -36,7 +36,7 @@ class Transport(C.Structure):
                 ('gLD', C.c_void_p), ('gRD', C.c_void_p), ('invjac', C.c_void_p),
                 ('cx', C.c_void_p), ('cy', C.c_void_p), ('cz', C.c_void_p),
                 ('cw', C.c_double), ('left', Wall), ('right', Wall),
-                ('work', C.c_void_p), ('nw', C.c_void_p)]
+                ('work', C.c_void_p), ('nw', C.c_void_p), ('Dm', C.c_void_p)]
 
 
 _l = C.c_long
@@ -91,6 +91,7 @@ SIGNATURES = {
     'dgfs_bi_collide': (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
     'dgfs_collision_timing_read': (_i, [_vp, _dp, _ip]),
     'dgfs_limit_linear': (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'dgfs_limit_positivity': (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'dgfs_transport_work_bytes': (_sz, [_i]),
     'dgfs_transport_rhs': (_i, [C.POINTER(Transport), _vp, _vp, _d, _vp]),
     'dgfs_matop': (_i, [_i, _i, _i, _l, _vp, _vp, _vp, _i, _vp]),

@@ -46,6 +46,24 @@ def svd_modes(T):
 class Basis:
     basis_kind = None
 
+    def upwind_ops(self, sigma=-1.0):
+        """Upwind summation-by-parts derivative operators of the nodal-sem-gll
+        basis (reference basis.py:303-312): Vo from the QR factorisation of
+        the orthonormal basis at the nodes, S = Vo diag(0, .., 0, sigma) Vo^T,
+        D+- = D +- S P^-1 / 2 with P = diag(w); returned in the 'ux = A u'
+        orientation (D+^T, D-^T), as the reference's GiMMiK kernels take them."""
+        if self.basis_kind != 'nodal-sem-gll':
+            raise ValueError('the upwind operators are defined for nodal-sem-gll only')
+        V = ortho_basis_at(self.K, self.z).T
+        Vo, _ = np.linalg.qr(V)
+        sig = np.zeros(self.Nq)
+        sig[-1] = sigma
+        S = Vo @ np.diag(sig) @ Vo.T
+        invP = np.diag(1.0 / self.w)
+        Dp = self.Dn + 0.5 * (S @ invP)
+        Dm = self.Dn - 0.5 * (S @ invP)
+        return Dp.T, Dm.T
+
     def limiter_ops(self):
         """(cell-average operator, slope operator of the linear part) of the
         nodal bases (reference basis.py:289-298, :408-417, :598-607):

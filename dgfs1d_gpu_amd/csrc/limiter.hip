@@ -55,8 +55,48 @@ k_limit_linear(int K, int Ne, int Nv, const double* __restrict__ xrel, const dou
     }
 }
 
+// Positivity limiter of the upwind driver (reference limitPos,
+// limiters.mako:178-203, applied by abstdu.py:432-442 after limitLin): per
+// element, theta = min(1, |(a - 1e-13) / (a - min_k u_k)|) and
+// ulim_k = a + theta (u_k - a).  The reference takes the absolute value with
+// fabsf, so theta is rounded to float; a NaN quotient leaves theta = 1.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_limit_positivity(int K, int Ne, int Nv, const T* __restrict__ u, const T* __restrict__ uavg,
+                   T* __restrict__ ulim) {
+    const long gid = long(blockIdx.x) * 256 + threadIdx.x;
+    if (gid >= long(Ne) * Nv) return;
+    const size_t ev = size_t(gid), KNe = size_t(Ne) * Nv;
+    double fmn = 1e10;
+    for (int k = 0; k < K; ++k) fmn = fmin(fmn, double(u[k * KNe + ev]));
+    const double a = double(uavg[ev]);
+    const double th = fmin(1.0, double(fabsf(float((a - 1e-13) / (a - fmn)))));
+    const T at = uavg[ev];
+    for (int k = 0; k < K; ++k) {
+        const T uk = u[k * KNe + ev];
+        ulim[k * KNe + ev] = T(double(at) + th * double(uk - at));
+    }
+}
+
 }  // namespace
 }  // namespace dgfs
+
+extern "C" int dgfs_limit_positivity(int dtype, int K, int Ne, int Nv, const void* u,
+                                     const void* uavg, void* ulim, hipStream_t stream) {
+    DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "limit_positivity: bad dtype");
+    DGFS_REQUIRE(K > 0 && Ne > 0 && Nv > 0, "limit_positivity: bad sizes");
+    DGFS_REQUIRE(u && uavg && ulim, "limit_positivity: null pointer");
+    const long nt = long(Ne) * Nv;
+    const dim3 grid((nt + 255) / 256);
+    if (dtype == DGFS_F64)
+        hipLaunchKernelGGL(dgfs::k_limit_positivity<double>, grid, dim3(256), 0, stream, K, Ne, Nv,
+                           (const double*)u, (const double*)uavg, (double*)ulim);
+    else
+        hipLaunchKernelGGL(dgfs::k_limit_positivity<float>, grid, dim3(256), 0, stream, K, Ne, Nv,
+                           (const float*)u, (const float*)uavg, (float*)ulim);
+    DGFS_HIP_CHECK(hipGetLastError());
+    return 0;
+}
 
 extern "C" int dgfs_limit_linear(int dtype, int K, int Ne, int Nv, const double* xrel,
                                  const double* h, const void* u, const void* uavg,

@@ -146,6 +146,7 @@ struct AdvDev {
     const double* gRD;    // K
     const double* invjac; // Ne
     double cw;
+    const double* Dm;     // K x K or null: upwind split form (D used for c < 0, Dm for c > 0)
 };
 
 template <typename T>
@@ -236,16 +237,34 @@ k_advect(AdvDev a, const T* __restrict__ u, const T* __restrict__ cx, const T* _
     if (e < Ne - 1) uRp = face_value(a, u, a.BL, e + 1, v);
     else uRp = ghost_value<T>(wr, nw[1], uLp, v, c, cy[v], cz[v]);
     const T ac = fabs(c);
-    // flux kernel (std.mako:115-152)
-    const T jR_e = T(0.5) * ac * (uLe - uRe) + T(0.5) * c * (uLe - uRe);
-    const T jL_p = T(0.5) * ac * (uLp - uRp) + T(0.5) * c * (uRp - uLp);
     const T ij = T(a.invjac[e]);
+    if (a.Dm == nullptr) {
+        // flux kernel (std.mako:115-152)
+        const T jR_e = T(0.5) * ac * (uLe - uRe) + T(0.5) * c * (uLe - uRe);
+        const T jL_p = T(0.5) * ac * (uLp - uRp) + T(0.5) * c * (uRp - uLp);
+        for (int l = 0; l < K; ++l) {
+            T d = T(0);
+            for (int m = 0; m < K; ++m) d += T(a.D[l * K + m]) * uk[m];
+            T r = -c * d;                                           // mul_by_adv
+            r += -jL_p * T(a.gRD[l]) + jR_e * T(a.gLD[l]);          // totalFlux
+            ux[(size_t(l) * Ne + e) * n + v] = r * ij;              // mul_by_invjac
+        }
+        return;
+    }
+    // upwind summation-by-parts form (abstdu.py:360-378, upwind.mako): flux
+    // splitting f = c+ u, g = c- u, ux = -(Dm f + Dp g), face jumps
+    // jL = c- (uR - uL), jR = -c+ (uR - uL)
+    const T cp = c > T(0) ? c : T(0), cm = c < T(0) ? c : T(0);
+    const T jR_e = (uRe - uLe) * -cp;
+    const T jL_p = (uRp - uLp) * cm;
+    const double* Dsel = c > T(0) ? a.Dm : a.D;
+    const T cs = c > T(0) ? cp : cm;
     for (int l = 0; l < K; ++l) {
         T d = T(0);
-        for (int m = 0; m < K; ++m) d += T(a.D[l * K + m]) * uk[m];
-        T r = -c * d;                                           // mul_by_adv
-        r += -jL_p * T(a.gRD[l]) + jR_e * T(a.gLD[l]);          // totalFlux
-        ux[(size_t(l) * Ne + e) * n + v] = r * ij;              // mul_by_invjac
+        for (int m = 0; m < K; ++m) d += T(Dsel[l * K + m]) * (cs * uk[m]);
+        T r = -d;
+        r += -jL_p * T(a.gRD[l]) + jR_e * T(a.gLD[l]);              // totalFlux
+        ux[(size_t(l) * Ne + e) * n + v] = r * ij;
     }
 }
 
@@ -360,7 +379,8 @@ template <typename T>
 static int transport_rhs_t(const ::dgfs_transport_t* tp, const void* u, void* ux, double t,
                            hipStream_t st) {
     (void)t;
-    AdvDev a{tp->K, tp->Ne, tp->Nv, tp->D, tp->BL, tp->BR, tp->gLD, tp->gRD, tp->invjac, tp->cw};
+    AdvDev a{tp->K, tp->Ne, tp->Nv, tp->D, tp->BL, tp->BR, tp->gLD, tp->gRD, tp->invjac, tp->cw,
+             tp->Dm};
     WallDev w[2];
     for (int s = 0; s < 2; ++s) {
         const ::dgfs_wall_t& x = s == 0 ? tp->left : tp->right;

@@ -71,13 +71,16 @@ def axnpby_compensated(x0, x0_lo, terms, a0, stream=None):
 class Transport:
     """Fused DG transport operator (std.py:283-348) bound to one basis/mesh."""
 
-    def __init__(self, basis, invjac, vm, left, right, dtype, device):
+    def __init__(self, basis, invjac, vm, left, right, dtype, device, upwind=None):
+        """upwind = (Dp, Dm), the upwind summation-by-parts operators in the
+        'ux = A u' orientation: selects the split form of the upwind driver."""
         import torch
         self.torch = torch
         self.K, self.Ne, self.Nv = basis.K, len(invjac), vm.vsize()
         self.device = device
         self.dtype = dtype
-        keep = dict(D=_dev_f64(basis.mat_deriv, device), BL=_dev_f64(basis.BL, device),
+        D = basis.mat_deriv if upwind is None else upwind[0]
+        keep = dict(D=_dev_f64(D, device), BL=_dev_f64(basis.BL, device),
                     BR=_dev_f64(basis.BR, device), gLD=_dev_f64(basis.gLD, device),
                     gRD=_dev_f64(basis.gRD, device), invjac=_dev_f64(invjac, device))
         nbytes = _lib.lib().dgfs_transport_work_bytes(self.Nv)
@@ -90,8 +93,11 @@ class Transport:
         tp = _lib.Transport()
         tp.dtype = _lib.dtype_code(dtype)
         tp.K, tp.Ne, tp.Nv = self.K, self.Ne, self.Nv
+        if upwind is not None:
+            keep['Dm'] = _dev_f64(upwind[1], device)
         for name in ('D', 'BL', 'BR', 'gLD', 'gRD', 'invjac', 'work', 'nw'):
             setattr(tp, name, keep[name].data_ptr())
+        tp.Dm = keep['Dm'].data_ptr() if upwind is not None else None
         tp.cx, tp.cy, tp.cz = self.cx.data_ptr(), self.cy.data_ptr(), self.cz.data_ptr()
         tp.cw = float(self.cw)
         self.tp = tp
@@ -146,6 +152,24 @@ class Limiter:
             _lib.ptr(self.h),
             _lib.ptr(u_in), _lib.ptr(self.uavg), _lib.ptr(self.ulx), _lib.ptr(u_out),
             _lib.stream_ptr(stream)))
+
+
+class PositivityLimiter:
+    """Positivity limiter of the upwind driver (dgfs_limit_positivity;
+    reference abstdu.py:432-442): cell means by the basis' operator."""
+
+    def __init__(self, basis, Ne, Nv, dtype, device):
+        import torch
+        A, _ = basis.limiter_ops()
+        self.avg = MatOp(A, device)
+        self.K, self.Ne, self.Nv = basis.K, int(Ne), int(Nv)
+        self.uavg = torch.zeros(self.K * self.Ne * self.Nv, dtype=dtype, device=device)
+
+    def __call__(self, u_in, u_out, stream=None):
+        self.avg(u_in, self.uavg, stream=stream)
+        _lib.check(_lib.lib().dgfs_limit_positivity(
+            _lib.dtype_code(u_in.dtype), self.K, self.Ne, self.Nv, _lib.ptr(u_in),
+            _lib.ptr(self.uavg), _lib.ptr(u_out), _lib.stream_ptr(stream)))
 
 
 class Residual:

@@ -147,6 +147,15 @@ int dgfs_limit_linear(int dtype, int K, int Ne, int Nv, const double* xrel, cons
                       const void* u, const void* uavg, const void* ulx, void* ulim,
                       hipStream_t stream);
 
+/* Positivity limiter of the upwind IMEX driver (reference limitPos,
+ * dgfs1D/astd/kernels/limiters.mako:178-203, applied by astd/abstdu.py:432-442
+ * after the linear limiter): per element and velocity, with a the cell mean
+ * (uavg, as for dgfs_limit_linear) and m the smallest nodal value,
+ * theta = min(1, |(a - 1e-13)/(a - m)|) (absolute value in float, as the
+ * reference's fabsf) and ulim = a + theta (u - a).  ulim may alias u. */
+int dgfs_limit_positivity(int dtype, int K, int Ne, int Nv, const void* u, const void* uavg,
+                          void* ulim, hipStream_t stream);
+
 /* ---------------------------------------------------------------- transport
  * Boundary kinds (dgfs1D/std/bc.py):
  *   COPY          'dgfs-periodic' / 'dgfs-cyclic' on one rank: ghost = interior
@@ -187,6 +196,12 @@ typedef struct {
     dgfs_wall_t left, right;
     double* work;              /* device, dgfs_transport_work_bytes(Nv) */
     double* nw;                /* device, 2 doubles: wall number densities */
+    const double* Dm;          /* device K*K or NULL.  Non-null selects the upwind
+                                * summation-by-parts form of the upwind driver
+                                * (dgfs1D/astd/abstdu.py:360-378, upwind.mako):
+                                * ux = -(Dm c+ u + D c- u) with D = Dp^T, Dm = Dm^T of
+                                * basis.py:303-312, face jumps jL = c- (uR - uL),
+                                * jR = -c+ (uR - uL), then the same lift and 1/J */
 } dgfs_transport_t;
 
 size_t dgfs_transport_work_bytes(int Nv);

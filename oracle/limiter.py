@@ -52,3 +52,27 @@ def limit_linear(u, z, D, xsol, invjac):
         for k in range(K):
             out[k, e] = np.where(flag, a + (xsol[k, e] - x0) * s, u[k, e])
     return out
+
+
+def upwind_ops(K, z, w, D, sigma=-1.0):
+    """Upwind SBP operators (basis.py:303-312) in the 'ux = A u' orientation."""
+    V = Q.ortho_basis_at(K, z).T
+    Vo, _ = np.linalg.qr(V)
+    sig = np.zeros(K)
+    sig[-1] = sigma
+    S = Vo @ np.diag(sig) @ Vo.T
+    invP = np.diag(1.0 / w)
+    return (D + 0.5 * S @ invP).T, (D - 0.5 * S @ invP).T
+
+
+def limit_positivity(u, z, D):
+    """limitPos (limiters.mako:178-203): theta = min(1, |(a - 1e-13)/(a - min u)|)
+    with the absolute value rounded to float, ulim = a + theta (u - a)."""
+    K = u.shape[0]
+    A, _ = limiter_ops(K, z, D)
+    a = np.einsum('kl,len->ken', A, u)[0]
+    m = np.minimum(u.min(axis=0), 1e10)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        q = np.abs(((a - 1e-13) / (a - m)).astype(np.float32)).astype(np.float64)
+    th = np.where(np.isnan(q), 1.0, np.minimum(1.0, q))
+    return a[None] + th[None] * (u - a[None])

@@ -88,6 +88,33 @@ class StdOracle:
         ux *= self.invjac[None, :, None]                                # mul_by_invjac
         return ux
 
+    def transport_upwind(self, t, u, Dp, Dm):
+        """Split upwind summation-by-parts transport (astd/abstdu.py:295-380,
+        astd/kernels/upwind.mako): f = c+ u, g = c- u, ux = -(Dm f + Dp g)
+        with Dp, Dm in the 'ux = A u' orientation, jumps jL = c- (uR - uL),
+        jR = -c+ (uR - uL), lift and 1/J as the standard transport."""
+        K, Ne, Nqf = self.K, self.Ne, self.Nqf
+        n = u.shape[-1]
+        c = self.vm.cv[0]
+        usolF = (self.b.face @ u.reshape(K, -1)).reshape(Nqf * Ne, n)
+        mapL = np.arange(Ne + 1) + (Nqf - 1) * Ne - 1
+        mapR = np.arange(Ne + 1)
+        mapL[0], mapR[-1] = 0, Ne * Nqf - 1
+        uL, uR = usolF[mapL].copy(), usolF[mapR].copy()
+        bcL, bcR = uL[0].copy(), uR[Ne].copy()
+        uL[0] = self.wl.apply(bcL)
+        uR[Ne] = self.wr.apply(bcR)
+        cp, cm = np.maximum(c, 0.0), np.minimum(c, 0.0)
+        jL = cm * (uR - uL)                                              # uflux
+        jR = -cp * (uR - uL)
+        f, g = cp * u, cm * u                                            # splitFlux
+        ux = -((Dm @ f.reshape(K, -1)) + (Dp @ g.reshape(K, -1))).reshape(K, Ne, n)
+        for k in range(K):                                               # totalFlux
+            for e in range(Ne):
+                ux[k, e] += -jL[e + 1] * self.b.gRD[k] + jR[e] * self.b.gLD[k]
+        ux *= self.invjac[None, :, None]
+        return ux
+
     def collisions(self, u, ux):
         K, Ne = self.K, self.Ne
         vm = self.vm

@@ -61,3 +61,55 @@ def test_planted_slope_slip_is_caught():
     assert np.abs(good - u).max() > 1e-3
     bad = limit_linear(u, z, D, xsol, np.ones_like(invjac))   # h and slope without 1/J
     assert np.abs(good - bad).max() > 1e-3
+
+
+@pytest.mark.parametrize('K', [3, 4, 5])
+def test_upwind_operators(K):
+    """D+ + D- = 2D; both differentiate polynomials of degree <= K-2 exactly;
+    P (D+ - D-)^T = S is symmetric negative semidefinite (dissipative)."""
+    from oracle.limiter import upwind_ops
+    z, w, D, xsol, invjac = _setup(K)
+    Ap, Am = upwind_ops(K, z, w, D)
+    assert np.abs((Ap + Am) - 2 * D.T).max() < 1e-13
+    for deg in range(K - 1):
+        p, dp = z ** deg, deg * z ** max(deg - 1, 0)
+        assert np.abs(Ap @ p - dp).max() < 1e-12 and np.abs(Am @ p - dp).max() < 1e-12
+    S = np.diag(w) @ (Ap - Am)
+    assert np.abs(S - S.T).max() < 1e-13
+    assert np.linalg.eigvalsh(0.5 * (S + S.T)).max() < 1e-13
+
+
+def test_upwind_transport_without_dissipation_is_the_standard_one():
+    """With sigma = 0 (D+ = D- = D) the split upwind transport equals the
+    standard upwind-flux transport exactly (the flux splitting reproduces
+    1/2 |c| (uL - uR) +- 1/2 c (...) term by term)."""
+    from oracle.deck import Deck
+    from oracle.limiter import upwind_ops
+    from tests.test_gpu_solver import _ini
+    deck = Deck(_ini({('mesh', 'Ne'): 4}))
+    b = deck.basis
+    Ap, Am = upwind_ops(b.K, b.z, b.w, b.D, sigma=0.0)
+    rng = np.random.default_rng(2)
+    u = deck.u0 * (1 + 0.1 * rng.uniform(-1, 1, deck.u0.shape))
+    ref = deck.op.transport(0.0, u)
+    got = deck.op.transport_upwind(0.0, u, Ap, Am)
+    assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max()
+    Ap1, Am1 = upwind_ops(b.K, b.z, b.w, b.D, sigma=-1.0)
+    assert np.abs(deck.op.transport_upwind(0.0, u, Ap1, Am1) - ref).max() > 1e-6 * np.abs(ref).max()
+
+
+def test_positivity_limiter_pins():
+    """Cell means kept, an element with a negative node value scaled towards
+    its mean until the minimum is non-negative, other elements untouched."""
+    from oracle.limiter import limit_positivity
+    z, w, D, xsol, invjac = _setup(3, Ne=6)
+    rng = np.random.default_rng(5)
+    u = rng.uniform(0.1, 1.0, (3, 6, 20))
+    u[0, 2, 3] = -0.05                    # one negative node value, mean still positive
+    out = limit_positivity(u, z, D)
+    mean = lambda v: np.einsum('q,qen->en', w, v) / 2
+    assert np.abs(mean(out) - mean(u)).max() < 1e-13
+    assert out[:, 2, 3].min() >= -1e-7 and out[:, 2, 3].min() < 1e-6
+    untouched = np.ones((6, 20), bool)
+    untouched[2, 3] = False
+    assert np.abs(out - u)[:, untouched].max() < 1e-6 * np.abs(u).max()
