@@ -174,12 +174,14 @@ def steady_state(dev, tau=1e-5, max_steps=4000):
     (checked every 10 steps) falls below tau on the Mach-2 hard-sphere normal
     shock (Ne=32, K=3, N=32^3, M=12, N_rho=16, FP32, SSP-RK2 dt=5e-4), plain
     spectral versus the learned closure (100-call spectral warm start, checks
-    every 10 calls on 25 % of the nodes, tolerance 0.05)."""
+    every 10 calls on 25 % of the nodes, tolerance 0.05, the network
+    re-evaluated every 4th RHS call and its output reused in between)."""
     from dgfs1d_gpu_amd.closure_train import shock_deck
     from dgfs1d_gpu_amd.steady import run_to_steady
     weights = os.path.join(ROOT, 'dgfs1d_gpu_amd', 'data', 'closure_hs_n32.npz')
     extra = ('closure = learned\nclosure-weights = %s\nclosure-check-every = 10\n'
-             'closure-tol = 0.05\nclosure-sample = 0.25\nclosure-start = 100\n' % weights)
+             'closure-tol = 0.05\nclosure-sample = 0.25\nclosure-start = 100\n'
+             'closure-net-every = 4\n' % weights)
     out = {'workload': 'Mach-2 hard-sphere normal shock Ne=32 K=3 N=32^3 M=12 Nrho=16 FP32 '
                        'SSP-RK2 dt=5e-4', 'tau': tau}
     for tag, text in (('spectral', shock_deck(mach=2.0)),
@@ -192,7 +194,8 @@ def steady_state(dev, tau=1e-5, max_steps=4000):
         if tag == 'learned':
             cl = s.sm.closure
             out[tag].update(nodes_learned=int(cl.learned.sum()), nodes=int(cl.ns),
-                            fallbacks=int(cl.fallbacks), readmissions=int(cl.readmissions))
+                            fallbacks=int(cl.fallbacks), readmissions=int(cl.readmissions),
+                            net_every=int(cl.net_every))
         del s
     if out['spectral']['wall_s'] and out['learned']['wall_s']:
         out['speedup'] = out['spectral']['wall_s'] / out['learned']['wall_s']

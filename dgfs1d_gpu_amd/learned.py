@@ -230,7 +230,10 @@ class LearnedClosure:
     model plus candidate correction is within ``tol * readmit`` of the
     spectral result returns to the closure with a fresh correction.
     ``use_net=False`` keeps the same schedule without the network
-    (Q_Shakov + C), the method the network is measured against.  The first
+    (Q_Shakov + C), the method the network is measured against.
+    ``net_every = k > 1`` evaluates the network on the learned nodes every
+    k-th call and reuses its last output R_net in between (Q_Shakov is
+    evaluated every call; checks always use a fresh evaluation).  The first
     ``start`` calls use the spectral operator for every node (the initial
     transient, where the residual changes fastest).
 
@@ -239,7 +242,8 @@ class LearnedClosure:
     prefac: the spectral operator's prefactor (scales R)."""
 
     def __init__(self, net, G, nslice, shakov, spectral, prefac, cw, device, check_every=10,
-                 sample=0.25, tol=0.05, seed=0, use_net=True, start=0, readmit=0.5):
+                 sample=0.25, tol=0.05, seed=0, use_net=True, start=0, readmit=0.5,
+                 net_every=1):
         import torch
         from .collision import task_table
         self.torch, self.net, self.G, self.ns = torch, net, int(G), int(nslice)
@@ -249,6 +253,9 @@ class LearnedClosure:
         self.start = int(start)              # calls served by the spectral operator first
         self.task_table = task_table
         self.readmit = float(readmit)
+        self.net_every = max(1, int(net_every))
+        self.R = None                        # last network output on the learned nodes
+        self._rstale = True
         self.learned = np.ones(self.ns, bool)
         self.calls = 0
         self.cursor = 0
@@ -279,6 +286,7 @@ class LearnedClosure:
         self.sh_tab = self._tab(idx, self.shakov.tdtype) if len(idx) else None
         self.sp_tab = self._tab(spec, self.spectral.tdtype) if len(spec) else None
         self.nl = len(idx)
+        self._rstale = True
 
     def _engine(self, nb):
         if nb not in self._eng:
@@ -317,7 +325,18 @@ class LearnedClosure:
             self._model(u, self.C, allnodes, t.tensor(allnodes, device=self.device),
                         self._tab(allnodes, self.shakov.tdtype))
             self.C.neg_().add_(qs)
-        if self.nl:
+        if self.nl and self.use_net and self.net_every > 1:
+            self.shakov(u, Q, tasks=self.sh_tab)
+            if self._rstale or (self.calls - self.start) % self.net_every == 0:
+                if self.R is None:
+                    self.R = t.zeros_like(u)
+                self.R.zero_()
+                sc, osc = self._scales(u, self.idx)
+                self._engine(self.nl).forward(u, self.idx, sc, mode=1, dst=self.idx, oscale=osc,
+                                              Q=self.R)
+                self._rstale = False
+            Q.add_(self.R)
+        elif self.nl:
             self._model(u, Q, self.idx_np, self.idx, self.sh_tab)
         Q.add_(self.C)
         if self.sp_tab is not None:

@@ -168,7 +168,8 @@ class DGFSVHSGLLScatteringModelStd(DGFSScatteringModelStd):
             use_net=cfg.lookupordefault('scattering-model', 'closure-use-net', 'true').lower()
             in ('1', 'true', 'yes'),
             start=int(cfg.lookupordefault('scattering-model', 'closure-start', 0)),
-            readmit=float(cfg.lookupordefault('scattering-model', 'closure-readmit', 0.5)))
+            readmit=float(cfg.lookupordefault('scattering-model', 'closure-readmit', 0.5)),
+            net_every=int(cfg.lookupordefault('scattering-model', 'closure-net-every', 1)))
 
     def _collide(self, u, Q, stream=None):
         if self.closure is not None:

@@ -71,6 +71,9 @@ def main(argv=None):
     ap.add_argument('--sparse', default=None, metavar='CHECK_EVERY:SAMPLE',
                     help='also run the closure with and without the network at this sparser '
                          'check schedule (first --start value)')
+    ap.add_argument('--net-every', default='1',
+                    help='closure-net-every of the learned runs (comma-separated: one learned '
+                         'run per value, first --start value)')
     ap.add_argument('--out', default=None)
     args = ap.parse_args(argv)
     import torch
@@ -86,15 +89,19 @@ def main(argv=None):
     ce, sa = args.check_every, args.sample
     plan = [('learned' if i == 0 else 'learned_start%d' % st, True, st, ce, sa)
             for i, st in enumerate(starts)] + [('shakov_corrected', False, starts[0], ce, sa)]
+    nets = [int(x) for x in str(args.net_every).split(',')]
+    plan = [p + (nets[0],) for p in plan] + [('learned_net%d' % k, True, starts[0], ce, sa, k)
+                                            for k in nets[1:]]
     if args.sparse:
         ce2, sa2 = args.sparse.split(':')
         plan += [('learned_sparse', True, starts[0], int(ce2), float(sa2)),
                  ('shakov_corrected_sparse', False, starts[0], int(ce2), float(sa2))]
-    for tag, use_net, start, ce_, sa_ in plan:
+        plan = [p if len(p) == 6 else p + (nets[0],) for p in plan]
+    for tag, use_net, start, ce_, sa_, ne_ in plan:
         extra = ('closure = learned\nclosure-weights = %s\nclosure-check-every = %d\n'
                  'closure-tol = %g\nclosure-sample = %g\nclosure-use-net = %s\n'
-                 'closure-start = %d\n'
-                 % (args.weights, ce_, args.tol, sa_, 'true' if use_net else 'false', start))
+                 'closure-start = %d\nclosure-net-every = %d\n'
+                 % (args.weights, ce_, args.tol, sa_, 'true' if use_net else 'false', start, ne_))
         s, rep, m = run_to_steady(shock_deck(mach=args.mach, extra=extra), taus, args.max_steps,
                                   args.nres, dev, log)
         cl = s.sm.closure
@@ -109,6 +116,7 @@ def main(argv=None):
         rep['start'] = start
         rep['check_every'] = ce_
         rep['sample'] = sa_
+        rep['net_every'] = ne_
         rep['speedup_to_tau'] = {k: spec['reached'][k]['wall_s'] / v['wall_s']
                                  for k, v in rep['reached'].items() if k in spec['reached']}
         log('%s: %s' % (tag, json.dumps(rep)))

@@ -152,11 +152,13 @@ def _closure_solvers(cuda, tmp_path, **opts):
     base = base.replace('precision = double', 'precision = single')
     w = str(tmp_path / 'net16.npz')
     ClosureNet(cuda, seed=5).save(w)
-    o = dict(check_every=1000, sample=1.0, tol=0.05, readmit=0.5, start=0)
+    o = dict(check_every=1000, sample=1.0, tol=0.05, readmit=0.5, start=0, net_every=1)
     o.update(opts)
     extra = ('closure = learned\nclosure-weights = %s\nclosure-check-every = %d\n'
              'closure-sample = %g\nclosure-tol = %g\nclosure-readmit = %g\nclosure-start = %d\n'
-             % (w, o['check_every'], o['sample'], o['tol'], o['readmit'], o['start']))
+             'closure-net-every = %d\n'
+             % (w, o['check_every'], o['sample'], o['tol'], o['readmit'], o['start'],
+                o['net_every']))
     text = base.replace('[scattering-model]\n', '[scattering-model]\n' + extra)
     return (DGFSStd1D(Dictionary(text), device=cuda, writers=False),
             DGFSStd1D(Dictionary(base), device=cuda, writers=False))
@@ -206,3 +208,38 @@ def test_closure_fallback_and_readmission(cuda, tmp_path):
     q.zero_()
     sc.sm.collide(u, q)                  # learned again, corrections fresh at this u
     assert float((q - qs).abs().max() / qs.abs().max()) < 1e-5
+
+
+def test_closure_net_every_reuses_network_output(cuda, tmp_path):
+    """closure-net-every = k: the network runs on the first call and every
+    k-th call after it, and its output is reused in between; on an unchanged
+    state the result equals evaluating it every call (to FP32 rounding)."""
+    import torch
+    import dgfs1d_gpu_amd.learned as L
+    counts = {'n': 0}
+    orig = L.ClosureEngine.forward
+
+    def counting(self, *a, **k):
+        counts['n'] += 1
+        return orig(self, *a, **k)
+    L.ClosureEngine.forward = counting
+    try:
+        outs = {}
+        for ne in (1, 3):
+            s, _ = _closure_solvers(cuda, tmp_path, net_every=ne)
+            cl = s.sm.closure
+            assert cl.net_every == ne
+            counts['n'] = 0
+            Q = torch.zeros_like(s.u)
+            for _ in range(6):
+                Q.zero_()
+                cl(s.u, Q)
+            torch.cuda.synchronize()
+            outs[ne] = (Q.clone(), counts['n'])
+        print('net-every forward counts', outs[1][1], outs[3][1], flush=True)
+        assert outs[1][1] == 7 and outs[3][1] == 4     # + the first call's full model
+        # same values; the rounding differs (the default path fuses Q += s*net)
+        d = float((outs[1][0] - outs[3][0]).abs().max() / outs[1][0].abs().max())
+        assert d < 1e-6, d
+    finally:
+        L.ClosureEngine.forward = orig
