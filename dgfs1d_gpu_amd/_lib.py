@@ -36,7 +36,8 @@ class Transport(C.Structure):
                 ('gLD', C.c_void_p), ('gRD', C.c_void_p), ('invjac', C.c_void_p),
                 ('cx', C.c_void_p), ('cy', C.c_void_p), ('cz', C.c_void_p),
                 ('cw', C.c_double), ('left', Wall), ('right', Wall),
-                ('work', C.c_void_p), ('nw', C.c_void_p), ('Dm', C.c_void_p)]
+                ('work', C.c_void_p), ('nw', C.c_void_p), ('Dm', C.c_void_p),
+                ('De', C.c_void_p)]
 
 
 _l = C.c_long

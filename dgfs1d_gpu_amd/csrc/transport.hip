@@ -147,7 +147,23 @@ struct AdvDev {
     const double* invjac; // Ne
     double cw;
     const double* Dm;     // K x K or null: upwind split form (D used for c < 0, Dm for c > 0)
+    const double* De;     // K x K or null: entropy flux-differencing volume term
 };
+
+// logarithmic mean (a - b)/(ln a - ln b), evaluated as the entropy driver's
+// aveL (astd/kernels/limiters.mako, entropy section): with xi = |a/(b+1e-20)|,
+// f = (xi-1)/(xi+1), u = f^2: (a+b)/2 / (1 + u/3 + u^2/5 + u^3/7) for u < 1e-2,
+// else (a+b)/2 / (ln(xi+1e-20) / 2 / (f+1e-20)); in the dtype
+template <typename T>
+DGFS_DEV T log_mean(T a, T b) {
+    const T tiny = T(1e-20);
+    const T xi = fabs(a / (b + tiny));
+    const T f = (xi - T(1)) / (xi + T(1));
+    const T u = f * f;
+    const T den = u < T(1e-2) ? (T(1) + u / T(3) + u * u / T(5) + u * u * u / T(7))
+                              : log(xi + tiny) / T(2) / (f + tiny);
+    return T(0.5) * (a + b) / den;
+}
 
 template <typename T>
 DGFS_DEV T face_value(const AdvDev& a, const T* u, const double* B, int e, int v) {
@@ -238,6 +254,19 @@ k_advect(AdvDev a, const T* __restrict__ u, const T* __restrict__ cx, const T* _
     else uRp = ghost_value<T>(wr, nw[1], uLp, v, c, cy[v], cz[v]);
     const T ac = fabs(c);
     const T ij = T(a.invjac[e]);
+    if (a.De != nullptr) {
+        // entropy-stable form (abstde2.py:214-217, eDeriv): volume term
+        // -2 sum_m De[l][m] c L(u_l, u_m), standard upwind flux and lift
+        const T jR_e = T(0.5) * ac * (uLe - uRe) + T(0.5) * c * (uLe - uRe);
+        const T jL_p = T(0.5) * ac * (uLp - uRp) + T(0.5) * c * (uRp - uLp);
+        for (int l = 0; l < K; ++l) {
+            T r = T(0);
+            for (int m = 0; m < K; ++m) r += T(-2.0 * a.De[l * K + m]) * c * log_mean(uk[l], uk[m]);
+            r += -jL_p * T(a.gRD[l]) + jR_e * T(a.gLD[l]);
+            ux[(size_t(l) * Ne + e) * n + v] = r * ij;
+        }
+        return;
+    }
     if (a.Dm == nullptr) {
         // flux kernel (std.mako:115-152)
         const T jR_e = T(0.5) * ac * (uLe - uRe) + T(0.5) * c * (uLe - uRe);
@@ -380,7 +409,7 @@ static int transport_rhs_t(const ::dgfs_transport_t* tp, const void* u, void* ux
                            hipStream_t st) {
     (void)t;
     AdvDev a{tp->K, tp->Ne, tp->Nv, tp->D, tp->BL, tp->BR, tp->gLD, tp->gRD, tp->invjac, tp->cw,
-             tp->Dm};
+             tp->Dm, tp->De};
     WallDev w[2];
     for (int s = 0; s < 2; ++s) {
         const ::dgfs_wall_t& x = s == 0 ? tp->left : tp->right;

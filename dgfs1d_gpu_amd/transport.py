@@ -71,9 +71,11 @@ def axnpby_compensated(x0, x0_lo, terms, a0, stream=None):
 class Transport:
     """Fused DG transport operator (std.py:283-348) bound to one basis/mesh."""
 
-    def __init__(self, basis, invjac, vm, left, right, dtype, device, upwind=None):
+    def __init__(self, basis, invjac, vm, left, right, dtype, device, upwind=None,
+                 entropy=False):
         """upwind = (Dp, Dm), the upwind summation-by-parts operators in the
-        'ux = A u' orientation: selects the split form of the upwind driver."""
+        'ux = A u' orientation: selects the split form of the upwind driver;
+        entropy=True selects the entropy-stable volume term (basis derivMat)."""
         import torch
         self.torch = torch
         self.K, self.Ne, self.Nv = basis.K, len(invjac), vm.vsize()
@@ -98,6 +100,9 @@ class Transport:
         for name in ('D', 'BL', 'BR', 'gLD', 'gRD', 'invjac', 'work', 'nw'):
             setattr(tp, name, keep[name].data_ptr())
         tp.Dm = keep['Dm'].data_ptr() if upwind is not None else None
+        if entropy:
+            keep['De'] = _dev_f64(basis.derivMat, device)
+        tp.De = keep['De'].data_ptr() if entropy else None
         tp.cx, tp.cy, tp.cz = self.cx.data_ptr(), self.cy.data_ptr(), self.cz.data_ptr()
         tp.cw = float(self.cw)
         self.tp = tp

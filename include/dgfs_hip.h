@@ -202,6 +202,12 @@ typedef struct {
                                 * ux = -(Dm c+ u + D c- u) with D = Dp^T, Dm = Dm^T of
                                 * basis.py:303-312, face jumps jL = c- (uR - uL),
                                 * jR = -c+ (uR - uL), then the same lift and 1/J */
+    const double* De;          /* device K*K or NULL.  Non-null (and Dm NULL) selects the
+                                * entropy-stable volume term of the entropy driver
+                                * (dgfs1D/astd/abstde2.py:214-217, eDeriv of
+                                * limiters.mako): -2 sum_m De[l][m] c L(u_l, u_m) with
+                                * L the logarithmic mean and De the basis derivMat,
+                                * with the standard flux, lift and 1/J */
 } dgfs_transport_t;
 
 size_t dgfs_transport_work_bytes(int Nv);

@@ -115,6 +115,45 @@ class StdOracle:
         ux *= self.invjac[None, :, None]
         return ux
 
+    @staticmethod
+    def log_mean(a, b):
+        """aveL of the entropy driver (astd/kernels/limiters.mako): a stable
+        evaluation of the logarithmic mean (a - b)/(ln a - ln b)."""
+        xi = np.abs(a / (b + 1e-20))
+        f = (xi - 1) / (xi + 1)
+        u = f * f
+        with np.errstate(divide='ignore', invalid='ignore'):
+            den = np.where(u < 1e-2, 1 + u / 3 + u * u / 5 + u * u * u / 7,
+                           np.log(xi + 1e-20) / 2 / (f + 1e-20))
+        return 0.5 * (a + b) / den
+
+    def transport_entropy(self, t, u, Dr, mean=None):
+        """Entropy-stable transport (astd/abstde2.py:214-217, eDeriv): volume
+        term -2 sum_l Dr[k][l] c L(u_k, u_l) with the logarithmic mean (or
+        ``mean``), the standard upwind flux (std.mako:115-152), lift and 1/J."""
+        mean = mean or self.log_mean
+        K, Ne, Nqf = self.K, self.Ne, self.Nqf
+        n = u.shape[-1]
+        c = self.vm.cv[0]
+        usolF = (self.b.face @ u.reshape(K, -1)).reshape(Nqf * Ne, n)
+        mapL = np.arange(Ne + 1) + (Nqf - 1) * Ne - 1
+        mapR = np.arange(Ne + 1)
+        mapL[0], mapR[-1] = 0, Ne * Nqf - 1
+        uL, uR = usolF[mapL].copy(), usolF[mapR].copy()
+        bcL, bcR = uL[0].copy(), uR[Ne].copy()
+        uL[0] = self.wl.apply(bcL)
+        uR[Ne] = self.wr.apply(bcR)
+        jL = 0.5 * np.abs(c) * (uL - uR) + 0.5 * c * (uR - uL)
+        jR = 0.5 * np.abs(c) * (uL - uR) + 0.5 * c * (uL - uR)
+        ux = np.zeros_like(u)
+        for k in range(K):                                               # eDeriv
+            for l in range(K):
+                ux[k] += -2.0 * Dr[k, l] * c * mean(u[k], u[l])
+        for k in range(K):                                               # totalFlux
+            for e in range(Ne):
+                ux[k, e] += -jL[e + 1] * self.b.gRD[k] + jR[e] * self.b.gLD[k]
+        return ux * self.invjac[None, :, None]
+
     def collisions(self, u, ux):
         K, Ne = self.K, self.Ne
         vm = self.vm

@@ -113,3 +113,30 @@ def test_positivity_limiter_pins():
     untouched = np.ones((6, 20), bool)
     untouched[2, 3] = False
     assert np.abs(out - u)[:, untouched].max() < 1e-6 * np.abs(u).max()
+
+
+def test_entropy_transport_pins():
+    """The log mean: L(a, a) = a, the closed form (a - b)/(ln a - ln b) away
+    from a = b, symmetry.  The flux-differencing volume term with the
+    arithmetic mean in place of L is exactly -c D u, so the entropy transport
+    then equals the standard transport; with L it differs slightly on smooth
+    positive data."""
+    from oracle.deck import Deck
+    from oracle.std import StdOracle
+    from tests.test_gpu_solver import _ini
+    a = np.array([0.3, 1.0, 2.0, 5e-3])
+    b = np.array([0.7, 1.0, 2.0 * (1 + 1e-3), 4e-3])
+    L = StdOracle.log_mean(a, b)
+    closed = np.where(a == b, a, (a - b) / (np.log(a) - np.log(b)))
+    assert np.allclose(L, closed, rtol=1e-12)
+    assert np.allclose(StdOracle.log_mean(b, a), L, rtol=1e-15)
+    deck = Deck(_ini({('mesh', 'Ne'): 4}))
+    rng = np.random.default_rng(4)
+    u = deck.u0 * (1 + 0.1 * rng.uniform(-1, 1, deck.u0.shape))
+    Dr = deck.basis.deriv                       # the 'ux = A u' derivative (derivMat)
+    arith = lambda x, y: 0.5 * (x + y)
+    ref = deck.op.transport(0.0, u)
+    assert np.abs(deck.op.transport_entropy(0.0, u, Dr, mean=arith) - ref).max() < \
+        1e-12 * np.abs(ref).max()
+    d = np.abs(deck.op.transport_entropy(0.0, u, Dr) - ref).max() / np.abs(ref).max()
+    assert 1e-9 < d < 1e-1
