@@ -254,3 +254,48 @@ def test_bi_steps_projected(cuda, scheme):
             err = _rel(s.us[p].cpu().numpy().reshape(us[p].shape), us[p])
             print('bi projected step', scheme, k, p, err, flush=True)
             assert err < 1e-11
+
+
+@pytest.mark.parametrize('over,tol', [
+    ({('config', 'precision'): 'single'}, 2e-5),
+    ({('soln-bcs-xlo', 'type'): 'dgfs-inlet', ('soln-bcs-xlo', 'nden1'): 1.7e21,
+      ('soln-bcs-xlo', 'nden2'): 7.9e20, ('soln-bcs-xlo', 'ux'): 30.0}, 1e-11),
+], ids=['fp32', 'inlet'])
+def test_bi_rhs_more(cuda, over, tol):
+    """FP32 state and collisions (config 5's precision) against the FP64
+    oracle, and the mixture inlet (no velocity shift in its fac)."""
+    import torch
+    deck, s, us = _pair(cuda, over)
+    for p in range(2):
+        out = torch.zeros_like(s.us[p])
+        s.rhs(p, 0.0, s.us, out)
+        torch.cuda.synchronize()
+        ref = deck.op.rhs(p, 0.0, [s.us[q].double().cpu().numpy().reshape(us[q].shape)
+                                    for q in range(2)])
+        err = _rel(out.double().cpu().numpy().reshape(ref.shape), ref)
+        print('bi rhs', list(over.values())[0], p, err, flush=True)
+        assert err < tol
+
+
+def test_bi_expr_wall_matches_diffuse_wall(cuda):
+    """dgfs-wall-expr-diffuse with constant expressions equals
+    dgfs-wall-diffuse with the same wall state (the expression wall's
+    exp(-m |c-u|^2/T) enters the kernel as T/m)."""
+    import torch
+    over = {('soln-bcs-xlo', 'T'): 263, ('soln-bcs-xlo', 'uy'): -40.0}
+    _, s1, us = _pair(cuda, over)
+    over2 = dict(over)
+    over2[('soln-bcs-xlo', 'type')] = 'dgfs-wall-expr-diffuse'
+    from dgfs1d_gpu_amd.bi.bi import DGFSBi1D
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    s2 = DGFSBi1D(Dictionary(_ini(over2)), device=cuda, writers=False)
+    for p in range(2):
+        s2.us[p].copy_(s1.us[p])
+    for p in range(2):
+        o1, o2 = torch.zeros_like(s1.us[p]), torch.zeros_like(s2.us[p])
+        s1.rhs(p, 0.0, s1.us, o1)
+        s2.rhs(p, 0.0, s2.us, o2)
+        torch.cuda.synchronize()
+        err = float((o1 - o2).abs().max() / o1.abs().max())
+        print('bi expr wall', p, err, flush=True)
+        assert err < 1e-12
