@@ -1,5 +1,7 @@
 """Host-side logic without a GPU: INI dictionary semantics, CLI overrides,
 restricted expressions, plugin registry, spherical rules, velocity mesh."""
+import os
+
 import numpy as np
 import pytest
 
@@ -263,3 +265,14 @@ def test_bi_moments_of_a_maxwellian_state():
         m = vm.masses()[p]
         assert np.allclose(out[..., o + 5], m * m * nden[p] * 8.3144598 / 6.0221409e+23 * 273.0,
                            rtol=1e-6)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N with N > 1 exits with a message (one-GPU build)."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, os.path.join(root, 'bench.py'), '--gpus', '2'],
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 2
+    assert 'one GPU' in p.stderr
