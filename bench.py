@@ -410,7 +410,7 @@ def main(argv=None):
     if not args.no_steady:
         out['time_to_steady_state'] = steady_state(dev)
     if not args.no_bi:
-        out['bi_couette_published'] = bi_couette_published(dev)
+        out['bi_couette_published'] = bi_couette_published(dev, ((72, 3, 20), (72, 3, 32)))
     if not args.no_cpu_baseline:
         cores = min(16, os.cpu_count() or 1)
         out['cpu_baseline'] = cpu_baseline(text, cores)

@@ -276,3 +276,26 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=300)
     assert p.returncode == 2
     assert 'one GPU' in p.stderr
+
+
+def test_bi_maxwellian_expr_with_constants_equals_maxwellian():
+    """The mixture's maxwellian-expr initial condition with constant
+    expressions gives the maxwellian one at every solution point."""
+    from dgfs1d_gpu_amd.bi.initcond import (DGFSMaxwellianExprInitConditionBi,
+                                            DGFSMaxwellianInitConditionBi)
+    from dgfs1d_gpu_amd.bi.velocitymesh import DGFSVelocityMeshBi
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    from tests.test_gpu_bi_solver import _ini
+    cfg = Dictionary(_ini({('soln-ics', 'ux'): 20.0, ('velocity-mesh', 'Nv'): 24}))
+    vm = DGFSVelocityMeshBi(cfg)
+    a = DGFSMaxwellianInitConditionBi(cfg, vm, 'soln-ics')
+    cfg2 = Dictionary(_ini({('soln-ics', 'type'): 'maxwellian-expr', ('soln-ics', 'ux'): 20.0,
+                            ('velocity-mesh', 'Nv'): 24}))
+    b = DGFSMaxwellianExprInitConditionBi(cfg2, vm, 'soln-ics')
+    xsol = np.linspace(0, 1, 6).reshape(3, 2)
+    for p in range(2):
+        u1 = np.zeros((3, 2, vm.vsize()))
+        u2 = np.zeros_like(u1)
+        a.apply_init_vals(p, u1, 3, 2, xsol)
+        b.apply_init_vals(p, u2, 3, 2, xsol)
+        assert np.allclose(u1, u2, rtol=1e-13, atol=0)
