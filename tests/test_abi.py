@@ -36,3 +36,29 @@ def test_errors_are_reported():
                                                C.byref(h))
     assert st == 2
     assert b'unsupported N=7' in _lib.lib().dgfs_last_error()
+
+
+def test_new_entry_points_reject_bad_arguments():
+    """Argument checks of the mixture, limiter and projection entry points
+    fail before any device work (status 2, message kept)."""
+    import ctypes as C
+    from dgfs1d_gpu_amd import _lib
+    L = _lib.lib()
+    h = C.c_void_p()
+    one = (C.c_double * 3)(1.0, 0.0, 0.0)
+    st = L.dgfs_bi_collision_plan_create(10, 1, 1, 1.0, one, one, one, 1.0, 1, one, 1.0, 1.0,
+                                         1.0, 0.5, 1.0, 0, 1, C.byref(h))
+    assert st == 2 and b'unsupported N=10' in L.dgfs_last_error()
+    st = L.dgfs_bi_collision_plan_create(48, 1, 1, 1.0, one, one, one, 1.0, 1, one, 1.0, 1.0,
+                                         1.0, 0.5, 1.0, 0, 1, C.byref(h))
+    assert st == 2 and b'DGFS_F32 only' in L.dgfs_last_error()
+    st = L.dgfs_bi_collision_plan_create(8, 1, 1, 1.0, one, one, one, 1.0, 1, one, -1.0, 1.0,
+                                         1.0, 0.5, 1.0, 0, 1, C.byref(h))
+    assert st == 2 and b'masses must be positive' in L.dgfs_last_error()
+    assert L.dgfs_limit_linear(0, 0, 3, 8, None, None, None, None, None, None, None) == 2
+    assert b'bad sizes' in L.dgfs_last_error()
+    assert L.dgfs_limit_positivity(5, 3, 3, 8, None, None, None, None) == 2
+    assert b'bad dtype' in L.dgfs_last_error()
+    assert L.dgfs_project_conservative_mixture(0, 1, 8, None, None, None, 1.0, -1.0, None, None,
+                                               None, None, None, 0, None, None) == 2
+    assert b'masses must be positive' in L.dgfs_last_error()
