@@ -148,7 +148,9 @@ extern "C" int dgfs_bi_collision_plan_create(int N, int M, int Nrho, double L, c
         for (int pp = 0; pp < Mpre; ++pp) {
             const double base = 1.0 + sz[3 * y] * szpre[3 * pp] + sz[3 * y + 1] * szpre[3 * pp + 1] +
                                 sz[3 * y + 2] * szpre[3 * pp + 2];
-            Bt[size_t(y) * Mpre + pp] = std::pow(base, eta);
+            // an exactly antipodal pair (1 + s.s' = 0) can round below zero,
+            // where pow with a fractional exponent is NaN: clamp at its exact value
+            Bt[size_t(y) * Mpre + pp] = std::pow(base > 0.0 ? base : 0.0, eta);
         }
     for (int pp = 0; pp < Mpre; ++pp)
         for (int y = 0; y < M; ++y) Bsum[pp] += Bt[size_t(y) * Mpre + pp];

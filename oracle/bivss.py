@@ -105,7 +105,7 @@ class BiPrecomp:
         self.l = l
         fac = np.pi / L * self.qz                                     # vss.mako:40
         self.a = fac[:, None, None] * (self.sz @ l)[None]             # (Nrho, M, n)
-        B = (1.0 + self.sz @ self.szpre.T) ** eta                     # (M, Mpre)
+        B = np.maximum(1.0 + self.sz @ self.szpre.T, 0.0) ** eta      # (M, Mpre)
         spl = self.szpre @ l                                          # (Mpre, n)
         mpq = mq / (mp + mq)                                          # vss.mako:54
         fac_b = swpre * self.qz ** (gam + 2.0)                        # scattering.py:197

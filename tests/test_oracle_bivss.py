@@ -158,3 +158,13 @@ def test_mixture_projection_pins(mixture):
     # planted slip: species 0's mass on both momentum rows
     S0, S1 = project_mixture(Q0, Q1, cv, m[0], m[0])
     assert np.abs(resid(S0, S1)[2:5]).max() > 1e3 * np.abs(resid(P0, P1)[2:5]).max()
+
+
+def test_antipodal_rules_give_finite_kernels():
+    """The same antipodal design as both the M rule and the pre-rule: the
+    exactly opposite pairs give (1 + s.s') = 0 (possibly rounded below zero);
+    the kernels stay finite."""
+    full = load_design('womersley', 12)
+    qz, qw = _radial(4.0, 3)
+    pre = BiPrecomp(8, 4.0, qz, qw, full, 4 * np.pi / 12, full, 1.0, 2.0, 1.4, 0.8)
+    assert np.isfinite(pre.b).all() and np.isfinite(pre.c).all()
