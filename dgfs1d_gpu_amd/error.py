@@ -1,9 +1,9 @@
 """L1 difference between a nodal DG solution on Ne elements and one on 2 Ne
 elements (reference dgfs1D/error.py:9-48, ``dgfsStdError1D``), for the
-distribution files this package writes (``.npz`` with coeff, K, Ne, Nv; the
-reference reads its HDF5 files).
+distribution files the solvers write (HDF5 with dataset coeff and attributes
+K, Ne, Nv, as the reference's; ``.npz`` files of the same keys also work).
 
-    python -m dgfs1d_gpu_amd.error coarse.npz fine.npz
+    python -m dgfs1d_gpu_amd.error coarse.h5_rank=0 fine.h5_rank=0
 
 The coarse solution's nodal polynomial on each element (GLL nodes) is
 evaluated at the GLL nodes of its two halves and compared with the fine
@@ -17,14 +17,21 @@ from pathlib import Path
 
 import numpy as np
 
+from . import h5
 from .quadratures import nodal_basis_at, zwglj
 
 
 def decode(path):
-    """(Ne, K, coeff (K, Ne, Nv)) of a written distribution."""
-    with np.load(path, allow_pickle=False) as d:
-        K, Ne, Nv = int(d['K']), int(d['Ne']), int(d['Nv'])
-        return Ne, K, np.array(d['coeff']).reshape(K, Ne, Nv)
+    """(Ne, K, coeff (K, Ne, Nv)) of a written distribution (error.py:9-14):
+    an HDF5 file of the reference's layout, or a ``.npz`` of the same keys."""
+    if h5.is_hdf5(path):
+        arr, d = h5.read(path)['coeff']
+    else:
+        with np.load(path, allow_pickle=False) as z:
+            d = {k: z[k] for k in ('K', 'Ne', 'Nv')}
+            arr = np.array(z['coeff'])
+    K, Ne, Nv = int(d['K']), int(d['Ne']), int(d['Nv'])
+    return Ne, K, np.asarray(arr, dtype=np.float64).reshape(K, Ne, Nv)
 
 
 def compare(K, f, f2):

@@ -194,8 +194,11 @@ basename = dist-{t:.2f}
     assert len(s.residual.history) == 3 and all(np.all(r > 0) for _, _, r in s.residual.history)
     mom = np.loadtxt(tmp_path / 'mom-0.03.txt')
     assert mom.shape == (2 * 4, 1 + 14)
-    assert (tmp_path / 'dist-0.02.npz').exists()
-    r = DGFSBi1D(Dictionary(text), device=cuda, restart=str(tmp_path / 'dist-0.02.npz'),
+    dist = tmp_path / 'dist-0.02.h5_rank=0'         # HDF5, the reference's name and layout
+    assert dist.exists()
+    from dgfs1d_gpu_amd import h5
+    assert sorted(h5.read(str(dist))) == ['coeff0', 'coeff1']
+    r = DGFSBi1D(Dictionary(text), device=cuda, restart=str(dist),
                  writers=False)
     assert abs(r.time - 0.02) < 1e-12
     s2 = DGFSBi1D(Dictionary(text), device=cuda, writers=False)
