@@ -210,3 +210,54 @@ def test_moments(cuda):
     err = float((np.abs(got - ref).max(axis=(0, 1)) / scale).max())
     print('moments', err)
     assert err < 1e-12
+
+
+def test_writers_and_restart(cuda, tmp_path):
+    """The moment writer's text table (std/moments.py:153-160, :195-220) at an
+    output time holds the positions and the dimensional oracle moments of the
+    state in the HDF5 distribution file written at that time
+    (std/distribution.py:36-41); a restart from that file (std/std.py:235-247)
+    resumes at its time and reproduces the uninterrupted run."""
+    import torch
+    from dgfs1d_gpu_amd import h5
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    from dgfs1d_gpu_amd.std.moments import dimensional
+    from dgfs1d_gpu_amd.std.std import DGFSStd1D
+    text = _ini({('time-integrator', 'scheme'): 'ssp-rk2'}) + """
+[dgfsresidual]
+nsteps = 1
+[dgfsmomwriter]
+dt-out = 0.002
+basedir = %s
+basename = mom-{t:.3f}
+[dgfsdistwriter]
+dt-out = 0.002
+basedir = %s
+basename = dist-{t:.3f}
+""" % (tmp_path, tmp_path)
+    deck = Deck(_ini({}))
+    s = DGFSStd1D(Dictionary(text), device=cuda)
+    s.run(4)
+    dist = tmp_path / 'dist-0.002.h5_rank=0'
+    data = h5.read(str(dist))
+    coeff, attrs = data['coeff']
+    assert attrs == {'time': pytest.approx(0.002, abs=1e-15), 'K': 3, 'Ne': 3, 'Nv': 512}
+    lines = open(tmp_path / 'mom-0.002.txt').read().splitlines()
+    # numpy.savetxt of the reference's header 't={0} \n {1}' with comments '#'
+    assert lines[0] == '#t=0.002 '
+    assert lines[1] == '# x0, rho, U:x, U:y, T, Q:x, Q:y, P:xx, P:yy, P:xy, p'
+    tab = np.loadtxt(tmp_path / 'mom-0.002.txt')
+    nd = OS.moments(coeff.reshape(3, 3, 512), deck.basis.interp, deck.vm)   # (Nqr, Ne, 9)
+    ref = dimensional(nd, s.vm).swapaxes(0, 1).reshape(-1, 10)
+    err = float((np.abs(tab[:, 1:] - ref).max(axis=0) / np.abs(ref).max(axis=0)).max())
+    print('moment table', err, flush=True)
+    assert err < 1e-12
+    # positions: the 4 Gauss-Lobatto points (+-1, +-1/sqrt 5) of each of the 3 elements
+    r = np.array([-1.0, -1.0 / np.sqrt(5.0), 1.0 / np.sqrt(5.0), 1.0])
+    x = -0.5 + (np.arange(3)[:, None] + 0.5 * (r[None, :] + 1.0)) / 3.0
+    assert tab.shape == (4 * 3, 11)
+    assert np.abs(tab[:, 0] - x.ravel()).max() < 1e-15
+    r = DGFSStd1D(Dictionary(text), device=cuda, restart=str(dist), writers=False)
+    assert abs(r.time - 0.002) < 1e-15
+    r.run(2)
+    assert torch.equal(r.u, s.u)
