@@ -194,6 +194,25 @@ class DGFSVHSScatteringModelStd(DGFSVHSGLLScatteringModelStd):
     _variant = _lib.VHS
 
 
+class DGFSMaxwellGLLScatteringModelStd(DGFSVHSGLLScatteringModelStd):
+    """Maxwell molecules on the vhs-gll operator (std/scattering.py:707-743):
+    gamma = 0, omega = 1 and prefactor H0 u0^2 / n0 / eps, eps read from
+    [scattering-model]; the BGK and ES-BGK prefactors equal it.  With
+    prefactor 1/(4 pi) the operator is the Boltzmann operator of the kernel
+    B = 1/(4 pi), whose closed-form Bobylev-Krook-Wu solution is the 'bkw'
+    initial condition.  As in the reference, the nodal-basis check applies to
+    'vhs-gll' only."""
+    scattering_model = 'maxwell-gll'
+
+    def load_parameters(self):
+        vm = self.vm
+        self._gamma, self._omega = 0.0, 1.0
+        nd = vm.H0() * vm.u0() ** 2 / vm.n0()
+        eps = float(self.cfg.lookupfloat('scattering-model', 'eps'))
+        self._prefactor = nd / eps
+        self._prefactorBGK = self._prefactorESBGK = self._prefactor
+
+
 class DGFSShakovDirectGLLScatteringModelStd(DGFSScatteringModelStd):
     """BGK-Shakov relaxation, iteration-free (std/scattering.py:1636-1833)."""
     scattering_model = 'shakov-direct-gll'
@@ -241,6 +260,20 @@ class DGFSShakovDirectGLLScatteringModelStd(DGFSScatteringModelStd):
         tab = _single_task(self._single, self._Ne, self.vm.vsize(), d_Q, elem, modein, modeout,
                            self.model.tdtype, self._device)
         self.model(d_f, d_Q, tasks=tab)
+
+
+class DGFSBGKDirectGLLScatteringModelStd(DGFSShakovDirectGLLScatteringModelStd):
+    """BGK relaxation, iteration-free (std/scattering.py:1186-1369): the
+    Maxwellian of the node's (rho, u, T) as target, frequency
+    prefac rho T^(1-omega), prefac = t0 Pr p0 / mu(T0) with Pr = 1.  That is
+    the Shakov term with Pr = 1 (its heat-flux factor (1 - Pr)/5 vanishes and
+    its prefactor t0 p0 / mu(T0) is the same), so the Shakov kernels run it;
+    a Pr in the deck is ignored, as in the reference."""
+    scattering_model = 'bgk-direct-gll'
+
+    def load_parameters(self):
+        super().load_parameters()
+        self._Pr = 1.0
 
 
 def _single_task(cache, Ne, Nv, d_Q, elem, modein, modeout, tdtype, device):

@@ -12,6 +12,8 @@ kernels of std/kernels/scattering/shakov-direct-gll.mako:
   equiSkDistCompute              :106-142  f_S
   output                         :145-167  Q += prefac rho / T^(omega-1) (f_S - f)
 
+``q_bgk`` restates 'bgk-direct-gll' (a Maxwellian target) the same way.
+
 ``project_conservative`` is the Euclidean-nearest slice with vanishing
 mass, momentum and energy moments (no reference counterpart), computed with
 a least-squares solve so that it does not share the product's Gram-inverse
@@ -46,6 +48,21 @@ def q_shakov(f, cv, cw, prefac, omega, Pr):
     """Increment added to Q[modeout, elem] by one fs() call."""
     rho, _, T, _ = shakov_moments(f, cv, cw)
     return prefac * (rho / T ** (omega - 1.0)) * (shakov_fS(f, cv, cw, Pr) - f)
+
+
+def q_bgk(f, cv, cw, prefac, omega):
+    """One 'bgk-direct-gll' fs() call (std/scattering.py:1313-1369, kernels of
+    std/kernels/scattering/bgk-direct-gll.mako): mom1/mom01Norm (:32-61) give
+    u = sum f c / sum f and rho = cw sum f, mom2/equiDistInit (:63-92) give
+    T = cw sum f |c - u|^2 / (1.5 rho), and output (:95-122) adds
+    prefac rho / T^(omega-1) (rho/(pi T)^1.5 exp(-|c - u|^2/T) - f)."""
+    s0 = f.sum()
+    ux, uy, uz = (f * cv[0]).sum() / s0, (f * cv[1]).sum() / s0, (f * cv[2]).sum() / s0
+    rho = s0 * cw
+    C2 = (cv[0] - ux) ** 2 + (cv[1] - uy) ** 2 + (cv[2] - uz) ** 2
+    T = (f * C2).sum() * cw / (1.5 * rho)
+    fe = rho / (np.pi * T) ** 1.5 * np.exp(-C2 / T)
+    return prefac * (rho / T ** (omega - 1.0)) * (fe - f)
 
 
 def shakov_prefactor(H0, u0, T0, n0, muRef, Tref, omega, R0=8.3144598, NA=6.0221409e+23):

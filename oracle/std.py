@@ -170,6 +170,8 @@ class StdOracle:
                     elif self.model == 'shakov-direct-gll':
                         omega, Pr = self.shakov
                         acc += OR.q_shakov(f[r, e], vm.cv, vm.cw, self.prefac, omega, Pr)
+                    elif self.model == 'bgk-direct-gll':
+                        acc += OR.q_bgk(f[r, e], vm.cv, vm.cw, self.prefac, self.shakov[0])
                     else:
                         raise ValueError(self.model)
                 ux[m, e] += OR.project_conservative(acc, vm.cv) if self.project else acc

@@ -40,7 +40,7 @@ def test_registries():
     for k in ('dgfs-wall-diffuse', 'dgfs-inlet', 'dgfs-periodic', 'dgfs-cyclic', 'dgfs-vanish',
               'dgfs-wall-expr-diffuse', 'dgfs-inlet-nondim'):
         subclass_where(DGFSBCStd, type=k)
-    for k in ('vhs', 'vhs-gll'):
+    for k in ('vhs', 'vhs-gll', 'maxwell-gll', 'shakov-direct-gll'):
         subclass_where(DGFSScatteringModelStd, scattering_model=k)
     with pytest.raises(KeyError):
         subclass_where(Basis, basis_kind='nope')
@@ -299,3 +299,22 @@ def test_bi_maxwellian_expr_with_constants_equals_maxwellian():
         a.apply_init_vals(p, u1, 3, 2, xsol)
         b.apply_init_vals(p, u2, 3, 2, xsol)
         assert np.allclose(u1, u2, rtol=1e-13, atol=0)
+
+
+def test_maxwell_gll_parameters():
+    """'maxwell-gll' (std/scattering.py:707-743): gamma 0, omega 1, prefactor
+    H0 u0^2 / n0 / eps (here 1/(4 pi)), BGK and ES-BGK prefactors equal."""
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    from dgfs1d_gpu_amd.std.scattering import DGFSMaxwellGLLScatteringModelStd
+    from dgfs1d_gpu_amd.std.velocitymesh import DGFSVelocityMeshStd
+    from tests.test_gpu_solver import BKW
+    R0, NA = 8.3144598, 6.0221409e+23
+    u0 = np.sqrt(2 * R0 / 39.96e-3 * 273.0)
+    n0 = 2.3e-4 / 39.96e-3 * NA
+    cfg = Dictionary(BKW % (4 * np.pi * u0 ** 2 / n0))
+    m = DGFSMaxwellGLLScatteringModelStd.__new__(DGFSMaxwellGLLScatteringModelStd)
+    m.cfg, m.vm = cfg, DGFSVelocityMeshStd(cfg, device='cpu')
+    m.load_parameters()
+    assert m._gamma == 0.0 and m.omega == 1.0
+    assert m.prefac == pytest.approx(1 / (4 * np.pi), rel=1e-12)
+    assert m.prefacBGK == m.prefacESBGK == m.prefac

@@ -108,3 +108,38 @@ def test_planted_mistake(monkeypatch):
     QG = OC._ifft_u((pre.weight()[:, None, :] * t1).sum(axis=(0, 1)), N)
     bad2 = QG.real - f * OC._ifft_u(pre.bb2 * FTg, N).real
     assert np.abs(bad2 - ref).max() > 1e-3 * np.abs(ref).max()
+
+
+def bkw(c2, t):
+    """Bobylev-Krook-Wu solution of the space-homogeneous Boltzmann equation
+    for Maxwell molecules with kernel 1/(4 pi) (std/initcond.py:121-141 for f),
+    and its time derivative, K = 1 - exp(-t/6), dK/dt = (1 - K)/6."""
+    K = 1.0 - np.exp(-t / 6.0)
+    A = 1.0 / (2.0 * (2.0 * np.pi * K) ** 1.5)
+    E = np.exp(-c2 / (2.0 * K))
+    f = A * E * ((5.0 * K - 3.0) / K + (1.0 - K) / K ** 2 * c2)
+    dfdK = f * (-1.5 / K + c2 / (2.0 * K * K)) + A * E * (3.0 / K ** 2 + c2 * (K - 2.0) / K ** 3)
+    return f, dfdK * (1.0 - K) / 6.0
+
+
+def test_bkw_closed_form_pins_normalisation():
+    """gamma = 0 and prefactor 1/(4 pi) ('maxwell-gll'): the operator applied to
+    f_BKW(t) is d f_BKW/dt, to spectral accuracy that improves with N.  This
+    fixes every constant of the pipeline at once (1/n scalings, radial and
+    sphere weights, bb1, bb2): a factor-2 slip in the sphere weight fails."""
+    errs = []
+    for N, Nrho in ((16, 8), (24, 12)):
+        vm = OVM.VelocityMesh(1e-3, 273.0, 2.3e-4, 39.96e-3, N, Nrho, 0.0, 273.0, 6.0, 6)
+        pre = OC.Precomp(N, vm.L, vm.qz, vm.qw, vm.sz, vm.sw, 0.0)
+        c2 = (vm.cv ** 2).sum(0)
+        e = 0.0
+        for t in (6.0, 7.0):
+            f, dfdt = bkw(c2, t)
+            q = OC.q_vhs_gll(pre, f, 1.0 / (4.0 * np.pi))
+            e = max(e, np.abs(q - dfdt).max() / np.abs(dfdt).max())
+            if N == 24:
+                slip = OC.Precomp(N, vm.L, vm.qz, vm.qw, vm.sz, 2.0 * vm.sw, 0.0)
+                qs = OC.q_vhs_gll(slip, f, 1.0 / (4.0 * np.pi))
+                assert np.abs(qs - dfdt).max() > 0.5 * np.abs(dfdt).max()
+        errs.append(e)
+    assert errs[1] < 5e-4 and errs[1] < errs[0] / 10

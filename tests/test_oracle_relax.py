@@ -132,3 +132,29 @@ def test_projection_properties():
     # minimal norm: any other conservative candidate is farther from Q
     other = Qc + 1e-3 * OR.project_conservative(rng.standard_normal(Q.shape), cv)
     assert np.linalg.norm(Q - other) > np.linalg.norm(Q - Qc)
+
+
+def test_bgk_pins():
+    """oracle.relax.q_bgk ('bgk-direct-gll'): a Maxwellian is a fixed point;
+    the target f + Q/nu keeps the node's rho, u and T (independent moment
+    loop) and has zero heat flux; mass, momentum and energy are conserved;
+    Q(2f) = 4 Q(f); and the term equals Shakov's at Pr = 1 (the two oracle
+    restatements, from different reference kernels, agree)."""
+    cv, cw = grid()
+    m = maxw(cv, 1.3, (0.2, -0.1, 0.05), 0.9)
+    assert np.abs(OR.q_bgk(m, cv, cw, 2.0, 0.81)).max() < 1e-10 * np.abs(m).max()
+    f = bimodal(cv)
+    rho, u, T, q = moments_of(f, cv, cw)
+    assert np.abs(q).max() > 1e-2
+    nu = rho * T ** (1.0 - 0.81)
+    target = f + OR.q_bgk(f, cv, cw, 1.0, 0.81) / nu
+    rB, uB, TB, qB = moments_of(target, cv, cw)
+    assert abs(rB - rho) / rho < 1e-11 and np.abs(uB - u).max() < 1e-11
+    assert abs(TB - T) / T < 1e-11 and np.abs(qB).max() < 1e-10 * np.abs(q).max()
+    Q = OR.q_bgk(f, cv, cw, 1.7, 0.81)
+    phi = OR.invariants(cv)
+    assert np.abs((phi @ Q) / (np.abs(phi) @ np.abs(Q))).max() < 1e-11
+    Q2 = OR.q_bgk(2.0 * f, cv, cw, 1.7, 0.81)
+    assert np.abs(Q2 - 4.0 * Q).max() < 1e-12 * np.abs(Q2).max()
+    QS = OR.q_shakov(f, cv, cw, 1.7, 0.81, 1.0)
+    assert np.abs(Q - QS).max() < 1e-13 * np.abs(Q).max()
