@@ -126,6 +126,9 @@ SIGNATURES = {
     'dgfs_adam': (_i, [_i, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _vp]),
     'dgfs_shakov': (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                          _d, _d, _d, _d, _vp, _vp, _vp]),
+    'dgfs_esbgk_work_bytes': (_sz, [_i, _i]),
+    'dgfs_esbgk': (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp,
+                        _d, _d, _d, _d, _vp, _vp, _vp]),
 }
 
 

@@ -4,6 +4,8 @@
 // Reference behaviour (jaisw7/dgfs1D_gpu):
 //   Shakov model 'shakov-direct-gll'   dgfs1D/std/scattering.py:1636-1833
 //     moments, f_S, output              std/kernels/scattering/shakov-direct-gll.mako:15-167
+//   ES-BGK model 'esbgk-direct-gll'    dgfs1D/std/scattering.py:1372-1632
+//     moments, tensor, Gaussian, output std/kernels/scattering/esbgk-direct-gll.mako:15-229
 //   conservation projection             no counterpart (the reference's only discrete
 //                                       conservation mechanism is the BGK Newton solve,
 //                                       std/scattering.py:647-695)
@@ -279,6 +281,109 @@ k_shk_apply(int K, int n, const int* __restrict__ nterm, const int* __restrict__
     Q[o] = T(double(Q[o]) + acc);
 }
 
+// ------------------------------------------------------------ ES-BGK model
+// 'esbgk-direct-gll' (std/scattering.py:1372-1632, esbgk-direct-gll.mako):
+// target G = rho / ((2 pi)^1.5 sqrt(det L)) exp(-C^T L^{-1} C / 2) with
+// L = cw/rho ((1 - 1/Pr) sum f C C^T + (1/Pr) sum f_M C C^T), f_M the node's
+// discrete Maxwellian of (rho, u, T); rate prefac rho T^{1-omega}.  The f part
+// of L comes from the first pass's raw moments; the f_M part is a second pass
+// over the grid (the reference sums the discrete Maxwellian, mako:97-162).
+constexpr int kEsMom = 6;     // xx yy zz xy yz zx
+constexpr int kEsCst = 16;
+
+// per 256-point block: sum f_M C_i C_j with f_M = rho/(pi T)^1.5 exp(-C^2/T)
+template <typename T>
+__global__ void __launch_bounds__(kRB)
+k_es_partial(int n, const T* __restrict__ cx, const T* __restrict__ cy,
+             const T* __restrict__ cz, const double* __restrict__ cst,
+             double* __restrict__ part /*[task][nb][6]*/) {
+    __shared__ double sh[kEsMom * 4];
+    const int nb = gridDim.x, t = blockIdx.y;
+    const int v = blockIdx.x * kRB + threadIdx.x;
+    double a[kEsMom];
+    for (int i = 0; i < kEsMom; ++i) a[i] = 0.0;
+    if (v < n) {
+        const double* c = cst + size_t(t) * kShkCst;
+        const double Cx = double(cx[v]) - c[0], Cy = double(cy[v]) - c[1],
+                     Cz = double(cz[v]) - c[2];
+        const double fm = c[4] * exp(-(Cx * Cx + Cy * Cy + Cz * Cz) * c[3]);
+        a[0] = fm * Cx * Cx; a[1] = fm * Cy * Cy; a[2] = fm * Cz * Cz;
+        a[3] = fm * Cx * Cy; a[4] = fm * Cy * Cz; a[5] = fm * Cz * Cx;
+    }
+    block_sum_all<kEsMom>(a, sh);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kEsMom; ++i) part[(size_t(t) * nb + blockIdx.x) * kEsMom + i] = a[i];
+}
+
+// est[t] = {ux, uy, uz, A_xx, A_yy, A_zz, A_xy, A_yz, A_zx (A = L^{-1}),
+//           rho/((2 pi)^1.5 sqrt(det L)), prefac rho T^(1-omega)}  (mako:143-229)
+__global__ void __launch_bounds__(kRB)
+k_es_final(int nb, ShkParams P, const double* __restrict__ part1,
+           const double* __restrict__ part2, const double* __restrict__ cst,
+           double* __restrict__ est) {
+    __shared__ double sh[kShkMom * 4];
+    const int t = blockIdx.x;
+    double m[kShkMom], b[kEsMom];
+    reduce_partials<kShkMom>(part1 + size_t(t) * nb * kShkMom, nb, m, sh);
+    __syncthreads();
+    reduce_partials<kEsMom>(part2 + size_t(t) * nb * kEsMom, nb, b, sh);
+    if (threadIdx.x != 0) return;
+    const double* c = cst + size_t(t) * kShkCst;
+    const double ux = c[0], uy = c[1], uz = c[2], m0 = m[0];
+    const double rho = m0 * P.cw;
+    // sum f C_i C_j = M2_ij - u_i m1_j - u_j m1_i + u_i u_j m0
+    const double F[kEsMom] = {m[4] - 2.0 * ux * m[1] + ux * ux * m0,
+                              m[5] - 2.0 * uy * m[2] + uy * uy * m0,
+                              m[6] - 2.0 * uz * m[3] + uz * uz * m0,
+                              m[7] - ux * m[2] - uy * m[1] + ux * uy * m0,
+                              m[8] - uy * m[3] - uz * m[2] + uy * uz * m0,
+                              m[9] - uz * m[1] - ux * m[3] + uz * ux * m0};
+    const double wf = 1.0 - 1.0 / P.Pr, wm = 1.0 / P.Pr, sc = P.cw / rho;
+    double L[kEsMom];
+    for (int i = 0; i < kEsMom; ++i) L[i] = sc * (wf * F[i] + wm * b[i]);
+    const double xx = L[0], yy = L[1], zz = L[2], xy = L[3], yz = L[4], zx = L[5];
+    const double det = xx * (yy * zz - yz * yz) - xy * (xy * zz - yz * zx) +
+                       zx * (xy * yz - yy * zx);
+    double* e = est + size_t(t) * kEsCst;
+    e[0] = ux;
+    e[1] = uy;
+    e[2] = uz;
+    e[3] = (yy * zz - yz * yz) / det;
+    e[4] = (xx * zz - zx * zx) / det;
+    e[5] = (xx * yy - xy * xy) / det;
+    e[6] = (yz * zx - zz * xy) / det;
+    e[7] = (xy * zx - xx * yz) / det;
+    e[8] = (xy * yz - yy * zx) / det;
+    e[9] = rho / (pow(2.0 * kPi, 1.5) * sqrt(det));
+    e[10] = c[9];
+}
+
+// Q[gslice[g]] += sum_{t in group g} prefac rho T^{1-omega} (G(t) - f_t)
+template <typename T>
+__global__ void __launch_bounds__(kRB)
+k_es_apply(int K, int n, const int* __restrict__ nterm, const int* __restrict__ slice,
+           const T* __restrict__ coef, const int* __restrict__ gstart,
+           const int* __restrict__ gslice, const T* __restrict__ u, const T* __restrict__ cx,
+           const T* __restrict__ cy, const T* __restrict__ cz,
+           const double* __restrict__ est, T* __restrict__ Q) {
+    const int g = blockIdx.y;
+    const int v = blockIdx.x * kRB + threadIdx.x;
+    if (v >= n) return;
+    const double x = cx[v], y = cy[v], z = cz[v];
+    double acc = 0.0;
+    for (int t = gstart[g]; t < gstart[g + 1]; ++t) {
+        const double* e = est + size_t(t) * kEsCst;
+        const double Cx = x - e[0], Cy = y - e[1], Cz = z - e[2];
+        const double qf = e[3] * Cx * Cx + e[4] * Cy * Cy + e[5] * Cz * Cz +
+                          2.0 * (e[6] * Cx * Cy + e[7] * Cy * Cz + e[8] * Cz * Cx);
+        const double G = e[9] * exp(-0.5 * qf);
+        const double f = shk_task_value(nterm, slice, coef, K, n, u, t, v);
+        acc += e[10] * (G - f);
+    }
+    const size_t o = size_t(gslice[g]) * n + v;
+    Q[o] = T(double(Q[o]) + acc);
+}
+
 }  // namespace
 }  // namespace dgfs
 
@@ -432,6 +537,77 @@ extern "C" int dgfs_shakov(int dtype, int K, int Ne, int Nv, int ntask, const in
                                Nv, task_nterm, task_slice, (const float*)task_coef, group_start,
                                group_slice, (const float*)u, (const float*)cx,
                                (const float*)cy, (const float*)cz, (const double*)cst,
+                               (float*)Q);
+        DGFS_HIP_CHECK(hipGetLastError());
+        (void)Ne;
+        return 0;
+    }
+}
+
+extern "C" size_t dgfs_esbgk_work_bytes(int ntask, int Nv) {
+    const int nb = (Nv + kRB - 1) / kRB;
+    return size_t(ntask) * (nb * (kShkMom + kEsMom) + kShkCst + kEsCst) * sizeof(double);
+}
+
+extern "C" int dgfs_esbgk(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nterm,
+                          const int* task_slice, const void* task_coef, int ngroup,
+                          const int* group_start, const int* group_slice, const void* u,
+                          const void* cx, const void* cy, const void* cz, double cw,
+                          double prefac, double omega, double Pr, void* Q, double* work,
+                          hipStream_t stream) {
+    {
+        DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "dtype must be DGFS_F64 or DGFS_F32");
+        DGFS_REQUIRE(K > 0 && Ne > 0 && Nv > 0 && ntask >= 0 && ngroup >= 0, "bad sizes");
+        DGFS_REQUIRE(Pr > 0.0, "esbgk: Pr must be positive");
+        if (ntask == 0 || ngroup == 0) return 0;
+        DGFS_REQUIRE(ntask <= 65535 && ngroup <= 65535, "too many tasks");
+        DGFS_REQUIRE(task_nterm && task_slice && task_coef && group_start && group_slice && u &&
+                         cx && cy && cz && Q && work,
+                     "null pointer");
+        const int nb = (Nv + kRB - 1) / kRB;
+        const ShkParams P{prefac, omega, Pr, cw};
+        double* part1 = work;
+        double* part2 = part1 + size_t(ntask) * nb * kShkMom;
+        double* cst = part2 + size_t(ntask) * nb * kEsMom;
+        double* est = cst + size_t(ntask) * kShkCst;
+        if (dtype == DGFS_F64) {
+            hipLaunchKernelGGL(k_shk_partial<double>, dim3(nb, ntask), dim3(kRB), 0, stream, K,
+                               Nv, task_nterm, task_slice, (const double*)task_coef,
+                               (const double*)u, (const double*)cx, (const double*)cy,
+                               (const double*)cz, part1);
+        } else {
+            hipLaunchKernelGGL(k_shk_partial<float>, dim3(nb, ntask), dim3(kRB), 0, stream, K,
+                               Nv, task_nterm, task_slice, (const float*)task_coef,
+                               (const float*)u, (const float*)cx, (const float*)cy,
+                               (const float*)cz, part1);
+        }
+        DGFS_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_shk_final, dim3(ntask), dim3(kRB), 0, stream, nb, P,
+                           (const double*)part1, cst);
+        DGFS_HIP_CHECK(hipGetLastError());
+        if (dtype == DGFS_F64)
+            hipLaunchKernelGGL(k_es_partial<double>, dim3(nb, ntask), dim3(kRB), 0, stream, Nv,
+                               (const double*)cx, (const double*)cy, (const double*)cz,
+                               (const double*)cst, part2);
+        else
+            hipLaunchKernelGGL(k_es_partial<float>, dim3(nb, ntask), dim3(kRB), 0, stream, Nv,
+                               (const float*)cx, (const float*)cy, (const float*)cz,
+                               (const double*)cst, part2);
+        DGFS_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_es_final, dim3(ntask), dim3(kRB), 0, stream, nb, P,
+                           (const double*)part1, (const double*)part2, (const double*)cst, est);
+        DGFS_HIP_CHECK(hipGetLastError());
+        if (dtype == DGFS_F64)
+            hipLaunchKernelGGL(k_es_apply<double>, dim3(nb, ngroup), dim3(kRB), 0, stream, K,
+                               Nv, task_nterm, task_slice, (const double*)task_coef, group_start,
+                               group_slice, (const double*)u, (const double*)cx,
+                               (const double*)cy, (const double*)cz, (const double*)est,
+                               (double*)Q);
+        else
+            hipLaunchKernelGGL(k_es_apply<float>, dim3(nb, ngroup), dim3(kRB), 0, stream, K,
+                               Nv, task_nterm, task_slice, (const float*)task_coef, group_start,
+                               group_slice, (const float*)u, (const float*)cx,
+                               (const float*)cy, (const float*)cz, (const double*)est,
                                (float*)Q);
         DGFS_HIP_CHECK(hipGetLastError());
         (void)Ne;

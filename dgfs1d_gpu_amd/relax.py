@@ -119,9 +119,14 @@ class MixtureProjection:
 
 
 class Shakov:
-    """Batched Shakov relaxation term Q += prefac rho T^{1-omega} (f_S - f)."""
+    """Batched Shakov relaxation term Q += prefac rho T^{1-omega} (f_S - f);
+    with target='esbgk' the ES-BGK term (dgfs_esbgk), f_S replaced by the
+    anisotropic Gaussian of the node."""
 
-    def __init__(self, n, cx, cy, cz, cw, prefac, omega, Pr, tdtype, device):
+    def __init__(self, n, cx, cy, cz, cw, prefac, omega, Pr, tdtype, device, target='shakov'):
+        if target not in ('shakov', 'esbgk'):
+            raise ValueError("target must be 'shakov' or 'esbgk'")
+        self._fn = 'dgfs_' + target
         import torch
         self.torch = torch
         self.n = int(n)
@@ -140,7 +145,7 @@ class Shakov:
         return self.tasks
 
     def work(self, ntask):
-        nb = _lib.lib().dgfs_shakov_work_bytes(max(1, int(ntask)), self.n)
+        nb = getattr(_lib.lib(), self._fn + '_work_bytes')(max(1, int(ntask)), self.n)
         if self._work is None or self._work.numel() * 8 < nb:
             self._work = self.torch.empty(max(1, nb // 8), dtype=self.torch.float64,
                                           device=self.device)
@@ -155,7 +160,7 @@ class Shakov:
         nmin = t['K'] * t['Ne'] * self.n
         if u.numel() < nmin or Q.numel() < nmin:
             raise ValueError('Shakov operands smaller than K*Ne*Nv')
-        _lib.check(_lib.lib().dgfs_shakov(
+        _lib.check(getattr(_lib.lib(), self._fn)(
             _lib.dtype_code(self.tdtype), t['K'], t['Ne'], self.n, t['ntask'],
             _lib.ptr(t['nterm']), _lib.ptr(t['slice']), _lib.ptr(t['coef']), t['ngroup'],
             _lib.ptr(t['gstart']), _lib.ptr(t['gslice']), _lib.ptr(u), _lib.ptr(self.cx),

@@ -276,6 +276,26 @@ class DGFSBGKDirectGLLScatteringModelStd(DGFSShakovDirectGLLScatteringModelStd):
         self._Pr = 1.0
 
 
+class DGFSESBGKDirectGLLScatteringModelStd(DGFSShakovDirectGLLScatteringModelStd):
+    """ES-BGK relaxation, iteration-free (std/scattering.py:1372-1632): the
+    node's anisotropic Gaussian with tensor
+    L = cw/rho ((1 - 1/Pr) sum f C C^T + (1/Pr) sum f_M C C^T) as target
+    (dgfs_esbgk), frequency prefac rho T^(1-omega), prefac = t0 Pr p0 / mu(T0).
+    Nodal bases only, as the reference."""
+    scattering_model = 'esbgk-direct-gll'
+
+    def load_parameters(self):
+        super().load_parameters()
+        self._prefactor *= self._Pr
+
+    def perform_precomputation(self):
+        vm = self.vm
+        self.model = Shakov(vm.vsize(), vm.d_cvx(), vm.d_cvy(), vm.d_cvz(), vm.cw(),
+                            self._prefactor, self._omega, self._Pr, _torch_dtype(self.cfg),
+                            self._device, target='esbgk')
+        self._single = {}
+
+
 def _single_task(cache, Ne, Nv, d_Q, elem, modein, modeout, tdtype, device):
     """Cached one-task table: slice (modein, elem) -> slice (modeout, elem)."""
     K = d_Q.numel() // (Ne * Nv)

@@ -315,6 +315,23 @@ int dgfs_shakov(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nte
                 const void* cy, const void* cz, double cw, double prefac, double omega,
                 double Pr, void* Q, double* work, hipStream_t stream);
 
+/* ES-BGK relaxation 'esbgk-direct-gll' (replaces fs() of
+ * DGFSESBGKDirectGLLScatteringModelStd, dgfs1D/std/scattering.py:1540-1584,
+ * kernels std/kernels/scattering/esbgk-direct-gll.mako:15-229), with the task
+ * and group semantics of dgfs_shakov:
+ *   Q[group slice] += prefac rho T^{1-omega} (G_t - f_t),
+ *   G = rho / ((2 pi)^{3/2} sqrt(det L)) exp(-(c-u)^T L^{-1} (c-u) / 2),
+ *   L = cw/rho ((1 - 1/Pr) sum f C C^T + (1/Pr) sum f_M C C^T), C = c - u,
+ * f_M the node's discrete Maxwellian of (rho, u, T); prefac is the caller's
+ * (t0 Pr p0 / mu(T0) in the reference).  Moments and L in FP64; Pr > 0.
+ * work: dgfs_esbgk_work_bytes(ntask, Nv) bytes. */
+size_t dgfs_esbgk_work_bytes(int ntask, int Nv);
+int dgfs_esbgk(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nterm,
+               const int* task_slice, const void* task_coef, int ngroup,
+               const int* group_start, const int* group_slice, const void* u, const void* cx,
+               const void* cy, const void* cz, double cw, double prefac, double omega, double Pr,
+               void* Q, double* work, hipStream_t stream);
+
 /* ---------------------------------------------------------------------------
  * IMEX penalisation with the 14-moment Shakov model
  * (DGFSShakovDirectGLLScatteringModelAstd, dgfs1D/astd/scattering.py:355-397,

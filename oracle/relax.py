@@ -12,7 +12,8 @@ kernels of std/kernels/scattering/shakov-direct-gll.mako:
   equiSkDistCompute              :106-142  f_S
   output                         :145-167  Q += prefac rho / T^(omega-1) (f_S - f)
 
-``q_bgk`` restates 'bgk-direct-gll' (a Maxwellian target) the same way.
+``q_bgk`` restates 'bgk-direct-gll' (a Maxwellian target) and ``q_esbgk``
+'esbgk-direct-gll' (an anisotropic Gaussian target) the same way.
 
 ``project_conservative`` is the Euclidean-nearest slice with vanishing
 mass, momentum and energy moments (no reference counterpart), computed with
@@ -63,6 +64,30 @@ def q_bgk(f, cv, cw, prefac, omega):
     T = (f * C2).sum() * cw / (1.5 * rho)
     fe = rho / (np.pi * T) ** 1.5 * np.exp(-C2 / T)
     return prefac * (rho / T ** (omega - 1.0)) * (fe - f)
+
+
+def q_esbgk(f, cv, cw, prefac, omega, Pr):
+    """One 'esbgk-direct-gll' fs() call (std/scattering.py:1540-1584, kernels
+    of std/kernels/scattering/esbgk-direct-gll.mako): rho, u, T as BGK
+    (:32-94), the discrete Maxwellian fe of them (:97-118), the tensor
+    L_ij = cw/rho sum (c_i - u_i)(c_j - u_j) ((1 - 1/Pr) f + (1/Pr) fe)
+    (mom2ES, mom2ESNorm :120-162), the Gaussian
+    rho / ((2 pi)^1.5 sqrt(det L)) exp(-C^T L^-1 C / 2) (:164-207; inverted
+    here with numpy.linalg, the reference writes cofactors), and output
+    (:209-229): prefac rho / T^(omega-1) (G - f)."""
+    s0 = f.sum()
+    u = np.array([(f * cv[0]).sum(), (f * cv[1]).sum(), (f * cv[2]).sum()]) / s0
+    rho = s0 * cw
+    C = cv - u[:, None]
+    C2 = (C * C).sum(0)
+    T = (f * C2).sum() * cw / (1.5 * rho)
+    fe = rho / (np.pi * T) ** 1.5 * np.exp(-C2 / T)
+    w = (1.0 - 1.0 / Pr) * f + (1.0 / Pr) * fe
+    L = np.array([[(C[i] * C[j] * w).sum() for j in range(3)] for i in range(3)]) * cw / rho
+    A = np.linalg.inv(L)
+    G = rho / ((2.0 * np.pi) ** 1.5 * np.sqrt(np.linalg.det(L))) * \
+        np.exp(-0.5 * np.einsum('iv,ij,jv->v', C, A, C))
+    return prefac * (rho / T ** (omega - 1.0)) * (G - f)
 
 
 def shakov_prefactor(H0, u0, T0, n0, muRef, Tref, omega, R0=8.3144598, NA=6.0221409e+23):

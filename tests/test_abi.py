@@ -39,8 +39,8 @@ def test_errors_are_reported():
 
 
 def test_new_entry_points_reject_bad_arguments():
-    """Argument checks of the mixture, limiter and projection entry points
-    fail before any device work (status 2, message kept)."""
+    """Argument checks of the mixture, limiter, projection and ES-BGK entry
+    points fail before any device work (status 2, message kept)."""
     import ctypes as C
     from dgfs1d_gpu_amd import _lib
     L = _lib.lib()
@@ -62,3 +62,11 @@ def test_new_entry_points_reject_bad_arguments():
     assert L.dgfs_project_conservative_mixture(0, 1, 8, None, None, None, 1.0, -1.0, None, None,
                                                None, None, None, 0, None, None) == 2
     assert b'masses must be positive' in L.dgfs_last_error()
+    # ES-BGK: Pr must be positive; null operands are refused (no device work)
+    assert L.dgfs_esbgk(0, 1, 1, 8, 1, None, None, None, 1, None, None, None, None, None, None,
+                        1.0, 1.0, 0.81, 0.0, None, None, None) == 2
+    assert b'Pr must be positive' in L.dgfs_last_error()
+    assert L.dgfs_esbgk(0, 1, 1, 8, 1, None, None, None, 1, None, None, None, None, None, None,
+                        1.0, 1.0, 0.81, 0.5, None, None, None) == 2
+    assert b'null pointer' in L.dgfs_last_error()
+    assert L.dgfs_esbgk_work_bytes(2, 600) == 2 * (3 * (13 + 6) + 32) * 8

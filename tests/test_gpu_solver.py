@@ -149,11 +149,13 @@ def test_rhs_boundaries(cuda, lo, hi, model):
 
 
 @pytest.mark.parametrize('kind', ['nodal-sem-gll', 'nodal-gll'])
-@pytest.mark.parametrize('model', ['shakov-direct-gll', 'bgk-direct-gll'])
+@pytest.mark.parametrize('model', ['shakov-direct-gll', 'bgk-direct-gll', 'esbgk-direct-gll'])
 def test_rhs_shakov(cuda, kind, model):
-    """'shakov-direct-gll' (std/scattering.py:1636-1833) and 'bgk-direct-gll'
-    (:1186-1369; its oracle is written from the BGK kernels, a Maxwellian
-    target, and a Pr in the deck is ignored) in the full RHS."""
+    """The relaxation models in the full RHS: 'shakov-direct-gll'
+    (std/scattering.py:1636-1833), 'bgk-direct-gll' (:1186-1369; its oracle is
+    written from the BGK kernels, a Maxwellian target, and a Pr in the deck is
+    ignored) and 'esbgk-direct-gll' (:1372-1632; anisotropic Gaussian target,
+    prefactor times Pr)."""
     import torch
     deck, solver, u = _pair(cuda, {('scattering-model', 'type'): model,
                                    ('scattering-model', 'muRef'): 2.117e-5,
@@ -165,6 +167,23 @@ def test_rhs_shakov(cuda, kind, model):
     err = _rel(out.cpu().numpy().reshape(ref.shape), ref)
     print(model, kind, err)
     assert err < 1e-11
+
+
+@pytest.mark.parametrize('model', ['shakov-direct-gll', 'esbgk-direct-gll'])
+def test_rhs_relaxation_fp32(cuda, model):
+    """The FP32 instantiations of the relaxation kernels (FP64 moments and
+    tensor): the RHS of the rounded state against the FP64 oracle, 2e-5."""
+    import torch
+    deck, solver, u = _pair(cuda, {('scattering-model', 'type'): model,
+                                   ('scattering-model', 'muRef'): 2.117e-5,
+                                   ('config', 'precision'): 'single'})
+    assert solver.u.dtype == torch.float32
+    out = torch.zeros_like(solver.u)
+    solver.rhs(0.0, solver.u, out)
+    ref = deck.op.rhs(0.0, solver.u.double().cpu().numpy().reshape(u.shape))
+    err = _rel(out.double().cpu().numpy().reshape(ref.shape), ref)
+    print(model, 'fp32', err)
+    assert err < 2e-5
 
 
 @pytest.mark.parametrize('model', ['vhs', 'shakov-direct-gll'])

@@ -158,3 +158,51 @@ def test_bgk_pins():
     assert np.abs(Q2 - 4.0 * Q).max() < 1e-12 * np.abs(Q2).max()
     QS = OR.q_shakov(f, cv, cw, 1.7, 0.81, 1.0)
     assert np.abs(Q - QS).max() < 1e-13 * np.abs(Q).max()
+
+
+def _dev(X):
+    return X - np.trace(X) / 3.0 * np.eye(3)
+
+
+def _stress(g, cv, cw, u):
+    C = cv - np.asarray(u)[:, None]
+    return np.array([[(g * C[i] * C[j]).sum() for j in range(3)] for i in range(3)]) * cw
+
+
+def test_esbgk_pins():
+    """oracle.relax.q_esbgk ('esbgk-direct-gll'): a Maxwellian is a fixed point;
+    at Pr = 1 it is the BGK term; mass, momentum and energy are conserved; and
+    the deviatoric stress relaxes at nu/Pr (the model's Prandtl number), the
+    fact the (1 - 1/Pr, 1/Pr) weights of the tensor fix: a planted swap of the
+    two weights fails it."""
+    cv, cw = grid()
+    Pr, om, pf = 2.0 / 3.0, 0.81, 1.7
+    m = maxw(cv, 1.3, (0.2, -0.1, 0.05), 0.9)
+    assert np.abs(OR.q_esbgk(m, cv, cw, 2.0, om, Pr)).max() < 1e-10 * np.abs(m).max()
+    f = bimodal(cv)
+    QB = OR.q_bgk(f, cv, cw, pf, om)
+    assert np.abs(OR.q_esbgk(f, cv, cw, pf, om, 1.0) - QB).max() < 1e-10 * np.abs(QB).max()
+    Q = OR.q_esbgk(f, cv, cw, pf, om, Pr)
+    phi = OR.invariants(cv)
+    assert np.abs((phi @ Q) / (np.abs(phi) @ np.abs(Q))).max() < 1e-11
+    rho, u, T, _ = moments_of(f, cv, cw)
+    nu = pf * rho * T ** (1.0 - om)
+    Sf, SQ = _dev(_stress(f, cv, cw, u)), _dev(_stress(Q, cv, cw, u))
+    assert np.abs(Sf).max() > 1e-2 * rho * T                     # anisotropic stress
+    assert np.abs(SQ + nu / Pr * Sf).max() < 1e-9 * nu * np.abs(Sf).max()
+
+    def swapped(f, Pr):
+        s0 = f.sum()
+        uu = np.array([(f * cv[i]).sum() for i in range(3)]) / s0
+        r = s0 * cw
+        C = cv - uu[:, None]
+        C2 = (C * C).sum(0)
+        TT = (f * C2).sum() * cw / (1.5 * r)
+        fe = r / (np.pi * TT) ** 1.5 * np.exp(-C2 / TT)
+        w = (1.0 / Pr) * f + (1.0 - 1.0 / Pr) * fe
+        L = np.array([[(C[i] * C[j] * w).sum() for j in range(3)] for i in range(3)]) * cw / r
+        G = r / ((2 * np.pi) ** 1.5 * np.sqrt(np.linalg.det(L))) * \
+            np.exp(-0.5 * np.einsum('iv,ij,jv->v', C, np.linalg.inv(L), C))
+        return pf * r * TT ** (1.0 - om) * (G - f)
+    SX = _dev(_stress(swapped(f, Pr), cv, cw, u))
+    assert np.abs(SX + nu / Pr * Sf).max() > 1e-2 * nu * np.abs(Sf).max()
