@@ -174,3 +174,18 @@ class DGFSShakovDirectGLLScatteringModelAstd(DGFSScatteringModelAstd):
             a.M[i + 1] = M[i].data_ptr()
         _lib.check(_lib.lib().dgfs_imex_ars_update(self.dt_code, ns, self.Nv, _lib.C.byref(a),
                                                    1, _lib.stream_ptr(stream)))
+
+
+class DGFSBGKDirectGLLScatteringModelAstd(DGFSShakovDirectGLLScatteringModelAstd):
+    """BGK penalisation P = nu (M - f) (astd/scattering.py:44-286): the
+    Maxwellian of (rho, u, T) as target, nu = prefac rho T^(1-omega) with
+    prefac = t0 p0 / mu(T0).  The reference evolves its five conserved
+    moments; the 14-moment Shakov update with Pr = 1 gives the same
+    distribution (its target then depends on rho, u and T only, and the
+    relaxation leaves the conserved moments to the explicit part), so that
+    update runs here.  A Pr in the deck is ignored, as in the reference."""
+    scattering_model = 'bgk-direct-gll'
+
+    def load_parameters(self):
+        super().load_parameters()
+        self._Pr = 1.0

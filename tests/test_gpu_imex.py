@@ -19,17 +19,17 @@ type = shakov-direct-gll
 """
 
 
-def _text(scheme, model='vhs'):
+def _text(scheme, model='vhs', penal='shakov-direct-gll'):
     from tests.test_gpu_solver import _ini
     return _ini({('time-integrator', 'scheme'): scheme, ('scattering-model', 'type'): model,
-                 ('scattering-model', 'muRef'): 2.117e-5}) + EXTRA
+                 ('scattering-model', 'muRef'): 2.117e-5}) + EXTRA.replace('shakov-direct-gll',
+                                                                          penal)
 
 
-def _oracle(deck):
+def _oracle(deck, Pr=2.0 / 3.0):
     b, op = deck.basis, deck.op
     K, Ne = deck.K, deck.Ne
     n = deck.vm.vsize
-    Pr = 2.0 / 3.0
     from oracle import relax as OR
     cp = deck.cp
     prefac = OR.shakov_prefactor(float(cp.get('mesh', 'H0')), deck.vm.u0, deck.vm.T0,
@@ -101,6 +101,29 @@ def test_ars_steps(cuda, scheme):
         t += deck.dt
     err = np.abs(s.u.cpu().numpy().reshape(f.shape) - f).max() / np.abs(f).max()
     print(scheme, err)
+    assert err < 1e-11
+
+
+@pytest.mark.parametrize('scheme', ['ars-222', 'imex-lm-bdf2'])
+def test_bgk_penalisation_steps(cuda, scheme):
+    """[penalized-scattering-model] type = bgk-direct-gll
+    (astd/scattering.py:44-286): the Maxwellian target, i.e. the oracle's
+    Shakov penalisation at Pr = 1 (its heat-flux term vanishes)."""
+    text = _text(scheme, penal='bgk-direct-gll')
+    deck = Deck(text)
+    s = _solver(cuda, text)
+    assert s.psm._Pr == 1.0
+    pm, explicit, boltz = _oracle(deck, Pr=1.0)
+    f = deck.u0.reshape(s.K * s.Ne, -1).copy()
+    it = OI.LMIntegrator(scheme, pm, explicit, boltz, 0.0, deck.dt, f.copy()) \
+        if scheme.startswith('imex-lm') else None
+    t = 0.0
+    for k in range(3):
+        f = it.step(t, deck.dt, k, f) if it else OI.ars_step(scheme, pm, explicit, t, deck.dt, f)
+        s.step()
+        t += deck.dt
+    err = np.abs(s.u.cpu().numpy().reshape(f.shape) - f).max() / np.abs(f).max()
+    print('bgk penalisation', scheme, err)
     assert err < 1e-11
 
 

@@ -40,8 +40,12 @@ def test_registries():
     for k in ('dgfs-wall-diffuse', 'dgfs-inlet', 'dgfs-periodic', 'dgfs-cyclic', 'dgfs-vanish',
               'dgfs-wall-expr-diffuse', 'dgfs-inlet-nondim'):
         subclass_where(DGFSBCStd, type=k)
-    for k in ('vhs', 'vhs-gll', 'maxwell-gll', 'shakov-direct-gll'):
+    for k in ('vhs', 'vhs-gll', 'maxwell-gll', 'shakov-direct-gll', 'bgk-direct-gll',
+              'esbgk-direct-gll'):
         subclass_where(DGFSScatteringModelStd, scattering_model=k)
+    from dgfs1d_gpu_amd.astd.scattering import DGFSScatteringModelAstd
+    for k in ('shakov-direct-gll', 'bgk-direct-gll'):
+        subclass_where(DGFSScatteringModelAstd, scattering_model=k)
     with pytest.raises(KeyError):
         subclass_where(Basis, basis_kind='nope')
 
