@@ -53,7 +53,8 @@ class LM(C.Structure):
                 ('U', C.c_void_p * IMEX_MAXS), ('LU', C.c_void_p * IMEX_MAXS),
                 ('LM', C.c_void_p * IMEX_MAXS), ('Unew', C.c_void_p),
                 ('F', C.c_void_p * IMEX_MAXS), ('L', C.c_void_p * IMEX_MAXS),
-                ('Pn', C.c_void_p * IMEX_MAXS), ('Mnew', C.c_void_p), ('fnew', C.c_void_p)]
+                ('Pn', C.c_void_p * IMEX_MAXS), ('Mnew', C.c_void_p), ('fnew', C.c_void_p),
+                ('srate', C.c_double)]
 
 
 class ARS(C.Structure):
@@ -62,7 +63,7 @@ class ARS(C.Structure):
                 ('ah', C.c_double * IMEX_MAXS), ('ai', C.c_double * (IMEX_MAXS + 1)),
                 ('U', C.c_void_p * (IMEX_MAXS + 1)), ('LU', C.c_void_p * IMEX_MAXS),
                 ('F', C.c_void_p * (IMEX_MAXS + 1)), ('L', C.c_void_p * IMEX_MAXS),
-                ('M', C.c_void_p * (IMEX_MAXS + 1))]
+                ('M', C.c_void_p * (IMEX_MAXS + 1)), ('srate', C.c_double)]
 
 
 class ClosureNet(C.Structure):
@@ -112,6 +113,7 @@ SIGNATURES = {
     'dgfs_mom14_work_bytes': (_sz, [_i, _i]),
     'dgfs_mom14': (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     'dgfs_shakov_target': (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _d, _vp, _vp]),
+    'dgfs_esbgk_target': (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _d, _vp, _vp]),
     'dgfs_relax_collide': (_i, [_i, _i, _i, _vp, _vp, _vp, _d, _d, _vp, _vp]),
     'dgfs_imex_lm_update': (_i, [_i, _i, _i, C.POINTER(LM), _i, _vp]),
     'dgfs_imex_ars_update': (_i, [_i, _i, _i, C.POINTER(ARS), _i, _vp]),

@@ -38,6 +38,7 @@ class DGFSShakovDirectGLLScatteringModelAstd(DGFSScatteringModelAstd):
     """14-moment Shakov penalisation P = nu (M_S - f)."""
     scattering_model = 'shakov-direct-gll'
     nalph = 14
+    srate = 1.0          # the stresses relax at nu
 
     def load_parameters(self):
         """t0 p0 / mu(T0), parameters of [scattering-model] (astd/scattering.py:362-376)."""
@@ -112,6 +113,7 @@ class DGFSShakovDirectGLLScatteringModelAstd(DGFSScatteringModelAstd):
         a = _lib.LM()
         a.nstages, a.dt, a.prefac = int(s), float(dt), float(self._prefactor)
         a.omega, a.Pr = float(self._omega), float(self._Pr)
+        a.srate = float(self.srate)
         for i in range(s):
             a.a[i], a.b[i] = float(A[i]), float(B[i])
         for i in range(s + 1):
@@ -144,6 +146,7 @@ class DGFSShakovDirectGLLScatteringModelAstd(DGFSScatteringModelAstd):
         a = _lib.ARS()
         a.stage, a.dt, a.prefac = int(s), float(dt), float(self._prefactor)
         a.omega, a.Pr = float(self._omega), float(self._Pr)
+        a.srate = float(self.srate)
         for i in range(s):
             a.ah[i] = float(ah[i])
             a.ai[i + 1] = float(ai[i])
@@ -189,3 +192,26 @@ class DGFSBGKDirectGLLScatteringModelAstd(DGFSShakovDirectGLLScatteringModelAstd
     def load_parameters(self):
         super().load_parameters()
         self._Pr = 1.0
+
+
+class DGFSESBGKDirectGLLScatteringModelAstd(DGFSShakovDirectGLLScatteringModelAstd):
+    """ES-BGK penalisation P = nu (G - f) (astd/scattering.py:287-354, kernels
+    astd/kernels/scattering/esbgk-direct-gll.mako): G the Gaussian of
+    Theta = T/Pr I + 2 (1 - 1/Pr) (P/rho - u u^T) (dgfs_esbgk_target),
+    nu = prefac rho T^(1-omega) with prefac = t0 Pr p0 / mu(T0); the
+    stresses relax at nu/Pr.  The reference evolves 11 moments; they are the
+    first 11 of the 14 kept here (the heat flux is carried along, unused)."""
+    scattering_model = 'esbgk-direct-gll'
+
+    def load_parameters(self):
+        super().load_parameters()
+        self._prefactor *= self._Pr
+        self.srate = 1.0 / self._Pr
+
+    def constructMaxwellian(self, t, U, M, stream=None):
+        """M = ES-BGK target of U (momentNorm + cmaxwellian)."""
+        ns = self._nslice(M)
+        self._check_U(U, ns)
+        _lib.check(_lib.lib().dgfs_esbgk_target(
+            self.dt_code, ns, self.Nv, _lib.ptr(U), _lib.ptr(self.cx), _lib.ptr(self.cy),
+            _lib.ptr(self.cz), float(self._Pr), _lib.ptr(M), _lib.stream_ptr(stream)))

@@ -120,6 +120,33 @@ k_target(int n, const double* __restrict__ U, const T* __restrict__ cx,
     M[size_t(s) * n + v] = T(shakov_target(m, Pr, cx[v], cy[v], cz[v]));
 }
 
+// ES-BGK target (esbgk-direct-gll.mako:157-255): Theta = T/Pr I
+// + 2 (1 - 1/Pr) (P/rho - u u^T), M = rho/(pi^1.5 sqrt det) exp(-C^T Theta^-1 C)
+template <typename T>
+__global__ void __launch_bounds__(kIB)
+k_es_target(int n, const double* __restrict__ U, const T* __restrict__ cx,
+            const T* __restrict__ cy, const T* __restrict__ cz, double Pr, T* __restrict__ M) {
+    const int s = blockIdx.y, v = blockIdx.x * kIB + threadIdx.x;
+    if (v >= n) return;
+    const double* u = U + size_t(s) * kNA;
+    const Macro m = macro_of(u);
+    const double a = 1.0 / Pr, b = 2.0 * (1.0 - 1.0 / Pr), r = m.rho;
+    const double xx = a * m.T + b * (u[5] / r - m.ux * m.ux);
+    const double yy = a * m.T + b * (u[6] / r - m.uy * m.uy);
+    const double zz = a * m.T + b * (u[7] / r - m.uz * m.uz);
+    const double xy = b * (u[8] / r - m.ux * m.uy);
+    const double yz = b * (u[9] / r - m.uy * m.uz);
+    const double zx = b * (u[10] / r - m.uz * m.ux);
+    const double det = xx * (yy * zz - yz * yz) - xy * (xy * zz - yz * zx) +
+                       zx * (xy * yz - yy * zx);
+    const double Cx = double(cx[v]) - m.ux, Cy = double(cy[v]) - m.uy, Cz = double(cz[v]) - m.uz;
+    const double q = (yy * zz - yz * yz) * Cx * Cx + (xx * zz - zx * zx) * Cy * Cy +
+                     (xx * yy - xy * xy) * Cz * Cz +
+                     2.0 * ((yz * zx - zz * xy) * Cx * Cy + (xy * zx - xx * yz) * Cy * Cz +
+                            (xy * yz - yy * zx) * Cz * Cx);
+    M[size_t(s) * n + v] = T(r / (pow(kPi, 1.5) * sqrt(det)) * exp(-q / det));
+}
+
 // Q = nu (M - f), nu = prefac rho T^{1-omega}  (collide, mako:266-305)
 template <typename T>
 __global__ void __launch_bounds__(kIB)
@@ -190,8 +217,9 @@ k_lm_moments(int nslice, dgfs_lm_t P) {
     double Us[kNA];
     for (int k = 0; k < kNA; ++k) Us[k] = E[k];
     Targets t = targets_of(Us, T, P.Pr);
+    const double nus = nu * P.srate;
     for (int j = 0; j < 6; ++j) {
-        Us[5 + j] = (E[5 + j] + nu * t.P[j]) / (1.0 + nu);
+        Us[5 + j] = (E[5 + j] + nus * t.P[j]) / (1.0 + nus);
         Un[5 + j] = Us[5 + j];
     }
     t = targets_of(Us, T, P.Pr);
@@ -242,7 +270,7 @@ k_ars_moments(int nslice, dgfs_ars_t P) {
         const double Ti = temp_of(Ui);
         const double w = P.ai[i] * P.dt * P.prefac * Ui[0] * pow(Ti, 1.0 - P.omega);
         const Targets t = targets_of(Ui, Ti, P.Pr);
-        for (int j = 0; j < 6; ++j) E[5 + j] += w * (t.P[j] - Ui[5 + j]);
+        for (int j = 0; j < 6; ++j) E[5 + j] += P.srate * w * (t.P[j] - Ui[5 + j]);
         for (int q = 0; q < 3; ++q) E[11 + q] += w * (t.Qf[q] - P.Pr * Ui[11 + q]);
     }
     // implicit stage (conserved moments of U_s are already final)
@@ -255,8 +283,9 @@ k_ars_moments(int nslice, dgfs_ars_t P) {
     for (int j = 0; j < 6; ++j) Us[5 + j] = 0.0;
     for (int q = 0; q < 3; ++q) Us[11 + q] = 0.0;
     Targets t = targets_of(Us, Ts, P.Pr);
+    const double ws = w * P.srate;
     for (int j = 0; j < 6; ++j) {
-        Un[5 + j] = (E[5 + j] + w * t.P[j]) / (1.0 + w);
+        Un[5 + j] = (E[5 + j] + ws * t.P[j]) / (1.0 + ws);
         Us[5 + j] = Un[5 + j];
     }
     t = targets_of(Us, Ts, P.Pr);
@@ -332,6 +361,26 @@ extern "C" int dgfs_shakov_target(int dtype, int nslice, int Nv, const double* U
     return 0;
 }
 
+extern "C" int dgfs_esbgk_target(int dtype, int nslice, int Nv, const double* U, const void* cx,
+                                 const void* cy, const void* cz, double Pr, void* M,
+                                 hipStream_t stream) {
+    DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "dtype must be DGFS_F64 or DGFS_F32");
+    DGFS_REQUIRE(nslice >= 0 && nslice <= 65535 && Nv > 0, "bad sizes");
+    DGFS_REQUIRE(Pr > 0.0, "esbgk_target: Pr must be positive");
+    if (nslice == 0) return 0;
+    DGFS_REQUIRE(U && cx && cy && cz && M, "null pointer");
+    const dim3 g((Nv + kIB - 1) / kIB, nslice);
+    if (dtype == DGFS_F64)
+        hipLaunchKernelGGL(k_es_target<double>, g, dim3(kIB), 0, stream, Nv, U,
+                           (const double*)cx, (const double*)cy, (const double*)cz, Pr,
+                           (double*)M);
+    else
+        hipLaunchKernelGGL(k_es_target<float>, g, dim3(kIB), 0, stream, Nv, U,
+                           (const float*)cx, (const float*)cy, (const float*)cz, Pr, (float*)M);
+    DGFS_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 extern "C" int dgfs_relax_collide(int dtype, int nslice, int Nv, const double* U, const void* M,
                                   const void* f, double prefac, double omega, void* Q,
                                   hipStream_t stream) {
@@ -358,6 +407,7 @@ extern "C" int dgfs_imex_lm_update(int dtype, int nslice, int Nv, const dgfs_lm_
     DGFS_REQUIRE(what == 0 || what == 1, "what must be 0 (moments) or 1 (distribution)");
     if (nslice == 0) return 0;
     const dgfs_lm_t P = *args;
+    DGFS_REQUIRE(P.srate > 0.0, "srate must be positive");
     DGFS_REQUIRE(P.Unew, "null U_s");
     for (int i = 0; i < P.nstages; ++i) {
         if (what == 0) DGFS_REQUIRE(P.U[i] && P.LU[i] && P.LM[i], "null moment history");
@@ -386,6 +436,7 @@ extern "C" int dgfs_imex_ars_update(int dtype, int nslice, int Nv, const dgfs_ar
     DGFS_REQUIRE(what == 0 || what == 1, "what must be 0 (moments) or 1 (distribution)");
     if (nslice == 0) return 0;
     const dgfs_ars_t P = *args;
+    DGFS_REQUIRE(P.srate > 0.0, "srate must be positive");
     for (int i = 0; i <= P.stage; ++i) DGFS_REQUIRE(P.U[i], "null moment stage");
     for (int i = 0; i < P.stage; ++i) DGFS_REQUIRE(P.LU[i] || what == 1, "null LU");
     if (what == 0) {

@@ -349,6 +349,15 @@ int dgfs_mom14(int dtype, int nslice, int Nv, const void* f, const void* cx, con
                const void* cz, double cw, double* work, double* U, hipStream_t stream);
 int dgfs_shakov_target(int dtype, int nslice, int Nv, const double* U, const void* cx,
                        const void* cy, const void* cz, double Pr, void* M, hipStream_t stream);
+/* ES-BGK target of the moments (the first 11 of the 14; esbgk penalisation,
+ * dgfs1D/astd/scattering.py:287-354, momentNorm + cmaxwellian of
+ * astd/kernels/scattering/esbgk-direct-gll.mako:157-255):
+ *   Theta = T/Pr I + 2 (1 - 1/Pr) (P/rho - u u^T),
+ *   M = rho / (pi^{3/2} sqrt(det Theta)) exp(-(c-u)^T Theta^{-1} (c-u)).
+ * With srate = 1/Pr, dgfs_imex_lm_update / dgfs_imex_ars_update relax the
+ * stresses at rate nu/Pr, the reference's ES-BGK moment update. */
+int dgfs_esbgk_target(int dtype, int nslice, int Nv, const double* U, const void* cx,
+                      const void* cy, const void* cz, double Pr, void* M, hipStream_t stream);
 int dgfs_relax_collide(int dtype, int nslice, int Nv, const double* U, const void* M,
                        const void* f, double prefac, double omega, void* Q, hipStream_t stream);
 
@@ -376,6 +385,7 @@ typedef struct {
     const void* Pn[DGFS_IMEX_MAXS];
     const void* Mnew;
     void* fnew;
+    double srate;   /* stress relaxation rate / nu: 1 (Shakov), 1/Pr (ES-BGK); > 0 */
 } dgfs_lm_t;
 int dgfs_imex_lm_update(int dtype, int nslice, int Nv, const dgfs_lm_t* args, int what,
                         hipStream_t stream);
@@ -399,6 +409,7 @@ typedef struct {
     void* F[DGFS_IMEX_MAXS + 1];
     const void* L[DGFS_IMEX_MAXS];
     const void* M[DGFS_IMEX_MAXS + 1];
+    double srate;   /* stress relaxation rate / nu: 1 (Shakov), 1/Pr (ES-BGK); > 0 */
 } dgfs_ars_t;
 int dgfs_imex_ars_update(int dtype, int nslice, int Nv, const dgfs_ars_t* args, int what,
                          hipStream_t stream);

@@ -87,14 +87,16 @@ def _targets(U, T, Pr):
     return P, np.stack(Q, 1)
 
 
-def lm_moments(dt, A, B, C, LU, LM, U, s, prefac, omega, Pr):
+def lm_moments(dt, A, B, C, LU, LM, U, s, prefac, omega, Pr, srate=1.0):
+    """srate: the stresses relax at srate nu (1 Shakov, 1/Pr ES-BGK)."""
     E = sum(-A[s - 1 - i] * U[i] + dt * B[s - 1 - i] * LU[i] + dt * C[s - i] * LM[i]
             for i in range(s))
     T = temperature(E)
     w = C[0] * dt * prefac * E[:, 0] * T ** (1.0 - omega)
     Un = E.copy()
     P, _ = _targets(E, T, Pr)
-    Un[:, 5:11] = (E[:, 5:11] + w[:, None] * P) / (1.0 + w[:, None])
+    ws = srate * w[:, None]
+    Un[:, 5:11] = (E[:, 5:11] + ws * P) / (1.0 + ws)
     _, Q = _targets(Un, T, Pr)
     Un[:, 11:14] = (E[:, 11:14] + w[:, None] * Q) / (1.0 + Pr * w[:, None])
     return Un
@@ -107,20 +109,21 @@ def lm_dist(dt, A, B, C, L, Pn, F, Mnew, Unew, s, prefac, omega):
     return (acc + cdn * Mnew) / (1.0 + cdn)
 
 
-def ars_moments(dt, ah, ai, LU, U, s, prefac, omega, Pr):
+def ars_moments(dt, ah, ai, LU, U, s, prefac, omega, Pr, srate=1.0):
     """U has s entries U_0..U_{s-1}; returns U_s (ai indexed 1..s)."""
     E = U[0] + sum(dt * ah[i] * LU[i] for i in range(s))
     for i in range(1, s):
         Ti = temperature(U[i])
         w = ai[i] * dt * prefac * U[i][:, 0] * Ti ** (1.0 - omega)
         P, Q = _targets(U[i], Ti, Pr)
-        E[:, 5:11] += w[:, None] * (P - U[i][:, 5:11])
+        E[:, 5:11] += srate * w[:, None] * (P - U[i][:, 5:11])
         E[:, 11:14] += w[:, None] * (Q - Pr * U[i][:, 11:14])
     Ts = temperature(E)
     w = ai[s] * dt * prefac * E[:, 0] * Ts ** (1.0 - omega)
     Un = E.copy()
     P, _ = _targets(E, Ts, Pr)
-    Un[:, 5:11] = (E[:, 5:11] + w[:, None] * P) / (1.0 + w[:, None])
+    ws = srate * w[:, None]
+    Un[:, 5:11] = (E[:, 5:11] + ws * P) / (1.0 + ws)
     _, Q = _targets(Un, Ts, Pr)
     Un[:, 11:14] = (E[:, 11:14] + w[:, None] * Q) / (1.0 + Pr * w[:, None])
     return Un
@@ -179,7 +182,7 @@ BDF = {
 }
 
 
-def bdf_moments(dt, A, G, B, U, LU, prefac, omega, Pr):
+def bdf_moments(dt, A, G, B, U, LU, prefac, omega, Pr, srate=1.0):
     """updateMom{q}_BDF: the conserved moments from the history, then the
     stresses and (with the new stresses) the heat flux relaxed implicitly
     with weight B dt nu(U_new)."""
@@ -190,7 +193,7 @@ def bdf_moments(dt, A, G, B, U, LU, prefac, omega, Pr):
     T = temperature(Un)
     w = (B * dt * prefac * Un[:, 0] * T ** (1.0 - omega))[:, None]
     P, _ = _targets(Un, T, Pr)
-    Un[:, 5:11] = (N[:, 5:11] + w * P) / (aq + w)
+    Un[:, 5:11] = (N[:, 5:11] + srate * w * P) / (aq + srate * w)
     _, Q = _targets(Un, T, Pr)
     Un[:, 11:14] = (N[:, 11:14] + w * Q) / (aq + Pr * w)
     return Un
@@ -217,16 +220,41 @@ SSPL = {
 }
 
 
-class Penal:
-    """The penalisation model bound to a grid: prefac, omega, Pr, cv, cw."""
+def es_target(U, cv, Pr):
+    """ES-BGK target of the moments (astd/kernels/scattering/esbgk-direct-gll.mako
+    momentNorm :157-219, cmaxwellian :221-255): Theta = T/Pr I + 2 (1 - 1/Pr)
+    (P/rho - u u^T), M = rho/(pi^1.5 sqrt det Theta) exp(-C^T Theta^-1 C);
+    inverted here with numpy.linalg (the reference writes cofactors)."""
+    rho = U[:, 0]
+    u = U[:, 1:4] / rho[:, None]
+    T = temperature(U)
+    idx = [[5, 8, 10], [8, 6, 9], [10, 9, 7]]
+    out = np.empty((U.shape[0], cv.shape[1]))
+    for s in range(U.shape[0]):
+        P = np.array([[U[s, idx[i][j]] for j in range(3)] for i in range(3)])
+        Th = T[s] / Pr * np.eye(3) + 2.0 * (1.0 - 1.0 / Pr) * (P / rho[s] - np.outer(u[s], u[s]))
+        C = cv - u[s][:, None]
+        q = np.einsum('iv,ij,jv->v', C, np.linalg.inv(Th), C)
+        out[s] = rho[s] / (np.pi ** 1.5 * np.sqrt(np.linalg.det(Th))) * np.exp(-q)
+    return out
 
-    def __init__(self, cv, cw, prefac, omega, Pr):
+
+class Penal:
+    """The penalisation model bound to a grid: prefac, omega, Pr, cv, cw;
+    model 'shakov' (14 moments) or 'esbgk' (the ES-BGK target, stresses
+    relaxing at nu/Pr, astd/scattering.py:287-354)."""
+
+    def __init__(self, cv, cw, prefac, omega, Pr, model='shakov'):
         self.cv, self.cw, self.prefac, self.omega, self.Pr = cv, cw, prefac, omega, Pr
+        self.model = model
+        self.srate = 1.0 / Pr if model == 'esbgk' else 1.0
 
     def moment(self, f):
         return moment(f, self.cv, self.cw)
 
     def target(self, U):
+        if self.model == 'esbgk':
+            return es_target(U, self.cv, self.Pr)
         return target(U, self.cv, self.Pr)
 
     def penal(self, U, f):
@@ -249,7 +277,7 @@ def ars_step(kind, pm, explicit, t, dt, f, penalize=True):
         L.append(Li)
         LU.append(pm.moment(Li))
         ah, ai = _A[i + 1][0:i + 1], [0.0] + A[i + 1][1:i + 2]
-        U.append(ars_moments(dt, ah, ai, LU, U, i + 1, pm.prefac, pm.omega, pm.Pr))
+        U.append(ars_moments(dt, ah, ai, LU, U, i + 1, pm.prefac, pm.omega, pm.Pr, pm.srate))
         M.append(pm.target(U[i + 1]))
         F.append(ars_dist(dt, ah, ai, L, U, M, F, i + 1, pm.prefac, pm.omega))
     return F[-1]
@@ -297,7 +325,7 @@ class LMIntegrator:
         if nacptsteps < s - 1:
             return self.F[nacptsteps + 1].copy()
         Un = lm_moments(dt, self.A, self.B, self.C, self.LU, self.LM, self.U, s,
-                        pm.prefac, pm.omega, pm.Pr)
+                        pm.prefac, pm.omega, pm.Pr, pm.srate)
         Mn = pm.target(Un)
         fn = lm_dist(dt, self.A, self.B, self.C, self.L, self.P, self.F, Mn, Un, s,
                      pm.prefac, pm.omega)
@@ -326,14 +354,16 @@ def sspl_step(kind, pm, explicit, t, dt, f):
     zero, zeroU = np.zeros_like(f), np.zeros((f.shape[0], NALPH))
     F, U, M = [f], [pm.moment(f)], [None]
     L, LU = [], []
-    U.append(ars_moments(dt, [0.0], [0.0, A[0][0]], [zeroU], U, 1, pm.prefac, pm.omega, pm.Pr))
+    U.append(ars_moments(dt, [0.0], [0.0, A[0][0]], [zeroU], U, 1, pm.prefac, pm.omega, pm.Pr,
+                             pm.srate))
     M.append(pm.target(U[1]))
     F.append(ars_dist(dt, [0.0], [0.0, A[0][0]], [zero], U, M, F, 1, pm.prefac, pm.omega))
     for i in range(1, ns):
         L.append(explicit(t, F[i]))
         LU.append(pm.moment(L[-1]))
         ah, ai = _A[i][:i + 1], [0.0] + A[i][:i + 1]
-        U.append(ars_moments(dt, ah, ai, LU + [zeroU], U, i + 1, pm.prefac, pm.omega, pm.Pr))
+        U.append(ars_moments(dt, ah, ai, LU + [zeroU], U, i + 1, pm.prefac, pm.omega, pm.Pr,
+                             pm.srate))
         M.append(pm.target(U[i + 1]))
         F.append(ars_dist(dt, ah, ai, L + [zero], U, M, F, i + 1, pm.prefac, pm.omega))
     L.append(explicit(t, F[ns]))
@@ -396,6 +426,6 @@ class BDFIntegrator:
         L = [h[1] for h in self.hist]
         U = [h[2] for h in self.hist]
         LU = [h[3] for h in self.hist]
-        Un = bdf_moments(dt, self.A, self.G, self.B, U, LU, pm.prefac, pm.omega, pm.Pr)
+        Un = bdf_moments(dt, self.A, self.G, self.B, U, LU, pm.prefac, pm.omega, pm.Pr, pm.srate)
         Mn = pm.target(Un)
         return bdf_dist(dt, self.A, self.G, self.B, F, L, Mn, Un, pm.prefac, pm.omega)
