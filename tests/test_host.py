@@ -44,7 +44,7 @@ def test_registries():
               'esbgk-direct-gll'):
         subclass_where(DGFSScatteringModelStd, scattering_model=k)
     from dgfs1d_gpu_amd.astd.scattering import DGFSScatteringModelAstd
-    for k in ('shakov-direct-gll', 'bgk-direct-gll'):
+    for k in ('shakov-direct-gll', 'bgk-direct-gll', 'esbgk-direct-gll'):
         subclass_where(DGFSScatteringModelAstd, scattering_model=k)
     with pytest.raises(KeyError):
         subclass_where(Basis, basis_kind='nope')
