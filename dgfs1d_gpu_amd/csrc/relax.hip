@@ -384,6 +384,119 @@ k_es_apply(int K, int n, const int* __restrict__ nterm, const int* __restrict__ 
     Q[o] = T(double(Q[o]) + acc);
 }
 
+// --------------------------------------------------------- conservative BGK
+// 'bgk-gll' (std/scattering.py:445-706, bgk-gll.mako:83-229): per task the
+// discrete Maxwellian fe = a0 exp(-a1 |C|^2 + a2 Cx + a3 Cy + a4 Cz), C = c - u,
+// whose discrete mass, momentum and energy (sums times cw) equal f's, found
+// by Newton's method from a = (rho/(pi T)^1.5, 1/T, 0, 0, 0); then
+// Q += prefac rho T^{1-omega} (fe - f).  One work-group per task; every thread
+// holds the same block sums and solves the same 5 x 5 system, so all leave the
+// (at most kBgkIters) iterations together.
+constexpr int kBgkIters = 20;
+constexpr double kBgkTol = 1e-12;
+
+template <typename T>
+__global__ void __launch_bounds__(kRB)
+k_bgk_newton(int n, const T* __restrict__ cx, const T* __restrict__ cy,
+             const T* __restrict__ cz, double cw, const double* __restrict__ cst,
+             double* __restrict__ est /*[task][16]: ux uy uz a0..a4 rate iters*/) {
+    __shared__ double sh[30 * 4];
+    const int t = blockIdx.x;
+    const double* c = cst + size_t(t) * kShkCst;
+    const double ux = c[0], uy = c[1], uz = c[2], Tn = 1.0 / c[3];
+    const double rho = c[4] * pow(kPi * Tn, 1.5);
+    const double m[5] = {rho, rho * ux, rho * uy, rho * uz,
+                         1.5 * rho * Tn + rho * (ux * ux + uy * uy + uz * uz)};
+    // residual scales: |rho| (a task's f may enter with a negative coefficient,
+    // e.g. the basis' U/V terms, so that rho, a0 and the rate are negative)
+    const double ar = fabs(rho);
+    const double sc[5] = {ar, ar * sqrt(Tn), ar * sqrt(Tn), ar * sqrt(Tn), ar * Tn};
+    double a[5] = {c[4], c[3], 0.0, 0.0, 0.0};
+    int it = 0;
+    for (; it < kBgkIters; ++it) {
+        double v[30];
+        for (int i = 0; i < 30; ++i) v[i] = 0.0;
+        for (int p = threadIdx.x; p < n; p += kRB) {
+            const double x = cx[p], y = cy[p], z = cz[p];
+            const double Cx = x - ux, Cy = y - uy, Cz = z - uz;
+            const double C2 = Cx * Cx + Cy * Cy + Cz * Cz;
+            const double fe = a[0] * exp(-a[1] * C2 + a[2] * Cx + a[3] * Cy + a[4] * Cz);
+            const double phi[5] = {1.0, x, y, z, x * x + y * y + z * z};
+            const double d[5] = {fe / a[0], -fe * C2, fe * Cx, fe * Cy, fe * Cz};
+            for (int i = 0; i < 5; ++i) {
+                v[i] += phi[i] * fe;
+                for (int j = 0; j < 5; ++j) v[5 + 5 * i + j] += phi[i] * d[j];
+            }
+        }
+        block_sum_all<30>(v, sh);
+        double F[5], J[5][6], res = 0.0;
+        for (int i = 0; i < 5; ++i) {
+            F[i] = cw * v[i] - m[i];
+            res = fmax(res, fabs(F[i]) / sc[i]);
+            for (int j = 0; j < 5; ++j) J[i][j] = cw * v[5 + 5 * i + j];
+            J[i][5] = -F[i];
+        }
+        if (res < kBgkTol) break;
+        // Gaussian elimination with partial pivoting on [J | -F]
+        for (int k = 0; k < 5; ++k) {
+            int piv = k;
+            for (int i = k + 1; i < 5; ++i)
+                if (fabs(J[i][k]) > fabs(J[piv][k])) piv = i;
+            if (piv != k)
+                for (int j = 0; j < 6; ++j) {
+                    const double tmp = J[k][j];
+                    J[k][j] = J[piv][j];
+                    J[piv][j] = tmp;
+                }
+            for (int i = k + 1; i < 5; ++i) {
+                const double f = J[i][k] / J[k][k];
+                for (int j = k; j < 6; ++j) J[i][j] -= f * J[k][j];
+            }
+        }
+        double dlt[5];
+        for (int i = 4; i >= 0; --i) {
+            double acc = J[i][5];
+            for (int j = i + 1; j < 5; ++j) acc -= J[i][j] * dlt[j];
+            dlt[i] = acc / J[i][i];
+        }
+        for (int i = 0; i < 5; ++i) a[i] += dlt[i];
+    }
+    if (threadIdx.x == 0) {
+        double* e = est + size_t(t) * kEsCst;
+        e[0] = ux;
+        e[1] = uy;
+        e[2] = uz;
+        for (int i = 0; i < 5; ++i) e[3 + i] = a[i];
+        e[8] = c[9];
+        e[9] = double(it);
+    }
+}
+
+// Q[gslice[g]] += sum_{t in group g} prefac rho T^{1-omega} (fe(t) - f_t)
+template <typename T>
+__global__ void __launch_bounds__(kRB)
+k_bgk_apply(int K, int n, const int* __restrict__ nterm, const int* __restrict__ slice,
+            const T* __restrict__ coef, const int* __restrict__ gstart,
+            const int* __restrict__ gslice, const T* __restrict__ u, const T* __restrict__ cx,
+            const T* __restrict__ cy, const T* __restrict__ cz,
+            const double* __restrict__ est, T* __restrict__ Q) {
+    const int g = blockIdx.y;
+    const int v = blockIdx.x * kRB + threadIdx.x;
+    if (v >= n) return;
+    const double x = cx[v], y = cy[v], z = cz[v];
+    double acc = 0.0;
+    for (int t = gstart[g]; t < gstart[g + 1]; ++t) {
+        const double* e = est + size_t(t) * kEsCst;
+        const double Cx = x - e[0], Cy = y - e[1], Cz = z - e[2];
+        const double fe =
+            e[3] * exp(-e[4] * (Cx * Cx + Cy * Cy + Cz * Cz) + e[5] * Cx + e[6] * Cy + e[7] * Cz);
+        const double f = shk_task_value(nterm, slice, coef, K, n, u, t, v);
+        acc += e[8] * (fe - f);
+    }
+    const size_t o = size_t(gslice[g]) * n + v;
+    Q[o] = T(double(Q[o]) + acc);
+}
+
 }  // namespace
 }  // namespace dgfs
 
@@ -605,6 +718,71 @@ extern "C" int dgfs_esbgk(int dtype, int K, int Ne, int Nv, int ntask, const int
                                (double*)Q);
         else
             hipLaunchKernelGGL(k_es_apply<float>, dim3(nb, ngroup), dim3(kRB), 0, stream, K,
+                               Nv, task_nterm, task_slice, (const float*)task_coef, group_start,
+                               group_slice, (const float*)u, (const float*)cx,
+                               (const float*)cy, (const float*)cz, (const double*)est,
+                               (float*)Q);
+        DGFS_HIP_CHECK(hipGetLastError());
+        (void)Ne;
+        return 0;
+    }
+}
+
+extern "C" size_t dgfs_bgk_work_bytes(int ntask, int Nv) {
+    const int nb = (Nv + kRB - 1) / kRB;
+    return size_t(ntask) * (nb * kShkMom + kShkCst + kEsCst) * sizeof(double);
+}
+
+extern "C" int dgfs_bgk_conservative(int dtype, int K, int Ne, int Nv, int ntask,
+                                     const int* task_nterm, const int* task_slice,
+                                     const void* task_coef, int ngroup, const int* group_start,
+                                     const int* group_slice, const void* u, const void* cx,
+                                     const void* cy, const void* cz, double cw, double prefac,
+                                     double omega, void* Q, double* work, hipStream_t stream) {
+    {
+        DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "dtype must be DGFS_F64 or DGFS_F32");
+        DGFS_REQUIRE(K > 0 && Ne > 0 && Nv > 0 && ntask >= 0 && ngroup >= 0, "bad sizes");
+        if (ntask == 0 || ngroup == 0) return 0;
+        DGFS_REQUIRE(ntask <= 65535 && ngroup <= 65535, "too many tasks");
+        DGFS_REQUIRE(task_nterm && task_slice && task_coef && group_start && group_slice && u &&
+                         cx && cy && cz && Q && work,
+                     "null pointer");
+        const int nb = (Nv + kRB - 1) / kRB;
+        const ShkParams P{prefac, omega, 1.0, cw};
+        double* part = work;
+        double* cst = part + size_t(ntask) * nb * kShkMom;
+        double* est = cst + size_t(ntask) * kShkCst;
+        if (dtype == DGFS_F64)
+            hipLaunchKernelGGL(k_shk_partial<double>, dim3(nb, ntask), dim3(kRB), 0, stream, K,
+                               Nv, task_nterm, task_slice, (const double*)task_coef,
+                               (const double*)u, (const double*)cx, (const double*)cy,
+                               (const double*)cz, part);
+        else
+            hipLaunchKernelGGL(k_shk_partial<float>, dim3(nb, ntask), dim3(kRB), 0, stream, K,
+                               Nv, task_nterm, task_slice, (const float*)task_coef,
+                               (const float*)u, (const float*)cx, (const float*)cy,
+                               (const float*)cz, part);
+        DGFS_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_shk_final, dim3(ntask), dim3(kRB), 0, stream, nb, P,
+                           (const double*)part, cst);
+        DGFS_HIP_CHECK(hipGetLastError());
+        if (dtype == DGFS_F64)
+            hipLaunchKernelGGL(k_bgk_newton<double>, dim3(ntask), dim3(kRB), 0, stream, Nv,
+                               (const double*)cx, (const double*)cy, (const double*)cz, cw,
+                               (const double*)cst, est);
+        else
+            hipLaunchKernelGGL(k_bgk_newton<float>, dim3(ntask), dim3(kRB), 0, stream, Nv,
+                               (const float*)cx, (const float*)cy, (const float*)cz, cw,
+                               (const double*)cst, est);
+        DGFS_HIP_CHECK(hipGetLastError());
+        if (dtype == DGFS_F64)
+            hipLaunchKernelGGL(k_bgk_apply<double>, dim3(nb, ngroup), dim3(kRB), 0, stream, K,
+                               Nv, task_nterm, task_slice, (const double*)task_coef, group_start,
+                               group_slice, (const double*)u, (const double*)cx,
+                               (const double*)cy, (const double*)cz, (const double*)est,
+                               (double*)Q);
+        else
+            hipLaunchKernelGGL(k_bgk_apply<float>, dim3(nb, ngroup), dim3(kRB), 0, stream, K,
                                Nv, task_nterm, task_slice, (const float*)task_coef, group_start,
                                group_slice, (const float*)u, (const float*)cx,
                                (const float*)cy, (const float*)cz, (const double*)est,

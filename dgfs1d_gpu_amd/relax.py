@@ -121,12 +121,17 @@ class MixtureProjection:
 class Shakov:
     """Batched Shakov relaxation term Q += prefac rho T^{1-omega} (f_S - f);
     with target='esbgk' the ES-BGK term (dgfs_esbgk), f_S replaced by the
-    anisotropic Gaussian of the node."""
+    anisotropic Gaussian of the node; with target='bgk' the conservative BGK
+    term (dgfs_bgk_conservative), f_S replaced by the discrete Maxwellian with
+    f's discrete mass, momentum and energy."""
 
     def __init__(self, n, cx, cy, cz, cw, prefac, omega, Pr, tdtype, device, target='shakov'):
-        if target not in ('shakov', 'esbgk'):
-            raise ValueError("target must be 'shakov' or 'esbgk'")
-        self._fn = 'dgfs_' + target
+        if target not in ('shakov', 'esbgk', 'bgk'):
+            raise ValueError("target must be 'shakov', 'esbgk' or 'bgk'")
+        self._fn = {'shakov': 'dgfs_shakov', 'esbgk': 'dgfs_esbgk',
+                    'bgk': 'dgfs_bgk_conservative'}[target]
+        self._wfn = {'shakov': 'dgfs_shakov', 'esbgk': 'dgfs_esbgk',
+                     'bgk': 'dgfs_bgk'}[target] + '_work_bytes'
         import torch
         self.torch = torch
         self.n = int(n)
@@ -145,7 +150,7 @@ class Shakov:
         return self.tasks
 
     def work(self, ntask):
-        nb = getattr(_lib.lib(), self._fn + '_work_bytes')(max(1, int(ntask)), self.n)
+        nb = getattr(_lib.lib(), self._wfn)(max(1, int(ntask)), self.n)
         if self._work is None or self._work.numel() * 8 < nb:
             self._work = self.torch.empty(max(1, nb // 8), dtype=self.torch.float64,
                                           device=self.device)
@@ -160,9 +165,10 @@ class Shakov:
         nmin = t['K'] * t['Ne'] * self.n
         if u.numel() < nmin or Q.numel() < nmin:
             raise ValueError('Shakov operands smaller than K*Ne*Nv')
+        pr = () if self._fn == 'dgfs_bgk_conservative' else (self.Pr,)
         _lib.check(getattr(_lib.lib(), self._fn)(
             _lib.dtype_code(self.tdtype), t['K'], t['Ne'], self.n, t['ntask'],
             _lib.ptr(t['nterm']), _lib.ptr(t['slice']), _lib.ptr(t['coef']), t['ngroup'],
             _lib.ptr(t['gstart']), _lib.ptr(t['gslice']), _lib.ptr(u), _lib.ptr(self.cx),
-            _lib.ptr(self.cy), _lib.ptr(self.cz), self.cw, self.prefac, self.omega, self.Pr,
+            _lib.ptr(self.cy), _lib.ptr(self.cz), self.cw, self.prefac, self.omega, *pr,
             _lib.ptr(Q), _lib.ptr(self.work(t['ntask'])), _lib.stream_ptr(stream)))

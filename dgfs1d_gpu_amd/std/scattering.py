@@ -1,7 +1,8 @@
 """Scattering-model plugins (reference dgfs1D/std/scattering.py).
 
-``vhs-gll`` (:51-313), ``vhs`` (:317-429) and ``shakov-direct-gll``
-(:1636-1833) keep the reference constructor ``(cfg, vm, Ne=Ne)``, the
+``vhs-gll`` (:51-313), ``vhs`` (:317-429), ``maxwell-gll`` (:707-743),
+``bgk-gll`` (:445-706), ``bgk-direct-gll`` (:1186-1369), ``esbgk-direct-gll``
+(:1372-1632) and ``shakov-direct-gll`` (:1636-1833) keep the reference constructor ``(cfg, vm, Ne=Ne)``, the
 ``prefac``/``omega`` properties and the single-node
 ``fs(d_f, d_g, d_Q, elem, modein, modeout)``; the solver uses the batched
 ``collide(u, Q)``, which evaluates every (m, r in sigModes[m], e) collision of
@@ -274,6 +275,31 @@ class DGFSBGKDirectGLLScatteringModelStd(DGFSShakovDirectGLLScatteringModelStd):
     def load_parameters(self):
         super().load_parameters()
         self._Pr = 1.0
+
+
+class DGFSBGKGLLScatteringModelStd(DGFSShakovDirectGLLScatteringModelStd):
+    """Conservative BGK (std/scattering.py:445-706): the discrete Maxwellian
+    whose discrete mass, momentum and energy equal the node's, found by
+    Newton's method (dgfs_bgk_conservative), as target; frequency
+    prefac rho T^(1-omega), prefac = t0 p0 / mu(T0).  nodal-sem-gll only, as
+    the reference."""
+    scattering_model = 'bgk-gll'
+
+    def __init__(self, cfg, velocitymesh, **kwargs):
+        if cfg.lookupordefault('basis', 'kind', 'nodal-sem-gll') != 'nodal-sem-gll':
+            raise RuntimeError('Only tested for nodal basis')
+        super().__init__(cfg, velocitymesh, **kwargs)
+
+    def load_parameters(self):
+        super().load_parameters()
+        self._Pr = 1.0
+
+    def perform_precomputation(self):
+        vm = self.vm
+        self.model = Shakov(vm.vsize(), vm.d_cvx(), vm.d_cvy(), vm.d_cvz(), vm.cw(),
+                            self._prefactor, self._omega, 1.0, _torch_dtype(self.cfg),
+                            self._device, target='bgk')
+        self._single = {}
 
 
 class DGFSESBGKDirectGLLScatteringModelStd(DGFSShakovDirectGLLScatteringModelStd):

@@ -326,6 +326,23 @@ int dgfs_shakov(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nte
  * (t0 Pr p0 / mu(T0) in the reference).  Moments and L in FP64; Pr > 0.
  * work: dgfs_esbgk_work_bytes(ntask, Nv) bytes. */
 size_t dgfs_esbgk_work_bytes(int ntask, int Nv);
+
+/* Conservative BGK 'bgk-gll' (replaces fs() of DGFSBGKGLLScatteringModelStd,
+ * dgfs1D/std/scattering.py:600-706, kernels bgk-gll.mako:83-229), task and
+ * group semantics of dgfs_shakov: per task the discrete Maxwellian
+ * fe = a0 exp(-a1 |c-u|^2 + a2 (c-u)_x + a3 (c-u)_y + a4 (c-u)_z) whose
+ * discrete mass, momentum and energy equal f's (Newton's method from
+ * a = (rho/(pi T)^1.5, 1/T, 0, 0, 0), at most 20 iterations, until every
+ * moment residual is below 1e-12 of its scale), then
+ *   Q[group slice] += prefac rho T^{1-omega} (fe - f_t).
+ * FP64 moments and solve.  work: dgfs_bgk_work_bytes(ntask, Nv) bytes. */
+size_t dgfs_bgk_work_bytes(int ntask, int Nv);
+int dgfs_bgk_conservative(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nterm,
+                          const int* task_slice, const void* task_coef, int ngroup,
+                          const int* group_start, const int* group_slice, const void* u,
+                          const void* cx, const void* cy, const void* cz, double cw,
+                          double prefac, double omega, void* Q, double* work,
+                          hipStream_t stream);
 int dgfs_esbgk(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nterm,
                const int* task_slice, const void* task_coef, int ngroup,
                const int* group_start, const int* group_slice, const void* u, const void* cx,

@@ -57,7 +57,7 @@ class Deck:
         omega = g('scattering-model', 'omega')
         self.model = cp.get('scattering-model', 'type')
         shakov = None
-        if self.model in ('shakov-direct-gll', 'bgk-direct-gll', 'esbgk-direct-gll'):
+        if self.model in ('shakov-direct-gll', 'bgk-direct-gll', 'esbgk-direct-gll', 'bgk-gll'):
             Pr = g('scattering-model', 'Pr') if cp.has_option('scattering-model', 'Pr') \
                 else 2.0 / 3.0
             self.prefac = OR.shakov_prefactor(H0, vm.u0, vm.T0, vm.n0,

@@ -90,6 +90,37 @@ def q_esbgk(f, cv, cw, prefac, omega, Pr):
     return prefac * (rho / T ** (omega - 1.0)) * (G - f)
 
 
+def bgk_conservative_target(f, cv, cw):
+    """'bgk-gll' target (std/scattering.py:600-706, bgk-gll.mako:83-229): the
+    discrete Maxwellian a0 exp(-a1 |C|^2 + a2 Cx + a3 Cy + a4 Cz), C = c - u,
+    whose discrete mass, momentum and energy equal f's.  Solved here with
+    scipy.optimize.root (MINPACK hybrd) from the reference's starting point
+    (rho/(pi T)^1.5, 1/T, 0, 0, 0); the reference iterates Newton steps."""
+    from scipy.optimize import root
+    s0 = f.sum()
+    u = np.array([(f * cv[0]).sum(), (f * cv[1]).sum(), (f * cv[2]).sum()]) / s0
+    rho = s0 * cw
+    C = cv - u[:, None]
+    C2 = (C * C).sum(0)
+    T = (f * C2).sum() * cw / (1.5 * rho)
+    phi = np.vstack([np.ones_like(C2), cv, (cv * cv).sum(0)]) * cw
+    m = phi @ f
+    ar = abs(rho)
+    sc = np.array([ar, ar * np.sqrt(T), ar * np.sqrt(T), ar * np.sqrt(T), ar * T])
+
+    def fe(a):
+        return a[0] * np.exp(-a[1] * C2 + a[2] * C[0] + a[3] * C[1] + a[4] * C[2])
+    sol = root(lambda a: (phi @ fe(a) - m) / sc, [rho / (np.pi * T) ** 1.5, 1.0 / T, 0, 0, 0],
+               method='hybr', options={'xtol': 1e-15})
+    return fe(sol.x), rho, T
+
+
+def q_bgk_conservative(f, cv, cw, prefac, omega):
+    """One 'bgk-gll' fs() call: prefac rho / T^(omega-1) (fe - f)."""
+    fe, rho, T = bgk_conservative_target(f, cv, cw)
+    return prefac * (rho / T ** (omega - 1.0)) * (fe - f)
+
+
 def shakov_prefactor(H0, u0, T0, n0, muRef, Tref, omega, R0=8.3144598, NA=6.0221409e+23):
     """t0 p0 / mu(T0) with t0 = H0/u0, p0 = n0 R0/NA T0 (std/scattering.py:1645-1661)."""
     t0 = H0 / u0

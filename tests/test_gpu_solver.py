@@ -169,6 +169,31 @@ def test_rhs_shakov(cuda, kind, model):
     assert err < 1e-11
 
 
+@pytest.mark.parametrize('prec,tol', [('double', 1e-10), ('single', 2e-5)])
+def test_rhs_bgk_conservative(cuda, prec, tol):
+    """'bgk-gll' (std/scattering.py:445-706): the GPU's Newton solve for the
+    conserving discrete Maxwellian against the oracle's MINPACK solve of the
+    same equations, in the full RHS (the solves agree to their 1e-12 stopping
+    residual); the collision output keeps each node's discrete mass, momentum
+    and energy."""
+    import torch
+    deck, solver, u = _pair(cuda, {('scattering-model', 'type'): 'bgk-gll',
+                                   ('scattering-model', 'muRef'): 2.117e-5,
+                                   ('config', 'precision'): prec})
+    out = torch.zeros_like(solver.u)
+    solver.rhs(0.0, solver.u, out)
+    ref = deck.op.rhs(0.0, solver.u.double().cpu().numpy().reshape(u.shape))
+    err = _rel(out.double().cpu().numpy().reshape(ref.shape), ref)
+    Q = torch.zeros_like(solver.u)
+    solver.sm.collide(solver.u, Q)
+    q = Q.double().cpu().numpy().reshape(-1, deck.vm.vsize)
+    cv, cw = deck.vm.cv, deck.vm.cw
+    phi = np.vstack([np.ones(cv.shape[1]), cv, (cv * cv).sum(0)]) * cw
+    mom = np.abs(q @ phi.T).max() / (np.abs(q) @ np.abs(phi.T)).max()
+    print('bgk-gll', prec, err, 'moments of Q', mom, flush=True)
+    assert err < tol and mom < (1e-11 if prec == 'double' else 1e-5)
+
+
 @pytest.mark.parametrize('model', ['shakov-direct-gll', 'esbgk-direct-gll'])
 def test_rhs_relaxation_fp32(cuda, model):
     """The FP32 instantiations of the relaxation kernels (FP64 moments and

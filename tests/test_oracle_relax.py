@@ -206,3 +206,43 @@ def test_esbgk_pins():
         return pf * r * TT ** (1.0 - om) * (G - f)
     SX = _dev(_stress(swapped(f, Pr), cv, cw, u))
     assert np.abs(SX + nu / Pr * Sf).max() > 1e-2 * nu * np.abs(Sf).max()
+
+
+def test_bgk_conservative_pins():
+    """oracle.relax.bgk_conservative_target ('bgk-gll'): its discrete mass,
+    momentum and energy equal f's to round-off (the defining equations, on a
+    grid too coarse for the plain Maxwellian to conserve them); a discrete
+    Maxwellian of that exact form is returned unchanged; on a resolved grid it
+    tends to the plain BGK target."""
+    cv, cw = grid(N=12, L=4.0)                         # coarse: the plain Maxwellian leaks
+    f = bimodal(cv)
+    fe, rho, T = OR.bgk_conservative_target(f, cv, cw)
+    phi = OR.invariants(cv)
+    mf, me = phi @ f * cw, phi @ fe * cw
+    assert np.abs(me - mf).max() < 1e-12 * np.abs(mf).max()
+    Qb = OR.q_bgk(f, cv, cw, 1.0, 0.81)
+    plain = phi @ (f + Qb / (rho * T ** 0.19)) * cw
+    assert np.abs(plain - mf).max() > 1e-7 * np.abs(mf).max()   # the direct target does not
+    C = cv - np.array([0.1, -0.2, 0.05])[:, None]
+    g = 0.8 * np.exp(-1.3 * (C * C).sum(0) + 0.2 * C[0] - 0.1 * C[2])
+    ge, _, _ = OR.bgk_conservative_target(g, cv, cw)
+    assert np.abs(ge - g).max() < 1e-10 * g.max()
+    cv2, cw2 = grid()
+    f2 = bimodal(cv2)
+    Q1 = OR.q_bgk_conservative(f2, cv2, cw2, 1.7, 0.81)
+    Q2 = OR.q_bgk(f2, cv2, cw2, 1.7, 0.81)
+    assert np.abs(Q1 - Q2).max() < 1e-8 * np.abs(Q2).max()
+
+
+def test_bgk_conservative_negative_node():
+    """A task's f may enter with coefficient -1 (basis U/V terms): the
+    conservative BGK term is quadratic, Q(-f) = Q(f), and the target of -f is
+    minus the target of f."""
+    cv, cw = grid(N=12, L=4.0)
+    f = bimodal(cv)
+    fe, _, _ = OR.bgk_conservative_target(f, cv, cw)
+    fm, _, _ = OR.bgk_conservative_target(-f, cv, cw)
+    assert np.abs(fm + fe).max() < 1e-12 * fe.max()
+    Q1 = OR.q_bgk_conservative(f, cv, cw, 1.3, 0.81)
+    Q2 = OR.q_bgk_conservative(-f, cv, cw, 1.3, 0.81)
+    assert np.abs(Q1 - Q2).max() < 1e-12 * np.abs(Q1).max()
