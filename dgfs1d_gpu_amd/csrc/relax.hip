@@ -497,6 +497,157 @@ k_bgk_apply(int K, int n, const int* __restrict__ nterm, const int* __restrict__
     Q[o] = T(double(Q[o]) + acc);
 }
 
+// ------------------------------------------------------- conservative ES-BGK
+// 'esbgk-gll' (std/scattering.py:752-1185, esbgk-gll.mako:218-416): after the
+// conservative BGK target fe (k_bgk_newton), the second moments
+// T_ij = cw/rho sum C_i C_j ((1 - 1/Pr) f + (1/Pr) fe) set the targets
+// {rho, rho u, rho (u_i u_j + T_ij)}; the discrete Gaussian
+// G = p0 exp(sum_{l=1..9} p_l phi_l(c)), phi = {c_x, c_y, c_z, c_x^2, c_y^2,
+// c_z^2, c_x c_y, c_y c_z, c_z c_x}, with those ten moments is found by
+// Newton's method (at most 40 iterations, residual < 1e-12 of the scales).
+// The Jacobian cw sum phi_k phi_l G (phi_0 = 1) is symmetric: 55 block sums.
+// One work-group per task; thread 0 solves, the others read the result.
+constexpr int kEsgIters = 40;
+constexpr int kEsgCst = 16;
+
+DGFS_DEV void esg_phi(double x, double y, double z, double (&ph)[10]) {
+    ph[0] = 1.0; ph[1] = x; ph[2] = y; ph[3] = z;
+    ph[4] = x * x; ph[5] = y * y; ph[6] = z * z;
+    ph[7] = x * y; ph[8] = y * z; ph[9] = z * x;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRB)
+k_esg_newton(int K, int n, const int* __restrict__ nterm, const int* __restrict__ slice,
+             const T* __restrict__ coef, const T* __restrict__ u, const T* __restrict__ cx,
+             const T* __restrict__ cy, const T* __restrict__ cz, double cw, double Pr,
+             const double* __restrict__ cst, const double* __restrict__ ebgk,
+             double* __restrict__ eesg /*[task][16]: p0..p9 rate iters*/) {
+    __shared__ double sh[55 * 4];
+    __shared__ double sp[11];
+    const int t = blockIdx.x;
+    const double* c = cst + size_t(t) * kShkCst;
+    const double* b = ebgk + size_t(t) * kEsCst;
+    const double ux = c[0], uy = c[1], uz = c[2], Tn = 1.0 / c[3];
+    const double rho = c[4] * pow(kPi * Tn, 1.5);
+    // tensor moments of (1 - 1/Pr) f + (1/Pr) fe
+    double S[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int p = threadIdx.x; p < n; p += kRB) {
+        const double Cx = double(cx[p]) - ux, Cy = double(cy[p]) - uy, Cz = double(cz[p]) - uz;
+        const double fe = b[3] * exp(-b[4] * (Cx * Cx + Cy * Cy + Cz * Cz) + b[5] * Cx +
+                                     b[6] * Cy + b[7] * Cz);
+        const double w = (1.0 - 1.0 / Pr) * shk_task_value(nterm, slice, coef, K, n, u, t, p) +
+                         (1.0 / Pr) * fe;
+        S[0] += w * Cx * Cx; S[1] += w * Cy * Cy; S[2] += w * Cz * Cz;
+        S[3] += w * Cx * Cy; S[4] += w * Cy * Cz; S[5] += w * Cz * Cx;
+    }
+    block_sum_all<6>(S, sh);
+    const double m[10] = {rho, rho * ux, rho * uy, rho * uz,
+                          rho * ux * ux + cw * S[0], rho * uy * uy + cw * S[1],
+                          rho * uz * uz + cw * S[2], rho * ux * uy + cw * S[3],
+                          rho * uy * uz + cw * S[4], rho * uz * ux + cw * S[5]};
+    const double ar = fabs(rho), sT = sqrt(Tn);
+    const double sc[10] = {ar, ar * sT, ar * sT, ar * sT, ar * Tn, ar * Tn, ar * Tn,
+                           ar * Tn, ar * Tn, ar * Tn};
+    // start: the Maxwellian of (rho, u, T) written in the monomials
+    double q[10] = {c[4] * exp(-(ux * ux + uy * uy + uz * uz) / Tn), 2.0 * ux / Tn, 2.0 * uy / Tn,
+                    2.0 * uz / Tn, -1.0 / Tn, -1.0 / Tn, -1.0 / Tn, 0.0, 0.0, 0.0};
+    int it = 0;
+    for (; it < kEsgIters; ++it) {
+        double v[55];
+        for (int i = 0; i < 55; ++i) v[i] = 0.0;
+        for (int p = threadIdx.x; p < n; p += kRB) {
+            double ph[10];
+            esg_phi(cx[p], cy[p], cz[p], ph);
+            double e = 0.0;
+            for (int l = 1; l < 10; ++l) e += q[l] * ph[l];
+            const double G = q[0] * exp(e);
+            int k = 0;
+            for (int i = 0; i < 10; ++i)
+                for (int j = i; j < 10; ++j) v[k++] += ph[i] * ph[j] * G;
+        }
+        block_sum_all<55>(v, sh);
+        if (threadIdx.x == 0) {
+            double J[10][11], res = 0.0;
+            int k = 0;
+            for (int i = 0; i < 10; ++i)
+                for (int j = i; j < 10; ++j) {
+                    J[i][j] = J[j][i] = cw * v[k];
+                    ++k;
+                }
+            for (int i = 0; i < 10; ++i) {
+                const double F = J[i][0] - m[i];          // cw sum phi_i G
+                res = fmax(res, fabs(F) / sc[i]);
+                J[i][10] = -F;
+                J[i][0] /= q[0];                          // d G / d p0 = G / p0
+            }
+            sp[10] = res;
+            if (res >= kBgkTol) {
+                for (int kk = 0; kk < 10; ++kk) {
+                    int piv = kk;
+                    for (int i = kk + 1; i < 10; ++i)
+                        if (fabs(J[i][kk]) > fabs(J[piv][kk])) piv = i;
+                    if (piv != kk)
+                        for (int j = 0; j < 11; ++j) {
+                            const double tmp = J[kk][j];
+                            J[kk][j] = J[piv][j];
+                            J[piv][j] = tmp;
+                        }
+                    for (int i = kk + 1; i < 10; ++i) {
+                        const double f = J[i][kk] / J[kk][kk];
+                        for (int j = kk; j < 11; ++j) J[i][j] -= f * J[kk][j];
+                    }
+                }
+                double d[10];
+                for (int i = 9; i >= 0; --i) {
+                    double acc = J[i][10];
+                    for (int j = i + 1; j < 10; ++j) acc -= J[i][j] * d[j];
+                    d[i] = acc / J[i][i];
+                }
+                for (int i = 0; i < 10; ++i) sp[i] = q[i] + d[i];
+            } else {
+                for (int i = 0; i < 10; ++i) sp[i] = q[i];
+            }
+        }
+        __syncthreads();
+        const double res = sp[10];
+        for (int i = 0; i < 10; ++i) q[i] = sp[i];
+        __syncthreads();
+        if (res < kBgkTol) break;
+    }
+    if (threadIdx.x == 0) {
+        double* e = eesg + size_t(t) * kEsgCst;
+        for (int i = 0; i < 10; ++i) e[i] = q[i];
+        e[10] = c[9];
+        e[11] = double(it);
+    }
+}
+
+// Q[gslice[g]] += sum_{t in group g} prefac rho T^{1-omega} (G(t) - f_t)
+template <typename T>
+__global__ void __launch_bounds__(kRB)
+k_esg_apply(int K, int n, const int* __restrict__ nterm, const int* __restrict__ slice,
+            const T* __restrict__ coef, const int* __restrict__ gstart,
+            const int* __restrict__ gslice, const T* __restrict__ u, const T* __restrict__ cx,
+            const T* __restrict__ cy, const T* __restrict__ cz,
+            const double* __restrict__ eesg, T* __restrict__ Q) {
+    const int g = blockIdx.y;
+    const int v = blockIdx.x * kRB + threadIdx.x;
+    if (v >= n) return;
+    double ph[10];
+    esg_phi(cx[v], cy[v], cz[v], ph);
+    double acc = 0.0;
+    for (int t = gstart[g]; t < gstart[g + 1]; ++t) {
+        const double* e = eesg + size_t(t) * kEsgCst;
+        double x = 0.0;
+        for (int l = 1; l < 10; ++l) x += e[l] * ph[l];
+        const double f = shk_task_value(nterm, slice, coef, K, n, u, t, v);
+        acc += e[10] * (e[0] * exp(x) - f);
+    }
+    const size_t o = size_t(gslice[g]) * n + v;
+    Q[o] = T(double(Q[o]) + acc);
+}
+
 }  // namespace
 }  // namespace dgfs
 
@@ -787,6 +938,87 @@ extern "C" int dgfs_bgk_conservative(int dtype, int K, int Ne, int Nv, int ntask
                                group_slice, (const float*)u, (const float*)cx,
                                (const float*)cy, (const float*)cz, (const double*)est,
                                (float*)Q);
+        DGFS_HIP_CHECK(hipGetLastError());
+        (void)Ne;
+        return 0;
+    }
+}
+
+extern "C" size_t dgfs_esbgk_conservative_work_bytes(int ntask, int Nv) {
+    const int nb = (Nv + kRB - 1) / kRB;
+    return size_t(ntask) * (nb * kShkMom + kShkCst + kEsCst + kEsgCst) * sizeof(double);
+}
+
+extern "C" int dgfs_esbgk_conservative(int dtype, int K, int Ne, int Nv, int ntask,
+                                       const int* task_nterm, const int* task_slice,
+                                       const void* task_coef, int ngroup, const int* group_start,
+                                       const int* group_slice, const void* u, const void* cx,
+                                       const void* cy, const void* cz, double cw, double prefac,
+                                       double omega, double Pr, void* Q, double* work,
+                                       hipStream_t stream) {
+    {
+        DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "dtype must be DGFS_F64 or DGFS_F32");
+        DGFS_REQUIRE(K > 0 && Ne > 0 && Nv > 0 && ntask >= 0 && ngroup >= 0, "bad sizes");
+        DGFS_REQUIRE(Pr > 0.0, "esbgk_conservative: Pr must be positive");
+        if (ntask == 0 || ngroup == 0) return 0;
+        DGFS_REQUIRE(ntask <= 65535 && ngroup <= 65535, "too many tasks");
+        DGFS_REQUIRE(task_nterm && task_slice && task_coef && group_start && group_slice && u &&
+                         cx && cy && cz && Q && work,
+                     "null pointer");
+        const int nb = (Nv + kRB - 1) / kRB;
+        const ShkParams P{prefac, omega, 1.0, cw};
+        double* part = work;
+        double* cst = part + size_t(ntask) * nb * kShkMom;
+        double* ebgk = cst + size_t(ntask) * kShkCst;
+        double* eesg = ebgk + size_t(ntask) * kEsCst;
+        if (dtype == DGFS_F64) {
+            hipLaunchKernelGGL(k_shk_partial<double>, dim3(nb, ntask), dim3(kRB), 0, stream, K,
+                               Nv, task_nterm, task_slice, (const double*)task_coef,
+                               (const double*)u, (const double*)cx, (const double*)cy,
+                               (const double*)cz, part);
+        } else {
+            hipLaunchKernelGGL(k_shk_partial<float>, dim3(nb, ntask), dim3(kRB), 0, stream, K,
+                               Nv, task_nterm, task_slice, (const float*)task_coef,
+                               (const float*)u, (const float*)cx, (const float*)cy,
+                               (const float*)cz, part);
+        }
+        DGFS_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_shk_final, dim3(ntask), dim3(kRB), 0, stream, nb, P,
+                           (const double*)part, cst);
+        DGFS_HIP_CHECK(hipGetLastError());
+        if (dtype == DGFS_F64) {
+            hipLaunchKernelGGL(k_bgk_newton<double>, dim3(ntask), dim3(kRB), 0, stream, Nv,
+                               (const double*)cx, (const double*)cy, (const double*)cz, cw,
+                               (const double*)cst, ebgk);
+            DGFS_HIP_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(k_esg_newton<double>, dim3(ntask), dim3(kRB), 0, stream, K, Nv,
+                               task_nterm, task_slice, (const double*)task_coef,
+                               (const double*)u, (const double*)cx, (const double*)cy,
+                               (const double*)cz, cw, Pr, (const double*)cst,
+                               (const double*)ebgk, eesg);
+            DGFS_HIP_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(k_esg_apply<double>, dim3(nb, ngroup), dim3(kRB), 0, stream, K,
+                               Nv, task_nterm, task_slice, (const double*)task_coef, group_start,
+                               group_slice, (const double*)u, (const double*)cx,
+                               (const double*)cy, (const double*)cz, (const double*)eesg,
+                               (double*)Q);
+        } else {
+            hipLaunchKernelGGL(k_bgk_newton<float>, dim3(ntask), dim3(kRB), 0, stream, Nv,
+                               (const float*)cx, (const float*)cy, (const float*)cz, cw,
+                               (const double*)cst, ebgk);
+            DGFS_HIP_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(k_esg_newton<float>, dim3(ntask), dim3(kRB), 0, stream, K, Nv,
+                               task_nterm, task_slice, (const float*)task_coef,
+                               (const float*)u, (const float*)cx, (const float*)cy,
+                               (const float*)cz, cw, Pr, (const double*)cst,
+                               (const double*)ebgk, eesg);
+            DGFS_HIP_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(k_esg_apply<float>, dim3(nb, ngroup), dim3(kRB), 0, stream, K,
+                               Nv, task_nterm, task_slice, (const float*)task_coef, group_start,
+                               group_slice, (const float*)u, (const float*)cx,
+                               (const float*)cy, (const float*)cz, (const double*)eesg,
+                               (float*)Q);
+        }
         DGFS_HIP_CHECK(hipGetLastError());
         (void)Ne;
         return 0;

@@ -126,12 +126,12 @@ class Shakov:
     f's discrete mass, momentum and energy."""
 
     def __init__(self, n, cx, cy, cz, cw, prefac, omega, Pr, tdtype, device, target='shakov'):
-        if target not in ('shakov', 'esbgk', 'bgk'):
-            raise ValueError("target must be 'shakov', 'esbgk' or 'bgk'")
-        self._fn = {'shakov': 'dgfs_shakov', 'esbgk': 'dgfs_esbgk',
-                    'bgk': 'dgfs_bgk_conservative'}[target]
-        self._wfn = {'shakov': 'dgfs_shakov', 'esbgk': 'dgfs_esbgk',
-                     'bgk': 'dgfs_bgk'}[target] + '_work_bytes'
+        fns = {'shakov': 'dgfs_shakov', 'esbgk': 'dgfs_esbgk', 'bgk': 'dgfs_bgk_conservative',
+               'esbgk-conservative': 'dgfs_esbgk_conservative'}
+        if target not in fns:
+            raise ValueError('target must be one of %s' % ', '.join(sorted(fns)))
+        self._fn = fns[target]
+        self._wfn = ('dgfs_bgk' if target == 'bgk' else fns[target]) + '_work_bytes'
         import torch
         self.torch = torch
         self.n = int(n)

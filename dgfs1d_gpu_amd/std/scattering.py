@@ -1,7 +1,7 @@
 """Scattering-model plugins (reference dgfs1D/std/scattering.py).
 
 ``vhs-gll`` (:51-313), ``vhs`` (:317-429), ``maxwell-gll`` (:707-743),
-``bgk-gll`` (:445-706), ``bgk-direct-gll`` (:1186-1369), ``esbgk-direct-gll``
+``bgk-gll`` (:445-706), ``esbgk-gll`` (:752-1185), ``bgk-direct-gll`` (:1186-1369), ``esbgk-direct-gll``
 (:1372-1632) and ``shakov-direct-gll`` (:1636-1833) keep the reference constructor ``(cfg, vm, Ne=Ne)``, the
 ``prefac``/``omega`` properties and the single-node
 ``fs(d_f, d_g, d_Q, elem, modein, modeout)``; the solver uses the batched
@@ -299,6 +299,26 @@ class DGFSBGKGLLScatteringModelStd(DGFSShakovDirectGLLScatteringModelStd):
         self.model = Shakov(vm.vsize(), vm.d_cvx(), vm.d_cvy(), vm.d_cvz(), vm.cw(),
                             self._prefactor, self._omega, 1.0, _torch_dtype(self.cfg),
                             self._device, target='bgk')
+        self._single = {}
+
+
+class DGFSESBGKGLLScatteringModelStd(DGFSBGKGLLScatteringModelStd):
+    """Conservative ES-BGK (std/scattering.py:752-1185): the discrete
+    Gaussian whose mass, momentum and second moments equal those of the ES
+    tensor built from f and the conservative BGK target
+    (dgfs_esbgk_conservative); prefac = t0 Pr p0 / mu(T0).  nodal-sem-gll
+    only."""
+    scattering_model = 'esbgk-gll'
+
+    def load_parameters(self):
+        DGFSShakovDirectGLLScatteringModelStd.load_parameters(self)
+        self._prefactor *= self._Pr
+
+    def perform_precomputation(self):
+        vm = self.vm
+        self.model = Shakov(vm.vsize(), vm.d_cvx(), vm.d_cvy(), vm.d_cvz(), vm.cw(),
+                            self._prefactor, self._omega, self._Pr, _torch_dtype(self.cfg),
+                            self._device, target='esbgk-conservative')
         self._single = {}
 
 

@@ -337,6 +337,23 @@ size_t dgfs_esbgk_work_bytes(int ntask, int Nv);
  *   Q[group slice] += prefac rho T^{1-omega} (fe - f_t).
  * FP64 moments and solve.  work: dgfs_bgk_work_bytes(ntask, Nv) bytes. */
 size_t dgfs_bgk_work_bytes(int ntask, int Nv);
+
+/* Conservative ES-BGK 'esbgk-gll' (replaces fs() of
+ * DGFSESBGKGLLScatteringModelStd, dgfs1D/std/scattering.py:752-1185, kernels
+ * esbgk-gll.mako:218-416), semantics of dgfs_shakov: the conservative BGK
+ * target fe of dgfs_bgk_conservative, then T_ij = cw/rho sum (c-u)_i (c-u)_j
+ * ((1 - 1/Pr) f + (1/Pr) fe) and the discrete Gaussian G whose mass,
+ * momentum and second moments equal rho, rho u, rho (u_i u_j + T_ij)
+ * (Newton's method, at most 40 iterations); Q += prefac rho T^{1-omega} (G - f),
+ * prefac the caller's (t0 Pr p0 / mu(T0) in the reference).  Pr > 0.
+ * work: dgfs_esbgk_conservative_work_bytes(ntask, Nv) bytes. */
+size_t dgfs_esbgk_conservative_work_bytes(int ntask, int Nv);
+int dgfs_esbgk_conservative(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nterm,
+                            const int* task_slice, const void* task_coef, int ngroup,
+                            const int* group_start, const int* group_slice, const void* u,
+                            const void* cx, const void* cy, const void* cz, double cw,
+                            double prefac, double omega, double Pr, void* Q, double* work,
+                            hipStream_t stream);
 int dgfs_bgk_conservative(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nterm,
                           const int* task_slice, const void* task_coef, int ngroup,
                           const int* group_start, const int* group_slice, const void* u,

@@ -57,13 +57,14 @@ class Deck:
         omega = g('scattering-model', 'omega')
         self.model = cp.get('scattering-model', 'type')
         shakov = None
-        if self.model in ('shakov-direct-gll', 'bgk-direct-gll', 'esbgk-direct-gll', 'bgk-gll'):
+        if self.model in ('shakov-direct-gll', 'bgk-direct-gll', 'esbgk-direct-gll', 'bgk-gll',
+                          'esbgk-gll'):
             Pr = g('scattering-model', 'Pr') if cp.has_option('scattering-model', 'Pr') \
                 else 2.0 / 3.0
             self.prefac = OR.shakov_prefactor(H0, vm.u0, vm.T0, vm.n0,
                                               g('scattering-model', 'muRef'),
                                               g('scattering-model', 'Tref'), omega)
-            if self.model == 'esbgk-direct-gll':
+            if self.model in ('esbgk-direct-gll', 'esbgk-gll'):
                 self.prefac *= Pr                 # t0 Pr p0 / mu(T0)
             shakov = (omega, Pr)
             self.pre = None

@@ -121,6 +121,41 @@ def q_bgk_conservative(f, cv, cw, prefac, omega):
     return prefac * (rho / T ** (omega - 1.0)) * (fe - f)
 
 
+def esbgk_conservative_target(f, cv, cw, Pr):
+    """'esbgk-gll' target (std/scattering.py:752-1185, esbgk-gll.mako:218-416):
+    fe the conservative BGK target, T_ij = cw/rho sum C_i C_j ((1 - 1/Pr) f
+    + (1/Pr) fe), and the discrete Gaussian in the reference's parameters
+    a0 exp(-a1 Cx^2 + a2 Cx - a3 Cy^2 + a4 Cy - a5 Cz^2 + a6 Cz + a7 cx cy
+    + a8 cx cz + a9 cy cz) whose mass, momentum and second moments equal
+    rho, rho u, rho (u_i u_j + T_ij); solved with scipy.optimize.root."""
+    from scipy.optimize import root
+    fe, rho, T = bgk_conservative_target(f, cv, cw)
+    s0 = f.sum()
+    u = np.array([(f * cv[0]).sum(), (f * cv[1]).sum(), (f * cv[2]).sum()]) / s0
+    C = cv - u[:, None]
+    w = (1.0 - 1.0 / Pr) * f + (1.0 / Pr) * fe
+    pr = [(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (2, 0)]
+    Tij = [(C[i] * C[j] * w).sum() * cw / rho for i, j in pr]
+    m = np.array([rho, *(rho * u)] + [rho * (u[i] * u[j] + Tij[k]) for k, (i, j) in enumerate(pr)])
+    phi = np.vstack([np.ones(cv.shape[1]), cv] + [cv[i] * cv[j] for i, j in pr]) * cw
+    ar = abs(rho)
+    sc = np.array([ar] + [ar * np.sqrt(T)] * 3 + [ar * T] * 6)
+
+    def G(a):
+        return a[0] * np.exp(-a[1] * C[0] ** 2 + a[2] * C[0] - a[3] * C[1] ** 2 + a[4] * C[1]
+                             - a[5] * C[2] ** 2 + a[6] * C[2] + a[7] * cv[0] * cv[1]
+                             + a[8] * cv[0] * cv[2] + a[9] * cv[1] * cv[2])
+    a0 = [rho / (np.pi * T) ** 1.5, 1 / T, 0, 1 / T, 0, 1 / T, 0, 0, 0, 0]
+    sol = root(lambda a: (phi @ G(a) - m) / sc, a0, method='hybr', options={'xtol': 1e-15})
+    return G(sol.x), rho, T
+
+
+def q_esbgk_conservative(f, cv, cw, prefac, omega, Pr):
+    """One 'esbgk-gll' fs() call: prefac rho / T^(omega-1) (G - f)."""
+    G, rho, T = esbgk_conservative_target(f, cv, cw, Pr)
+    return prefac * (rho / T ** (omega - 1.0)) * (G - f)
+
+
 def shakov_prefactor(H0, u0, T0, n0, muRef, Tref, omega, R0=8.3144598, NA=6.0221409e+23):
     """t0 p0 / mu(T0) with t0 = H0/u0, p0 = n0 R0/NA T0 (std/scattering.py:1645-1661)."""
     t0 = H0 / u0

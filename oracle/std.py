@@ -175,6 +175,10 @@ class StdOracle:
                     elif self.model == 'bgk-gll':
                         acc += OR.q_bgk_conservative(f[r, e], vm.cv, vm.cw, self.prefac,
                                                      self.shakov[0])
+                    elif self.model == 'esbgk-gll':
+                        omega, Pr = self.shakov
+                        acc += OR.q_esbgk_conservative(f[r, e], vm.cv, vm.cw, self.prefac,
+                                                       omega, Pr)
                     elif self.model == 'esbgk-direct-gll':
                         omega, Pr = self.shakov
                         acc += OR.q_esbgk(f[r, e], vm.cv, vm.cw, self.prefac, omega, Pr)

@@ -74,3 +74,7 @@ def test_new_entry_points_reject_bad_arguments():
                                    None, None, 1.0, 1.0, 0.81, None, None, None) == 2
     assert b'null pointer' in L.dgfs_last_error()
     assert L.dgfs_bgk_work_bytes(2, 600) == 2 * (3 * 13 + 32) * 8
+    assert L.dgfs_esbgk_conservative(0, 1, 1, 8, 1, None, None, None, 1, None, None, None, None,
+                                     None, None, 1.0, 1.0, 0.81, -1.0, None, None, None) == 2
+    assert b'Pr must be positive' in L.dgfs_last_error()
+    assert L.dgfs_esbgk_conservative_work_bytes(2, 600) == 2 * (3 * 13 + 48) * 8

@@ -170,14 +170,15 @@ def test_rhs_shakov(cuda, kind, model):
 
 
 @pytest.mark.parametrize('prec,tol', [('double', 1e-10), ('single', 2e-5)])
-def test_rhs_bgk_conservative(cuda, prec, tol):
-    """'bgk-gll' (std/scattering.py:445-706): the GPU's Newton solve for the
-    conserving discrete Maxwellian against the oracle's MINPACK solve of the
-    same equations, in the full RHS (the solves agree to their 1e-12 stopping
-    residual); the collision output keeps each node's discrete mass, momentum
-    and energy."""
+@pytest.mark.parametrize('model', ['bgk-gll', 'esbgk-gll'])
+def test_rhs_bgk_conservative(cuda, prec, tol, model):
+    """'bgk-gll' (std/scattering.py:445-706) and 'esbgk-gll' (:752-1185): the
+    GPU's Newton solves for the conserving discrete Maxwellian / Gaussian
+    against the oracle's MINPACK solves of the same equations (the ES one in
+    the reference's parameters), in the full RHS; the collision output keeps
+    each node's discrete mass, momentum and energy."""
     import torch
-    deck, solver, u = _pair(cuda, {('scattering-model', 'type'): 'bgk-gll',
+    deck, solver, u = _pair(cuda, {('scattering-model', 'type'): model,
                                    ('scattering-model', 'muRef'): 2.117e-5,
                                    ('config', 'precision'): prec})
     out = torch.zeros_like(solver.u)
@@ -190,7 +191,7 @@ def test_rhs_bgk_conservative(cuda, prec, tol):
     cv, cw = deck.vm.cv, deck.vm.cw
     phi = np.vstack([np.ones(cv.shape[1]), cv, (cv * cv).sum(0)]) * cw
     mom = np.abs(q @ phi.T).max() / (np.abs(q) @ np.abs(phi.T)).max()
-    print('bgk-gll', prec, err, 'moments of Q', mom, flush=True)
+    print(model, prec, err, 'moments of Q', mom, flush=True)
     assert err < tol and mom < (1e-11 if prec == 'double' else 1e-5)
 
 

@@ -41,7 +41,7 @@ def test_registries():
               'dgfs-wall-expr-diffuse', 'dgfs-inlet-nondim'):
         subclass_where(DGFSBCStd, type=k)
     for k in ('vhs', 'vhs-gll', 'maxwell-gll', 'shakov-direct-gll', 'bgk-direct-gll',
-              'esbgk-direct-gll', 'bgk-gll'):
+              'esbgk-direct-gll', 'bgk-gll', 'esbgk-gll'):
         subclass_where(DGFSScatteringModelStd, scattering_model=k)
     from dgfs1d_gpu_amd.astd.scattering import DGFSScatteringModelAstd
     for k in ('shakov-direct-gll', 'bgk-direct-gll', 'esbgk-direct-gll'):

@@ -246,3 +246,31 @@ def test_bgk_conservative_negative_node():
     Q1 = OR.q_bgk_conservative(f, cv, cw, 1.3, 0.81)
     Q2 = OR.q_bgk_conservative(-f, cv, cw, 1.3, 0.81)
     assert np.abs(Q1 - Q2).max() < 1e-12 * np.abs(Q1).max()
+
+
+def test_esbgk_conservative_pins():
+    """oracle.relax.esbgk_conservative_target ('esbgk-gll'): the Gaussian's
+    mass, momentum and second moments equal rho, rho u and rho (u u + T_ij)
+    of the ES tensor (a strongly anisotropic node); at Pr = 1 the tensor is
+    the conservative BGK target's and the Gaussian is that target; the term
+    is quadratic, Q(-f) = Q(f)."""
+    cv, cw = grid(N=16, L=5.0)
+    C = cv - np.array([0.2, -0.1, 0.05])[:, None]
+    f = np.exp(-C[0] ** 2 / 0.7 - C[1] ** 2 / 1.3 - C[2] ** 2 / 1.0 - 0.3 * C[0] * C[1])
+    Pr = 2.0 / 3.0
+    G, rho, T = OR.esbgk_conservative_target(f, cv, cw, Pr)
+    fe, _, _ = OR.bgk_conservative_target(f, cv, cw)
+    u = np.array([(f * cv[i]).sum() for i in range(3)]) / f.sum()
+    Cu = cv - u[:, None]
+    w = (1 - 1 / Pr) * f + (1 / Pr) * fe
+    for i, j in [(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (2, 0)]:
+        want = rho * u[i] * u[j] + (Cu[i] * Cu[j] * w).sum() * cw
+        got = (cv[i] * cv[j] * G).sum() * cw
+        assert abs(got - want) < 1e-11 * rho * T, (i, j)
+    assert abs(G.sum() * cw - rho) < 1e-12 * rho
+    assert np.abs(cv @ G * cw - rho * u).max() < 1e-11 * rho
+    G1, _, _ = OR.esbgk_conservative_target(f, cv, cw, 1.0)
+    assert np.abs(G1 - fe).max() < 1e-9 * fe.max()
+    Q1 = OR.q_esbgk_conservative(f, cv, cw, 1.2, 0.81, Pr)
+    Q2 = OR.q_esbgk_conservative(-f, cv, cw, 1.2, 0.81, Pr)
+    assert np.abs(Q1 - Q2).max() < 1e-10 * np.abs(Q1).max()
