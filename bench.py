@@ -208,6 +208,46 @@ def steady_state(dev, tau=1e-6, tau2=1e-5, max_steps=6000):
     return out
 
 
+def relax_models(dev, reps, config):
+    """One collision call of each relaxation model on the Couette workload's
+    grid (Ne=4, K=3, N=32^3, FP32: 12 node slices), timed with HIP events on
+    the current stream over reps calls: ms per call and the rate of its
+    algorithmic bytes, f read twice and Q read and written (4 x 12 x N^3 x 4 B;
+    the Newton models re-read the velocity grid on every iteration)."""
+    import configparser
+    import io
+    import torch
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    from dgfs1d_gpu_amd.std.std import DGFSStd1D
+    out = {'workload': 'Couette grid Ne=4 K=3 N=32^3 FP32, one collision call (12 slices)'}
+    for model in ('shakov-direct-gll', 'bgk-direct-gll', 'esbgk-direct-gll', 'bgk-gll',
+                  'esbgk-gll'):
+        cp = configparser.ConfigParser(inline_comment_prefixes=[';'])
+        cp.optionxform = str
+        cp.read(config)
+        cp.set('scattering-model', 'type', model)
+        if not cp.has_option('scattering-model', 'muRef'):
+            cp.set('scattering-model', 'muRef', '2.117e-5')
+        buf = io.StringIO()
+        cp.write(buf)
+        s = DGFSStd1D(Dictionary(buf.getvalue()), device=dev, writers=False)
+        Q = torch.zeros_like(s.u)
+        for _ in range(3):
+            s.sm.collide(s.u, Q)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            s.sm.collide(s.u, Q)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        nbytes = 4 * s.u.numel() * s.u.element_size()
+        out[model] = {'ms_per_call': ms, 'GB_per_s': nbytes / (ms * 1e-3) / 1e9}
+        del s
+    return out
+
+
 # the reference's published single-GPU times (README.md:104-115): binary VSS
 # Couette, FP64, M=12, N_rho=8, first 52 steps on one Tesla P100
 BI_PUBLISHED = {(72, 3, 20): 47.58, (72, 3, 32): 126.601, (72, 3, 48): 391.943,
@@ -311,6 +351,9 @@ def main(argv=None):
                     help='only time STEPS steps of the N=48 mixture deck (configs/bi_fourier_n48.ini)')
     ap.add_argument('--closure-forward', type=int, default=0, metavar='NODES',
                     help='only time the learned-closure network forward on NODES slices of 32^3')
+    ap.add_argument('--relax-models', type=int, default=0, metavar='REPS',
+                    help='only time REPS collision calls of each relaxation model on the '
+                         'Couette grid')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run 5 device-to-device copies of MIB MiB (the known-size '
                          'kernel that scales the FETCH_SIZE / WRITE_SIZE counters)')
@@ -327,6 +370,9 @@ def main(argv=None):
     if args.bi_only:
         cases = [tuple(int(x) for x in c.split('/')) for c in args.bi_only.split(',')]
         print(json.dumps(bi_couette_published(torch.device('cuda:0'), cases)))
+        return 0
+    if args.relax_models:
+        print(json.dumps(relax_models(torch.device('cuda:0'), args.relax_models, args.config)))
         return 0
     if args.closure_forward:
         print(json.dumps(closure_forward(args.closure_forward, 32, 10, torch.device('cuda:0'))))
