@@ -8,7 +8,9 @@ discretisation must satisfy, independent of its formulas:
     order on u' = lambda u and integrate u' = t^(q-1) exactly at their order
     (stage times);
   * discrete moments of a resolved Maxwellian return its (rho, u, T) with
-    zero heat flux and shear stress.
+    zero heat flux and shear stress; those of an anisotropic drifting
+    Gaussian return its covariance as T and P (with the factor 2), and a
+    Grad-13 perturbation returns its closed-form heat flux.
 """
 import numpy as np
 import pytest
@@ -114,3 +116,64 @@ def test_moments_of_a_maxwellian():
     assert np.allclose(m[..., 1], 0.1, atol=1e-7) and np.allclose(m[..., 2], -0.2, atol=1e-7)
     assert np.allclose(m[..., 3], 1.1, rtol=1e-6)
     assert np.abs(m[..., 4:6]).max() < 1e-7 and np.abs(m[..., 8]).max() < 1e-7
+
+
+def _gaussian(cv, rho, u, Sig):
+    """rho times the normalised Gaussian of mean u and covariance Sig."""
+    d = cv - np.asarray(u, float).reshape(3, 1)
+    Si = np.linalg.inv(Sig)
+    e = np.einsum('iv,ij,jv->v', d, Si, d)
+    return rho * np.exp(-0.5 * e) / ((2 * np.pi) ** 1.5 * np.sqrt(np.linalg.det(Sig)))
+
+
+def _moments_of(f, Nv=32):
+    from tests.test_gpu_solver import _ini
+    d = Deck(_ini({('velocity-mesh', 'Nv'): Nv}))
+    vm, b = d.vm, d.basis
+    u = np.broadcast_to(np.einsum('kq,q->k', b.fwd, np.ones(b.Nq))[:, None, None] * f(vm.cv),
+                        (d.K, d.Ne, vm.vsize)).copy()
+    return OS.moments(u, b.interp, vm)
+
+
+def test_moments_of_an_anisotropic_drifting_gaussian():
+    """Closed forms for f = rho N(u, Sig) (u_z = 0): T = (2/3) tr Sig and
+    P_ij = 2 rho Sig_ij (std/moments.py:58-61 carries the factor 2), q = 0.
+    A dropped factor 2 on P, a P built from absolute instead of peculiar
+    velocities, or a T without the 2/3 all fail."""
+    rho, u = 0.7, (0.25, -0.35, 0.0)
+    Sig = np.array([[0.55, 0.12, 0.0], [0.12, 0.40, 0.0], [0.0, 0.0, 0.45]])
+    m = _moments_of(lambda cv: _gaussian(cv, rho, u, Sig))
+    tol = 3e-8          # the grid's truncation at |c_i| = 5 (N = 32)
+    assert np.abs(m[..., 0] - rho).max() < tol
+    assert np.abs(m[..., 1] - u[0]).max() < tol and np.abs(m[..., 2] - u[1]).max() < tol
+    assert np.abs(m[..., 3] - 2.0 / 3.0 * np.trace(Sig)).max() < tol
+    assert np.abs(m[..., 4:6]).max() < tol
+    assert np.abs(m[..., 6] - 2 * rho * Sig[0, 0]).max() < tol
+    assert np.abs(m[..., 7] - 2 * rho * Sig[1, 1]).max() < tol
+    assert np.abs(m[..., 8] - 2 * rho * Sig[0, 1]).max() < tol
+
+
+def test_moments_heat_flux_of_a_grad13_perturbation():
+    """f = M (1 + a (c.e)(|c|^2/T - 5/2)) around a resting Maxwellian
+    M = rho/(pi T)^{3/2} exp(-|c|^2/T) (variance T/2 per axis): the
+    perturbation carries no mass, momentum, energy or stress, and its heat
+    flux is q = int f |c|^2 c = (5/4) rho a T^2 e (from E[c_x^2|c|^2] = 5 s^4,
+    E[c_x^2|c|^4] = 35 s^6, s^2 = T/2)."""
+    rho, T, a = 0.9, 1.1, 0.08
+    e = np.array([0.6, 0.8, 0.0])
+
+    def f(cv):
+        M = OS.maxwellian(cv, rho, 0.0, 0.0, 0.0, T)
+        ce = e @ cv
+        return M * (1 + a * ce * ((cv * cv).sum(0) / T - 2.5))
+
+    m = _moments_of(f)
+    tol = 3e-8          # the grid's truncation at |c_i| = 5 (N = 32)
+    assert np.abs(m[..., 0] - rho).max() < tol
+    assert np.abs(m[..., 1:3]).max() < tol
+    assert np.abs(m[..., 3] - T).max() < tol
+    q = 1.25 * rho * a * T * T
+    assert np.abs(m[..., 4] - q * e[0]).max() < tol
+    assert np.abs(m[..., 5] - q * e[1]).max() < tol
+    assert np.abs(m[..., 6] - rho * T).max() < tol and np.abs(m[..., 7] - rho * T).max() < tol
+    assert np.abs(m[..., 8]).max() < tol
