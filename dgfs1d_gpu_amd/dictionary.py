@@ -70,7 +70,7 @@ class Dictionary:
     def lookupexpr(self, section, option, subs=None):
         expr = self.lookup(section, option)
         if not re.match(r'[A-Za-z0-9 \t\n\r.,+\-*/%()<>=\{\}\$]+$', expr):
-            raise ValueError('Invalid characters in expression')
+            raise ValueError('expression %r holds characters outside the arithmetic set' % expr)
         if subs:
             expr = re.sub(r'\b({0})\b'.format('|'.join(subs)), lambda m: subs[m.group(1)], expr)
         expr = re.sub(r'\b((\d+\.?\d*)|(\.\d+))([eE][+-]?\d+)?(?!\s*])', _ensure_float, expr)

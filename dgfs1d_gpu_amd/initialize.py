@@ -18,18 +18,18 @@ def overrides_to_dict(pairs):
 
 def initialize(argv=None):
     ap = ArgumentParser()
-    sp = ap.add_subparsers(dest='cmd', help='sub-command help')
-    ap_run = sp.add_parser('run', help='run --help')
-    ap_run.add_argument('inp', type=FileType('r'), help='input file')
+    sp = ap.add_subparsers(dest='cmd', help='what to do with the deck')
+    ap_run = sp.add_parser('run', help='start a run from the deck\'s initial condition')
+    ap_run.add_argument('inp', type=FileType('r'), help='the INI deck')
     ap_run.add_argument('-v', nargs=2, action='append', default=[],
-                        help='substitute variables. Example: -v mesh::Ne 4')
+                        help='override one deck entry, e.g. -v mesh::Ne 4 (repeatable)')
     ap_run.set_defaults(process_run=True)
-    ap_rs = sp.add_parser('restart', help='restart --help')
-    ap_rs.add_argument('inp', type=FileType('r'), help='input file')
+    ap_rs = sp.add_parser('restart', help='continue a run from distribution files')
+    ap_rs.add_argument('inp', type=FileType('r'), help='the INI deck')
     ap_rs.add_argument('dist', nargs='+', action='append', type=FileType('rb'), default=[],
-                       help='sorted list of distributions')
+                       help='distribution files, one per rank, in rank order')
     ap_rs.add_argument('-v', nargs=2, action='append', default=[],
-                       help='substitute variables. Example: -v mesh::Ne 4')
+                       help='override one deck entry, e.g. -v mesh::Ne 4 (repeatable)')
     ap_rs.set_defaults(process_restart=True)
     args = ap.parse_args(argv)
     if hasattr(args, 'process_run') or hasattr(args, 'process_restart'):

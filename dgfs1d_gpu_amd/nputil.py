@@ -22,12 +22,12 @@ _NPEVAL_SYMS = {
 def npeval(expr, local_vars):
     """Evaluate a restricted arithmetic expression (nputil.py:22-36 semantics)."""
     if '^' in expr or '**' in expr:
-        raise ValueError('Direct exponentiation is not supported; use pow')
+        raise ValueError('powers are written pow(a, b); the operators ** and ^ are refused')
     if not re.match(r'[A-Za-z0-9 \t\n\r.,+\-*/%()<>=]+$', expr):
-        raise ValueError('Invalid characters in expression')
+        raise ValueError('expression %r holds characters outside the arithmetic set' % expr)
     names = '|'.join(re.escape(k) for k in it.chain(_NPEVAL_SYMS, local_vars))
     if re.search(r'(%s|\))\s*\.' % names, expr):
-        raise ValueError('Invalid expression')
+        raise ValueError('expression %r reads an attribute, which is refused' % expr)
     return eval(expr, dict(_NPEVAL_SYMS), dict(local_vars))  # noqa: S307 (restricted)
 
 
@@ -49,4 +49,4 @@ def subclass_where(cls, **kwargs):
     for s in subclasses(cls):
         if getattr(s, k, None) == v:
             return s
-    raise KeyError("No subclasses of {0} with cls.{1} == '{2}'".format(cls.__name__, k, v))
+    raise KeyError('no plugin derived from %s has %s = %r' % (cls.__name__, k, v))
