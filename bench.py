@@ -14,8 +14,8 @@ state with an FP64 run of the same steps (after the timed region).
 ``roofline`` is for the dominant kernel (the gain
 kernel of the collision operator), timed with HIP events recorded around it
 on its launch stream over the timed region.  ``cpu_baseline`` is the FP64
-NumPy/SciPy oracle on the host cores, timed on a bounded sample (2 collision
-integrals + 1 transport evaluation of the same case) and scaled to one step.
+NumPy/SciPy oracle on the host cores (the job's CPU share), timed over 3 full
+SSP-RK2 steps of the same deck after one warm-up step (median step time).
 """
 import argparse
 import json
@@ -44,30 +44,64 @@ def fft_flops(n):
     return 5.0 * n * math.log2(n)
 
 
-def cpu_baseline(deck_text, cores, nq=2):
+def host_cpu():
+    """CPU model, logical CPUs of the host (nproc) and the CPUs this process may
+    run on (its affinity mask: the job's share of the host)."""
+    model = None
+    try:
+        with open('/proc/cpuinfo') as fh:
+            for line in fh:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, share
+
+
+def cpu_baseline(deck_text, steps=3):
+    """The FP64 oracle (the reference's algorithm restated in NumPy/SciPy, FFTs
+    through scipy.fft with one worker per CPU of this job) advancing the same
+    deck by full time steps: one untimed step, then ``steps`` timed steps;
+    the median step time gives the rate.  Threads: the CPUs in this process'
+    affinity mask, capped by OMP_NUM_THREADS when that is set (the job's
+    share of a shared host)."""
     import numpy as np
     from oracle import collision as OC
     from oracle.deck import Deck
+    model, nproc, share = host_cpu()
+    cores = share
+    omp = os.environ.get('OMP_NUM_THREADS')
+    if omp and omp.isdigit() and int(omp) > 0:
+        cores = min(cores, int(omp))
     OC.set_fft_workers(cores)
-    d = Deck(deck_text)
-    u = d.u0
-    t0 = time.perf_counter()
-    d.op.transport(0.0, u)
-    t_tr = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    for i in range(nq):
-        f = u[i % d.K, i % d.Ne]
-        OC.q_vhs(d.pre, f, f, d.prefac)
-    t_q = (time.perf_counter() - t0) / nq
-    OC.set_fft_workers(None)
-    ncoll = d.K * d.Ne                                   # nodal basis: one Q per node
-    step_s = 2 * (ncoll * t_q + t_tr)                    # SSP-RK2: 2 RHS per step
+    try:
+        d = Deck(deck_text)
+        u = d.u0
+        t = d.t0
+        u = d.step(t, u)                              # untimed warm-up step
+        t += d.dt
+        times = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            u = d.step(t, u)
+            times.append(time.perf_counter() - t0)
+            t += d.dt
+    finally:
+        OC.set_fft_workers(None)
+    step_s = float(np.median(times))
     pts = d.Ne * d.K * d.vm.vsize
     return dict(value=pts / step_s, unit='phase-space point-steps/s', cores=cores,
-                kind='port',
-                sample='%d FP64 collision integrals + 1 transport of the same deck '
-                       '(%.2f s/collision), scaled to one SSP-RK2 step (%d collisions)'
-                       % (nq, t_q, 2 * ncoll))
+                kind='port', cpu_model=model, nproc=nproc, affinity_cpus=share,
+                step_s=times,
+                sample='%d full %s steps of the same deck by the FP64 oracle after one '
+                       'warm-up step (%d collision integrals per step, scipy.fft with %d '
+                       'workers), median %.2f s per step'
+                       % (steps, d.scheme, 2 * d.K * d.Ne, cores, step_s))
 
 
 MOMENT_NAMES = ('rho', 'ux', 'uy', 'T', 'qx', 'qy', 'Pxx', 'Pyy', 'Pxy')
@@ -464,8 +498,7 @@ def main(argv=None):
     if not args.no_bi:
         out['bi_couette_published'] = bi_couette_published(dev, ((72, 3, 20), (72, 3, 32)))
     if not args.no_cpu_baseline:
-        cores = min(16, os.cpu_count() or 1)
-        out['cpu_baseline'] = cpu_baseline(text, cores)
+        out['cpu_baseline'] = cpu_baseline(text)
         out['gpu_over_cpu'] = value / out['cpu_baseline']['value']
     print(json.dumps(out))
     return 0
