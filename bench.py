@@ -448,7 +448,9 @@ def main(argv=None):
     n = N ** 3
     B = s.vm.M() * s.vm.Nrho()
     ntask = s.ntasks
-    fg_same = plan.tasks['fg_same']
+    ref = getattr(s.sm, '_ref', None)
+    # the reference-state form runs the bilinear gain on (f - f0, f + f0)
+    fg_same = plan.tasks['fg_same'] and ref is None
     # k_gain: one packed inverse 3D transform per (task, p, q) and function
     # (f, and g when f != g) plus the survey's pointwise count per (p, q, k)
     nvar, pw = (1, 14) if fg_same else (2, 26)
@@ -461,8 +463,9 @@ def main(argv=None):
     q_survey = (4 + 5 * B) * fft_flops(n) + 26 * B * n   # per Q, reference-structured count
     value = Ne * K * n * args.steps / wall
     q = plan.query()
-    workload = 'couette Ne=%d K=%d N=%d^3 M=%d Nrho=%d %s dt=%g' % (
-        Ne, K, N, s.vm.M(), s.vm.Nrho(), cfg.lookup('time-integrator', 'scheme'), s.dt)
+    workload = 'couette Ne=%d K=%d N=%d^3 M=%d Nrho=%d %s dt=%g%s' % (
+        Ne, K, N, s.vm.M(), s.vm.Nrho(), cfg.lookup('time-integrator', 'scheme'), s.dt,
+        '' if ref is None else ' reference-state R=%d' % ref['R'])
     tr = GAIN_TRAFFIC_KIB.get(workload)
     traffic = None if tr is None else (tr[0] * tr[1] + tr[2] * tr[3]) * 1024.0
     out = {
@@ -477,7 +480,9 @@ def main(argv=None):
         'config': {'workload': workload,
             'Ne': Ne, 'K': K, 'N': N, 'M': s.vm.M(), 'Nrho': s.vm.Nrho(),
             'collisions_per_rhs': ntask, 'split_S': q['split'], 'qgroups': q['qgroups'],
-            'pgroups': q['pgroups']},
+            'pgroups': q['pgroups'],
+            'collision_form': 'direct' if ref is None else
+            'reference-state, FP64 Q(f0,f0) refreshed every %d calls' % ref['R']},
         'roofline': {'bound': 'fp32-valu', 'kernel': 'k_gain', 'achieved': achieved,
                      'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / PEAK_FP32_TFLOPS, 'traffic': traffic,

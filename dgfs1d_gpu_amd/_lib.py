@@ -85,6 +85,8 @@ SIGNATURES = {
     'dgfs_collision_plan_set_qgroups': (_i, [_vp, _i]),
     'dgfs_collide': (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _i,
                           _vp]),
+    'dgfs_collide_reference': (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _vp,
+                                    _vp, _vp]),
     'dgfs_collision_timing': (_i, [_vp, _i]),
     'dgfs_bi_collision_plan_create': (_i, [_i, _i, _i, _d, _dp, _dp, _dp, _d, _i, _dp, _d, _d,
                                            _d, _d, _d, _i, _i, C.POINTER(_vp)]),
@@ -132,9 +134,9 @@ SIGNATURES = {
     'dgfs_bgk_work_bytes': (_sz, [_i, _i]),
     'dgfs_esbgk_conservative_work_bytes': (_sz, [_i, _i]),
     'dgfs_esbgk_conservative': (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp,
-                                     _vp, _vp, _d, _d, _d, _d, _vp, _vp, _vp]),
+                                     _vp, _vp, _d, _d, _d, _d, _vp, _vp, _vp, _vp]),
     'dgfs_bgk_conservative': (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp,
-                                   _vp, _vp, _d, _d, _d, _vp, _vp, _vp]),
+                                   _vp, _vp, _d, _d, _d, _vp, _vp, _vp, _vp]),
     'dgfs_esbgk': (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                         _d, _d, _d, _d, _vp, _vp, _vp]),
 }

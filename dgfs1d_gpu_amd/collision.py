@@ -124,6 +124,29 @@ class CollisionPlan:
             _lib.ptr(t['gstart']), _lib.ptr(t['gslice']), fg_same,
             _lib.stream_ptr(stream)))
 
+    def reference(self, u, u0, Q0, Q, stream=None, tasks=None):
+        """Q += Q(f, f) in the reference-state form (dgfs_collide_reference):
+        Q0 (FP64, (K, Ne, Nv)) must hold Q(f0, f0) of the reference state u0;
+        the FP32 rounding error then scales with |f - f0|.  f = g tasks only."""
+        import torch
+        t = self.tasks if tasks is None else tasks
+        if t is None:
+            raise RuntimeError('set_tasks() must be called first')
+        if not t['fg_same']:
+            raise ValueError('the reference-state form evaluates Q(f, f): every task needs f = g')
+        for x in (u, u0, Q):
+            if x.dtype != self.tdtype:
+                raise TypeError('collision operands must be %s' % self.tdtype)
+        if Q0.dtype != torch.float64:
+            raise TypeError('Q0 must be float64')
+        n = t['K'] * t['Ne'] * self.n
+        if min(u.numel(), u0.numel(), Q0.numel(), Q.numel()) < n:
+            raise ValueError('collision operands smaller than K*Ne*Nv')
+        _lib.check(_lib.lib().dgfs_collide_reference(
+            self._h, _lib.ptr(u), _lib.ptr(u0), _lib.ptr(Q0), _lib.ptr(Q), t['K'], t['Ne'],
+            t['ntask'], _lib.ptr(t['nterm']), _lib.ptr(t['slice']), _lib.ptr(t['coef']),
+            t['ngroup'], _lib.ptr(t['gstart']), _lib.ptr(t['gslice']), _lib.stream_ptr(stream)))
+
     def timing(self, max_calls):
         """Record per-stage HIP events for the next max_calls calls (0 = off)."""
         _lib.check(_lib.lib().dgfs_collision_timing(self._h, int(max_calls)))

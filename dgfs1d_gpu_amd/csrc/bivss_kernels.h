@@ -294,8 +294,8 @@ int bi_launch(BiPlan* pl, const void* up, const void* uq, void* Q, TaskView tv, 
         tc.coef = static_cast<const T*>(tv.coef) + size_t(c0) * 2 * tv.K;
         const int XG = bi_xgroups<N>(nt, NXY);
         const long npl = long(nt) * 2 * N;
-        hipLaunchKernelGGL((k_fwd_plane<T, N>), dim3((npl + PL - 1) / PL), dim3(PL * N), 0, st,
-                           (const T*)up, (const T*)uq, tc, 2, npl, (cplx<T>*)pl->Iw);
+        hipLaunchKernelGGL((k_fwd_plane<T, N, false>), dim3((npl + PL - 1) / PL), dim3(PL * N),
+                           0, st, (const T*)up, (const T*)uq, tc, 2, npl, (cplx<T>*)pl->Iw);
         DGFS_HIP_CHECK(hipGetLastError());
         const long nli = long(nt) * 2 * NN;
         hipLaunchKernelGGL((k_fwd_line<T, N>), dim3((nli + 255) / 256), dim3(256), 0, st, nli, 2,
@@ -331,9 +331,9 @@ int bi_launch(BiPlan* pl, const void* up, const void* uq, void* Q, TaskView tv, 
         DGFS_HIP_CHECK(hipGetLastError());
     }
     const long nlo = long(ngroup) * NN;
-    hipLaunchKernelGGL((k_line_output<T, N>), dim3((nlo + 255) / 256), dim3(256), 0, st,
-                       (const T*)up, tv, (const cplx<T>*)pl->H, nlo, gstart, gslice,
-                       T(pl->prefac), (T*)Q);
+    hipLaunchKernelGGL((k_line_output<T, N, false>), dim3((nlo + 255) / 256), dim3(256), 0, st,
+                       (const T*)up, (const T*)nullptr, (const double*)nullptr, tv,
+                       (const cplx<T>*)pl->H, nlo, gstart, gslice, T(pl->prefac), (T*)Q);
     DGFS_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -25,18 +25,16 @@ const char* last_error() { return g_err.c_str(); }
 template <typename T>
 static int dispatch(CollisionPlan* pl, const void* u, const void* ug, void* Q, TaskView tv,
                     int ntask, int ngroup, const int* gs, const int* gsl, int fgs,
-                    hipStream_t st) {
+                    hipStream_t st, const double* Q0) {
+#define DGFS_CASE(NN)                                                                     \
+    case NN:                                                                              \
+        return collide_launch<T, NN>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st, Q0);
     switch (pl->N) {
-        case 4: return collide_launch<T, 4>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
-        case 6: return collide_launch<T, 6>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
-        case 8: return collide_launch<T, 8>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
-        case 12: return collide_launch<T, 12>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
-        case 16: return collide_launch<T, 16>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
-        case 20: return collide_launch<T, 20>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
-        case 24: return collide_launch<T, 24>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
-        case 32: return collide_launch<T, 32>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+        DGFS_CASE(4) DGFS_CASE(6) DGFS_CASE(8) DGFS_CASE(12) DGFS_CASE(16) DGFS_CASE(20)
+        DGFS_CASE(24) DGFS_CASE(32)
         default: break;
     }
+#undef DGFS_CASE
     set_error("collision: unsupported velocity grid N=%d", pl->N);
     return 2;
 }
@@ -141,7 +139,7 @@ extern "C" int dgfs_collision_plan_create(int N, int M, int Nrho, double L, cons
     hipError_t e2 = hipMalloc(&pl->acc, size_t(ntask_max) * Nrho * pl->QG * n * cs);
     pl->QG_cap = pl->QG;
     hipError_t e3 = hipMalloc(&pl->Iw, size_t(ntask_max) * 2 * n * cs);
-    hipError_t e4 = hipMalloc(&pl->H, size_t(ntask_max) * 2 * n * cs);
+    hipError_t e4 = hipMalloc(&pl->H, size_t(ntask_max) * 3 * n * cs);   // 3 planes: reference-state form
     hipError_t e5 = hipMalloc(&pl->nyq, size_t(ntask_max) * 2 * Nrho * M * 6 * N * N * es);
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess ||
         e5 != hipSuccess) {
@@ -212,7 +210,7 @@ extern "C" int dgfs_collision_plan_query(dgfs_collision_plan_t plan, int* split,
     if (qgroups) *qgroups = pl->QG;
     if (pgroups) *pgroups = pl->SD;
     if (scratch_bytes)
-        *scratch_bytes = size_t(pl->ntask_max) * pl->n * cs * (6 + pl->Nrho * pl->QG);
+        *scratch_bytes = size_t(pl->ntask_max) * pl->n * cs * (7 + pl->Nrho * pl->QG);
     return 0;
 }
 
@@ -252,7 +250,32 @@ extern "C" int dgfs_collide(dgfs_collision_plan_t plan, const void* u, const voi
     dgfs::TaskView tv{K, Ne, int(pl->n), task_nterm, task_slice, task_coef};
     if (pl->dtype == DGFS_F64)
         return dgfs::dispatch<double>(pl, u, ug, Q, tv, ntask, ngroup, group_start,
-                                      group_slice, fg_same, stream);
+                                      group_slice, fg_same, stream, nullptr);
     return dgfs::dispatch<float>(pl, u, ug, Q, tv, ntask, ngroup, group_start, group_slice,
-                                 fg_same, stream);
+                                 fg_same, stream, nullptr);
+}
+
+extern "C" int dgfs_collide_reference(dgfs_collision_plan_t plan, const void* u, const void* u0,
+                                      const double* Q0, void* Q, int K, int Ne, int ntask,
+                                      const int* task_nterm, const int* task_slice,
+                                      const void* task_coef, int ngroup,
+                                      const int* group_start, const int* group_slice,
+                                      hipStream_t stream) {
+    auto* pl = static_cast<CollisionPlan*>(plan);
+    DGFS_REQUIRE(pl, "collide_reference: null plan");
+    DGFS_REQUIRE(ntask >= 0 && ntask <= pl->ntask_max,
+                 "collide_reference: ntask=%d exceeds plan max %d", ntask, pl->ntask_max);
+    DGFS_REQUIRE(ngroup >= 0 && ngroup <= ntask, "collide_reference: bad ngroup=%d", ngroup);
+    DGFS_REQUIRE(K > 0 && Ne > 0, "collide_reference: bad K/Ne");
+    if (ntask == 0) return 0;
+    DGFS_REQUIRE(u && u0 && Q0 && Q && task_nterm && task_slice && task_coef && group_start &&
+                     group_slice,
+                 "collide_reference: null pointer argument");
+    DGFS_REQUIRE(u0 != u, "collide_reference: the reference state must be its own array");
+    dgfs::TaskView tv{K, Ne, int(pl->n), task_nterm, task_slice, task_coef};
+    if (pl->dtype == DGFS_F64)
+        return dgfs::dispatch<double>(pl, u, u0, Q, tv, ntask, ngroup, group_start,
+                                      group_slice, 0, stream, Q0);
+    return dgfs::dispatch<float>(pl, u, u0, Q, tv, ntask, ngroup, group_start, group_slice, 0,
+                                 stream, Q0);
 }

@@ -87,7 +87,9 @@ DGFS_DEV void lds_fft_col(cplx<T>* P, int i) {
 // ================================================================ kernel A
 // A1: plane x of task t's f (or g): 2D forward transform over (y, z);
 //     Iw[t][which][x][l3][l2]
-template <typename T, int N>
+//     DEV (reference-state form): ug is the reference state u0; which = 0
+//     loads h = f - f0, which = 1 loads s = f + f0 (both with the f terms)
+template <typename T, int N, bool DEV>
 __global__ void __launch_bounds__(planes_per_wg(N) * N)
 k_fwd_plane(const T* __restrict__ u, const T* __restrict__ ug, TaskView tv, int nfg,
             long nplanes, cplx<T>* __restrict__ Iw) {
@@ -102,7 +104,13 @@ k_fwd_plane(const T* __restrict__ u, const T* __restrict__ ug, TaskView tv, int 
             const int x = int(pid % N);
             const long tw = pid / N;
             const int which = int(tw % nfg), t = int(tw / nfg);
-            v.x = task_value<T>(tv, which ? ug : u, t, which, x * NN + yz);
+            if constexpr (DEV) {
+                const T f = task_value<T>(tv, u, t, 0, x * NN + yz);
+                const T f0 = task_value<T>(tv, ug, t, 0, x * NN + yz);
+                v.x = which ? f + f0 : f - f0;
+            } else {
+                v.x = task_value<T>(tv, which ? ug : u, t, which, x * NN + yz);
+            }
         }
         L[pl * N * (N + 1) + (yz / N) * (N + 1) + yz % N] = v;
     }
@@ -375,13 +383,16 @@ k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
 // p = slot, slot+PL, ...:  G^(k1,:,:) = sum_p wtab_p(k1,:,:) FFT2_{y,z}(sum_qg acc[t][p][qg][k1]),
 // reduced over the slots through LDS; then Hg = IFFT2(G^(k1)) (slot 0) and
 // Hl = IFFT2(bb2 G^_g(k1)) (slot 1).  H[t][0 = loss | 1 = gain][k1][y][z].
-template <typename T, int N>
+// DEV (reference-state form, F^ = {h^, s^}): slot 1 inverts bb2 f^ with
+// f^ = (h^ + s^)/2 and slot 2 inverts bb2 h^:  H[t][0 = L[f] | 1 = gain | 2 = L[h]].
+template <typename T, int N, bool DEV>
 __global__ void __launch_bounds__(planes_per_wg(N) * N)
 k_plane_weight(const cplx<T>* __restrict__ acc_in, const T* __restrict__ wtab,
                const cplx<T>* __restrict__ Fhat, const T* __restrict__ bb2, int fg_same,
                int Nrho, int QG, int SB, cplx<T>* __restrict__ H) {
     constexpr int PL = planes_per_wg(N), NT = PL * N, NN = N * N, n = N * NN, PS = N * (N + 1);
-    static_assert(PL >= 2, "plane kernel needs two slots");
+    constexpr int NH = DEV ? 3 : 2;
+    static_assert(PL >= NH, "plane kernel needs a slot per output plane");
     __shared__ cplx<T> L[PL * PS];
     const int k1 = blockIdx.x % N, t = blockIdx.x / N;
     const int slot = threadIdx.x / N, c = threadIdx.x - slot * N;
@@ -427,20 +438,35 @@ k_plane_weight(const cplx<T>* __restrict__ acc_in, const T* __restrict__ wtab,
     } else if (slot == 1) {
         // loss plane: bb2 * G^_g at l1 = k1, column c = l3
         const cplx<T>* Fg = Fhat + (size_t(t) * 2 + (fg_same ? 0 : 1)) * n;
+        if constexpr (DEV) {
+            const cplx<T>* Fh = Fhat + size_t(t) * 2 * n;
+            sfor<0, N>([&](auto L2) {
+                const size_t o = (size_t(c) * N + k1) * N + L2;
+                P[L2 * (N + 1) + c] = (T(0.5) * bb2[size_t(k1) * NN + L2 * N + c]) *
+                                      (Fh[o] + Fh[o + n]);
+            });
+        } else {
+            sfor<0, N>([&](auto L2) {
+                P[L2 * (N + 1) + c] = bb2[size_t(k1) * NN + L2 * N + c] *
+                                      Fg[(size_t(c) * N + k1) * N + L2];
+            });
+        }
+    } else if (DEV && slot == 2) {
+        const cplx<T>* Fh = Fhat + size_t(t) * 2 * n;
         sfor<0, N>([&](auto L2) {
             P[L2 * (N + 1) + c] = bb2[size_t(k1) * NN + L2 * N + c] *
-                                  Fg[(size_t(c) * N + k1) * N + L2];
+                                  Fh[(size_t(c) * N + k1) * N + L2];
         });
     }
     __syncthreads();
-    if (slot < 2) lds_fft_col<T, N, +1>(P, c);      // k2 -> y
+    if (slot < NH) lds_fft_col<T, N, +1>(P, c);      // k2 -> y
     __syncthreads();
-    if (slot < 2) lds_fft_row<T, N, +1>(P, c);      // k3 -> z
+    if (slot < NH) lds_fft_row<T, N, +1>(P, c);      // k3 -> z
     __syncthreads();
-    for (int i = threadIdx.x; i < 2 * NN; i += NT) {
-        const int which = i / NN, r = i - which * NN;     // slot 0 = gain, slot 1 = loss
-        H[((size_t(t) * 2 + (which == 0 ? 1 : 0)) * N + k1) * NN + r] =
-            L[which * PS + (r / N) * (N + 1) + r % N];
+    for (int i = threadIdx.x; i < NH * NN; i += NT) {
+        const int which = i / NN, r = i - which * NN;     // slot 0 = gain, 1 = loss, 2 = L[h]
+        const int dst = which == 0 ? 1 : (which == 1 ? 0 : 2);
+        H[((size_t(t) * NH + dst) * N + k1) * NN + r] = L[which * PS + (r / N) * (N + 1) + r % N];
     }
 }
 
@@ -448,12 +474,16 @@ k_plane_weight(const cplx<T>* __restrict__ acc_in, const T* __restrict__ wtab,
 // Line (g, y, z): gain(x) = Re IFFT_k1(sum_{t in g} Hg[t](:,y,z)),
 // loss(x) = sum_{t in g} f_t(x,y,z) Re IFFT_k1(Hl[t](:,y,z));
 // Q[gslice[g]](x,y,z) += prefac (gain - loss).
-template <typename T, int N>
+// DEV (reference-state form, u0 the reference state, Q0 = Q(f0, f0) per
+// output slice in FP64): loss = sum_t (h_t L[f_t] + f0_t L[h_t]) and
+// Q[gslice[g]] += Q0[gslice[g]] + prefac (gain - loss), gain = G(h, s).
+template <typename T, int N, bool DEV>
 __global__ void __launch_bounds__(256)
-k_line_output(const T* __restrict__ u, TaskView tv, const cplx<T>* __restrict__ H, long nlines,
+k_line_output(const T* __restrict__ u, const T* __restrict__ u0, const double* __restrict__ Q0,
+              TaskView tv, const cplx<T>* __restrict__ H, long nlines,
               const int* __restrict__ gstart, const int* __restrict__ gslice, T prefac,
               T* __restrict__ Q) {
-    constexpr int NN = N * N, n = N * NN;
+    constexpr int NN = N * N, n = N * NN, NH = DEV ? 3 : 2;
     const long gid = long(blockIdx.x) * 256 + threadIdx.x;
     if (gid >= nlines) return;
     const int g = int(gid / NN), yz = int(gid - long(g) * NN);
@@ -462,17 +492,39 @@ k_line_output(const T* __restrict__ u, TaskView tv, const cplx<T>* __restrict__ 
     T loss[N];
     sfor<0, N>([&](auto I) { loss[I] = T(0); a[I] = mk(T(0), T(0)); });
     for (int t = ta; t < tb; ++t) {
-        const cplx<T>* hl = H + (size_t(t) * 2) * n + yz;
+        const cplx<T>* hl = H + (size_t(t) * NH) * n + yz;
         cplx<T> b[N];
         sfor<0, N>([&](auto K1) { b[K1] = hl[size_t(K1) * NN]; });
         RegDFT<T, N, +1>::template run<1, 0>(b, o);
-        sfor<0, N>([&](auto X) { loss[X] += task_value<T>(tv, u, t, 0, X * NN + yz) * o[X].x; });
+        if constexpr (DEV) {
+            T lh[N];
+            sfor<0, N>([&](auto K1) { b[K1] = hl[2 * size_t(n) + size_t(K1) * NN]; });
+            {
+                cplx<T> o2[N];
+                RegDFT<T, N, +1>::template run<1, 0>(b, o2);
+                sfor<0, N>([&](auto X) { lh[X] = o2[X].x; });
+            }
+            sfor<0, N>([&](auto X) {
+                const T f = task_value<T>(tv, u, t, 0, X * NN + yz);
+                const T f0 = task_value<T>(tv, u0, t, 0, X * NN + yz);
+                loss[X] += (f - f0) * o[X].x + f0 * lh[X];
+            });
+        } else {
+            sfor<0, N>([&](auto X) { loss[X] += task_value<T>(tv, u, t, 0, X * NN + yz) * o[X].x; });
+        }
         const cplx<T>* hg = hl + n;
         sfor<0, N>([&](auto K1) { a[K1] = a[K1] + hg[size_t(K1) * NN]; });
     }
     RegDFT<T, N, +1>::template run<1, 0>(a, o);
     T* dst = Q + size_t(gslice[g]) * n + yz;
-    sfor<0, N>([&](auto X) { dst[size_t(X) * NN] += prefac * (o[X].x - loss[X]); });
+    if constexpr (DEV) {
+        const double* q0 = Q0 + size_t(gslice[g]) * n + yz;
+        sfor<0, N>([&](auto X) {
+            dst[size_t(X) * NN] += T(q0[size_t(X) * NN] + double(prefac * (o[X].x - loss[X])));
+        });
+    } else {
+        sfor<0, N>([&](auto X) { dst[size_t(X) * NN] += prefac * (o[X].x - loss[X]); });
+    }
 }
 
 // ================================================================ the plan
@@ -512,15 +564,25 @@ struct Impl {
     static constexpr int S2 = pick_split_gain<T, N, 2>();   // gain residue split, bilinear
     static constexpr int PL = planes_per_wg(N);
 
+    // Q0 == nullptr: Q += Q(f, g).  Q0 != nullptr (reference-state form, ug the
+    // reference state u0): Q += Q0 + G(h, s) - h L[f] - f0 L[h], h = f - f0,
+    // s = f + f0, which equals Q(f, f) when Q0 = Q(f0, f0).
     static int launch(CollisionPlan* pl, const void* u, const void* ug, void* Q, TaskView tv,
                       int ntask, int ngroup, const int* gstart, const int* gslice, int fg_same,
-                      hipStream_t st) {
+                      hipStream_t st, const double* Q0) {
+        const bool dev = Q0 != nullptr;
+        if (dev) fg_same = 0;
         const int nfg = fg_same ? 1 : 2;
         constexpr long NN = long(N) * N;
         timing_mark(pl, 0, st);
         const long npl = long(ntask) * nfg * N;
-        hipLaunchKernelGGL((k_fwd_plane<T, N>), dim3((npl + PL - 1) / PL), dim3(PL * N), 0, st,
-                           (const T*)u, (const T*)ug, tv, nfg, npl, (cplx<T>*)pl->Iw);
+        if (dev)
+            hipLaunchKernelGGL((k_fwd_plane<T, N, true>), dim3((npl + PL - 1) / PL), dim3(PL * N),
+                               0, st, (const T*)u, (const T*)ug, tv, nfg, npl, (cplx<T>*)pl->Iw);
+        else
+            hipLaunchKernelGGL((k_fwd_plane<T, N, false>), dim3((npl + PL - 1) / PL),
+                               dim3(PL * N), 0, st, (const T*)u, (const T*)ug, tv, nfg, npl,
+                               (cplx<T>*)pl->Iw);
         DGFS_HIP_CHECK(hipGetLastError());
         const long nli = long(ntask) * nfg * NN;
         hipLaunchKernelGGL((k_fwd_line<T, N>), dim3((nli + 255) / 256), dim3(256), 0, st, nli,
@@ -544,16 +606,27 @@ struct Impl {
                                (const T*)pl->nyq, pl->M, pl->Nrho, pl->QG, (cplx<T>*)pl->acc);
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 3, st);
-        hipLaunchKernelGGL((k_plane_weight<T, N>), dim3(ntask * N), dim3(PL * N), 0, st,
-                           (const cplx<T>*)pl->acc, (const T*)pl->wtab,
-                           (const cplx<T>*)pl->Fhat, (const T*)pl->bb2, fg_same, pl->Nrho,
-                           pl->QG, S, (cplx<T>*)pl->H);
+        if (dev)
+            hipLaunchKernelGGL((k_plane_weight<T, N, true>), dim3(ntask * N), dim3(PL * N), 0, st,
+                               (const cplx<T>*)pl->acc, (const T*)pl->wtab,
+                               (const cplx<T>*)pl->Fhat, (const T*)pl->bb2, fg_same, pl->Nrho,
+                               pl->QG, S, (cplx<T>*)pl->H);
+        else
+            hipLaunchKernelGGL((k_plane_weight<T, N, false>), dim3(ntask * N), dim3(PL * N), 0,
+                               st, (const cplx<T>*)pl->acc, (const T*)pl->wtab,
+                               (const cplx<T>*)pl->Fhat, (const T*)pl->bb2, fg_same, pl->Nrho,
+                               pl->QG, S, (cplx<T>*)pl->H);
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 4, st);
         const long nlo = long(ngroup) * NN;
-        hipLaunchKernelGGL((k_line_output<T, N>), dim3((nlo + 255) / 256), dim3(256), 0, st,
-                           (const T*)u, tv, (const cplx<T>*)pl->H, nlo, gstart, gslice,
-                           T(pl->prefac), (T*)Q);
+        if (dev)
+            hipLaunchKernelGGL((k_line_output<T, N, true>), dim3((nlo + 255) / 256), dim3(256), 0,
+                               st, (const T*)u, (const T*)ug, Q0, tv, (const cplx<T>*)pl->H, nlo,
+                               gstart, gslice, T(pl->prefac), (T*)Q);
+        else
+            hipLaunchKernelGGL((k_line_output<T, N, false>), dim3((nlo + 255) / 256), dim3(256),
+                               0, st, (const T*)u, (const T*)ug, Q0, tv, (const cplx<T>*)pl->H,
+                               nlo, gstart, gslice, T(pl->prefac), (T*)Q);
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 5, st);
         if (pl->ev && pl->ev_used < pl->ev_max) ++pl->ev_used;
@@ -563,18 +636,19 @@ struct Impl {
 
 template <typename T, int N>
 int collide_launch(CollisionPlan* pl, const void* u, const void* ug, void* Q, TaskView tv,
-                   int ntask, int ngroup, const int* gs, const int* gsl, int fgs, hipStream_t st) {
-    return Impl<T, N>::launch(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st);
+                   int ntask, int ngroup, const int* gs, const int* gsl, int fgs, hipStream_t st,
+                   const double* Q0) {
+    return Impl<T, N>::launch(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st, Q0);
 }
 
 #define DGFS_COLLIDE_EXTERN(T, N)                                                        \
     extern template int collide_launch<T, N>(CollisionPlan*, const void*, const void*, void*, \
                                              TaskView, int, int, const int*, const int*, int, \
-                                             hipStream_t);
+                                             hipStream_t, const double*);
 #define DGFS_COLLIDE_INST(T, N)                                                          \
     template int collide_launch<T, N>(CollisionPlan*, const void*, const void*, void*,     \
                                       TaskView, int, int, const int*, const int*, int,     \
-                                      hipStream_t);
+                                      hipStream_t, const double*);
 #define DGFS_COLLIDE_SIZES(X, T) X(T, 4) X(T, 6) X(T, 8) X(T, 12) X(T, 16) X(T, 20) X(T, 24) X(T, 32)
 
 DGFS_COLLIDE_SIZES(DGFS_COLLIDE_EXTERN, float)

@@ -17,6 +17,8 @@
 // order and turns them into the slice's constants (Lagrange multipliers,
 // Shakov state), and the apply kernel reads the constants (no atomics, no
 // host synchronisation).
+#include <algorithm>
+
 #include "dgfs_common.h"
 #include "dgfs_hip.h"
 
@@ -232,13 +234,20 @@ DGFS_DEV ShkState shk_state(const double (&m)[kShkMom], double cw) {
 // (1-Pr)/5 * 4/(rho T^2), qx, qy, qz, prefac rho T^(1-omega)}  (mako:50-141)
 constexpr int kShkCst = 16;
 
+// With cnt non-null it also zeroes the task's arrival counters cnt[i][t],
+// i < ncnt, and its status word (read by the Newton solves that follow on the stream).
 __global__ void __launch_bounds__(kRB)
-k_shk_final(int nb, ShkParams P, const double* __restrict__ part, double* __restrict__ cst) {
+k_shk_final(int nb, ShkParams P, const double* __restrict__ part, double* __restrict__ cst,
+            unsigned* __restrict__ cnt = nullptr, int ncnt = 0, int* __restrict__ status = nullptr) {
     __shared__ double sh[kShkMom * 4];
     const int t = blockIdx.x;
     double m[kShkMom];
     reduce_partials<kShkMom>(part + size_t(t) * nb * kShkMom, nb, m, sh);
     if (threadIdx.x != 0) return;
+    if (cnt) {
+        for (int i = 0; i < ncnt; ++i) cnt[size_t(i) * gridDim.x + t] = 0u;
+        status[t] = 0;
+    }
     const ShkState st = shk_state(m, P.cw);
     double* c = cst + size_t(t) * kShkCst;
     c[0] = st.ux;
@@ -384,24 +393,92 @@ k_es_apply(int K, int n, const int* __restrict__ nterm, const int* __restrict__ 
     Q[o] = T(double(Q[o]) + acc);
 }
 
+// ---------------------------------------------- multi-work-group Newton solves
+// The Newton solves of 'bgk-gll' and 'esbgk-gll' split each task's velocity
+// sums over nch work-groups: grid (nch, ntask), launched cooperatively, so
+// every work-group of the grid is resident at once.  Pass k writes one
+// partial vector per work-group into npart[t][k & 1][ch][NV]; the task's
+// work-groups then meet at the task's arrival counter (each adds 1 per pass;
+// the counter is zeroed by k_shk_final before the launch and never reset
+// inside it), and every one of them adds the task's partials in the same
+// order and solves the same small system, so all hold identical iterates and
+// leave the loop together.  The partials are double-buffered: buffer k & 1 is
+// written again only at pass k + 2, which a work-group reaches after the
+// meeting of pass k + 1, and every work-group arrives there only after it has
+// read pass k's partials.  The wait is bounded: a meeting that does not fill
+// within kMeetSpins polls (possible only if the grid were not resident) sets
+// kStMeetTimeout and ends the solve, so every wave exits.
+constexpr int kNchMax = 32;
+constexpr unsigned kMeetSpins = 1u << 22;
+
+// per-task status bits written by the Newton kernels
+constexpr int kStNonFinite = 1;      // a residual or iterate became NaN or infinite
+constexpr int kStNotConverged = 2;   // the iteration limit was reached above tolerance
+constexpr int kStMeetTimeout = 4;    // a work-group meeting gave up waiting
+
+DGFS_DEV bool task_meet(unsigned* cnt, unsigned target) {
+    __shared__ int ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();                  // this work-group's partial is visible
+        atomicAdd(cnt, 1u);
+        unsigned seen = __hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        for (unsigned s = 0; seen < target && s < kMeetSpins; ++s) {
+            __builtin_amdgcn_s_sleep(1);
+            seen = __hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        ok = seen >= target;
+    }
+    __syncthreads();
+    __threadfence();                      // the other work-groups' partials are visible
+    return ok != 0;
+}
+
+// Sum v over the task's work-groups (pass `pass`); every thread receives the
+// task totals.  base: the task's npart block [2][nch][NV]; tot: NV doubles of LDS.
+template <int NV>
+DGFS_DEV bool task_sum(double (&v)[NV], double* sh, double* tot, double* base, int nch,
+                       int pass, unsigned* cnt) {
+    block_sum_all<NV>(v, sh);
+    double* buf = base + size_t(pass & 1) * nch * NV;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NV; ++i) buf[size_t(blockIdx.x) * NV + i] = v[i];
+    const bool ok = task_meet(cnt, unsigned(pass + 1) * unsigned(nch));
+    if (threadIdx.x < NV) {
+        double s = 0.0;
+        for (int b = 0; b < nch; ++b) s += buf[size_t(b) * NV + threadIdx.x];
+        tot[threadIdx.x] = s;
+    }
+    __syncthreads();
+    for (int i = 0; i < NV; ++i) v[i] = tot[i];
+    __syncthreads();
+    return ok;
+}
+
+// residual max that keeps a NaN (fmax would drop it)
+DGFS_DEV double res_max(double res, double r) { return (r > res || r != r) ? r : res; }
+
 // --------------------------------------------------------- conservative BGK
 // 'bgk-gll' (std/scattering.py:445-706, bgk-gll.mako:83-229): per task the
 // discrete Maxwellian fe = a0 exp(-a1 |C|^2 + a2 Cx + a3 Cy + a4 Cz), C = c - u,
 // whose discrete mass, momentum and energy (sums times cw) equal f's, found
 // by Newton's method from a = (rho/(pi T)^1.5, 1/T, 0, 0, 0); then
-// Q += prefac rho T^{1-omega} (fe - f).  One work-group per task; every thread
-// holds the same block sums and solves the same 5 x 5 system, so all leave the
-// (at most kBgkIters) iterations together.
+// Q += prefac rho T^{1-omega} (fe - f).  At most kBgkIters iterations, as the
+// reference's loop (scattering.py:647-695), which raises on a NaN residual.
 constexpr int kBgkIters = 20;
 constexpr double kBgkTol = 1e-12;
+constexpr int kBgkSums = 30;
 
 template <typename T>
 __global__ void __launch_bounds__(kRB)
-k_bgk_newton(int n, const T* __restrict__ cx, const T* __restrict__ cy,
+k_bgk_newton(int n, int t0, const T* __restrict__ cx, const T* __restrict__ cy,
              const T* __restrict__ cz, double cw, const double* __restrict__ cst,
-             double* __restrict__ est /*[task][16]: ux uy uz a0..a4 rate iters*/) {
-    __shared__ double sh[30 * 4];
-    const int t = blockIdx.x;
+             double* __restrict__ npart /*[task][2][nch][30]*/, unsigned* __restrict__ cnt,
+             double* __restrict__ est /*[task][16]: ux uy uz a0..a4 rate iters*/,
+             int* __restrict__ status) {
+    __shared__ double sh[kBgkSums * 4];
+    __shared__ double tot[kBgkSums];
+    const int nch = gridDim.x, ch = blockIdx.x, t = t0 + blockIdx.y;
     const double* c = cst + size_t(t) * kShkCst;
     const double ux = c[0], uy = c[1], uz = c[2], Tn = 1.0 / c[3];
     const double rho = c[4] * pow(kPi * Tn, 1.5);
@@ -412,11 +489,13 @@ k_bgk_newton(int n, const T* __restrict__ cx, const T* __restrict__ cy,
     const double ar = fabs(rho);
     const double sc[5] = {ar, ar * sqrt(Tn), ar * sqrt(Tn), ar * sqrt(Tn), ar * Tn};
     double a[5] = {c[4], c[3], 0.0, 0.0, 0.0};
-    int it = 0;
+    double* base = npart + size_t(t) * 2 * nch * kBgkSums;
+    int it = 0, st = 0;
+    bool conv = false;
     for (; it < kBgkIters; ++it) {
-        double v[30];
-        for (int i = 0; i < 30; ++i) v[i] = 0.0;
-        for (int p = threadIdx.x; p < n; p += kRB) {
+        double v[kBgkSums];
+        for (int i = 0; i < kBgkSums; ++i) v[i] = 0.0;
+        for (int p = ch * kRB + threadIdx.x; p < n; p += nch * kRB) {
             const double x = cx[p], y = cy[p], z = cz[p];
             const double Cx = x - ux, Cy = y - uy, Cz = z - uz;
             const double C2 = Cx * Cx + Cy * Cy + Cz * Cz;
@@ -428,15 +507,25 @@ k_bgk_newton(int n, const T* __restrict__ cx, const T* __restrict__ cy,
                 for (int j = 0; j < 5; ++j) v[5 + 5 * i + j] += phi[i] * d[j];
             }
         }
-        block_sum_all<30>(v, sh);
+        if (!task_sum<kBgkSums>(v, sh, tot, base, nch, it, cnt + t)) {
+            st |= kStMeetTimeout;
+            break;
+        }
         double F[5], J[5][6], res = 0.0;
         for (int i = 0; i < 5; ++i) {
             F[i] = cw * v[i] - m[i];
-            res = fmax(res, fabs(F[i]) / sc[i]);
+            res = res_max(res, fabs(F[i]) / sc[i]);
             for (int j = 0; j < 5; ++j) J[i][j] = cw * v[5 + 5 * i + j];
             J[i][5] = -F[i];
         }
-        if (res < kBgkTol) break;
+        if (!isfinite(res)) {
+            st |= kStNonFinite;
+            break;
+        }
+        if (res < kBgkTol) {
+            conv = true;
+            break;
+        }
         // Gaussian elimination with partial pivoting on [J | -F]
         for (int k = 0; k < 5; ++k) {
             int piv = k;
@@ -461,7 +550,10 @@ k_bgk_newton(int n, const T* __restrict__ cx, const T* __restrict__ cy,
         }
         for (int i = 0; i < 5; ++i) a[i] += dlt[i];
     }
-    if (threadIdx.x == 0) {
+    if (!conv && st == 0) st = kStNotConverged;
+    for (int i = 0; i < 5; ++i)
+        if (!isfinite(a[i])) st |= kStNonFinite;
+    if (ch == 0 && threadIdx.x == 0) {
         double* e = est + size_t(t) * kEsCst;
         e[0] = ux;
         e[1] = uy;
@@ -469,6 +561,7 @@ k_bgk_newton(int n, const T* __restrict__ cx, const T* __restrict__ cy,
         for (int i = 0; i < 5; ++i) e[3 + i] = a[i];
         e[8] = c[9];
         e[9] = double(it);
+        status[t] |= st;
     }
 }
 
@@ -505,10 +598,13 @@ k_bgk_apply(int K, int n, const int* __restrict__ nterm, const int* __restrict__
 // G = p0 exp(sum_{l=1..9} p_l phi_l(c)), phi = {c_x, c_y, c_z, c_x^2, c_y^2,
 // c_z^2, c_x c_y, c_y c_z, c_z c_x}, with those ten moments is found by
 // Newton's method (at most 40 iterations, residual < 1e-12 of the scales).
-// The Jacobian cw sum phi_k phi_l G (phi_0 = 1) is symmetric: 55 block sums.
-// One work-group per task; thread 0 solves, the others read the result.
+// The Jacobian cw sum phi_k phi_l G (phi_0 = 1) is symmetric: 55 sums.
+// The task's work-groups share the sums as k_bgk_newton's do (pass 0: the
+// tensor, pass k >= 1: Newton iteration k - 1); in each work-group thread 0
+// solves the 10 x 10 system and the others read the result from LDS.
 constexpr int kEsgIters = 40;
 constexpr int kEsgCst = 16;
+constexpr int kEsgSums = 55;
 
 DGFS_DEV void esg_phi(double x, double y, double z, double (&ph)[10]) {
     ph[0] = 1.0; ph[1] = x; ph[2] = y; ph[3] = z;
@@ -518,21 +614,26 @@ DGFS_DEV void esg_phi(double x, double y, double z, double (&ph)[10]) {
 
 template <typename T>
 __global__ void __launch_bounds__(kRB)
-k_esg_newton(int K, int n, const int* __restrict__ nterm, const int* __restrict__ slice,
+k_esg_newton(int K, int n, int t0, const int* __restrict__ nterm, const int* __restrict__ slice,
              const T* __restrict__ coef, const T* __restrict__ u, const T* __restrict__ cx,
              const T* __restrict__ cy, const T* __restrict__ cz, double cw, double Pr,
              const double* __restrict__ cst, const double* __restrict__ ebgk,
-             double* __restrict__ eesg /*[task][16]: p0..p9 rate iters*/) {
-    __shared__ double sh[55 * 4];
+             double* __restrict__ npart /*[task][2][nch][55]*/, unsigned* __restrict__ cnt,
+             double* __restrict__ eesg /*[task][16]: p0..p9 rate iters*/,
+             int* __restrict__ status) {
+    __shared__ double sh[kEsgSums * 4];
+    __shared__ double tot[kEsgSums];
     __shared__ double sp[11];
-    const int t = blockIdx.x;
+    const int nch = gridDim.x, ch = blockIdx.x, t = t0 + blockIdx.y;
     const double* c = cst + size_t(t) * kShkCst;
     const double* b = ebgk + size_t(t) * kEsCst;
     const double ux = c[0], uy = c[1], uz = c[2], Tn = 1.0 / c[3];
     const double rho = c[4] * pow(kPi * Tn, 1.5);
-    // tensor moments of (1 - 1/Pr) f + (1/Pr) fe
+    double* base = npart + size_t(t) * 2 * nch * kEsgSums;
+    int st = 0;
+    // pass 0: tensor moments of (1 - 1/Pr) f + (1/Pr) fe
     double S[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int p = threadIdx.x; p < n; p += kRB) {
+    for (int p = ch * kRB + threadIdx.x; p < n; p += nch * kRB) {
         const double Cx = double(cx[p]) - ux, Cy = double(cy[p]) - uy, Cz = double(cz[p]) - uz;
         const double fe = b[3] * exp(-b[4] * (Cx * Cx + Cy * Cy + Cz * Cz) + b[5] * Cx +
                                      b[6] * Cy + b[7] * Cz);
@@ -541,7 +642,7 @@ k_esg_newton(int K, int n, const int* __restrict__ nterm, const int* __restrict_
         S[0] += w * Cx * Cx; S[1] += w * Cy * Cy; S[2] += w * Cz * Cz;
         S[3] += w * Cx * Cy; S[4] += w * Cy * Cz; S[5] += w * Cz * Cx;
     }
-    block_sum_all<6>(S, sh);
+    const bool met = task_sum<6>(S, sh, tot, base, nch, 0, cnt + t);
     const double m[10] = {rho, rho * ux, rho * uy, rho * uz,
                           rho * ux * ux + cw * S[0], rho * uy * uy + cw * S[1],
                           rho * uz * uz + cw * S[2], rho * ux * uy + cw * S[3],
@@ -553,10 +654,12 @@ k_esg_newton(int K, int n, const int* __restrict__ nterm, const int* __restrict_
     double q[10] = {c[4] * exp(-(ux * ux + uy * uy + uz * uz) / Tn), 2.0 * ux / Tn, 2.0 * uy / Tn,
                     2.0 * uz / Tn, -1.0 / Tn, -1.0 / Tn, -1.0 / Tn, 0.0, 0.0, 0.0};
     int it = 0;
-    for (; it < kEsgIters; ++it) {
-        double v[55];
-        for (int i = 0; i < 55; ++i) v[i] = 0.0;
-        for (int p = threadIdx.x; p < n; p += kRB) {
+    bool conv = false;
+    if (!met) st |= kStMeetTimeout;
+    for (; met && it < kEsgIters; ++it) {
+        double v[kEsgSums];
+        for (int i = 0; i < kEsgSums; ++i) v[i] = 0.0;
+        for (int p = ch * kRB + threadIdx.x; p < n; p += nch * kRB) {
             double ph[10];
             esg_phi(cx[p], cy[p], cz[p], ph);
             double e = 0.0;
@@ -566,7 +669,10 @@ k_esg_newton(int K, int n, const int* __restrict__ nterm, const int* __restrict_
             for (int i = 0; i < 10; ++i)
                 for (int j = i; j < 10; ++j) v[k++] += ph[i] * ph[j] * G;
         }
-        block_sum_all<55>(v, sh);
+        if (!task_sum<kEsgSums>(v, sh, tot, base, nch, it + 1, cnt + t)) {
+            st |= kStMeetTimeout;
+            break;
+        }
         if (threadIdx.x == 0) {
             double J[10][11], res = 0.0;
             int k = 0;
@@ -577,12 +683,12 @@ k_esg_newton(int K, int n, const int* __restrict__ nterm, const int* __restrict_
                 }
             for (int i = 0; i < 10; ++i) {
                 const double F = J[i][0] - m[i];          // cw sum phi_i G
-                res = fmax(res, fabs(F) / sc[i]);
+                res = res_max(res, fabs(F) / sc[i]);
                 J[i][10] = -F;
                 J[i][0] /= q[0];                          // d G / d p0 = G / p0
             }
             sp[10] = res;
-            if (res >= kBgkTol) {
+            if (isfinite(res) && res >= kBgkTol) {
                 for (int kk = 0; kk < 10; ++kk) {
                     int piv = kk;
                     for (int i = kk + 1; i < 10; ++i)
@@ -613,13 +719,24 @@ k_esg_newton(int K, int n, const int* __restrict__ nterm, const int* __restrict_
         const double res = sp[10];
         for (int i = 0; i < 10; ++i) q[i] = sp[i];
         __syncthreads();
-        if (res < kBgkTol) break;
+        if (!isfinite(res)) {
+            st |= kStNonFinite;
+            break;
+        }
+        if (res < kBgkTol) {
+            conv = true;
+            break;
+        }
     }
-    if (threadIdx.x == 0) {
+    if (!conv && st == 0) st = kStNotConverged;
+    for (int i = 0; i < 10; ++i)
+        if (!isfinite(q[i])) st |= kStNonFinite;
+    if (ch == 0 && threadIdx.x == 0) {
         double* e = eesg + size_t(t) * kEsgCst;
         for (int i = 0; i < 10; ++i) e[i] = q[i];
         e[10] = c[9];
         e[11] = double(it);
+        status[t] |= st;
     }
 }
 
@@ -879,9 +996,117 @@ extern "C" int dgfs_esbgk(int dtype, int K, int Ne, int Nv, int ntask, const int
     }
 }
 
+namespace dgfs {
+namespace {
+
+// work-groups of kRB threads of `kernel` that can be resident at once on the
+// current device (cached per kernel and device)
+int coresident_groups(const void* kernel) {
+    struct Entry { const void* k; int dev, groups; };
+    static Entry cache[32];
+    static int used = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    for (int i = 0; i < used; ++i)
+        if (cache[i].k == kernel && cache[i].dev == dev) return cache[i].groups;
+    int ncu = 0, per = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kRB, 0) != hipSuccess)
+        return 0;
+    if (used < 32) cache[used++] = Entry{kernel, dev, per * ncu};
+    return per * ncu;
+}
+
+template <typename... A>
+hipError_t launch_coop(const void* kernel, dim3 grid, hipStream_t st, A... args) {
+    void* p[] = {static_cast<void*>(&args)...};
+    return hipLaunchCooperativeKernel(kernel, grid, dim3(kRB), p, 0, st);
+}
+
+// Newton work-group layout: nch work-groups per task, tasks in launches of
+// `per` so that each launch's grid is resident
+struct NewtonGrid { int nch, per; };
+
+NewtonGrid newton_grid(const void* kernel, int ntask, int nb) {
+    const int cap = coresident_groups(kernel);
+    if (cap <= 0) return NewtonGrid{0, 0};
+    const int nch = std::max(1, std::min(std::min(kNchMax, nb), cap / ntask));
+    return NewtonGrid{nch, std::max(1, cap / nch)};
+}
+
+// status and arrival counters behind the sums of the Newton solves
+size_t newton_work_doubles(int ntask, int nsums) {
+    return size_t(ntask) * 2 * kNchMax * nsums + size_t(ntask);   // + 2 uint counters per task
+}
+
+template <typename T>
+int bgk_newton(int Nv, int ntask, const void* cx, const void* cy, const void* cz, double cw,
+               const double* cst, double* npart, unsigned* cnt, double* est, int* status,
+               hipStream_t stream) {
+    const void* k = reinterpret_cast<const void*>(&k_bgk_newton<T>);
+    const NewtonGrid g = newton_grid(k, ntask, (Nv + kRB - 1) / kRB);
+    DGFS_REQUIRE(g.nch > 0, "conservative BGK: cannot size the cooperative Newton launch");
+    for (int t0 = 0; t0 < ntask; t0 += g.per) {
+        const int nt = std::min(g.per, ntask - t0);
+        DGFS_HIP_CHECK(launch_coop(k, dim3(g.nch, nt), stream, Nv, t0, (const T*)cx,
+                                   (const T*)cy, (const T*)cz, cw, cst, npart, cnt, est, status));
+    }
+    return 0;
+}
+
+template <typename T>
+int esg_newton(int K, int Nv, int ntask, const int* nterm, const int* slice, const void* coef,
+               const void* u, const void* cx, const void* cy, const void* cz, double cw,
+               double Pr, const double* cst, const double* ebgk, double* npart, unsigned* cnt,
+               double* eesg, int* status, hipStream_t stream) {
+    const void* k = reinterpret_cast<const void*>(&k_esg_newton<T>);
+    const NewtonGrid g = newton_grid(k, ntask, (Nv + kRB - 1) / kRB);
+    DGFS_REQUIRE(g.nch > 0, "conservative ES-BGK: cannot size the cooperative Newton launch");
+    for (int t0 = 0; t0 < ntask; t0 += g.per) {
+        const int nt = std::min(g.per, ntask - t0);
+        DGFS_HIP_CHECK(launch_coop(k, dim3(g.nch, nt), stream, K, Nv, t0, nterm, slice,
+                                   (const T*)coef, (const T*)u, (const T*)cx, (const T*)cy,
+                                   (const T*)cz, cw, Pr, cst, ebgk, npart, cnt, eesg, status));
+    }
+    return 0;
+}
+
+template <typename T>
+void bgk_apply(int K, int Nv, int nb, int ngroup, const int* nterm, const int* slice,
+               const void* coef, const int* gstart, const int* gslice, const void* u,
+               const void* cx, const void* cy, const void* cz, const double* est, void* Q,
+               hipStream_t stream) {
+    hipLaunchKernelGGL(k_bgk_apply<T>, dim3(nb, ngroup), dim3(kRB), 0, stream, K, Nv, nterm,
+                       slice, (const T*)coef, gstart, gslice, (const T*)u, (const T*)cx,
+                       (const T*)cy, (const T*)cz, est, (T*)Q);
+}
+
+template <typename T>
+void esg_apply(int K, int Nv, int nb, int ngroup, const int* nterm, const int* slice,
+               const void* coef, const int* gstart, const int* gslice, const void* u,
+               const void* cx, const void* cy, const void* cz, const double* eesg, void* Q,
+               hipStream_t stream) {
+    hipLaunchKernelGGL(k_esg_apply<T>, dim3(nb, ngroup), dim3(kRB), 0, stream, K, Nv, nterm,
+                       slice, (const T*)coef, gstart, gslice, (const T*)u, (const T*)cx,
+                       (const T*)cy, (const T*)cz, eesg, (T*)Q);
+}
+
+template <typename T>
+void shk_partial(int K, int Nv, int nb, int ntask, const int* nterm, const int* slice,
+                 const void* coef, const void* u, const void* cx, const void* cy, const void* cz,
+                 double* part, hipStream_t stream) {
+    hipLaunchKernelGGL(k_shk_partial<T>, dim3(nb, ntask), dim3(kRB), 0, stream, K, Nv, nterm,
+                       slice, (const T*)coef, (const T*)u, (const T*)cx, (const T*)cy,
+                       (const T*)cz, part);
+}
+
+}  // namespace
+}  // namespace dgfs
+
 extern "C" size_t dgfs_bgk_work_bytes(int ntask, int Nv) {
     const int nb = (Nv + kRB - 1) / kRB;
-    return size_t(ntask) * (nb * kShkMom + kShkCst + kEsCst) * sizeof(double);
+    return (size_t(ntask) * (nb * kShkMom + kShkCst + kEsCst) +
+            newton_work_doubles(ntask, kBgkSums)) * sizeof(double);
 }
 
 extern "C" int dgfs_bgk_conservative(int dtype, int K, int Ne, int Nv, int ntask,
@@ -889,64 +1114,45 @@ extern "C" int dgfs_bgk_conservative(int dtype, int K, int Ne, int Nv, int ntask
                                      const void* task_coef, int ngroup, const int* group_start,
                                      const int* group_slice, const void* u, const void* cx,
                                      const void* cy, const void* cz, double cw, double prefac,
-                                     double omega, void* Q, double* work, hipStream_t stream) {
+                                     double omega, void* Q, int* status, double* work,
+                                     hipStream_t stream) {
     {
         DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "dtype must be DGFS_F64 or DGFS_F32");
         DGFS_REQUIRE(K > 0 && Ne > 0 && Nv > 0 && ntask >= 0 && ngroup >= 0, "bad sizes");
         if (ntask == 0 || ngroup == 0) return 0;
         DGFS_REQUIRE(ntask <= 65535 && ngroup <= 65535, "too many tasks");
         DGFS_REQUIRE(task_nterm && task_slice && task_coef && group_start && group_slice && u &&
-                         cx && cy && cz && Q && work,
+                         cx && cy && cz && Q && status && work,
                      "null pointer");
         const int nb = (Nv + kRB - 1) / kRB;
         const ShkParams P{prefac, omega, 1.0, cw};
         double* part = work;
         double* cst = part + size_t(ntask) * nb * kShkMom;
         double* est = cst + size_t(ntask) * kShkCst;
-        if (dtype == DGFS_F64)
-            hipLaunchKernelGGL(k_shk_partial<double>, dim3(nb, ntask), dim3(kRB), 0, stream, K,
-                               Nv, task_nterm, task_slice, (const double*)task_coef,
-                               (const double*)u, (const double*)cx, (const double*)cy,
-                               (const double*)cz, part);
-        else
-            hipLaunchKernelGGL(k_shk_partial<float>, dim3(nb, ntask), dim3(kRB), 0, stream, K,
-                               Nv, task_nterm, task_slice, (const float*)task_coef,
-                               (const float*)u, (const float*)cx, (const float*)cy,
-                               (const float*)cz, part);
+        double* npart = est + size_t(ntask) * kEsCst;
+        unsigned* cnt = reinterpret_cast<unsigned*>(npart + size_t(ntask) * 2 * kNchMax * kBgkSums);
+        const bool f64 = dtype == DGFS_F64;
+        (f64 ? shk_partial<double> : shk_partial<float>)(K, Nv, nb, ntask, task_nterm, task_slice,
+                                                         task_coef, u, cx, cy, cz, part, stream);
         DGFS_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_shk_final, dim3(ntask), dim3(kRB), 0, stream, nb, P,
-                           (const double*)part, cst);
+                           (const double*)part, cst, cnt, 1, status);
         DGFS_HIP_CHECK(hipGetLastError());
-        if (dtype == DGFS_F64)
-            hipLaunchKernelGGL(k_bgk_newton<double>, dim3(ntask), dim3(kRB), 0, stream, Nv,
-                               (const double*)cx, (const double*)cy, (const double*)cz, cw,
-                               (const double*)cst, est);
-        else
-            hipLaunchKernelGGL(k_bgk_newton<float>, dim3(ntask), dim3(kRB), 0, stream, Nv,
-                               (const float*)cx, (const float*)cy, (const float*)cz, cw,
-                               (const double*)cst, est);
+        const int rc = (f64 ? bgk_newton<double> : bgk_newton<float>)(
+            Nv, ntask, cx, cy, cz, cw, cst, npart, cnt, est, status, stream);
+        if (rc) return rc;
+        (f64 ? bgk_apply<double> : bgk_apply<float>)(K, Nv, nb, ngroup, task_nterm, task_slice,
+                                                     task_coef, group_start, group_slice, u, cx,
+                                                     cy, cz, est, Q, stream);
         DGFS_HIP_CHECK(hipGetLastError());
-        if (dtype == DGFS_F64)
-            hipLaunchKernelGGL(k_bgk_apply<double>, dim3(nb, ngroup), dim3(kRB), 0, stream, K,
-                               Nv, task_nterm, task_slice, (const double*)task_coef, group_start,
-                               group_slice, (const double*)u, (const double*)cx,
-                               (const double*)cy, (const double*)cz, (const double*)est,
-                               (double*)Q);
-        else
-            hipLaunchKernelGGL(k_bgk_apply<float>, dim3(nb, ngroup), dim3(kRB), 0, stream, K,
-                               Nv, task_nterm, task_slice, (const float*)task_coef, group_start,
-                               group_slice, (const float*)u, (const float*)cx,
-                               (const float*)cy, (const float*)cz, (const double*)est,
-                               (float*)Q);
-        DGFS_HIP_CHECK(hipGetLastError());
-        (void)Ne;
         return 0;
     }
 }
 
 extern "C" size_t dgfs_esbgk_conservative_work_bytes(int ntask, int Nv) {
     const int nb = (Nv + kRB - 1) / kRB;
-    return size_t(ntask) * (nb * kShkMom + kShkCst + kEsCst + kEsgCst) * sizeof(double);
+    return (size_t(ntask) * (nb * kShkMom + kShkCst + kEsCst + kEsgCst) +
+            newton_work_doubles(ntask, kEsgSums)) * sizeof(double);
 }
 
 extern "C" int dgfs_esbgk_conservative(int dtype, int K, int Ne, int Nv, int ntask,
@@ -954,8 +1160,8 @@ extern "C" int dgfs_esbgk_conservative(int dtype, int K, int Ne, int Nv, int nta
                                        const void* task_coef, int ngroup, const int* group_start,
                                        const int* group_slice, const void* u, const void* cx,
                                        const void* cy, const void* cz, double cw, double prefac,
-                                       double omega, double Pr, void* Q, double* work,
-                                       hipStream_t stream) {
+                                       double omega, double Pr, void* Q, int* status,
+                                       double* work, hipStream_t stream) {
     {
         DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "dtype must be DGFS_F64 or DGFS_F32");
         DGFS_REQUIRE(K > 0 && Ne > 0 && Nv > 0 && ntask >= 0 && ngroup >= 0, "bad sizes");
@@ -963,7 +1169,7 @@ extern "C" int dgfs_esbgk_conservative(int dtype, int K, int Ne, int Nv, int nta
         if (ntask == 0 || ngroup == 0) return 0;
         DGFS_REQUIRE(ntask <= 65535 && ngroup <= 65535, "too many tasks");
         DGFS_REQUIRE(task_nterm && task_slice && task_coef && group_start && group_slice && u &&
-                         cx && cy && cz && Q && work,
+                         cx && cy && cz && Q && status && work,
                      "null pointer");
         const int nb = (Nv + kRB - 1) / kRB;
         const ShkParams P{prefac, omega, 1.0, cw};
@@ -971,56 +1177,26 @@ extern "C" int dgfs_esbgk_conservative(int dtype, int K, int Ne, int Nv, int nta
         double* cst = part + size_t(ntask) * nb * kShkMom;
         double* ebgk = cst + size_t(ntask) * kShkCst;
         double* eesg = ebgk + size_t(ntask) * kEsCst;
-        if (dtype == DGFS_F64) {
-            hipLaunchKernelGGL(k_shk_partial<double>, dim3(nb, ntask), dim3(kRB), 0, stream, K,
-                               Nv, task_nterm, task_slice, (const double*)task_coef,
-                               (const double*)u, (const double*)cx, (const double*)cy,
-                               (const double*)cz, part);
-        } else {
-            hipLaunchKernelGGL(k_shk_partial<float>, dim3(nb, ntask), dim3(kRB), 0, stream, K,
-                               Nv, task_nterm, task_slice, (const float*)task_coef,
-                               (const float*)u, (const float*)cx, (const float*)cy,
-                               (const float*)cz, part);
-        }
+        double* npart = eesg + size_t(ntask) * kEsgCst;
+        unsigned* cnt = reinterpret_cast<unsigned*>(npart + size_t(ntask) * 2 * kNchMax * kEsgSums);
+        const bool f64 = dtype == DGFS_F64;
+        (f64 ? shk_partial<double> : shk_partial<float>)(K, Nv, nb, ntask, task_nterm, task_slice,
+                                                         task_coef, u, cx, cy, cz, part, stream);
         DGFS_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_shk_final, dim3(ntask), dim3(kRB), 0, stream, nb, P,
-                           (const double*)part, cst);
+                           (const double*)part, cst, cnt, 2, status);
         DGFS_HIP_CHECK(hipGetLastError());
-        if (dtype == DGFS_F64) {
-            hipLaunchKernelGGL(k_bgk_newton<double>, dim3(ntask), dim3(kRB), 0, stream, Nv,
-                               (const double*)cx, (const double*)cy, (const double*)cz, cw,
-                               (const double*)cst, ebgk);
-            DGFS_HIP_CHECK(hipGetLastError());
-            hipLaunchKernelGGL(k_esg_newton<double>, dim3(ntask), dim3(kRB), 0, stream, K, Nv,
-                               task_nterm, task_slice, (const double*)task_coef,
-                               (const double*)u, (const double*)cx, (const double*)cy,
-                               (const double*)cz, cw, Pr, (const double*)cst,
-                               (const double*)ebgk, eesg);
-            DGFS_HIP_CHECK(hipGetLastError());
-            hipLaunchKernelGGL(k_esg_apply<double>, dim3(nb, ngroup), dim3(kRB), 0, stream, K,
-                               Nv, task_nterm, task_slice, (const double*)task_coef, group_start,
-                               group_slice, (const double*)u, (const double*)cx,
-                               (const double*)cy, (const double*)cz, (const double*)eesg,
-                               (double*)Q);
-        } else {
-            hipLaunchKernelGGL(k_bgk_newton<float>, dim3(ntask), dim3(kRB), 0, stream, Nv,
-                               (const float*)cx, (const float*)cy, (const float*)cz, cw,
-                               (const double*)cst, ebgk);
-            DGFS_HIP_CHECK(hipGetLastError());
-            hipLaunchKernelGGL(k_esg_newton<float>, dim3(ntask), dim3(kRB), 0, stream, K, Nv,
-                               task_nterm, task_slice, (const float*)task_coef,
-                               (const float*)u, (const float*)cx, (const float*)cy,
-                               (const float*)cz, cw, Pr, (const double*)cst,
-                               (const double*)ebgk, eesg);
-            DGFS_HIP_CHECK(hipGetLastError());
-            hipLaunchKernelGGL(k_esg_apply<float>, dim3(nb, ngroup), dim3(kRB), 0, stream, K,
-                               Nv, task_nterm, task_slice, (const float*)task_coef, group_start,
-                               group_slice, (const float*)u, (const float*)cx,
-                               (const float*)cy, (const float*)cz, (const double*)eesg,
-                               (float*)Q);
-        }
+        int rc = (f64 ? bgk_newton<double> : bgk_newton<float>)(
+            Nv, ntask, cx, cy, cz, cw, cst, npart, cnt, ebgk, status, stream);
+        if (rc) return rc;
+        rc = (f64 ? esg_newton<double> : esg_newton<float>)(
+            K, Nv, ntask, task_nterm, task_slice, task_coef, u, cx, cy, cz, cw, Pr, cst, ebgk,
+            npart, cnt + ntask, eesg, status, stream);
+        if (rc) return rc;
+        (f64 ? esg_apply<double> : esg_apply<float>)(K, Nv, nb, ngroup, task_nterm, task_slice,
+                                                     task_coef, group_start, group_slice, u, cx,
+                                                     cy, cz, eesg, Q, stream);
         DGFS_HIP_CHECK(hipGetLastError());
-        (void)Ne;
         return 0;
     }
 }

@@ -7,10 +7,14 @@ and energy moments of collision output slices in FP64 (no reference
 counterpart).  Both call the C-ABI of libdgfs_hip.so and never fall back to
 host arithmetic.
 """
+import logging
+
 import numpy as np
 
 from . import _lib
 from .collision import task_table
+
+log = logging.getLogger('dgfs1d_gpu_amd')
 
 
 def _grid_as_device(cv, tdtype):
@@ -118,20 +122,36 @@ class MixtureProjection:
             _lib.ptr(self._work), _lib.stream_ptr(stream)))
 
 
+# per-task status bits of the Newton solves (dgfs_bgk_conservative)
+NEWTON_NONFINITE, NEWTON_NOT_CONVERGED, NEWTON_WAIT_TIMEOUT = 1, 2, 4
+
+
 class Shakov:
     """Batched Shakov relaxation term Q += prefac rho T^{1-omega} (f_S - f);
     with target='esbgk' the ES-BGK term (dgfs_esbgk), f_S replaced by the
     anisotropic Gaussian of the node; with target='bgk' the conservative BGK
     term (dgfs_bgk_conservative), f_S replaced by the discrete Maxwellian with
-    f's discrete mass, momentum and energy."""
+    f's discrete mass, momentum and energy; with target='esbgk-conservative'
+    the conservative ES-BGK term (dgfs_esbgk_conservative).
 
-    def __init__(self, n, cx, cy, cz, cw, prefac, omega, Pr, tdtype, device, target='shakov'):
+    The two conservative targets run Newton solves on the device.  With
+    check=True (default) every call reads their per-task status back and
+    raises RuntimeError('NaN encountered ...') when a solve produced a NaN or
+    an infinity, as the reference does (std/scattering.py:689, :1084); a solve
+    that stops at the iteration limit is logged, as the reference stops there
+    without an error.  check=False leaves the call free of host
+    synchronisation; ``check_status()`` then reads the last call's status."""
+
+    def __init__(self, n, cx, cy, cz, cw, prefac, omega, Pr, tdtype, device, target='shakov',
+                 check=True):
         fns = {'shakov': 'dgfs_shakov', 'esbgk': 'dgfs_esbgk', 'bgk': 'dgfs_bgk_conservative',
                'esbgk-conservative': 'dgfs_esbgk_conservative'}
         if target not in fns:
             raise ValueError('target must be one of %s' % ', '.join(sorted(fns)))
         self._fn = fns[target]
         self._wfn = ('dgfs_bgk' if target == 'bgk' else fns[target]) + '_work_bytes'
+        self.newton = target in ('bgk', 'esbgk-conservative')
+        self.check = bool(check)
         import torch
         self.torch = torch
         self.n = int(n)
@@ -141,6 +161,8 @@ class Shakov:
         self.device = device
         self.tasks = None
         self._work = None
+        self._status = None
+        self._ntask_last = 0
 
     def set_tasks(self, K, Ne, tasks):
         """tasks: (out_slice, f_terms[, g_terms]); only the f terms are used
@@ -166,9 +188,42 @@ class Shakov:
         if u.numel() < nmin or Q.numel() < nmin:
             raise ValueError('Shakov operands smaller than K*Ne*Nv')
         pr = () if self._fn == 'dgfs_bgk_conservative' else (self.Pr,)
+        st = ()
+        if self.newton:
+            if self._status is None or self._status.numel() < t['ntask']:
+                self._status = self.torch.zeros(max(1, t['ntask']), dtype=self.torch.int32,
+                                                device=self.device)
+            st = (_lib.ptr(self._status),)
+            self._ntask_last = t['ntask']
         _lib.check(getattr(_lib.lib(), self._fn)(
             _lib.dtype_code(self.tdtype), t['K'], t['Ne'], self.n, t['ntask'],
             _lib.ptr(t['nterm']), _lib.ptr(t['slice']), _lib.ptr(t['coef']), t['ngroup'],
             _lib.ptr(t['gstart']), _lib.ptr(t['gslice']), _lib.ptr(u), _lib.ptr(self.cx),
             _lib.ptr(self.cy), _lib.ptr(self.cz), self.cw, self.prefac, self.omega, *pr,
-            _lib.ptr(Q), _lib.ptr(self.work(t['ntask'])), _lib.stream_ptr(stream)))
+            _lib.ptr(Q), *st, _lib.ptr(self.work(t['ntask'])), _lib.stream_ptr(stream)))
+        if self.newton and self.check:
+            self.check_status()
+
+    def status(self):
+        """Per-task status bits of the last call's Newton solves (host array;
+        synchronises with the device)."""
+        import numpy as np
+        if not self.newton or self._status is None:
+            return np.zeros(0, dtype=np.int32)
+        return self._status[:self._ntask_last].cpu().numpy()
+
+    def check_status(self):
+        """Raise if the last call's Newton solves met a NaN or an infinity (or a
+        work-group wait gave up); log the tasks that hit the iteration limit."""
+        st = self.status()
+        bad = np.flatnonzero(st & (NEWTON_NONFINITE | NEWTON_WAIT_TIMEOUT))
+        if bad.size:
+            kind = 'NaN encountered' if np.any(st[bad] & NEWTON_NONFINITE) else \
+                'work-group wait gave up'
+            raise RuntimeError('%s in the %s Newton solve of task(s) %s'
+                               % (kind, self._fn[5:], ', '.join(str(int(i)) for i in bad[:8])))
+        slow = np.flatnonzero(st & NEWTON_NOT_CONVERGED)
+        if slow.size:
+            log.warning('%s: Newton solve stopped at the iteration limit for %d task(s)',
+                        self._fn[5:], slow.size)
+        return st

@@ -129,9 +129,51 @@ class DGFSVHSGLLScatteringModelStd(DGFSScatteringModelStd):
     def set_tasks(self, K, Ne, tasks):
         if self._variant == _lib.VHS_GLL:
             tasks = [(o, f, f) for o, f, _ in tasks]       # fs() uses d_arr_in only
-        self.plan.set_tasks(K, Ne, _fold_same_slice(tasks))
+        folded = _fold_same_slice(tasks)
+        tab = self.plan.set_tasks(K, Ne, folded)
         self._single = {}
         self.closure = self._make_closure(K, Ne, tasks)
+        self._make_reference_state(K, Ne, tab, folded)
+
+    def _make_reference_state(self, K, Ne, tab, tasks):
+        """[scattering-model] reference-refresh = R (R > 0, FP32 runs with f = g
+        tasks): every collide() evaluates Q(f, f) in the reference-state form
+        Q(f0, f0) + Gain(h, f + f0) - h Loss(f) - f0 Loss(h), h = f - f0
+        (dgfs_collide_reference), where f0 is the state of the most recent
+        refresh and Q(f0, f0) comes from an FP64 plan.  The form is exact; it
+        makes the FP32 rounding error scale with |f - f0| instead of |f|.  The
+        reference state is refreshed on the first call and every R-th call
+        after it.  R = 0 (default): the direct form."""
+        self._ref = None
+        R = int(self.cfg.lookupordefault('scattering-model', 'reference-refresh', 0))
+        if R < 0:
+            raise ValueError('scattering-model::reference-refresh must be >= 0')
+        if R == 0 or self.plan.dtype == _lib.F64:
+            return
+        if not tab['fg_same']:
+            raise ValueError('scattering-model::reference-refresh needs f = g collision tasks '
+                             '(a nodal basis)')
+        if self.closure is not None:
+            raise ValueError('scattering-model::reference-refresh and closure = learned are '
+                             'exclusive')
+        import torch
+        vm = self.vm
+        plan64 = CollisionPlan(vm.Nv(), vm.M(), vm.Nrho(), vm.L(), vm.qz(), vm.qw(), vm.sz(),
+                               vm.sw(), self._gamma, self._prefactor, _lib.F64, self._variant,
+                               ntask_max=self.plan.ntask_max, device=self._device)
+        plan64.set_tasks(K, Ne, tasks)
+        n = K * Ne * vm.vsize()
+        self._ref = dict(R=R, calls=0, plan64=plan64,
+                         u0=torch.zeros(n, dtype=self.plan.tdtype, device=self._device),
+                         Q0=torch.zeros(n, dtype=torch.float64, device=self._device))
+
+    def refresh_reference(self, u, stream=None):
+        """Make u the reference state: u0 = u, Q0 = Q(u, u) in FP64."""
+        r = self._ref
+        n = r['u0'].numel()
+        r['u0'].copy_(u[:n])
+        r['Q0'].zero_()
+        r['plan64'](r['u0'].double(), r['Q0'], stream)
 
     def _make_closure(self, K, Ne, tasks):
         """[scattering-model] closure = learned: Q = Q_Shakov + R_net per node,
@@ -175,6 +217,12 @@ class DGFSVHSGLLScatteringModelStd(DGFSScatteringModelStd):
     def _collide(self, u, Q, stream=None):
         if self.closure is not None:
             self.closure(u, Q)
+        elif self._ref is not None:
+            r = self._ref
+            if r['calls'] % r['R'] == 0:
+                self.refresh_reference(u, stream)
+            r['calls'] += 1
+            self.plan.reference(u, r['u0'], r['Q0'], Q, stream)
         else:
             self.plan(u, Q, stream)
 

@@ -87,6 +87,27 @@ int dgfs_collide(dgfs_collision_plan_t plan, const void* u, const void* ug, void
                  int ngroup, const int* group_start, const int* group_slice, int fg_same,
                  hipStream_t stream);
 
+/* Reference-state form of the same collision (no counterpart in the
+ * reference, whose fs() is FP64 end to end; it replaces dgfs_collide for
+ * f = g tasks when the plan computes in FP32).  With f_t the task's f
+ * contraction of u and f0_t the same contraction of the reference state u0:
+ *     Q[group_slice[g]] += Q0[group_slice[g]]
+ *         + sum_{t in g} prefac (Re Gain(h_t, s_t) - h_t Re Loss(f_t) - f0_t Re Loss(h_t)),
+ *     h = f - f0,  s = f + f0,
+ * which equals dgfs_collide's Q(f, f) exactly when Q0 holds Q(f0, f0) (Gain
+ * is a symmetric bilinear form and Loss is linear).  Q0: FP64, the (K, Ne,
+ * Nv) layout, typically one FP64 dgfs_collide call on u0.  Every FP32
+ * rounding error of the spectral pipeline then scales with |h| instead of
+ * |f|: near equilibrium the gain and the loss each exceed Q by orders of
+ * magnitude, and FP32 rounding of those two large terms is what limits the
+ * direct form.  The g terms of the task table are ignored.  u0 must be a
+ * different array from u. */
+int dgfs_collide_reference(dgfs_collision_plan_t plan, const void* u, const void* u0,
+                           const double* Q0, void* Q, int K, int Ne, int ntask,
+                           const int* task_nterm, const int* task_slice, const void* task_coef,
+                           int ngroup, const int* group_start, const int* group_slice,
+                           hipStream_t stream);
+
 /* Stage timing (no counterpart in the reference, which only times the whole
  * loop, std/std.py:394-423): dgfs_collision_timing(plan, max_calls) creates
  * 6*max_calls HIP events (0 frees them); the next max_calls dgfs_collide calls
@@ -335,7 +356,14 @@ size_t dgfs_esbgk_work_bytes(int ntask, int Nv);
  * a = (rho/(pi T)^1.5, 1/T, 0, 0, 0), at most 20 iterations, until every
  * moment residual is below 1e-12 of its scale), then
  *   Q[group slice] += prefac rho T^{1-omega} (fe - f_t).
- * FP64 moments and solve.  work: dgfs_bgk_work_bytes(ntask, Nv) bytes. */
+ * FP64 moments and solve.  Each task's velocity sums are split over up to 32
+ * work-groups of one cooperative launch.  status: device array of ntask ints,
+ * overwritten with per-task bits: 1 = a residual or the parameters became NaN
+ * or infinite (the reference raises 'NaN encountered', scattering.py:689),
+ * 2 = not converged within the iteration limit (the reference stops there
+ * silently), 4 = a work-group wait gave up.  The call itself does not
+ * synchronise; the caller reads status when it needs the verdict.
+ * work: dgfs_bgk_work_bytes(ntask, Nv) bytes. */
 size_t dgfs_bgk_work_bytes(int ntask, int Nv);
 
 /* Conservative ES-BGK 'esbgk-gll' (replaces fs() of
@@ -346,19 +374,20 @@ size_t dgfs_bgk_work_bytes(int ntask, int Nv);
  * momentum and second moments equal rho, rho u, rho (u_i u_j + T_ij)
  * (Newton's method, at most 40 iterations); Q += prefac rho T^{1-omega} (G - f),
  * prefac the caller's (t0 Pr p0 / mu(T0) in the reference).  Pr > 0.
+ * status: as dgfs_bgk_conservative's, the bits of both solves combined.
  * work: dgfs_esbgk_conservative_work_bytes(ntask, Nv) bytes. */
 size_t dgfs_esbgk_conservative_work_bytes(int ntask, int Nv);
 int dgfs_esbgk_conservative(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nterm,
                             const int* task_slice, const void* task_coef, int ngroup,
                             const int* group_start, const int* group_slice, const void* u,
                             const void* cx, const void* cy, const void* cz, double cw,
-                            double prefac, double omega, double Pr, void* Q, double* work,
-                            hipStream_t stream);
+                            double prefac, double omega, double Pr, void* Q, int* status,
+                            double* work, hipStream_t stream);
 int dgfs_bgk_conservative(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nterm,
                           const int* task_slice, const void* task_coef, int ngroup,
                           const int* group_start, const int* group_slice, const void* u,
                           const void* cx, const void* cy, const void* cz, double cw,
-                          double prefac, double omega, void* Q, double* work,
+                          double prefac, double omega, void* Q, int* status, double* work,
                           hipStream_t stream);
 int dgfs_esbgk(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nterm,
                const int* task_slice, const void* task_coef, int ngroup,

@@ -137,6 +137,29 @@ def q_vhs_gll(pre, f, prefac):
     return prefac * (QG.real - f * fC2.real)
 
 
+def loss_field(pre, g):
+    """L[g] = Re IDFT_u(bb2 * DFT(g) / n): the field that multiplies f in the
+    loss term of output_append_ (vhs-gll.mako ax kernel and the inverse
+    transform, std/scattering.py:418-425)."""
+    N, n = pre.N, pre.n
+    return _ifft_u(pre.bb2 * _fft(g.astype(np.complex128), N) / n, N).real
+
+
+def q_reference(pre, f, f0, Q0, prefac):
+    """Q(f, f) in the reference-state form used by the FP32 product path
+    (no reference counterpart):
+
+        Q0 + prefac (G(h, s) - h L[f] - f0 L[h]),   h = f - f0,  s = f + f0,
+
+    equal to q_vhs(f, f) when Q0 = q_vhs(f0, f0), because the gain G of
+    q_vhs is a symmetric bilinear form (its spectral kernel cos(a(l) - a(m))
+    is even) and L is linear.  G(h, s) is taken from q_vhs(h, s) =
+    prefac (G(h, s) - h L[s]) and L[s] - L[f0] = L[f]."""
+    h, s = f - f0, f + f0
+    return Q0 + q_vhs(pre, h, s, prefac) + prefac * (h * loss_field(pre, f0)
+                                                    - f0 * loss_field(pre, h))
+
+
 def q_bruteforce(pre, f, g, prefac):
     """The same operator as a direct spectral sum (no FFT of any product):
 

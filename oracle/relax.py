@@ -110,9 +110,19 @@ def bgk_conservative_target(f, cv, cw):
 
     def fe(a):
         return a[0] * np.exp(-a[1] * C2 + a[2] * C[0] + a[3] * C[1] + a[4] * C[2])
-    sol = root(lambda a: (phi @ fe(a) - m) / sc, [rho / (np.pi * T) ** 1.5, 1.0 / T, 0, 0, 0],
-               method='hybr', options={'xtol': 1e-15})
+    fun = lambda a: (phi @ fe(a) - m) / sc
+    sol = root(fun, [rho / (np.pi * T) ** 1.5, 1.0 / T, 0, 0, 0], method='hybr',
+               options={'xtol': 1e-15})
+    _check_root(sol, fun, 'bgk_conservative_target')
     return fe(sol.x), rho, T
+
+
+def _check_root(sol, fun, what):
+    """MINPACK may stop on 'no further progress' at machine precision: accept a
+    solve whose scaled residual is below 1e-11 either way, refuse the rest."""
+    res = float(np.abs(fun(sol.x)).max())
+    if not np.isfinite(res) or (not sol.success and res > 1e-11) or res > 1e-9:
+        raise RuntimeError('%s: root solve failed (%s, residual %.3e)' % (what, sol.message, res))
 
 
 def q_bgk_conservative(f, cv, cw, prefac, omega):
@@ -146,7 +156,9 @@ def esbgk_conservative_target(f, cv, cw, Pr):
                              - a[5] * C[2] ** 2 + a[6] * C[2] + a[7] * cv[0] * cv[1]
                              + a[8] * cv[0] * cv[2] + a[9] * cv[1] * cv[2])
     a0 = [rho / (np.pi * T) ** 1.5, 1 / T, 0, 1 / T, 0, 1 / T, 0, 0, 0, 0]
-    sol = root(lambda a: (phi @ G(a) - m) / sc, a0, method='hybr', options={'xtol': 1e-15})
+    fun = lambda a: (phi @ G(a) - m) / sc
+    sol = root(fun, a0, method='hybr', options={'xtol': 1e-15})
+    _check_root(sol, fun, 'esbgk_conservative_target')
     return G(sol.x), rho, T
 
 

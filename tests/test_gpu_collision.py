@@ -89,3 +89,52 @@ def test_collision_fp32(cuda, N, M, Nrho):
     err = _run(cuda, N, M, Nrho, _lib.F32, _lib.VHS, NODAL)
     print('fp32 nodal', N, err)
     assert err < 2e-5
+
+
+@pytest.mark.parametrize('N,M,Nrho', [(8, 6, 4), (16, 6, 8)])
+def test_collision_reference_form(cuda, N, M, Nrho):
+    """dgfs_collide_reference: FP64 -- equal to the direct Q(f, f) to 1e-12;
+    FP32 -- against the FP64 oracle on a state 1e-3 away from the reference
+    state, its error is the direct FP32 error scaled down by about |f - f0|/|f|
+    (the gain and the loss each exceed Q several times here)."""
+    import torch
+    from dgfs1d_gpu_amd import _lib
+    from dgfs1d_gpu_amd.collision import CollisionPlan
+    vm = _mesh(N, M, Nrho)
+    gamma = 2.0 * (1.0 - 0.81)
+    pre = OC.Precomp(N, vm.L, vm.qz, vm.qw, vm.sz, vm.sw, gamma)
+    prefac = 0.37
+    K, Ne = 2, 2
+    rng = np.random.default_rng(31)
+    u0 = np.stack([_field(vm, 40 + s, u=(0.2 - 0.1 * s, 0.05 * s, 0.0), T=0.9 + 0.05 * s)
+                   for s in range(K * Ne)])
+    u = u0 * (1.0 + 1e-3 * rng.uniform(-1, 1, u0.shape))
+    errs = {}
+    for dt in (_lib.F64, _lib.F32):
+        td = torch.float64 if dt == _lib.F64 else torch.float32
+        plan = CollisionPlan(N, M, Nrho, vm.L, vm.qz, vm.qw, vm.sz, vm.sw, gamma, prefac,
+                             dt, _lib.VHS, ntask_max=4, device=cuda)
+        plan64 = CollisionPlan(N, M, Nrho, vm.L, vm.qz, vm.qw, vm.sz, vm.sw, gamma, prefac,
+                               _lib.F64, _lib.VHS, ntask_max=4, device=cuda)
+        plan.set_tasks(K, Ne, NODAL)
+        plan64.set_tasks(K, Ne, NODAL)
+        ud = torch.tensor(u, dtype=td, device=cuda)
+        u0d = torch.tensor(u0, dtype=td, device=cuda)
+        Q0 = torch.zeros(u.shape, dtype=torch.float64, device=cuda)
+        plan64(u0d.double(), Q0)                       # Q(f0, f0) of the stored reference
+        Qr = torch.zeros_like(ud)
+        plan.reference(ud, u0d, Q0, Qr)
+        Qd = torch.zeros_like(ud)
+        plan(ud, Qd)
+        torch.cuda.synchronize()
+        ur = ud.double().cpu().numpy()                 # the operands as stored
+        ref = np.stack([OC.q_vhs(pre, ur[s], ur[s], prefac) for s in range(K * Ne)])
+        scale = np.abs(ref).max()
+        errs[dt] = (np.abs(Qr.double().cpu().numpy() - ref).max() / scale,
+                    np.abs(Qd.double().cpu().numpy() - ref).max() / scale)
+        plan.close()
+        plan64.close()
+    print('reference form', N, 'fp64 (ref, direct)', errs[_lib.F64], 'fp32', errs[_lib.F32])
+    assert errs[_lib.F64][0] < 1e-12
+    ref32, dir32 = errs[_lib.F32]
+    assert ref32 < 2e-8 and ref32 < dir32 / 30

@@ -3,10 +3,13 @@
 * One FP64 RHS of the full operator (transport, diffuse walls, 12 batched
   collisions, FP64 conservation projection) against the FP64 oracle at full
   size: 1e-11 relative.
-* The FP32 spectral path against the FP64 path after 10 SSP-RK2 steps from the
-  same state: moments n, u_y, T, P_xx, P_yy, P_xy within 1e-6 relative to the
-  field's maximum, and the collision output of the FP64 state conserving mass,
-  momentum and energy to 1e-12 after projection.
+* The FP32 spectral path (the deck's reference-state form, FP64 reference
+  collision refreshed every few RHS calls) against the FP64 path after the
+  deck's 52 SSP-RK2 steps from the same state: all nine moments (n, u_x, u_y,
+  T, q_x, q_y, P_xx, P_yy, P_xy) within 1e-6 relative to the field's maximum
+  (u_x, which stays near zero, relative to the wall speed), and the collision
+  output of the FP64 state conserving mass, momentum and energy to 1e-12
+  after projection.
 """
 import os
 
@@ -55,18 +58,21 @@ def test_prod_fp32_moments_and_conservation(cuda):
     from oracle import relax as OR
     s64 = _solver(cuda, 'double')
     s32 = _solver(cuda, 'float')
-    for _ in range(10):
+    assert s32.sm._ref is not None            # the deck runs the reference-state form
+    for _ in range(52):
         s64.step()
         s32.step()
     mom = Moments(s64.basis.interpMat, s64.K, s64.Ne, s64.vm, cuda)
     m64 = mom(s64.state_fp64()).reshape(-1, 9)
     m32 = mom(s32.state_fp64()).reshape(-1, 9)      # FP32 state (hi + lo), FP64 moments
     names = ('rho', 'ux', 'uy', 'T', 'qx', 'qy', 'Pxx', 'Pyy', 'Pxy')
+    uwall = 168.52743267214112 / s64.vm.u0()
     rel = {}
     for i, nm in enumerate(names):
-        rel[nm] = float(np.abs(m32[:, i] - m64[:, i]).max() / np.abs(m64[:, i]).max())
-    print('fp32 vs fp64 moments after 10 steps', rel)
-    for nm in ('rho', 'uy', 'T', 'Pxx', 'Pyy', 'Pxy'):
+        scale = uwall if nm == 'ux' else np.abs(m64[:, i]).max()
+        rel[nm] = float(np.abs(m32[:, i] - m64[:, i]).max() / scale)
+    print('fp32 vs fp64 moments after 52 steps', rel)
+    for nm in names:
         assert rel[nm] < 1e-6, nm
     Q = torch.zeros_like(s64.u)
     s64.sm.collide(s64.u, Q)

@@ -144,3 +144,87 @@ def test_axnpby_compensated(cuda):
     tot = (u.double() + ul.double()).cpu().numpy()
     assert np.abs(tot - (1.0 + 4000 * float(np.float32(1e-9)))).max() < 1e-10
     assert np.all(up.cpu().numpy() == 1.0)
+
+
+def _newton_model(cuda, td, N, target, Pr=0.6, check=True):
+    from dgfs1d_gpu_amd.relax import Shakov
+    cv, cw = _grid(N)
+    return Shakov(N ** 3, _dev(cv[0], td, cuda), _dev(cv[1], td, cuda), _dev(cv[2], td, cuda),
+                  cw, 1.0, 0.81, Pr, td, cuda, target=target, check=check), cv, cw
+
+
+@pytest.mark.parametrize('target', ['bgk', 'esbgk-conservative'])
+def test_newton_targets_node_by_node(cuda, target):
+    """'bgk-gll' (std/scattering.py:445-706) and 'esbgk-gll' (:752-1185): the
+    device Newton solves, with each task's sums spread over many work-groups,
+    against the oracle's MINPACK solves node by node (FP64, 12 tasks as the
+    Couette deck's K = 3, Ne = 4), and the moments of the device targets
+    against the conditions that define them: mass, momentum and energy of f
+    (bgk), mass, momentum and rho (u_i u_j + T_ij) (esbgk)."""
+    import torch
+    td = torch.float64
+    N, K, Ne, Pr = 16, 3, 4, 0.6
+    sh, cv, cw = _newton_model(cuda, td, N, target, Pr)
+    u = _state(cv, K, Ne, seed=21)
+    tasks = [(s, [(s, 1.0)]) for s in range(K * Ne)]
+    sh.set_tasks(K, Ne, [(o, f, f) for o, f in tasks])
+    Qd = torch.zeros(K * Ne * N ** 3, dtype=td, device=cuda)
+    sh(_dev(u, td, cuda), Qd)
+    assert not sh.status().any()
+    got = Qd.cpu().numpy().reshape(K * Ne, -1)
+    ur = u.reshape(K * Ne, -1)
+    pr = [(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (2, 0)]
+    worst, worst_m = 0.0, 0.0
+    for s in range(K * Ne):
+        f = ur[s]
+        if target == 'bgk':
+            tgt, rho, T = OR.bgk_conservative_target(f, cv, cw)
+        else:
+            tgt, rho, T = OR.esbgk_conservative_target(f, cv, cw, Pr)
+        rate = rho / T ** (0.81 - 1.0)
+        ref = rate * (tgt - f)
+        worst = max(worst, np.abs(got[s] - ref).max() / np.abs(ref).max())
+        G = f + got[s] / rate                       # the device's target
+        uu = np.array([(f * cv[i]).sum() for i in range(3)]) / f.sum()
+        if target == 'bgk':
+            phi = np.vstack([np.ones(cv.shape[1]), cv, (cv * cv).sum(0)])
+            want = phi @ f * cw
+            sc = np.array([rho] + [rho * np.sqrt(T)] * 3 + [rho * T])
+        else:
+            C = cv - uu[:, None]
+            fe, _, _ = OR.bgk_conservative_target(f, cv, cw)
+            w = (1.0 - 1.0 / Pr) * f + (1.0 / Pr) * fe
+            phi = np.vstack([np.ones(cv.shape[1]), cv] + [cv[i] * cv[j] for i, j in pr])
+            Tij = [(C[i] * C[j] * w).sum() * cw / rho for i, j in pr]
+            want = np.array([rho, *(rho * uu)] +
+                            [rho * (uu[i] * uu[j] + Tij[k]) for k, (i, j) in enumerate(pr)])
+            sc = np.array([rho] + [rho * np.sqrt(T)] * 3 + [rho * T] * 6)
+        worst_m = max(worst_m, float((np.abs(phi @ G * cw - want) / sc).max()))
+    print(target, 'Q vs oracle', worst, 'target moments', worst_m)
+    assert worst < 1e-10
+    assert worst_m < 1e-11
+
+
+@pytest.mark.parametrize('target', ['bgk', 'esbgk-conservative'])
+def test_newton_nan_raises(cuda, target):
+    """A task whose slice is zero has no temperature: its Newton solve meets a
+    NaN, the status says so and the call raises, as the reference's
+    RuntimeError('NaN encountered') (std/scattering.py:689, :1084).  The other
+    tasks of the same call are solved as usual."""
+    import torch
+    td = torch.float64
+    N, K, Ne = 8, 1, 3
+    sh, cv, cw = _newton_model(cuda, td, N, target, check=False)
+    u = _state(cv, K, Ne, seed=5)
+    u[0, 1] = 0.0
+    sh.set_tasks(K, Ne, [(s, [(s, 1.0)], [(s, 1.0)]) for s in range(Ne)])
+    Qd = torch.zeros(K * Ne * N ** 3, dtype=td, device=cuda)
+    sh(_dev(u, td, cuda), Qd)
+    st = sh.status()
+    print(target, 'status', st)
+    assert st[1] & 1 and st[0] == 0 and st[2] == 0
+    with pytest.raises(RuntimeError, match='NaN encountered'):
+        sh.check_status()
+    sh.check = True
+    with pytest.raises(RuntimeError, match='NaN encountered'):
+        sh(_dev(u, td, cuda), Qd)

@@ -143,3 +143,25 @@ def test_bkw_closed_form_pins_normalisation():
                 assert np.abs(qs - dfdt).max() > 0.5 * np.abs(dfdt).max()
         errs.append(e)
     assert errs[1] < 5e-4 and errs[1] < errs[0] / 10
+
+
+def test_reference_state_form_is_exact():
+    """The reference-state form Q(f0, f0) + G(h, f + f0) - h L[f] - f0 L[h]
+    (h = f - f0) reproduces Q(f, f) for any f0, also far from f: it rests only
+    on the gain being a symmetric bilinear form, which the brute-force sum
+    pins above.  A sign slip on either loss correction (or taking L[s] for
+    L[f]) breaks it by O(|h| |L|)."""
+    vm, pre = _setup(8, 6, 4)
+    f = _f(vm, (0.3, 0.1, 0), 0.9, 3)
+    for f0 in (_f(vm, (0.25, 0.1, 0), 0.95, 4), _f(vm, (-0.4, 0.3, 0.1), 1.3, 5)):
+        Q0 = OC.q_vhs(pre, f0, f0, 0.7)
+        ref = OC.q_vhs(pre, f, f, 0.7)
+        got = OC.q_reference(pre, f, f0, Q0, 0.7)
+        assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max()
+        h = f - f0
+        bad = Q0 + OC.q_vhs(pre, h, f + f0, 0.7) + 0.7 * (h * OC.loss_field(pre, f0)
+                                                         + f0 * OC.loss_field(pre, h))
+        assert np.abs(bad - ref).max() > 1e-6 * np.abs(ref).max()
+    # with f0 = f the form returns Q0 exactly
+    Qf = OC.q_vhs(pre, f, f, 0.7)
+    assert np.array_equal(OC.q_reference(pre, f, f.copy(), Qf, 0.7), Qf)
