@@ -371,8 +371,8 @@ def main(argv=None):
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-accuracy', action='store_true',
                     help='skip the FP64 comparison run after the timed region')
-    ap.add_argument('--batch', action='store_true',
-                    help='also run the seeded 64-problem uneven batch (autotuned, graph-captured)')
+    ap.add_argument('--no-batch', action='store_true',
+                    help='skip the seeded 64-problem uneven batch (autotuned, graph-captured)')
     ap.add_argument('--batch-seed', type=int, default=0)
     ap.add_argument('--no-steady', action='store_true',
                     help='skip the time-to-steady-state comparison (learned closure vs '
@@ -494,9 +494,9 @@ def main(argv=None):
     }
     if not args.no_accuracy:
         out['accuracy'] = fp64_accuracy(text, args.warmup + args.steps, s, dev)
-    if args.batch or not args.no_steady or not args.no_bi:
+    if not args.no_batch or not args.no_steady or not args.no_bi:
         del s
-    if args.batch:
+    if not args.no_batch:
         out['batched'] = batched_workload(args.batch_seed, 64, 20, args.warmup, dev)
     if not args.no_steady:
         out['time_to_steady_state'] = steady_state(dev)

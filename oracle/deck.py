@@ -114,6 +114,20 @@ class Deck:
             return OS.Wall('inlet', nl, vm.cv, vm.cw, f0=f0)
         if kind == 'dgfs-vanish':
             return OS.Wall('vanish', nl, vm.cv, vm.cw)
+        if kind == 'dgfs-inlet-nondim':                    # std/bc.py:269-324
+            ux = g('ux')
+            f0 = OS.maxwellian(vm.cv, g('rho'), ux, 0.0, 0.0, g('T'))
+            return OS.Wall('inlet', nl, vm.cv, vm.cw, f0=f0, ushift=ux)
+        if kind == 'dgfs-wall-expr-diffuse':               # std/bc.py:136-198
+            x = self.xmesh[0] if nl < 0 else self.xmesh[-1]
+            ex = [cp.get(sect, k) for k in ('ux', 'uy', 'uz', 'T')]
+            scale = (vm.u0, vm.u0, vm.u0, vm.T0)
+
+            def state(t, ex=ex, x=x, scale=scale):
+                env = {'__builtins__': {}, 't': t, 'x': x, 'pi': np.pi, 'sin': np.sin,
+                       'cos': np.cos, 'exp': np.exp, 'tanh': np.tanh, 'sqrt': np.sqrt}
+                return tuple(float(eval(e, env)) / sc for e, sc in zip(ex, scale))
+            return OS.Wall('diffuse-expr', nl, vm.cv, vm.cw, state=state)
         raise ValueError('oracle deck: unsupported boundary ' + kind)
 
     def step(self, t, u):

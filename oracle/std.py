@@ -27,15 +27,29 @@ def maxwellian(cv, rho, ux, uy, uz, T):
 
 
 class Wall:
-    """kind: 'copy' | 'diffuse' | 'inlet' | 'vanish' (bc.py)."""
+    """kind: 'copy' | 'diffuse' | 'diffuse-expr' | 'inlet' | 'vanish' (bc.py).
+    'diffuse-expr' (std/bc.py:136-198, dgfs-wall-expr-diffuse.mako): wall
+    velocity and temperature are functions of time (``state(t)`` returns
+    non-dimensional (ux, uy, uz, T)); the wall distribution is the
+    unnormalised exp(-|c - u|^2 / T), the density factor coming from the
+    zero-net-mass-flux condition as for the fixed wall."""
 
-    def __init__(self, kind, nl, cv, cw, f0=None, ushift=0.0):
+    def __init__(self, kind, nl, cv, cw, f0=None, ushift=0.0, state=None):
         self.kind, self.nl, self.cv, self.cw = kind, nl, cv, cw
-        self.f0, self.ushift = f0, ushift
+        self.f0, self.ushift, self.state = f0, ushift, state
         self.nw = None
 
-    def apply(self, ul):
+    def apply(self, ul, t=0.0):
         c = self.cv[0]
+        if self.kind == 'diffuse-expr':                             # mako:14-26, :40-52
+            ux, uy, uz, T = self.state(t)
+            u = np.array([ux, uy, uz]).reshape(3, 1)
+            mx = np.exp(-((self.cv - u) ** 2).sum(0) / T)
+            fac = (c - ux) * self.nl
+            num = np.sum(np.where(fac >= 0, fac * ul * self.cw, 0.0))
+            den = np.sum(np.where(fac < 0, fac * mx * self.cw, 0.0))
+            self.nw = -num / den
+            return np.where(fac < 0, mx * self.nw, ul)
         fac = (c - self.ushift) * self.nl
         if self.kind == 'copy':
             return ul.copy()
@@ -76,8 +90,8 @@ class StdOracle:
         mapL[0], mapR[-1] = 0, Ne * Nqf - 1
         uL, uR = usolF[mapL].copy(), usolF[mapR].copy()                 # extract_left/right
         bcL, bcR = uL[0].copy(), uR[Ne].copy()                          # transfer_bc_*
-        uL[0] = self.wl.apply(bcL)                                      # update/apply/insert
-        uR[Ne] = self.wr.apply(bcR)
+        uL[0] = self.wl.apply(bcL, t)                                      # update/apply/insert
+        uR[Ne] = self.wr.apply(bcR, t)
         jL = 0.5 * np.abs(c) * (uL - uR) + 0.5 * c * (uR - uL)          # flux
         jR = 0.5 * np.abs(c) * (uL - uR) + 0.5 * c * (uL - uR)
         ux = (self.b.deriv @ u.reshape(K, -1)).reshape(K, Ne, n)        # deriv
@@ -102,8 +116,8 @@ class StdOracle:
         mapL[0], mapR[-1] = 0, Ne * Nqf - 1
         uL, uR = usolF[mapL].copy(), usolF[mapR].copy()
         bcL, bcR = uL[0].copy(), uR[Ne].copy()
-        uL[0] = self.wl.apply(bcL)
-        uR[Ne] = self.wr.apply(bcR)
+        uL[0] = self.wl.apply(bcL, t)
+        uR[Ne] = self.wr.apply(bcR, t)
         cp, cm = np.maximum(c, 0.0), np.minimum(c, 0.0)
         jL = cm * (uR - uL)                                              # uflux
         jR = -cp * (uR - uL)
@@ -141,8 +155,8 @@ class StdOracle:
         mapL[0], mapR[-1] = 0, Ne * Nqf - 1
         uL, uR = usolF[mapL].copy(), usolF[mapR].copy()
         bcL, bcR = uL[0].copy(), uR[Ne].copy()
-        uL[0] = self.wl.apply(bcL)
-        uR[Ne] = self.wr.apply(bcR)
+        uL[0] = self.wl.apply(bcL, t)
+        uR[Ne] = self.wr.apply(bcR, t)
         jL = 0.5 * np.abs(c) * (uL - uR) + 0.5 * c * (uR - uL)
         jR = 0.5 * np.abs(c) * (uL - uR) + 0.5 * c * (uL - uR)
         ux = np.zeros_like(u)

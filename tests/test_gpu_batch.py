@@ -71,3 +71,77 @@ def test_autotune_keeps_results(cuda):
         g['plan'](b.u[g['lo']:g['hi']], got[g['lo']:g['hi']])
     err = float((got - ref).abs().max() / ref.abs().max())
     assert err < 1e-12
+
+
+# the seeded workload's grids reduced so that the FP64 oracle is cheap
+REDUCED_N = {16: 6, 24: 8, 32: 12}
+
+
+def test_seeded_batch_groups_vs_oracle(cuda):
+    """All 64 problems of the seeded uneven workload (make_problems(0, 64):
+    Couette / Fourier, Ne in {2, 4, 8}, K in {3, 5, 7}), each on a reduced
+    velocity grid (N 16/24/32 -> 6/8/12, N_rho = N/2, M = 12), batched in FP64:
+    one RHS of the whole batch from a perturbed state, every problem against
+    the oracle's RHS of the same deck (1e-11 of max|RHS|)."""
+    import torch
+    from dgfs1d_gpu_amd.batch import BatchedStd1D, make_problems, problem_deck
+    from oracle.deck import Deck
+    specs = [dict(p, N=REDUCED_N[p['N']]) for p in make_problems(0, 64)]
+    b = BatchedStd1D(specs, cuda, precision='double', dt=2e-4)
+    rng = np.random.default_rng(17)
+    u = b.u.cpu().numpy() * (1.0 + 0.05 * rng.uniform(-1, 1, b.total))
+    b.u.copy_(torch.tensor(u, dtype=torch.float64, device=cuda))
+    out = torch.zeros_like(b.u)
+    b.rhs(0.0, b.u, out)
+    got = out.cpu().numpy()
+    worst = 0.0
+    seen = set()
+    for k, p in enumerate(b.specs):
+        d = Deck(problem_deck(p['kind'], p['Ne'], p['K'], p['N'], p['uw'], p['Tl'], p['Tr'],
+                              precision='double', dt=2e-4))
+        lo, hi = int(b.offsets[k]), int(b.offsets[k + 1])
+        ref = d.op.rhs(0.0, u[lo:hi].reshape(d.u0.shape))
+        err = float(np.abs(got[lo:hi] - ref.ravel()).max() / np.abs(ref).max())
+        worst = max(worst, err)
+        seen.add((p['N'], p['K'], p['Ne']))
+    print('64 problems, %d (N, K, Ne) groups, worst RHS error %.3e' % (len(seen), worst))
+    assert worst < 1e-11
+
+
+def test_seeded_batch_full_size(cuda):
+    """The seeded workload at its real sizes (27.13 M phase-space points,
+    FP32, autotuned plans): a captured step replayed equals an eager step
+    from the same state bit for bit, the state stays finite and every
+    problem's mass (diffuse walls carry no net mass flux) stays within 1e-5
+    over 3 steps."""
+    import torch
+    from dgfs1d_gpu_amd.batch import BatchedStd1D, make_problems
+    b = BatchedStd1D(make_problems(0, 64), cuda, precision='float')
+    assert b.points > 2.7e7
+    b.autotune(candidates=(1, 2), reps=1)
+
+    def masses():
+        out = []
+        for k, s in enumerate(b.solvers):
+            x = b.solver_state(k).double().cpu().numpy().reshape(s.K, s.Ne, -1).sum(-1)
+            jac = np.asarray(s.mesh.jac).reshape(-1)          # nodal-sem-gll: GLL weights
+            out.append(float((x * s.basis.w[:, None] * jac[None, :]).sum()))
+        return np.array(out)
+    m0 = masses()
+    b.capture(warmup=1)
+    u0 = b.u.clone()
+    t0 = b.time
+    b.graph.replay()
+    torch.cuda.synchronize()
+    g = b.u.clone()
+    b.u.copy_(u0)
+    b.odes.integrate(t0, b.dt, b.u)
+    torch.cuda.synchronize()
+    assert torch.equal(g, b.u)
+    b.time = t0 + b.dt
+    b.step()
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(b.u).all())
+    drift = np.abs(masses() - m0) / np.abs(m0)
+    print('points', b.points, 'mass drift over 3 steps', drift.max())
+    assert drift.max() < 1e-5

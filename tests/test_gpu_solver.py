@@ -109,7 +109,10 @@ def _rel(a, b):
 
 @pytest.mark.parametrize('kind,K', [('nodal-sem-gll', 3), ('nodal-sem-gll', 4), ('nodal-gll', 3),
                                     ('modal-orthonormal-legendre', 3),
-                                    ('modal-modified-karniadakis', 3), ('nodal-sem-gll-2', 3)])
+                                    ('modal-modified-karniadakis', 3), ('nodal-sem-gll-2', 3),
+                                    ('nodal-sem-gll', 5), ('nodal-sem-gll', 7),
+                                    ('nodal-gll', 5), ('modal-orthonormal-legendre', 5),
+                                    ('modal-modified-karniadakis', 5)])
 def test_rhs_bases(cuda, kind, K):
     import torch
     deck, solver, u = _pair(cuda, {('basis', 'kind'): kind, ('basis', 'K'): K})
@@ -412,3 +415,53 @@ def test_maxwell_gll_bkw_relaxation(cuda):
     e_cf = float(np.abs(u - ref).max() / np.abs(ref).max())
     print('bkw run: oracle %.2e closed form %.2e' % (e_or, e_cf), flush=True)
     assert e_or < 1e-11 and e_cf < 1e-4
+
+
+EXPR_WALLS = {
+    ('soln-bcs-xlo', 'type'): 'dgfs-wall-expr-diffuse',
+    ('soln-bcs-xlo', 'ux'): '0',
+    ('soln-bcs-xlo', 'uy'): '-168.5*sin(2*pi*t/0.004)',
+    ('soln-bcs-xlo', 'uz'): '20*t/0.004',
+    ('soln-bcs-xlo', 'T'): '273*(1 + 0.2*t/0.004)',
+    ('soln-bcs-xhi', 'type'): 'dgfs-inlet-nondim',
+    ('soln-bcs-xhi', 'rho'): 1.1,
+    ('soln-bcs-xhi', 'ux'): -0.3,
+    ('soln-bcs-xhi', 'T'): 0.9,
+}
+
+
+@pytest.mark.parametrize('t', [0.0, 0.0013, 0.0031])
+@pytest.mark.parametrize('model', ['vhs', 'vhs-gll'])
+def test_rhs_expr_wall_and_nondim_inlet(cuda, t, model):
+    """'dgfs-wall-expr-diffuse' (std/bc.py:136-198: wall velocity and
+    temperature as expressions of t, evaluated at the stage time) on the left
+    and 'dgfs-inlet-nondim' (:269-324: a non-dimensional Maxwellian inflow
+    moving at ux) on the right, in the full RHS at three times, against the
+    oracle."""
+    import torch
+    over = dict(EXPR_WALLS)
+    over[('scattering-model', 'type')] = model
+    deck, solver, u = _pair(cuda, over)
+    out = torch.zeros_like(solver.u)
+    solver.rhs(t, solver.u, out)
+    ref = deck.op.rhs(t, u)
+    err = _rel(out.cpu().numpy().reshape(ref.shape), ref)
+    print('expr wall / nondim inlet', model, t, err)
+    assert err < 1e-11
+
+
+def test_expr_wall_steps(cuda):
+    """Three SSP-RK3 steps with the time-dependent wall: the stage times reach
+    the boundary (t, t + dt, t + dt/2), so the trajectory matches the oracle's
+    only if every stage evaluates the expressions at its own time."""
+    over = dict(EXPR_WALLS)
+    over[('time-integrator', 'scheme')] = 'ssp-rk3'
+    deck, solver, u = _pair(cuda, over)
+    t = 0.0
+    for _ in range(3):
+        u = OS.integrate(deck.op, 'ssp-rk3', t, deck.dt, u)
+        t += deck.dt
+        solver.step()
+    err = _rel(solver.u.cpu().numpy().reshape(u.shape), u)
+    print('expr wall ssp-rk3 3 steps', err)
+    assert err < 1e-11

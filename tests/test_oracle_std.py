@@ -71,6 +71,31 @@ def test_diffuse_wall_has_no_net_mass_flux():
         assert abs(flux) < 1e-13 * (np.abs(c) * face).sum() * vm.cw
 
 
+def test_expression_wall_matches_fixed_wall_and_carries_no_mass():
+    """The expression wall (std/bc.py:136-198) uses the unnormalised
+    exp(-|c - u(t)|^2 / T(t)); its density factor from the zero-mass-flux
+    condition makes the ghost state independent of that normalisation: with
+    constant expressions it equals the fixed diffuse wall's ghost, and at any
+    time its net mass flux vanishes."""
+    d = _deck()
+    vm = d.vm
+    c = vm.cv[0]
+    rng = np.random.default_rng(1)
+    interior = OS.maxwellian(vm.cv, 1.0, 0.1, 0.2, 0.0, 1.2) * rng.uniform(0.9, 1.1, vm.vsize)
+    for nl in (-1.0, 1.0):
+        fixed = OS.Wall('diffuse', nl, vm.cv, vm.cw,
+                        f0=OS.maxwellian(vm.cv, 1.0, 0.0, 0.3, 0.0, 0.95))
+        expr = OS.Wall('diffuse-expr', nl, vm.cv, vm.cw, state=lambda t: (0.0, 0.3, 0.0, 0.95))
+        g1, g2 = fixed.apply(interior), expr.apply(interior, 0.7)
+        assert np.abs(g1 - g2).max() < 1e-14 * np.abs(g1).max()
+        moving = OS.Wall('diffuse-expr', nl, vm.cv, vm.cw,
+                         state=lambda t: (0.0, np.sin(t), 0.1 * t, 1.0 + 0.2 * t))
+        for t in (0.0, 0.4, 1.3):
+            ghost = moving.apply(interior, t)
+            face = np.where((c * nl) > 0, interior, ghost)
+            assert abs((c * face).sum()) < 1e-13 * (np.abs(c) * face).sum()
+
+
 class _Lin:
     def __init__(self, lam):
         self.lam = lam
