@@ -210,7 +210,10 @@ def steady_state(dev, tau=1e-6, tau2=1e-5, max_steps=6000):
     shock (Ne=32, K=3, N=32^3, M=12, N_rho=16, FP32, SSP-RK2 dt=5e-4), plain
     spectral versus the learned closure (100-call spectral warm start, checks
     every 10 calls on 25 % of the nodes, tolerance 0.05, the network
-    re-evaluated every 4th RHS call and its output reused in between)."""
+    re-evaluated every 4th RHS call and its output reused in between), and
+    the same closure schedule without the network (Q_Shakov + the lagged
+    spectral correction: closure-use-net = false), the method the network is
+    measured against."""
     from dgfs1d_gpu_amd.closure_train import shock_deck
     from dgfs1d_gpu_amd.steady import run_to_steady
     weights = os.path.join(ROOT, 'dgfs1d_gpu_amd', 'data', 'closure_hs_n32.npz')
@@ -220,7 +223,9 @@ def steady_state(dev, tau=1e-6, tau2=1e-5, max_steps=6000):
     out = {'workload': 'Mach-2 hard-sphere normal shock Ne=32 K=3 N=32^3 M=12 Nrho=16 FP32 '
                        'SSP-RK2 dt=5e-4', 'tau': tau, 'tau2': tau2}
     for tag, text in (('spectral', shock_deck(mach=2.0)),
-                      ('learned', shock_deck(mach=2.0, extra=extra))):
+                      ('learned', shock_deck(mach=2.0, extra=extra)),
+                      ('no_network', shock_deck(mach=2.0,
+                                                extra=extra + 'closure-use-net = false\n'))):
         s, rep, _ = run_to_steady(text, [tau, tau2], max_steps, 10, dev)
         hit = rep['reached'].get('%g' % tau)
         hit2 = rep['reached'].get('%g' % tau2)
@@ -229,7 +234,7 @@ def steady_state(dev, tau=1e-6, tau2=1e-5, max_steps=6000):
                     'wall_s_tau2': hit2['wall_s'] if hit2 else None,
                     'steps_tau2': hit2['steps'] if hit2 else None,
                     'ms_per_step': rep['ms_per_step']}
-        if tag == 'learned':
+        if tag != 'spectral':
             cl = s.sm.closure
             out[tag].update(nodes_learned=int(cl.learned.sum()), nodes=int(cl.ns),
                             fallbacks=int(cl.fallbacks), readmissions=int(cl.readmissions),
@@ -239,6 +244,8 @@ def steady_state(dev, tau=1e-6, tau2=1e-5, max_steps=6000):
         out['speedup'] = out['spectral']['wall_s'] / out['learned']['wall_s']
     if out['spectral']['wall_s_tau2'] and out['learned']['wall_s_tau2']:
         out['speedup_tau2'] = out['spectral']['wall_s_tau2'] / out['learned']['wall_s_tau2']
+    if out['spectral']['wall_s'] and out['no_network']['wall_s']:
+        out['speedup_no_network'] = out['spectral']['wall_s'] / out['no_network']['wall_s']
     return out
 
 
@@ -332,16 +339,24 @@ def bi_couette_published(dev, cases=((72, 3, 20),), steps=52):
     return out
 
 
-def bi_fourier_n48(dev, steps=3, warmup=1):
+def bi_fourier_n48(dev, steps=3, warmup=1, scheme=None):
     """The largest mixture grid (configs/bi_fourier_n48.ini: Ne=8, modal K=5,
     N=48^3, M=24, N_rho=24, Mpre=192, FP32 collisions with the FP64 mixture
-    projection, SSP-RK2): set-up time and ms per step."""
+    projection, IMEX-BDF2 with species-wise Shakov penalties; scheme = an
+    explicit kind such as ssp-rk2 runs the explicit driver instead): set-up
+    time (including the IMEX start-up history) and ms per step."""
+    from dgfs1d_gpu_amd.bi.abi import DGFSBiAstd1D
     from dgfs1d_gpu_amd.bi.bi import DGFSBi1D
     from dgfs1d_gpu_amd.dictionary import Dictionary
     import torch
     text = open(os.path.join(ROOT, 'dgfs1d_gpu_amd', 'configs', 'bi_fourier_n48.ini')).read()
+    cfg = Dictionary(text)
+    if scheme:
+        cfg.set('time-integrator', 'scheme', scheme)
+    imex = cfg.lookup('time-integrator', 'scheme').startswith(('imex', 'ars', 'sspl', 'bdf',
+                                                                'boltz'))
     t0 = time.perf_counter()
-    s = DGFSBi1D(Dictionary(text), device=dev, writers=False)
+    s = (DGFSBiAstd1D if imex else DGFSBi1D)(cfg, device=dev, writers=False)
     torch.cuda.synchronize()
     setup = time.perf_counter() - t0
     s.run(warmup)
@@ -352,7 +367,7 @@ def bi_fourier_n48(dev, steps=3, warmup=1):
     wall = time.perf_counter() - t0
     K, Ne, n = s.K, s.Ne, s.Nv
     out = {'workload': 'binary VSS Fourier Ne=8 modal K=5 N=48^3 M=24 Nrho=24 Mpre=192 FP32 '
-                       '+ FP64 mixture projection, SSP-RK2',
+                       '+ FP64 mixture projection, %s' % cfg.lookup('time-integrator', 'scheme'),
            'tasks_per_pair': s.sm.tasks['ntask'], 'setup_s': setup, 'steps': steps,
            'ms_per_step': wall / steps * 1e3,
            'point_steps_per_s': Ne * K * n * steps / wall,
@@ -383,6 +398,8 @@ def main(argv=None):
                     help='only run the binary-mixture Couette benchmark at these phase spaces')
     ap.add_argument('--bi-n48', type=int, default=0, metavar='STEPS',
                     help='only time STEPS steps of the N=48 mixture deck (configs/bi_fourier_n48.ini)')
+    ap.add_argument('--bi-n48-scheme', default=None,
+                    help='time scheme of --bi-n48 (default: the deck\'s imex-lm-bdf2)')
     ap.add_argument('--closure-forward', type=int, default=0, metavar='NODES',
                     help='only time the learned-closure network forward on NODES slices of 32^3')
     ap.add_argument('--relax-models', type=int, default=0, metavar='REPS',
@@ -399,7 +416,8 @@ def main(argv=None):
 
     import torch
     if args.bi_n48:
-        print(json.dumps(bi_fourier_n48(torch.device('cuda:0'), args.bi_n48)))
+        print(json.dumps(bi_fourier_n48(torch.device('cuda:0'), args.bi_n48,
+                                        scheme=args.bi_n48_scheme)))
         return 0
     if args.bi_only:
         cases = [tuple(int(x) for x in c.split('/')) for c in args.bi_only.split(',')]

@@ -62,6 +62,7 @@ class DGFSShakovDirectGLLScatteringModelAstd(DGFSScatteringModelAstd):
         self.dt_code = _lib.dtype_code(self.tdtype)
         self.Nv = vm.vsize()
         self.cx, self.cy, self.cz = vm.d_cvx(), vm.d_cvy(), vm.d_cvz()
+        self.cw = float(vm.cw())
         self._work = None
 
     # ------------------------------------------------------------- helpers
@@ -90,7 +91,7 @@ class DGFSShakovDirectGLLScatteringModelAstd(DGFSScatteringModelAstd):
                                           device=self._device)
         _lib.check(_lib.lib().dgfs_mom14(
             self.dt_code, ns, self.Nv, _lib.ptr(f), _lib.ptr(self.cx), _lib.ptr(self.cy),
-            _lib.ptr(self.cz), float(self.vm.cw()), _lib.ptr(self._work), _lib.ptr(U),
+            _lib.ptr(self.cz), self.cw, _lib.ptr(self._work), _lib.ptr(U),
             _lib.stream_ptr(stream)))
 
     def constructMaxwellian(self, t, U, M, stream=None):

@@ -94,7 +94,9 @@ class DGFSBi1D:
             self._qc = [torch.zeros(n, dtype=self.tdtype, device=self.device)
                         for _ in range(ns)]
             self._pending = None
-        self.us = [torch.zeros(n, dtype=self.tdtype, device=self.device) for _ in range(ns)]
+        # both species in one buffer (species-major); us[p] are views of it
+        self.u = torch.zeros(ns * n, dtype=self.tdtype, device=self.device)
+        self.us = list(self.u.view(ns, n))
         self.ux = torch.zeros(n, dtype=self.tdtype, device=self.device)
         if restart is None:
             usol = np.empty((Nq, Ne, Nv))
@@ -112,9 +114,7 @@ class DGFSBi1D:
         self.uprevs = [u.clone() for u in self.us]
         self.nacptsteps = 0
 
-        odescls = subclass_where(DGFSIntegratorBi,
-                                 intg_kind=cfg.lookup('time-integrator', 'scheme'))
-        self.odes = odescls(self.rhs, (K, Ne, Nv), self.tdtype, ns, device=self.device)
+        self.odes = self._make_integrator(cfg, (K, Ne, Nv))
 
         self.residual = self.moments = self.distribution = None
         if writers:
@@ -125,6 +125,11 @@ class DGFSBi1D:
                                                cfg, 'dgfsmomwriter')
             if cfg.has_section('dgfsdistwriter'):
                 self.distribution = DGFSDistributionBi(ti, (K, Ne, Nv), cfg, 'dgfsdistwriter')
+
+    def _make_integrator(self, cfg, shape):
+        odescls = subclass_where(DGFSIntegratorBi,
+                                 intg_kind=cfg.lookup('time-integrator', 'scheme'))
+        return odescls(self.rhs, shape, self.tdtype, self.nspcs, device=self.device)
 
     def _collide(self, p, us_in, Q):
         if self.projection is None:
@@ -154,8 +159,11 @@ class DGFSBi1D:
             self._collide(p, us_in, self.ux)
             self.invmass(self.ux, u_out)
 
-    def step(self):
+    def _advance(self):
         self.odes.integrate(self.time, self.dt, self.us)
+
+    def step(self):
+        self._advance()
         self.time += self.dt
         self.nacptsteps += 1
         if self.residual is not None:

@@ -102,3 +102,68 @@ class BiDeck:
 
     def step(self, t, us):
         return OB.integrate_bi(self.op, self.scheme, t, self.dt, us)
+
+    # --------------------------------------------------------- IMEX path
+    def imex_parts(self):
+        """(explicit, boltz, MixPenal, to_points, to_coeffs) of the IMEX mixture
+        on the species-major stack F of shape (2 K Ne, n): explicit =
+        invMass(transport_p(f_p)), boltz = invMass(sum_q Q_pq) (with the deck's
+        projection), and the species' Shakov penalisations
+        ([penalized-scattering-model] Pr).  On a modal basis the stack holds
+        the values at the K Gauss-Legendre points of each element (the modes
+        evaluated there with numpy.polynomial.legendre); to_points/to_coeffs
+        map coefficient stacks to that state and back (identity on a nodal
+        basis)."""
+        from . import imex as OI
+        from .velocitymesh import NA
+        op, vm, cp = self.op, self.vm, self.cp
+        b, K, Ne, n = op.b, op.K, op.Ne, vm.vsize
+        kind = cp.get('basis', 'kind')
+        if kind.startswith('nodal'):
+            V = np.eye(K)
+        elif kind == 'modal-orthonormal-legendre':
+            from numpy.polynomial import legendre
+            z, _ = legendre.leggauss(K)
+            V = np.stack([np.sqrt(i + 0.5) * legendre.legval(z, np.eye(K)[i])
+                          for i in range(K)]).T                         # V[q, m]
+        else:
+            raise ValueError('oracle IMEX mixture: unsupported basis ' + kind)
+        Vi = np.linalg.inv(V)
+        amap = lambda A, F: np.concatenate(
+            [(A @ F[p * K * Ne:(p + 1) * K * Ne].reshape(K, -1)).reshape(K * Ne, n)
+             for p in range(2)])
+        to_points = lambda F: amap(V, F)
+        to_coeffs = lambda F: amap(Vi, F)
+        split = lambda F: [F[p * K * Ne:(p + 1) * K * Ne].reshape(K, Ne, n)
+                           for p in range(2)]
+        im = lambda x: (b.invmass @ x.reshape(K, -1)).reshape(K * Ne, n)
+
+        def explicit(t, W):
+            us = split(to_coeffs(W))
+            return to_points(np.concatenate([im(op.tr[p].transport(t, us[p]))
+                                             for p in range(2)]))
+
+        def boltz(t, W):
+            us = split(to_coeffs(W))
+            c = [op.collision(p, us) for p in range(2)]
+            if op.project is not None:
+                from .relax import project_mixture
+                cv, m0, m1 = op.project
+                for m in range(K):
+                    for e in range(Ne):
+                        c[0][m, e], c[1][m, e] = project_mixture(c[0][m, e], c[1][m, e], cv,
+                                                                 m0, m1)
+            return to_points(np.concatenate([im(c[p]) for p in range(2)]))
+
+        Pr = float(cp.get('penalized-scattering-model', 'Pr')) \
+            if cp.has_option('penalized-scattering-model', 'Pr') else 2.0 / 3.0
+        g = lambda k: float(cp.get('scattering-model', k))
+        sps = []
+        for p in range(2):
+            pp = '%d%d' % (p, p)
+            mm = float(cp.get('velocity-mesh', 'molarMass%d' % (p + 1)))
+            pre = OI.species_prefactor(float(cp.get('mesh', 'H0')), vm.u0, vm.n0, vm.T0, mm,
+                                       NA, g('alpha' + pp), g('omega' + pp), g('dRef' + pp),
+                                       g('Tref' + pp))
+            sps.append(OI.SpeciesPenal(vm.cv, vm.cw, vm.masses[p], pre, g('omega' + pp), Pr))
+        return explicit, boltz, OI.MixPenal(sps, K * Ne), to_points, to_coeffs

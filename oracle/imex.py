@@ -429,3 +429,114 @@ class BDFIntegrator:
         Un = bdf_moments(dt, self.A, self.G, self.B, U, LU, pm.prefac, pm.omega, pm.Pr, pm.srate)
         Mn = pm.target(Un)
         return bdf_dist(dt, self.A, self.G, self.B, F, L, Mn, Un, pm.prefac, pm.omega)
+
+
+# --------------------------------------------- binary mixture (no reference)
+# The reference's IMEX path is single-species; the mixture's species-wise
+# Shakov penalisation is this build's (dgfs1d_gpu_amd/bi/penalty.py).  Here it
+# is written in each species' physical variables: for species mass m and raw
+# velocity moments of f on the common grid (cw),
+#   n = sum f cw, u = sum f c cw / n, T = (2/3) m sum f |c-u|^2 cw / n,
+#   q_c = sum f |c-u|^2 (c-u) cw,
+#   S = n (m/(pi T))^1.5 exp(-m C^2/T) (1 + 4(1-Pr)/5 m^2 (q_c.C)/(n T^2) (m C^2/T - 5/2)),
+#   nu = prefac n T^(1-omega).
+# The linear-multistep and ARS moment updates act on the species' mass-scaled
+# moment vector D_m U (the moments of the unit-mass problem in c' = sqrt(m) c),
+# D_m = m^1.5 diag(1, m^.5 (x3), m, m (x6), m^1.5 (x3)).
+
+
+def mass_scale(m):
+    return m ** 1.5 * np.array([1.0] + [m ** 0.5] * 3 + [m] + [m] * 6 + [m ** 1.5] * 3)
+
+
+def species_target(U, cv, m, Pr):
+    """Shakov target of species mass m from its mass-scaled moments U."""
+    Ur = U / mass_scale(m)[None, :]                     # raw moments on the grid
+    n = Ur[:, 0]
+    u = Ur[:, 1:4] / n[:, None]
+    C = cv[None, :, :] - u[:, :, None]
+    C2 = (C * C).sum(1)
+    T = 2.0 / 3.0 * m * (Ur[:, 4] - n * (u * u).sum(1)) / n
+    qdot = (_central3(Ur, n, u)[:, :, None] * C).sum(1)
+    fm = (n * (m / (np.pi * T)) ** 1.5)[:, None] * np.exp(-m * C2 / T[:, None])
+    return fm * (1.0 + 0.8 * (1.0 - Pr) * (m * m * qdot / (n * T * T)[:, None])
+                 * (m * C2 / T[:, None] - 2.5))
+
+
+def _central3(Ur, n, u):
+    """sum f |c-u|^2 (c-u) cw from the raw moments (R3 = sum f |c|^2 c cw,
+    P = sum f c c^T cw, tr = sum f |c|^2 cw, j = sum f c cw), by the expansion
+    |c-u|^2 (c-u) = |c|^2 c - 2 (c.u) c - |c|^2 u + 2 (c.u) u + |u|^2 c - |u|^2 u."""
+    P = np.stack([np.stack([Ur[:, 5], Ur[:, 8], Ur[:, 10]], 1),
+                  np.stack([Ur[:, 8], Ur[:, 6], Ur[:, 9]], 1),
+                  np.stack([Ur[:, 10], Ur[:, 9], Ur[:, 7]], 1)], 1)
+    j = Ur[:, 1:4]
+    tr = Ur[:, 4]
+    u2 = (u * u).sum(1)
+    ju = (j * u).sum(1)
+    return (Ur[:, 11:14] - 2.0 * np.einsum('sij,sj->si', P, u) - tr[:, None] * u
+            + 2.0 * ju[:, None] * u + u2[:, None] * j - (u2 * n)[:, None] * u)
+
+
+class SpeciesPenal:
+    """Species penalisation: mass m, species prefactor (nu = prefac n T^(1-omega)),
+    omega, Pr on the common grid (cv, cw)."""
+
+    def __init__(self, cv, cw, m, prefac, omega, Pr):
+        self.cv, self.cw, self.m, self.omega, self.Pr = cv, cw, m, omega, Pr
+        self.species_prefac = prefac
+        self.prefac = prefac * m ** -1.5          # acting on the scaled density
+        self.srate = 1.0
+
+    def moment(self, f):
+        return moment(f, self.cv, self.cw) * mass_scale(self.m)[None, :]
+
+    def target(self, U):
+        return species_target(U, self.cv, self.m, self.Pr)
+
+    def penal(self, U, f):
+        return collide(U, self.target(U), f, self.prefac, self.omega)
+
+
+class MixPenal:
+    """Two SpeciesPenals over a species-major stack of 2 nslice slices; the
+    update functions see per-slice prefac and omega arrays."""
+
+    def __init__(self, species, nslice):
+        self.sp, self.ns = species, nslice
+        self.prefac = np.concatenate([np.full(nslice, s.prefac) for s in species])
+        self.omega = np.concatenate([np.full(nslice, s.omega) for s in species])
+        if len({s.Pr for s in species}) != 1:
+            raise ValueError('one Prandtl number for both species')
+        self.Pr = species[0].Pr
+        self.srate = 1.0
+
+    def _split(self, x):
+        return [x[p * self.ns:(p + 1) * self.ns] for p in range(len(self.sp))]
+
+    def moment(self, f):
+        return np.concatenate([s.moment(x) for s, x in zip(self.sp, self._split(f))])
+
+    def target(self, U):
+        return np.concatenate([s.target(x) for s, x in zip(self.sp, self._split(U))])
+
+    def penal(self, U, f):
+        return collide(U, self.target(U), f, self.prefac, self.omega)
+
+
+KB = 1.38064852e-23
+
+
+def vss_viscosity(m, alpha, omega, dRef, Tref, T):
+    """Variable-soft-sphere viscosity (Bird, Molecular Gas Dynamics, 1994,
+    eq. (3.74)): mu_ref = 5 (alpha+1)(alpha+2) sqrt(m k Tref / pi) /
+    (4 alpha (5 - 2 omega)(7 - 2 omega) dRef^2), mu = mu_ref (T/Tref)^omega."""
+    mu_ref = 5.0 * (alpha + 1.0) * (alpha + 2.0) * np.sqrt(m * KB * Tref / np.pi) / (
+        4.0 * alpha * (5.0 - 2.0 * omega) * (7.0 - 2.0 * omega) * dRef ** 2)
+    return mu_ref * (T / Tref) ** omega
+
+
+def species_prefactor(H0, u0, n0, T0, molar_mass, NA, alpha, omega, dRef, Tref):
+    """Shakov frequency prefactor of a pure species in the solver's units:
+    nu = prefac n T^(1-omega) with prefac = t0 n0 k T0 / mu(T0), t0 = H0/u0."""
+    return H0 / u0 * n0 * KB * T0 / vss_viscosity(molar_mass / NA, alpha, omega, dRef, Tref, T0)

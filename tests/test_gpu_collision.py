@@ -95,8 +95,10 @@ def test_collision_fp32(cuda, N, M, Nrho):
 def test_collision_reference_form(cuda, N, M, Nrho):
     """dgfs_collide_reference: FP64 -- equal to the direct Q(f, f) to 1e-12;
     FP32 -- against the FP64 oracle on a state 1e-3 away from the reference
-    state, its error is the direct FP32 error scaled down by about |f - f0|/|f|
-    (the gain and the loss each exceed Q several times here)."""
+    state, its error is the direct FP32 error scaled down by at least 30x, down
+    to the rounding of the FP32 output itself (half an ulp, 6e-8 of |Q|): the
+    computed part scales with |f - f0|/|f|, while the gain and the loss each
+    exceed Q several times here."""
     import torch
     from dgfs1d_gpu_amd import _lib
     from dgfs1d_gpu_amd.collision import CollisionPlan
@@ -137,4 +139,4 @@ def test_collision_reference_form(cuda, N, M, Nrho):
     print('reference form', N, 'fp64 (ref, direct)', errs[_lib.F64], 'fp32', errs[_lib.F32])
     assert errs[_lib.F64][0] < 1e-12
     ref32, dir32 = errs[_lib.F32]
-    assert ref32 < 2e-8 and ref32 < dir32 / 30
+    assert ref32 < 1e-7 and ref32 < dir32 / 30

@@ -260,3 +260,97 @@ def test_es_relaxation_rate():
     # the distribution follows the target, so the slip shows as a first-order
     # error (1e-4 here) instead of the scheme's second-order one (3e-6)
     assert np.abs(dev(OI.moment(f, cv, cw)) - exact).max() > 10 * errs[2]
+
+
+# ------------------------------------------------ binary-mixture penalisation
+def _species_maxw(cv, m, n, u, T):
+    C = cv - np.asarray(u, float).reshape(3, 1)
+    return n * (m / (np.pi * T)) ** 1.5 * np.exp(-m * (C * C).sum(0) / T)
+
+
+def _species_moments(h, cv, cw, m):
+    n = (h * cw).sum()
+    u = (cv * h).sum(1) * cw / n
+    C = cv - u[:, None]
+    C2 = (C * C).sum(0)
+    return n, u, 2.0 / 3.0 * m * (h * C2).sum() * cw / n, (h * C2 * C).sum(1) * cw
+
+
+@pytest.mark.parametrize('m', [1.0, 2.1])
+def test_species_target_moments(m):
+    """The species' Shakov target (mass m): a Maxwellian of mass m is its
+    own target; for a state with heat flux q the target keeps n, u and T and
+    carries (1 - Pr) q -- the Shakov model's defining moments -- with the
+    temperature and heat flux of a mass-m particle (a wrong power of m fails)."""
+    cv, cw = grid(28, 6.0)
+    n, u, T = 0.8, np.array([0.2, -0.1, 0.05]), 1.1
+    f = _species_maxw(cv, m, n, u, T)
+    sp = OI.SpeciesPenal(cv, cw, m, prefac=1.0, omega=0.8, Pr=2.0 / 3.0)
+    assert np.abs(sp.target(sp.moment(f[None]))[0] - f).max() < 1e-6 * f.max()
+    C = cv - u[:, None]
+    g = f * (1.0 + (C * np.array([0.05, 0.02, -0.03])[:, None]).sum(0)
+             * (m * (C * C).sum(0) / T - 2.5))
+    S = sp.target(sp.moment(g[None]))[0]
+    a, b = _species_moments(g, cv, cw, m), _species_moments(S, cv, cw, m)
+    assert abs(a[0] - b[0]) < 1e-10 and np.abs(a[1] - b[1]).max() < 1e-10
+    assert abs(a[2] - b[2]) < 1e-10 and abs(a[2] - T) < 1e-6
+    assert np.abs(b[3] - (1.0 - 2.0 / 3.0) * a[3]).max() < 1e-8 * np.abs(a[3]).max()
+    if m == 1.0:                      # unit mass: the single-species target
+        assert np.abs(OI.target(OI.moment(g[None], cv, cw), cv, 2.0 / 3.0)[0] - S).max() \
+            < 1e-14 * S.max()
+
+
+def test_vss_viscosity_against_tabulated_values():
+    """Bird (1994), Table A1 at 273 K: argon 2.117e-5 Pa s (VHS d = 4.17e-10,
+    omega = 0.81), krypton 2.328e-5 (VHS d = 4.76e-10, omega = 0.80); the VSS
+    argon parameters (alpha = 1.40, d = 4.11e-10) give the same viscosity."""
+    NA = 6.0221409e23
+    mAr, mKr = 39.948e-3 / NA, 83.80e-3 / NA
+    assert abs(OI.vss_viscosity(mAr, 1.0, 0.81, 4.17e-10, 273.0, 273.0) / 2.117e-5 - 1) < 2e-3
+    assert abs(OI.vss_viscosity(mKr, 1.0, 0.80, 4.76e-10, 273.0, 273.0) / 2.328e-5 - 1) < 2e-3
+    assert abs(OI.vss_viscosity(mAr, 1.40, 0.81, 4.11e-10, 273.0, 273.0) / 2.117e-5 - 1) < 2e-3
+    # (T/Tref)^omega
+    r = OI.vss_viscosity(mAr, 1.0, 0.81, 4.17e-10, 273.0, 546.0) / \
+        OI.vss_viscosity(mAr, 1.0, 0.81, 4.17e-10, 273.0, 273.0)
+    assert abs(r - 2.0 ** 0.81) < 1e-12
+
+
+@pytest.mark.parametrize('kind,order', [('imex-lm-bdf2', 2), ('imex-lm-bdf3', 3)])
+def test_mixture_relaxation_order(kind, order):
+    """Two species (masses 1 and 2.1, different prefactors and omegas) relax
+    by their own BGK penalisations (Pr = 1, no transport): n, u and T of each
+    species are invariants, so nu_p is constant and
+    f_p(t) = M_p + (f_p(0) - M_p) exp(-nu_p t) with M_p the mass-m_p Maxwellian
+    of f_p(0)'s n, u, T.  The linear-multistep integrator on the mixture stack
+    converges to it at the scheme's order (a wrong mass scaling of nu or of the
+    target fails)."""
+    cv, cw = grid(24, 6.0)      # resolves the heavier species' Maxwellian to 1e-8
+    ms, pre, om = (1.0, 2.1), (1.3, 0.8), (0.81, 0.75)
+    sps = [OI.SpeciesPenal(cv, cw, ms[p], pre[p], om[p], 1.0) for p in range(2)]
+    pm = OI.MixPenal(sps, 1)
+    f0, Ms, nus = [], [], []
+    for p, (n, u, T) in enumerate(((1.0, (0.1, 0, 0), 1.0), (0.4, (-0.05, 0.1, 0), 1.2))):
+        f = _species_maxw(cv, ms[p], n, u, T)
+        C = cv - np.asarray(u, float).reshape(3, 1)
+        g = f * (1.0 + 0.2 * np.cos(C[0] + 0.5 * C[1]))
+        nn, uu, TT, _ = _species_moments(g, cv, cw, ms[p])
+        f0.append(g)
+        Ms.append(_species_maxw(cv, ms[p], nn, uu, TT))
+        nus.append(pre[p] * nn * TT ** (1.0 - om[p]))
+    F0 = np.stack(f0)
+    M = np.stack(Ms)
+    nu = np.array(nus)[:, None]
+    tend = 0.5 / max(nus)
+    exact = M + (F0 - M) * np.exp(-nu * tend)
+    zero = lambda t, f: np.zeros_like(f)
+    errs = []
+    for nsteps in (16, 32):
+        dt = tend / nsteps
+        it = OI.LMIntegrator(kind, pm, zero, None, 0.0, dt, F0.copy(), penalize_init=False)
+        f = F0.copy()
+        for k in range(nsteps):
+            f = it.step(k * dt, dt, k, f)
+        errs.append(np.abs(f - exact).max() / np.abs(exact).max())
+    rate = np.log2(errs[0] / errs[1])
+    print(kind, errs, rate)
+    assert rate > order - 0.3 and errs[1] < 1e-3
