@@ -85,6 +85,7 @@ SIGNATURES = {
     'dgfs_collision_plan_set_qgroups': (_i, [_vp, _i]),
     'dgfs_collide': (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _i,
                           _vp]),
+    'dgfs_collision_reference_set': (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'dgfs_collide_reference': (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _vp,
                                     _vp, _vp]),
     'dgfs_collision_timing': (_i, [_vp, _i]),

@@ -124,9 +124,28 @@ class CollisionPlan:
             _lib.ptr(t['gstart']), _lib.ptr(t['gslice']), fg_same,
             _lib.stream_ptr(stream)))
 
+    def reference_set(self, u0, stream=None, tasks=None):
+        """Make u0 the plan's reference state (dgfs_collision_reference_set):
+        its F0^ and inverse transforms for the task table, used by every
+        reference() call until the next reference_set (u0 must keep its
+        storage and contents until then)."""
+        t = self.tasks if tasks is None else tasks
+        if t is None:
+            raise RuntimeError('set_tasks() must be called first')
+        if not t['fg_same']:
+            raise ValueError('the reference-state form evaluates Q(f, f): every task needs f = g')
+        if u0.dtype != self.tdtype:
+            raise TypeError('collision operands must be %s' % self.tdtype)
+        if u0.numel() < t['K'] * t['Ne'] * self.n:
+            raise ValueError('collision operands smaller than K*Ne*Nv')
+        _lib.check(_lib.lib().dgfs_collision_reference_set(
+            self._h, _lib.ptr(u0), t['K'], t['Ne'], t['ntask'], _lib.ptr(t['nterm']),
+            _lib.ptr(t['slice']), _lib.ptr(t['coef']), _lib.stream_ptr(stream)))
+
     def reference(self, u, u0, Q0, Q, stream=None, tasks=None):
         """Q += Q(f, f) in the reference-state form (dgfs_collide_reference):
-        Q0 (FP64, (K, Ne, Nv)) must hold Q(f0, f0) of the reference state u0;
+        Q0 (FP64, (K, Ne, Nv)) must hold Q(f0, f0) of the reference state u0,
+        which reference_set(u0) must have made the plan's reference state;
         the FP32 rounding error then scales with |f - f0|.  f = g tasks only."""
         import torch
         t = self.tasks if tasks is None else tasks

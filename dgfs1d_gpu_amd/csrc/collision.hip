@@ -39,6 +39,22 @@ static int dispatch(CollisionPlan* pl, const void* u, const void* ug, void* Q, T
     return 2;
 }
 
+template <typename T>
+static int dispatch_reference_set(CollisionPlan* pl, const void* u0, TaskView tv, int ntask,
+                                  hipStream_t st) {
+#define DGFS_CASE(NN)                                                                     \
+    case NN:                                                                              \
+        return reference_set_launch<T, NN>(pl, u0, tv, ntask, st);
+    switch (pl->N) {
+        DGFS_CASE(4) DGFS_CASE(6) DGFS_CASE(8) DGFS_CASE(12) DGFS_CASE(16) DGFS_CASE(20)
+        DGFS_CASE(24) DGFS_CASE(32)
+        default: break;
+    }
+#undef DGFS_CASE
+    set_error("collision: unsupported velocity grid N=%d", pl->N);
+    return 2;
+}
+
 static bool supported_N(int N) {
     return N == 4 || N == 6 || N == 8 || N == 12 || N == 16 || N == 20 || N == 24 || N == 32;
 }
@@ -195,7 +211,8 @@ extern "C" int dgfs_collision_plan_destroy(dgfs_collision_plan_t plan) {
     auto* pl = static_cast<CollisionPlan*>(plan);
     if (!pl) return 0;
     free_events(pl);
-    for (void* p : {pl->phase, pl->wtab, pl->bb2, pl->Fhat, pl->acc, pl->Iw, pl->H, pl->nyq})
+    for (void* p : {pl->phase, pl->wtab, pl->bb2, pl->Fhat, pl->acc, pl->Iw, pl->H, pl->nyq,
+                    pl->Fhat0, pl->ucache})
         if (p) (void)hipFree(p);
     delete pl;
     return 0;
@@ -255,6 +272,48 @@ extern "C" int dgfs_collide(dgfs_collision_plan_t plan, const void* u, const voi
                                  fg_same, stream, nullptr);
 }
 
+extern "C" int dgfs_collision_reference_set(dgfs_collision_plan_t plan, const void* u0, int K,
+                                            int Ne, int ntask, const int* task_nterm,
+                                            const int* task_slice, const void* task_coef,
+                                            hipStream_t stream) {
+    auto* pl = static_cast<CollisionPlan*>(plan);
+    DGFS_REQUIRE(pl, "collision_reference_set: null plan");
+    DGFS_REQUIRE(ntask > 0 && ntask <= pl->ntask_max,
+                 "collision_reference_set: ntask=%d outside 1..%d", ntask, pl->ntask_max);
+    DGFS_REQUIRE(K > 0 && Ne > 0, "collision_reference_set: bad K/Ne");
+    DGFS_REQUIRE(u0 && task_nterm && task_slice && task_coef,
+                 "collision_reference_set: null pointer argument");
+    pl->ref_u0 = nullptr;
+    pl->ref_ntask = 0;
+    if (pl->ref_cap < pl->ntask_max) {
+        // first use: the reference state's buffers for the plan's task capacity
+        const size_t es = pl->dtype == DGFS_F64 ? 8 : 4;
+        if (pl->Fhat0) (void)hipFree(pl->Fhat0);
+        if (pl->ucache) (void)hipFree(pl->ucache);
+        pl->Fhat0 = pl->ucache = nullptr;
+        const size_t nf = size_t(pl->ntask_max) * 2 * pl->n * 2 * es;
+        const size_t nc = size_t(pl->ntask_max) * pl->Nrho * pl->M * pl->n * 4 * es;
+        if (hipMalloc(&pl->Fhat0, nf) != hipSuccess || hipMalloc(&pl->ucache, nc) != hipSuccess) {
+            if (pl->Fhat0) (void)hipFree(pl->Fhat0);
+            pl->Fhat0 = nullptr;
+            pl->ucache = nullptr;
+            pl->ref_cap = 0;
+            dgfs::set_error("collision_reference_set: allocation of %zu bytes failed", nf + nc);
+            return 1;
+        }
+        pl->ref_cap = pl->ntask_max;
+    }
+    dgfs::TaskView tv{K, Ne, int(pl->n), task_nterm, task_slice, task_coef};
+    const int rc = pl->dtype == DGFS_F64
+                       ? dgfs::dispatch_reference_set<double>(pl, u0, tv, ntask, stream)
+                       : dgfs::dispatch_reference_set<float>(pl, u0, tv, ntask, stream);
+    if (rc) return rc;
+    pl->ref_u0 = u0;
+    pl->ref_ntask = ntask;
+    pl->ref_slice = task_slice;
+    return 0;
+}
+
 extern "C" int dgfs_collide_reference(dgfs_collision_plan_t plan, const void* u, const void* u0,
                                       const double* Q0, void* Q, int K, int Ne, int ntask,
                                       const int* task_nterm, const int* task_slice,
@@ -272,6 +331,9 @@ extern "C" int dgfs_collide_reference(dgfs_collision_plan_t plan, const void* u,
                      group_slice,
                  "collide_reference: null pointer argument");
     DGFS_REQUIRE(u0 != u, "collide_reference: the reference state must be its own array");
+    DGFS_REQUIRE(pl->ref_u0 == u0 && pl->ref_ntask == ntask && pl->ref_slice == task_slice,
+                 "collide_reference: the plan holds no reference state for this u0 and task "
+                 "table (call dgfs_collision_reference_set after every change of u0)");
     dgfs::TaskView tv{K, Ne, int(pl->n), task_nterm, task_slice, task_coef};
     if (pl->dtype == DGFS_F64)
         return dgfs::dispatch<double>(pl, u, u0, Q, tv, ntask, ngroup, group_start,

@@ -21,6 +21,14 @@
 //   D  line  : inverse along k1 per (y, z) line and
 //              Q[out] += prefac (Re IFFT3(sum G^) - f Re IFFT3(bb2 G^_g))
 //
+// Reference-state form (FP32 rounding error scaled by |f - f0|): with the
+// reference state f0 set once (dgfs_collision_reference_set: F0^ and
+// u+-_f0 = IFFT3(exp(+-i a_pq) F0^) for every (task, p, q) kept in the plan),
+// each call transforms h = f - f0 only and
+//   gain  = sum_q u+h (u-h + u-f0) + u-h u+f0  (= Gain(f) - Gain(f0)),
+//   loss  = h L[f] + f0 L[h],  Q += Q(f0, f0) + prefac (gain - loss)
+// with Q(f0, f0) supplied by the caller in FP64.
+//
 // Tables (phases, weights, bb2) are built once in FP64 on the host at plan
 // creation.
 #pragma once
@@ -248,12 +256,26 @@ constexpr int pick_split_gain() {
     else return 8;
 }
 
+// u+ and u- of one point (the reference state's cache entry)
+template <typename T>
+struct UPair {
+    cplx<T> p, m;
+};
+
+enum GainMode { kGainAccumulate = 0, kGainStore = 1, kGainDelta = 2 };
+
 // Gain: for (task, radius p, z-residue r, q-group), acc = sum_q t1 t2 + t3 t4
 // with t1 t2 + t3 t4 = u+f u-f (NW = 1, f = g) or 1/2 (u+f u-g + u-f u+g).
-template <typename T, int N, int S, int NW>
+// MODE kGainStore (NW = 1, F^ = F0^): write u+-_f0 of every (q, point) to
+//   ucache[t][p][q][x][r][P] (P = this thread's pencil) instead.
+// MODE kGainDelta (NW = 1, F^ = h^): acc = sum_q u+h (u-h + u-f0) + u-h u+f0
+//   with u+-_f0 read from ucache.
+template <typename T, int N, int S, int NW, int MODE = kGainAccumulate>
 __global__ void __launch_bounds__(round64(N * (N / S)))
 k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
-       const T* __restrict__ nyq, int M, int Nrho, int QG, cplx<T>* __restrict__ acc_out) {
+       const T* __restrict__ nyq, int M, int Nrho, int QG, cplx<T>* __restrict__ acc_out,
+       UPair<T>* __restrict__ ucache) {
+    static_assert(MODE == kGainAccumulate || NW == 1, "reference-state modes transform one function");
     using E = Fft3Engine<T, N, S, round64(N * (N / S)), +1>;
     using NL = NyLayout<N, S>;
     constexpr int n = N * N * N, NN = N * N;
@@ -342,7 +364,25 @@ k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
                         up = mk(wr + delta, beta);
                         um = mk(wi - beta, delta);
                     };
-                    if constexpr (NW == 1) {
+                    if constexpr (MODE == kGainStore) {
+                        UPair<T>* dst = ucache + ((((size_t(t) * Nrho + p) * M + q) * N) * S + r) *
+                                                     size_t(E::NPEN) + threadIdx.x;
+                        sfor<0, N>([&](auto I) {
+                            UPair<T> v;
+                            unpack(0, I, o[I], v.p, v.m);
+                            dst[size_t(int(I)) * S * E::NPEN] = v;
+                        });
+                    } else if constexpr (MODE == kGainDelta) {
+                        const UPair<T>* src = ucache +
+                            ((((size_t(t) * Nrho + p) * M + q) * N) * S + r) * size_t(E::NPEN) +
+                            threadIdx.x;
+                        sfor<0, N>([&](auto I) {
+                            cplx<T> up, um;
+                            unpack(0, I, o[I], up, um);
+                            const UPair<T> c0 = src[size_t(int(I)) * S * E::NPEN];
+                            acc[I] = acc[I] + up * (um + c0.m) + um * c0.p;
+                        });
+                    } else if constexpr (NW == 1) {
                         sfor<0, N>([&](auto I) {
                             cplx<T> up, um;
                             unpack(0, I, o[I], up, um);
@@ -361,6 +401,7 @@ k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
                 });
         });
     }
+    if constexpr (MODE == kGainStore) return;
     if (threadIdx.x < E::NPEN) {
         int o2, j;
         map3(threadIdx.x, o2, j);
@@ -383,13 +424,15 @@ k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
 // p = slot, slot+PL, ...:  G^(k1,:,:) = sum_p wtab_p(k1,:,:) FFT2_{y,z}(sum_qg acc[t][p][qg][k1]),
 // reduced over the slots through LDS; then Hg = IFFT2(G^(k1)) (slot 0) and
 // Hl = IFFT2(bb2 G^_g(k1)) (slot 1).  H[t][0 = loss | 1 = gain][k1][y][z].
-// DEV (reference-state form, F^ = {h^, s^}): slot 1 inverts bb2 f^ with
-// f^ = (h^ + s^)/2 and slot 2 inverts bb2 h^:  H[t][0 = L[f] | 1 = gain | 2 = L[h]].
+// DEV (reference-state form, F^ = h^, F0^ the reference state's): slot 1
+// inverts bb2 f^ with f^ = h^ + F0^ and slot 2 inverts bb2 h^:
+// H[t][0 = L[f] | 1 = gain | 2 = L[h]].
 template <typename T, int N, bool DEV>
 __global__ void __launch_bounds__(planes_per_wg(N) * N)
 k_plane_weight(const cplx<T>* __restrict__ acc_in, const T* __restrict__ wtab,
-               const cplx<T>* __restrict__ Fhat, const T* __restrict__ bb2, int fg_same,
-               int Nrho, int QG, int SB, cplx<T>* __restrict__ H) {
+               const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ Fhat0,
+               const T* __restrict__ bb2, int fg_same, int Nrho, int QG, int SB,
+               cplx<T>* __restrict__ H) {
     constexpr int PL = planes_per_wg(N), NT = PL * N, NN = N * N, n = N * NN, PS = N * (N + 1);
     constexpr int NH = DEV ? 3 : 2;
     static_assert(PL >= NH, "plane kernel needs a slot per output plane");
@@ -440,10 +483,10 @@ k_plane_weight(const cplx<T>* __restrict__ acc_in, const T* __restrict__ wtab,
         const cplx<T>* Fg = Fhat + (size_t(t) * 2 + (fg_same ? 0 : 1)) * n;
         if constexpr (DEV) {
             const cplx<T>* Fh = Fhat + size_t(t) * 2 * n;
+            const cplx<T>* F0 = Fhat0 + size_t(t) * 2 * n;
             sfor<0, N>([&](auto L2) {
                 const size_t o = (size_t(c) * N + k1) * N + L2;
-                P[L2 * (N + 1) + c] = (T(0.5) * bb2[size_t(k1) * NN + L2 * N + c]) *
-                                      (Fh[o] + Fh[o + n]);
+                P[L2 * (N + 1) + c] = bb2[size_t(k1) * NN + L2 * N + c] * (Fh[o] + F0[o]);
             });
         } else {
             sfor<0, N>([&](auto L2) {
@@ -490,37 +533,54 @@ k_line_output(const T* __restrict__ u, const T* __restrict__ u0, const double* _
     const int ta = gstart[g], tb = gstart[g + 1];
     cplx<T> a[N], o[N];
     T loss[N];
-    sfor<0, N>([&](auto I) { loss[I] = T(0); a[I] = mk(T(0), T(0)); });
+    sfor<0, N>([&](auto I) { loss[I] = T(0); });
+    // loss terms first, task by task (one inverse pencil live at a time),
+    // then the gain planes summed over the group and inverted once
     for (int t = ta; t < tb; ++t) {
         const cplx<T>* hl = H + (size_t(t) * NH) * n + yz;
-        cplx<T> b[N];
-        sfor<0, N>([&](auto K1) { b[K1] = hl[size_t(K1) * NN]; });
-        RegDFT<T, N, +1>::template run<1, 0>(b, o);
         if constexpr (DEV) {
-            T lh[N];
-            sfor<0, N>([&](auto K1) { b[K1] = hl[2 * size_t(n) + size_t(K1) * NN]; });
-            {
-                cplx<T> o2[N];
-                RegDFT<T, N, +1>::template run<1, 0>(b, o2);
-                sfor<0, N>([&](auto X) { lh[X] = o2[X].x; });
-            }
+            // scheduling barriers keep each phase's loads after the previous
+            // phase, so that one pencil's values are live at a time
+            sfor<0, N>([&](auto K1) { a[K1] = hl[2 * size_t(n) + size_t(K1) * NN]; });
+            RegDFT<T, N, +1>::template run<1, 0>(a, o);             // L[h]
+            __builtin_amdgcn_sched_barrier(0);
+            sfor<0, N>([&](auto X) {
+                loss[X] += task_value<T>(tv, u0, t, 0, X * NN + yz) * o[X].x;
+            });
+            __builtin_amdgcn_sched_barrier(0);
+            sfor<0, N>([&](auto K1) { a[K1] = hl[size_t(K1) * NN]; });
+            RegDFT<T, N, +1>::template run<1, 0>(a, o);             // L[f]
+            __builtin_amdgcn_sched_barrier(0);
             sfor<0, N>([&](auto X) {
                 const T f = task_value<T>(tv, u, t, 0, X * NN + yz);
                 const T f0 = task_value<T>(tv, u0, t, 0, X * NN + yz);
-                loss[X] += (f - f0) * o[X].x + f0 * lh[X];
+                loss[X] += (f - f0) * o[X].x;
             });
+            __builtin_amdgcn_sched_barrier(0);
         } else {
+            sfor<0, N>([&](auto K1) { a[K1] = hl[size_t(K1) * NN]; });
+            RegDFT<T, N, +1>::template run<1, 0>(a, o);
             sfor<0, N>([&](auto X) { loss[X] += task_value<T>(tv, u, t, 0, X * NN + yz) * o[X].x; });
         }
-        const cplx<T>* hg = hl + n;
+    }
+    sfor<0, N>([&](auto I) { a[I] = mk(T(0), T(0)); });
+    for (int t = ta; t < tb; ++t) {
+        const cplx<T>* hg = H + (size_t(t) * NH + 1) * n + yz;
         sfor<0, N>([&](auto K1) { a[K1] = a[K1] + hg[size_t(K1) * NN]; });
     }
     RegDFT<T, N, +1>::template run<1, 0>(a, o);
     T* dst = Q + size_t(gslice[g]) * n + yz;
     if constexpr (DEV) {
         const double* q0 = Q0 + size_t(gslice[g]) * n + yz;
-        sfor<0, N>([&](auto X) {
-            dst[size_t(X) * NN] += T(q0[size_t(X) * NN] + double(prefac * (o[X].x - loss[X])));
+        // in chunks of CH points, so the FP64 loads of Q0 are not all live at once
+        constexpr int CH = (N % 8 == 0) ? 8 : ((N % 4 == 0) ? 4 : 2);
+        sfor<0, N / CH>([&](auto C) {
+            sfor<0, CH>([&](auto Xc) {
+                constexpr int X = int(C) * CH + int(Xc);
+                dst[size_t(X) * NN] += T(q0[size_t(X) * NN] +
+                                         double(prefac * (o[X].x - loss[X])));
+            });
+            __builtin_amdgcn_sched_barrier(0);
         });
     } else {
         sfor<0, N>([&](auto X) { dst[size_t(X) * NN] += prefac * (o[X].x - loss[X]); });
@@ -542,6 +602,15 @@ struct CollisionPlan {
     void* Iw = nullptr;     // [ntask][2][n] cplx<T>  forward-transform intermediate
     void* H = nullptr;      // [ntask][2][n] cplx<T>  plane-inverted loss / gain
     void* nyq = nullptr;    // [ntask][2][Nrho][M][6][N*N] T  Nyquist-plane imaginary parts
+    // reference-state form: F0^ ([ntask][2][n] cplx<T>, slot 0 used) and
+    // u+-_f0 ([ntask][Nrho][M][n] UPair<T>) of the reference state, valid for
+    // ref_ntask tasks of the reference state at ref_u0 once set
+    void* Fhat0 = nullptr;
+    void* ucache = nullptr;
+    int ref_cap = 0;        // tasks the reference buffers hold
+    int ref_ntask = 0;
+    const void* ref_u0 = nullptr;
+    const int* ref_slice = nullptr;
     int num_cu = 256;
     // optional stage timing: 6 events per call (before forward, nyquist,
     // gain, plane, line, and after line)
@@ -565,29 +634,18 @@ struct Impl {
     static constexpr int PL = planes_per_wg(N);
 
     // Q0 == nullptr: Q += Q(f, g).  Q0 != nullptr (reference-state form, ug the
-    // reference state u0): Q += Q0 + G(h, s) - h L[f] - f0 L[h], h = f - f0,
-    // s = f + f0, which equals Q(f, f) when Q0 = Q(f0, f0).
+    // reference state u0 whose F0^ and u+-_f0 the plan holds): Q += Q0 +
+    // G(h) - h L[f] - f0 L[h], h = f - f0, which equals Q(f, f) when
+    // Q0 = Q(f0, f0).
     static int launch(CollisionPlan* pl, const void* u, const void* ug, void* Q, TaskView tv,
                       int ntask, int ngroup, const int* gstart, const int* gslice, int fg_same,
                       hipStream_t st, const double* Q0) {
         const bool dev = Q0 != nullptr;
-        if (dev) fg_same = 0;
+        if (dev) fg_same = 1;                  // one transform per (p, q): h
         const int nfg = fg_same ? 1 : 2;
         constexpr long NN = long(N) * N;
         timing_mark(pl, 0, st);
-        const long npl = long(ntask) * nfg * N;
-        if (dev)
-            hipLaunchKernelGGL((k_fwd_plane<T, N, true>), dim3((npl + PL - 1) / PL), dim3(PL * N),
-                               0, st, (const T*)u, (const T*)ug, tv, nfg, npl, (cplx<T>*)pl->Iw);
-        else
-            hipLaunchKernelGGL((k_fwd_plane<T, N, false>), dim3((npl + PL - 1) / PL),
-                               dim3(PL * N), 0, st, (const T*)u, (const T*)ug, tv, nfg, npl,
-                               (cplx<T>*)pl->Iw);
-        DGFS_HIP_CHECK(hipGetLastError());
-        const long nli = long(ntask) * nfg * NN;
-        hipLaunchKernelGGL((k_fwd_line<T, N>), dim3((nli + 255) / 256), dim3(256), 0, st, nli,
-                           nfg, (const cplx<T>*)pl->Iw, (cplx<T>*)pl->Fhat);
-        DGFS_HIP_CHECK(hipGetLastError());
+        if (int rc = forward(pl, u, ug, tv, ntask, nfg, dev, (cplx<T>*)pl->Fhat, st)) return rc;
         timing_mark(pl, 1, st);
         hipLaunchKernelGGL((k_nyq<T, N>), dim3(ntask * nfg * pl->Nrho * pl->M), dim3(6 * N), 0,
                            st, (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->phase, pl->M,
@@ -596,26 +654,33 @@ struct Impl {
         timing_mark(pl, 2, st);
         const int S = fg_same ? S1 : S2;
         const int nb = ntask * pl->Nrho * S * pl->QG;
-        if (fg_same)
+        if (dev)
+            hipLaunchKernelGGL((k_gain<T, N, S1, 1, kGainDelta>), dim3(nb),
+                               dim3(round64(N * (N / S1))), 0, st, (const cplx<T>*)pl->Fhat,
+                               (const cplx<T>*)pl->phase, (const T*)pl->nyq, pl->M, pl->Nrho,
+                               pl->QG, (cplx<T>*)pl->acc, (UPair<T>*)pl->ucache);
+        else if (fg_same)
             hipLaunchKernelGGL((k_gain<T, N, S1, 1>), dim3(nb), dim3(round64(N * (N / S1))), 0,
                                st, (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->phase,
-                               (const T*)pl->nyq, pl->M, pl->Nrho, pl->QG, (cplx<T>*)pl->acc);
+                               (const T*)pl->nyq, pl->M, pl->Nrho, pl->QG, (cplx<T>*)pl->acc,
+                               (UPair<T>*)nullptr);
         else
             hipLaunchKernelGGL((k_gain<T, N, S2, 2>), dim3(nb), dim3(round64(N * (N / S2))), 0,
                                st, (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->phase,
-                               (const T*)pl->nyq, pl->M, pl->Nrho, pl->QG, (cplx<T>*)pl->acc);
+                               (const T*)pl->nyq, pl->M, pl->Nrho, pl->QG, (cplx<T>*)pl->acc,
+                               (UPair<T>*)nullptr);
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 3, st);
         if (dev)
             hipLaunchKernelGGL((k_plane_weight<T, N, true>), dim3(ntask * N), dim3(PL * N), 0, st,
                                (const cplx<T>*)pl->acc, (const T*)pl->wtab,
-                               (const cplx<T>*)pl->Fhat, (const T*)pl->bb2, fg_same, pl->Nrho,
-                               pl->QG, S, (cplx<T>*)pl->H);
+                               (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->Fhat0,
+                               (const T*)pl->bb2, fg_same, pl->Nrho, pl->QG, S, (cplx<T>*)pl->H);
         else
             hipLaunchKernelGGL((k_plane_weight<T, N, false>), dim3(ntask * N), dim3(PL * N), 0,
                                st, (const cplx<T>*)pl->acc, (const T*)pl->wtab,
-                               (const cplx<T>*)pl->Fhat, (const T*)pl->bb2, fg_same, pl->Nrho,
-                               pl->QG, S, (cplx<T>*)pl->H);
+                               (const cplx<T>*)pl->Fhat, (const cplx<T>*)nullptr,
+                               (const T*)pl->bb2, fg_same, pl->Nrho, pl->QG, S, (cplx<T>*)pl->H);
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 4, st);
         const long nlo = long(ngroup) * NN;
@@ -632,7 +697,52 @@ struct Impl {
         if (pl->ev && pl->ev_used < pl->ev_max) ++pl->ev_used;
         return 0;
     }
+
+    // F^ of every task into dst ([t][2][n], nfg functions per task); dev: the
+    // single function h = f - f0 (u = f, ug = f0)
+    static int forward(CollisionPlan* pl, const void* u, const void* ug, TaskView tv, int ntask,
+                       int nfg, bool dev, cplx<T>* dst, hipStream_t st) {
+        constexpr long NN = long(N) * N;
+        const long npl = long(ntask) * nfg * N;
+        if (dev)
+            hipLaunchKernelGGL((k_fwd_plane<T, N, true>), dim3((npl + PL - 1) / PL), dim3(PL * N),
+                               0, st, (const T*)u, (const T*)ug, tv, nfg, npl, (cplx<T>*)pl->Iw);
+        else
+            hipLaunchKernelGGL((k_fwd_plane<T, N, false>), dim3((npl + PL - 1) / PL),
+                               dim3(PL * N), 0, st, (const T*)u, (const T*)ug, tv, nfg, npl,
+                               (cplx<T>*)pl->Iw);
+        DGFS_HIP_CHECK(hipGetLastError());
+        const long nli = long(ntask) * nfg * NN;
+        hipLaunchKernelGGL((k_fwd_line<T, N>), dim3((nli + 255) / 256), dim3(256), 0, st, nli,
+                           nfg, (const cplx<T>*)pl->Iw, dst);
+        DGFS_HIP_CHECK(hipGetLastError());
+        return 0;
+    }
+
+    // the reference state f0 (= task f-terms of u0): F0^ and u+-_f0 of every
+    // (task, p, q) into the plan's reference buffers
+    static int reference_set(CollisionPlan* pl, const void* u0, TaskView tv, int ntask,
+                             hipStream_t st) {
+        if (int rc = forward(pl, u0, u0, tv, ntask, 1, false, (cplx<T>*)pl->Fhat0, st)) return rc;
+        hipLaunchKernelGGL((k_nyq<T, N>), dim3(ntask * pl->Nrho * pl->M), dim3(6 * N), 0, st,
+                           (const cplx<T>*)pl->Fhat0, (const cplx<T>*)pl->phase, pl->M, pl->Nrho,
+                           1, (T*)pl->nyq);
+        DGFS_HIP_CHECK(hipGetLastError());
+        const int nb = ntask * pl->Nrho * S1 * pl->QG;
+        hipLaunchKernelGGL((k_gain<T, N, S1, 1, kGainStore>), dim3(nb), dim3(round64(N * (N / S1))),
+                           0, st, (const cplx<T>*)pl->Fhat0, (const cplx<T>*)pl->phase,
+                           (const T*)pl->nyq, pl->M, pl->Nrho, pl->QG, (cplx<T>*)nullptr,
+                           (UPair<T>*)pl->ucache);
+        DGFS_HIP_CHECK(hipGetLastError());
+        return 0;
+    }
 };
+
+template <typename T, int N>
+int reference_set_launch(CollisionPlan* pl, const void* u0, TaskView tv, int ntask,
+                         hipStream_t st) {
+    return Impl<T, N>::reference_set(pl, u0, tv, ntask, st);
+}
 
 template <typename T, int N>
 int collide_launch(CollisionPlan* pl, const void* u, const void* ug, void* Q, TaskView tv,
@@ -644,11 +754,15 @@ int collide_launch(CollisionPlan* pl, const void* u, const void* ug, void* Q, Ta
 #define DGFS_COLLIDE_EXTERN(T, N)                                                        \
     extern template int collide_launch<T, N>(CollisionPlan*, const void*, const void*, void*, \
                                              TaskView, int, int, const int*, const int*, int, \
-                                             hipStream_t, const double*);
+                                             hipStream_t, const double*);                     \
+    extern template int reference_set_launch<T, N>(CollisionPlan*, const void*, TaskView, int, \
+                                                   hipStream_t);
 #define DGFS_COLLIDE_INST(T, N)                                                          \
     template int collide_launch<T, N>(CollisionPlan*, const void*, const void*, void*,     \
                                       TaskView, int, int, const int*, const int*, int,     \
-                                      hipStream_t, const double*);
+                                      hipStream_t, const double*);                         \
+    template int reference_set_launch<T, N>(CollisionPlan*, const void*, TaskView, int,    \
+                                            hipStream_t);
 #define DGFS_COLLIDE_SIZES(X, T) X(T, 4) X(T, 6) X(T, 8) X(T, 12) X(T, 16) X(T, 20) X(T, 24) X(T, 32)
 
 DGFS_COLLIDE_SIZES(DGFS_COLLIDE_EXTERN, float)

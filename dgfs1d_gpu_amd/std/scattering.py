@@ -168,12 +168,14 @@ class DGFSVHSGLLScatteringModelStd(DGFSScatteringModelStd):
                          Q0=torch.zeros(n, dtype=torch.float64, device=self._device))
 
     def refresh_reference(self, u, stream=None):
-        """Make u the reference state: u0 = u, Q0 = Q(u, u) in FP64."""
+        """Make u the reference state: u0 = u, Q0 = Q(u, u) in FP64, and the
+        FP32 plan's transforms of u0."""
         r = self._ref
         n = r['u0'].numel()
         r['u0'].copy_(u[:n])
         r['Q0'].zero_()
         r['plan64'](r['u0'].double(), r['Q0'], stream)
+        self.plan.reference_set(r['u0'], stream)
 
     def _make_closure(self, K, Ne, tasks):
         """[scattering-model] closure = learned: Q = Q_Shakov + R_net per node,

@@ -92,16 +92,27 @@ int dgfs_collide(dgfs_collision_plan_t plan, const void* u, const void* ug, void
  * f = g tasks when the plan computes in FP32).  With f_t the task's f
  * contraction of u and f0_t the same contraction of the reference state u0:
  *     Q[group_slice[g]] += Q0[group_slice[g]]
- *         + sum_{t in g} prefac (Re Gain(h_t, s_t) - h_t Re Loss(f_t) - f0_t Re Loss(h_t)),
- *     h = f - f0,  s = f + f0,
+ *         + sum_{t in g} prefac (Re [Gain(f_t) - Gain(f0_t)] - h_t Re Loss(f_t)
+ *                                - f0_t Re Loss(h_t)),      h = f - f0,
  * which equals dgfs_collide's Q(f, f) exactly when Q0 holds Q(f0, f0) (Gain
- * is a symmetric bilinear form and Loss is linear).  Q0: FP64, the (K, Ne,
- * Nv) layout, typically one FP64 dgfs_collide call on u0.  Every FP32
- * rounding error of the spectral pipeline then scales with |h| instead of
- * |f|: near equilibrium the gain and the loss each exceed Q by orders of
- * magnitude, and FP32 rounding of those two large terms is what limits the
- * direct form.  The g terms of the task table are ignored.  u0 must be a
- * different array from u. */
+ * is a quadratic form and Loss is linear).  Q0: FP64, the (K, Ne, Nv)
+ * layout, typically one FP64 dgfs_collide call on u0.  Every FP32 rounding
+ * error of the spectral pipeline then scales with |h| instead of |f|: near
+ * equilibrium the gain and the loss each exceed Q by orders of magnitude,
+ * and FP32 rounding of those two large terms is what limits the direct form.
+ *
+ * dgfs_collision_reference_set(plan, u0, ...) must be called, on the same
+ * stream, after every change of u0 and before the dgfs_collide_reference
+ * calls that use it (same u0 pointer, ntask and task_slice pointer; other
+ * calls are refused): it keeps F0^ and the inverse transforms
+ * u+-_f0 = IFFT3(exp(+-i a_pq) F0^) of every (task, p, q) in the plan
+ * (allocated on first use: ntask_max*N_rho*M*N^3*4 values of the plan's
+ * dtype), so that each dgfs_collide_reference call transforms h alone:
+ * Gain(f) - Gain(f0) = sum_pq w (u+h (u-h + u-f0) + u-h u+f0).  The g terms of
+ * the task table are ignored.  u0 must be a different array from u. */
+int dgfs_collision_reference_set(dgfs_collision_plan_t plan, const void* u0, int K, int Ne,
+                                 int ntask, const int* task_nterm, const int* task_slice,
+                                 const void* task_coef, hipStream_t stream);
 int dgfs_collide_reference(dgfs_collision_plan_t plan, const void* u, const void* u0,
                            const double* Q0, void* Q, int K, int Ne, int ntask,
                            const int* task_nterm, const int* task_slice, const void* task_coef,

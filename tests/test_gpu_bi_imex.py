@@ -96,11 +96,11 @@ def test_bi_imex_against_oracle(cuda, case):
     assert np.abs(f - F0).max() > 1e-4 * np.abs(F0).max()          # the state moved
 
 
-def _n48_deck(**over):
+def _n48_deck(over=None):
     from dgfs1d_gpu_amd.dictionary import Dictionary
     text = open(os.path.join(ROOT, 'dgfs1d_gpu_amd', 'configs', 'bi_fourier_n48.ini')).read()
     cfg = Dictionary(text)
-    for (sect, key), v in over.items():
+    for (sect, key), v in (over or {}).items():
         cfg.set(sect, key, str(v))
     return cfg
 
@@ -138,8 +138,8 @@ def test_bi_imex_fp32_against_fp64(cuda):
     from dgfs1d_gpu_amd.bi.abi import DGFSBiAstd1D
     runs = {}
     for prec in ('double', 'single'):
-        cfg = _n48_deck(**{('velocity-mesh', 'Nv'): 16, ('velocity-mesh', 'Nrho'): 8,
-                           ('config', 'precision'): prec})
+        cfg = _n48_deck({('velocity-mesh', 'Nv'): 16, ('velocity-mesh', 'Nrho'): 8,
+                         ('config', 'precision'): prec})
         s = DGFSBiAstd1D(cfg, device=cuda, writers=False)
         for k in range(3):
             print(prec, 'step', k, flush=True)

@@ -124,6 +124,7 @@ def test_collision_reference_form(cuda, N, M, Nrho):
         u0d = torch.tensor(u0, dtype=td, device=cuda)
         Q0 = torch.zeros(u.shape, dtype=torch.float64, device=cuda)
         plan64(u0d.double(), Q0)                       # Q(f0, f0) of the stored reference
+        plan.reference_set(u0d)
         Qr = torch.zeros_like(ud)
         plan.reference(ud, u0d, Q0, Qr)
         Qd = torch.zeros_like(ud)
