@@ -109,6 +109,8 @@ SIGNATURES = {
     'dgfs_residual': (_i, [_i, _l, _vp, _vp, _vp, _vp, _vp]),
     'dgfs_projection_work_bytes': (_sz, [_i, _i]),
     'dgfs_project_conservative': (_i, [_i, _i, _i, _vp, _vp, _vp, _dp, _vp, _vp, _i, _vp, _vp]),
+    'dgfs_project_conservative_noalloc': (_i, [_i, _i, _i, _vp, _vp, _vp, _dp, _vp, _vp, _i,
+                                               _vp]),
     'dgfs_projection_mixture_work_bytes': (_sz, [_i, _i]),
     'dgfs_project_conservative_mixture': (_i, [_i, _i, _i, _vp, _vp, _vp, _d, _d, _dp, _vp, _vp,
                                                _vp, _vp, _i, _vp, _vp]),

@@ -18,6 +18,7 @@
 // Shakov state), and the apply kernel reads the constants (no atomics, no
 // host synchronisation).
 #include <algorithm>
+#include <mutex>
 
 #include "dgfs_common.h"
 #include "dgfs_hip.h"
@@ -775,44 +776,108 @@ extern "C" size_t dgfs_projection_work_bytes(int nslice, int Nv) {
     return size_t(nslice) * (nb * 5 + 8) * sizeof(double);
 }
 
+namespace {
+
+// Launches of the single-species projection on a caller-provided workspace
+// (work: dgfs_projection_work_bytes(nslice, Nv) bytes of device memory).
+int project_conservative_on(int dtype, int nslice, int Nv, const void* cx, const void* cy,
+                            const void* cz, const double* gram_inv, const void* Q, void* out,
+                            int accumulate, double* work, hipStream_t stream) {
+    Gram5 G;
+    for (int i = 0; i < 25; ++i) G.g[i] = gram_inv[i];
+    const int nb = (Nv + kRB - 1) / kRB;
+    const dim3 grid(nb, nslice);
+    double* lam = work + size_t(nslice) * nb * 5;
+    if (dtype == DGFS_F64)
+        hipLaunchKernelGGL(k_proj_partial<double>, grid, dim3(kRB), 0, stream, Nv,
+                           (const double*)Q, (const double*)cx, (const double*)cy,
+                           (const double*)cz, work);
+    else
+        hipLaunchKernelGGL(k_proj_partial<float>, grid, dim3(kRB), 0, stream, Nv,
+                           (const float*)Q, (const float*)cx, (const float*)cy,
+                           (const float*)cz, work);
+    DGFS_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_proj_final, dim3(nslice), dim3(kRB), 0, stream, nb, G,
+                       (const double*)work, lam);
+    DGFS_HIP_CHECK(hipGetLastError());
+    if (dtype == DGFS_F64)
+        hipLaunchKernelGGL(k_proj_apply<double>, grid, dim3(kRB), 0, stream, Nv,
+                           (const double*)Q, (const double*)cx, (const double*)cy,
+                           (const double*)cz, (const double*)lam, (double*)out, accumulate);
+    else
+        hipLaunchKernelGGL(k_proj_apply<float>, grid, dim3(kRB), 0, stream, Nv,
+                           (const float*)Q, (const float*)cx, (const float*)cy,
+                           (const float*)cz, (const double*)lam, (float*)out, accumulate);
+    DGFS_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// Workspace of dgfs_project_conservative_noalloc: one array per GPU, held
+// until the process ends.  Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in for the symbol): its device
+// address is looked up once per device with hipGetSymbolAddress and kept in
+// scratch_dev[].
+constexpr size_t kScratchBytes = DGFS_PROJECTION_SCRATCH_BYTES;
+__device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+double* scratch_dev[kMaxDevices] = {};
+
+// Device address of the current device's scratch array (looked up on the
+// device's first call).
+int scratch_address(double** out) {
+    int dev = -1;
+    DGFS_HIP_CHECK(hipGetDevice(&dev));
+    DGFS_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void* p = nullptr;
+        DGFS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        DGFS_REQUIRE(p != nullptr, "no device address for the projection scratch array");
+        scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = scratch_dev[dev];
+    return 0;
+}
+
+int project_check(int dtype, int nslice, int Nv, const void* cx, const void* cy, const void* cz,
+                  const double* gram_inv, const void* Q, void* out) {
+    DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "dtype must be DGFS_F64 or DGFS_F32");
+    DGFS_REQUIRE(nslice >= 0 && Nv > 0, "bad sizes");
+    DGFS_REQUIRE(gram_inv && Q && out && cx && cy && cz, "null pointer");
+    DGFS_REQUIRE(nslice <= 65535, "too many slices");
+    return 0;
+}
+
+}  // namespace
+
 extern "C" int dgfs_project_conservative(int dtype, int nslice, int Nv, const void* cx,
                                          const void* cy, const void* cz, const double* gram_inv,
                                          const void* Q, void* out, int accumulate, double* work,
                                          hipStream_t stream) {
-    {
-        DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "dtype must be DGFS_F64 or DGFS_F32");
-        DGFS_REQUIRE(nslice >= 0 && Nv > 0, "bad sizes");
-        DGFS_REQUIRE(gram_inv && Q && out && work && cx && cy && cz, "null pointer");
-        if (nslice == 0) return 0;
-        DGFS_REQUIRE(nslice <= 65535, "too many slices");
-        Gram5 G;
-        for (int i = 0; i < 25; ++i) G.g[i] = gram_inv[i];
-        const int nb = (Nv + kRB - 1) / kRB;
-        const dim3 grid(nb, nslice);
-        double* lam = work + size_t(nslice) * nb * 5;
-        if (dtype == DGFS_F64)
-            hipLaunchKernelGGL(k_proj_partial<double>, grid, dim3(kRB), 0, stream, Nv,
-                               (const double*)Q, (const double*)cx, (const double*)cy,
-                               (const double*)cz, work);
-        else
-            hipLaunchKernelGGL(k_proj_partial<float>, grid, dim3(kRB), 0, stream, Nv,
-                               (const float*)Q, (const float*)cx, (const float*)cy,
-                               (const float*)cz, work);
-        DGFS_HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_proj_final, dim3(nslice), dim3(kRB), 0, stream, nb, G,
-                           (const double*)work, lam);
-        DGFS_HIP_CHECK(hipGetLastError());
-        if (dtype == DGFS_F64)
-            hipLaunchKernelGGL(k_proj_apply<double>, grid, dim3(kRB), 0, stream, Nv,
-                               (const double*)Q, (const double*)cx, (const double*)cy,
-                               (const double*)cz, (const double*)lam, (double*)out, accumulate);
-        else
-            hipLaunchKernelGGL(k_proj_apply<float>, grid, dim3(kRB), 0, stream, Nv,
-                               (const float*)Q, (const float*)cx, (const float*)cy,
-                               (const float*)cz, (const double*)lam, (float*)out, accumulate);
-        DGFS_HIP_CHECK(hipGetLastError());
-        return 0;
-    }
+    if (int st = project_check(dtype, nslice, Nv, cx, cy, cz, gram_inv, Q, out)) return st;
+    DGFS_REQUIRE(work, "null pointer");
+    if (nslice == 0) return 0;
+    return project_conservative_on(dtype, nslice, Nv, cx, cy, cz, gram_inv, Q, out, accumulate,
+                                   work, stream);
+}
+
+extern "C" int dgfs_project_conservative_noalloc(int dtype, int nslice, int Nv, const void* cx,
+                                                 const void* cy, const void* cz,
+                                                 const double* gram_inv, const void* Q,
+                                                 void* out, int accumulate, hipStream_t stream) {
+    if (int st = project_check(dtype, nslice, Nv, cx, cy, cz, gram_inv, Q, out)) return st;
+    if (nslice == 0) return 0;
+    const size_t need = dgfs_projection_work_bytes(nslice, Nv);
+    DGFS_REQUIRE(need <= kScratchBytes,
+                 "nslice=%d, Nv=%d needs %zu bytes of workspace; the no-allocation projection "
+                 "holds %zu (use dgfs_project_conservative with a caller workspace)",
+                 nslice, Nv, need, kScratchBytes);
+    double* work = nullptr;
+    if (int st = scratch_address(&work)) return st;
+    return project_conservative_on(dtype, nslice, Nv, cx, cy, cz, gram_inv, Q, out, accumulate,
+                                   work, stream);
 }
 
 extern "C" size_t dgfs_projection_mixture_work_bytes(int nslice, int Nv) {

@@ -307,6 +307,22 @@ int dgfs_project_conservative(int dtype, int nslice, int Nv, const void* cx, con
                               const void* cz, const double* gram_inv, const void* Q, void* out,
                               int accumulate, double* work, hipStream_t stream);
 
+/* dgfs_project_conservative without the work argument: it allocates nothing.
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The workspace is a fixed array of
+ * DGFS_PROJECTION_SCRATCH_BYTES in the library's device image, one per GPU,
+ * held until the process ends; calls needing more
+ * (dgfs_projection_work_bytes(nslice, Nv) > DGFS_PROJECTION_SCRATCH_BYTES,
+ * e.g. nslice > 241 at Nv = 48^3) return status 2.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (concurrent streams
+ * or host threads) race.  Results are identical to dgfs_project_conservative. */
+#define DGFS_PROJECTION_SCRATCH_BYTES (4u << 20)
+int dgfs_project_conservative_noalloc(int dtype, int nslice, int Nv, const void* cx,
+                                      const void* cy, const void* cz, const double* gram_inv,
+                                      const void* Q, void* out, int accumulate,
+                                      hipStream_t stream);
+
 /* Mixture conservation projection (FP64; no reference counterpart).  Each
  * Q_pq of the binary VSS operator keeps species p's mass, but momentum and
  * energy are kept only by the pair sum, so for each slice s of the two
