@@ -124,8 +124,10 @@ def fp64_accuracy(text, nsteps, s32, dev):
     m64 = mom(s64.state_fp64()).reshape(-1, 9)
     m32 = mom(s32.state_fp64()).reshape(-1, 9)      # FP32 state (hi + lo), FP64 moments
     err = {}
+    speed = float(np.abs(m64[:, 1:3]).max())
     for i, name in enumerate(MOMENT_NAMES):
-        scale = float(np.abs(m64[:, i]).max())
+        # u_x stays near zero in this flow: relative to the largest flow speed
+        scale = speed if name == 'ux' else float(np.abs(m64[:, i]).max())
         err[name] = {'max_abs': float(np.abs(m32[:, i] - m64[:, i]).max()), 'field_max': scale}
         err[name]['rel'] = err[name]['max_abs'] / scale if scale > 0 else None
     cv = s64.vm.cv()
@@ -138,6 +140,33 @@ def fp64_accuracy(text, nsteps, s32, dev):
         cons[tag] = float((np.abs(phi @ q.T) / (np.abs(phi) @ np.abs(q).T)).max())
     return {'steps': nsteps, 'moments_fp32_vs_fp64': err,
             'collision_invariant_moments_rel': cons}
+
+
+def direct_form(text, steps, warmup, dev, accuracy=True):
+    """The same deck with the direct FP32 collision (reference-refresh = 0):
+    ms per step, phase-space updates/s and, unless accuracy is False, the
+    moments against the FP64 path after the same steps."""
+    import torch
+    from dgfs1d_gpu_amd.dictionary import Dictionary
+    from dgfs1d_gpu_amd.std.std import DGFSStd1D
+    cfg = Dictionary(text)
+    cfg.set('scattering-model', 'reference-refresh', '0')
+    s = DGFSStd1D(cfg, device=dev, writers=False)
+    for _ in range(warmup):
+        s.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = {'collision_form': 'direct (FP32 Q(f, f))', 'ms_per_step': wall / steps * 1e3,
+           'value': s.Ne * s.K * s.vm.vsize() * steps / wall}
+    if accuracy:
+        acc = fp64_accuracy(text, warmup + steps, s, dev)
+        out['moments_fp32_vs_fp64'] = {k: v['rel'] for k, v in acc['moments_fp32_vs_fp64'].items()}
+    del s
+    return out
 
 
 def batched_workload(seed, count, steps, warmup, dev):
@@ -467,12 +496,20 @@ def main(argv=None):
     B = s.vm.M() * s.vm.Nrho()
     ntask = s.ntasks
     ref = getattr(s.sm, '_ref', None)
-    # the reference-state form runs the bilinear gain on (f - f0, f + f0)
-    fg_same = plan.tasks['fg_same'] and ref is None
     # k_gain: one packed inverse 3D transform per (task, p, q) and function
-    # (f, and g when f != g) plus the survey's pointwise count per (p, q, k)
-    nvar, pw = (1, 14) if fg_same else (2, 26)
+    # (f, and g when f != g; h = f - f0 alone in the reference-state form)
+    # plus the survey's pointwise count per (p, q, k): 14 for f = g, 26 for the
+    # bilinear form, 22 for the reference-state form (f = g's 14 and one more
+    # complex product and sum with the stored transforms of f0)
+    if ref is not None:
+        nvar, pw = 1, 22
+    else:
+        nvar, pw = (1, 14) if plan.tasks['fg_same'] else (2, 26)
     gain_flops = ntask * B * (nvar * fft_flops(n) + pw * n)
+    # HBM bytes the reference-state gain streams per launch: u+ and u- of f0
+    # (two complex values) per (task, p, q, point)
+    gain_stream = ntask * B * n * 4 * (4 if cfg.dtype.__name__ == 'float32' else 8) \
+        if ref is not None else 0
     nc = max(ncalls, 1)
     stages = dict(zip(plan.STAGES, (m / nc for m in stage_ms)))
     gain_ms = stages['gain']
@@ -504,7 +541,10 @@ def main(argv=None):
         'roofline': {'bound': 'fp32-valu', 'kernel': 'k_gain', 'achieved': achieved,
                      'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / PEAK_FP32_TFLOPS, 'traffic': traffic,
-                     'flops_per_launch': gain_flops, 'avg_launch_ms': gain_ms},
+                     'flops_per_launch': gain_flops, 'avg_launch_ms': gain_ms,
+                     'streamed_bytes_per_launch': gain_stream,
+                     'streamed_GBs': gain_stream / (gain_ms * 1e-3) / 1e9,
+                     'streamed_frac_of_hbm': gain_stream / (gain_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
         'collision_stage_ms': stages,
         'collision_ms_per_rhs': coll_ms,
         'collision_integrals_per_s': ntask * ncalls / (sum(stage_ms) * 1e-3),
@@ -512,6 +552,9 @@ def main(argv=None):
     }
     if not args.no_accuracy:
         out['accuracy'] = fp64_accuracy(text, args.warmup + args.steps, s, dev)
+    if ref is not None:
+        out['direct_form'] = direct_form(text, args.steps, args.warmup, dev,
+                                         accuracy=not args.no_accuracy)
     if not args.no_batch or not args.no_steady or not args.no_bi:
         del s
     if not args.no_batch:

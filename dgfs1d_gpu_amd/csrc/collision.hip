@@ -4,6 +4,7 @@
 #include "collision_kernels.h"
 
 #include <cmath>
+#include <type_traits>
 #include <cstdarg>
 #include <cstdio>
 #include <string>
@@ -32,6 +33,11 @@ static int dispatch(CollisionPlan* pl, const void* u, const void* ug, void* Q, T
     switch (pl->N) {
         DGFS_CASE(4) DGFS_CASE(6) DGFS_CASE(8) DGFS_CASE(12) DGFS_CASE(16) DGFS_CASE(20)
         DGFS_CASE(24) DGFS_CASE(32)
+        case 48:
+            if constexpr (std::is_same<T, float>::value)
+                return collide_launch<T, 48>(pl, u, ug, Q, tv, ntask, ngroup, gs, gsl, fgs, st,
+                                             Q0);
+            break;
         default: break;
     }
 #undef DGFS_CASE
@@ -48,6 +54,10 @@ static int dispatch_reference_set(CollisionPlan* pl, const void* u0, TaskView tv
     switch (pl->N) {
         DGFS_CASE(4) DGFS_CASE(6) DGFS_CASE(8) DGFS_CASE(12) DGFS_CASE(16) DGFS_CASE(20)
         DGFS_CASE(24) DGFS_CASE(32)
+        case 48:
+            if constexpr (std::is_same<T, float>::value)
+                return reference_set_launch<T, 48>(pl, u0, tv, ntask, st);
+            break;
         default: break;
     }
 #undef DGFS_CASE
@@ -56,7 +66,8 @@ static int dispatch_reference_set(CollisionPlan* pl, const void* u0, TaskView tv
 }
 
 static bool supported_N(int N) {
-    return N == 4 || N == 6 || N == 8 || N == 12 || N == 16 || N == 20 || N == 24 || N == 32;
+    return N == 4 || N == 6 || N == 8 || N == 12 || N == 16 || N == 20 || N == 24 || N == 32 ||
+           N == 48;
 }
 
 template <typename T>
@@ -76,6 +87,9 @@ extern "C" int dgfs_collision_plan_create(int N, int M, int Nrho, double L, cons
     DGFS_REQUIRE(out != nullptr, "collision_plan_create: null output handle");
     *out = nullptr;
     DGFS_REQUIRE(dgfs::supported_N(N), "collision_plan_create: unsupported N=%d", N);
+    DGFS_REQUIRE(N != 48 || dtype == DGFS_F32,
+                 "collision_plan_create: N=48 is built for DGFS_F32 only (its FP64 plane "
+                 "buffers exceed a work-group's 160 KiB of LDS)");
     DGFS_REQUIRE(M > 0 && Nrho > 0 && ntask_max > 0, "collision_plan_create: bad sizes");
     DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "collision_plan_create: bad dtype");
     DGFS_REQUIRE(variant == DGFS_VHS || variant == DGFS_VHS_GLL,

@@ -246,21 +246,45 @@ template <typename T, int N, int S>
 constexpr int gain_lds_bytes(int NW) {
     return N * N * (N / S + 1) * 2 * int(sizeof(T)) + NW * NyLayout<N, S>::SIZE * int(sizeof(T));
 }
-template <typename T, int N, int NW>
+// smallest residue split S (a divisor of N) whose LDS image fits a work-group
+template <typename T, int N, int NW, int S = 1>
 constexpr int pick_split_gain() {
-    if constexpr (N % 1 == 0 && gain_lds_bytes<T, N, 1>(NW) <= 160 * 1024) return 1;
-    else if constexpr (N % 2 == 0 && gain_lds_bytes<T, N, 2>(NW) <= 160 * 1024) return 2;
-    else if constexpr (N % 3 == 0 && gain_lds_bytes<T, N, 3>(NW) <= 160 * 1024) return 3;
-    else if constexpr (N % 4 == 0 && gain_lds_bytes<T, N, 4>(NW) <= 160 * 1024) return 4;
-    else if constexpr (N % 6 == 0 && gain_lds_bytes<T, N, 6>(NW) <= 160 * 1024) return 6;
-    else return 8;
+    if constexpr (S > N) {
+        static_assert(S <= N, "no residue split fits the gain kernel in LDS");
+        return N;
+    } else if constexpr (N % S == 0 && gain_lds_bytes<T, N, S>(NW) <= 160 * 1024) {
+        return S;
+    } else {
+        return pick_split_gain<T, N, NW, S + 1>();
+    }
 }
 
 // u+ and u- of one point (the reference state's cache entry)
 template <typename T>
-struct UPair {
+struct alignas(4 * sizeof(T)) UPair {
     cplx<T> p, m;
 };
+
+// the cache is streamed once per call: non-temporal accesses keep it from
+// evicting the transforms the other work-groups of a task re-read from L2
+template <typename T>
+DGFS_DEV UPair<T> load_pair_nt(const UPair<T>* p) {
+    const T* q = reinterpret_cast<const T*>(p);
+    UPair<T> v;
+    v.p.x = __builtin_nontemporal_load(q);
+    v.p.y = __builtin_nontemporal_load(q + 1);
+    v.m.x = __builtin_nontemporal_load(q + 2);
+    v.m.y = __builtin_nontemporal_load(q + 3);
+    return v;
+}
+template <typename T>
+DGFS_DEV void store_pair_nt(UPair<T>* p, const UPair<T>& v) {
+    T* q = reinterpret_cast<T*>(p);
+    __builtin_nontemporal_store(v.p.x, q);
+    __builtin_nontemporal_store(v.p.y, q + 1);
+    __builtin_nontemporal_store(v.m.x, q + 2);
+    __builtin_nontemporal_store(v.m.y, q + 3);
+}
 
 enum GainMode { kGainAccumulate = 0, kGainStore = 1, kGainDelta = 2 };
 
@@ -370,7 +394,7 @@ k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
                         sfor<0, N>([&](auto I) {
                             UPair<T> v;
                             unpack(0, I, o[I], v.p, v.m);
-                            dst[size_t(int(I)) * S * E::NPEN] = v;
+                            store_pair_nt(dst + size_t(int(I)) * S * E::NPEN, v);
                         });
                     } else if constexpr (MODE == kGainDelta) {
                         const UPair<T>* src = ucache +
@@ -379,7 +403,7 @@ k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
                         sfor<0, N>([&](auto I) {
                             cplx<T> up, um;
                             unpack(0, I, o[I], up, um);
-                            const UPair<T> c0 = src[size_t(int(I)) * S * E::NPEN];
+                            const UPair<T> c0 = load_pair_nt(src + size_t(int(I)) * S * E::NPEN);
                             acc[I] = acc[I] + up * (um + c0.m) + um * c0.p;
                         });
                     } else if constexpr (NW == 1) {
@@ -766,6 +790,7 @@ int collide_launch(CollisionPlan* pl, const void* u, const void* ug, void* Q, Ta
 #define DGFS_COLLIDE_SIZES(X, T) X(T, 4) X(T, 6) X(T, 8) X(T, 12) X(T, 16) X(T, 20) X(T, 24) X(T, 32)
 
 DGFS_COLLIDE_SIZES(DGFS_COLLIDE_EXTERN, float)
+DGFS_COLLIDE_EXTERN(float, 48)        // N = 48: FP32 only (FP64 plane buffers exceed LDS)
 DGFS_COLLIDE_SIZES(DGFS_COLLIDE_EXTERN, double)
 
 }  // namespace dgfs

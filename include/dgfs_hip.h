@@ -43,7 +43,7 @@ const char* dgfs_last_error(void);
  * kernels (dgfs1D/std/kernels/scattering/vhs-gll.mako:43-99): builds the
  * phase, radial-weight (bb1) and loss (bb2) tables in FP64 on the host and
  * allocates scratch for up to ntask_max collision tasks.
- *   N      velocity points per axis (4, 6, 8, 12, 16, 20, 24, 32)
+ *   N      velocity points per axis (4, 6, 8, 12, 16, 20, 24, 32; 48 with DGFS_F32)
  *   qz,qw  N_rho radial nodes/weights on [0, R]   (velocitymesh.py:93-96)
  *   sz     M x 3 half-sphere directions, sw = 2 pi / M (velocitymesh.py:98-105)
  *   gamma  2 (1 - omega);  prefac as scattering.py:81-82

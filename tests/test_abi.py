@@ -38,6 +38,18 @@ def test_errors_are_reported():
     assert b'unsupported N=7' in _lib.lib().dgfs_last_error()
 
 
+def test_collision_n48_is_fp32_only():
+    """N = 48 single-species plans are built for FP32 only; FP64 is refused
+    before any device work."""
+    import ctypes as C
+    from dgfs1d_gpu_amd import _lib
+    h = C.c_void_p()
+    qz = (C.c_double * 3)(1.0, 0.0, 0.0)
+    st = _lib.lib().dgfs_collision_plan_create(48, 1, 1, 1.0, qz, qz, qz, 1.0, 0.0, 1.0,
+                                               _lib.F64, _lib.VHS, 1, C.byref(h))
+    assert st == 2 and b'DGFS_F32 only' in _lib.lib().dgfs_last_error()
+
+
 def test_new_entry_points_reject_bad_arguments():
     """Argument checks of the mixture, limiter, projection and ES-BGK entry
     points fail before any device work (status 2, message kept)."""

@@ -91,6 +91,17 @@ def test_collision_fp32(cuda, N, M, Nrho):
     assert err < 2e-5
 
 
+def test_collision_n48_fp32(cuda):
+    """The largest grid (N = 48, FP32 only): nodal f = g tasks and bilinear
+    grouped tasks against the FP64 oracle, 3e-5 of max|Q| (the N = 16..32
+    FP32 bound grows with the transform length)."""
+    from dgfs1d_gpu_amd import _lib
+    for name, spec in (('nodal', NODAL[:2]), ('bilinear', BILINEAR[:2])):
+        err = _run(cuda, 48, 6, 4, _lib.F32, _lib.VHS, spec)
+        print('fp32 N=48', name, err, flush=True)
+        assert err < 3e-5, name
+
+
 @pytest.mark.parametrize('N,M,Nrho', [(8, 6, 4), (16, 6, 8)])
 def test_collision_reference_form(cuda, N, M, Nrho):
     """dgfs_collide_reference: FP64 -- equal to the direct Q(f, f) to 1e-12;
