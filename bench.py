@@ -189,11 +189,15 @@ def batched_workload(seed, count, steps, warmup, dev):
     wall = time.perf_counter() - t0
     return {'problems': count, 'seed': seed, 'points': b.points,
             'groups': [{'N': g['N'], 'problems': len(g['solvers']), 'collisions_per_rhs':
-                        g['ntask'], 'qgroups': tune[g['N']][0]} for g in b.groups],
+                        g['ntask'], 'schedule': dict(zip(('qgroups', 'split_mult', 'xcd_remap'),
+                                                         tune[g['N']][0]))}
+                       for g in b.groups],
+            'transport': 'batched (3 launches per RHS for the 64 problems)',
             'steps': steps, 'ms_per_step': wall / steps * 1e3,
             'value': b.points * steps / wall, 'unit': 'phase-space point-steps/s',
             'graph': True, 'setup_s': setup,
-            'autotune_ms': {str(k): v[1] for k, v in tune.items()}}
+            'autotune_ms': {str(k): {'qg%d_split%d_remap%d' % sc: ms for sc, ms in v[1].items()}
+                            for k, v in tune.items()}}
 
 
 def closure_forward(nodes, G, reps, dev):
@@ -496,16 +500,20 @@ def main(argv=None):
     B = s.vm.M() * s.vm.Nrho()
     ntask = s.ntasks
     ref = getattr(s.sm, '_ref', None)
-    # k_gain: one packed inverse 3D transform per (task, p, q) and function
-    # (f, and g when f != g; h = f - f0 alone in the reference-state form)
-    # plus the survey's pointwise count per (p, q, k): 14 for f = g, 26 for the
-    # bilinear form, 22 for the reference-state form (f = g's 14 and one more
-    # complex product and sum with the stored transforms of f0)
+    fgs = plan.tasks['fg_same'] or ref is not None
+    # k_gain, algorithmic count (the survey's, of the work the kernel replaces):
+    # per (task, p, q) the reference's two inverse 3D transforms (cos and sin
+    # parts) per function (f, and g when f != g) and its pointwise count, 14
+    # per (p, q, k) for f = g, 26 for the bilinear form
+    gain_flops = ntask * B * ((2 if fgs else 4) * fft_flops(n) + (14 if fgs else 26) * n)
+    # executed count: one packed inverse transform per (p, q) and function (h
+    # alone in the reference-state form), and 22 pointwise flops there (f = g's
+    # 14 and one more complex product and sum with the stored transforms of f0)
     if ref is not None:
         nvar, pw = 1, 22
     else:
         nvar, pw = (1, 14) if plan.tasks['fg_same'] else (2, 26)
-    gain_flops = ntask * B * (nvar * fft_flops(n) + pw * n)
+    gain_exec = ntask * B * (nvar * fft_flops(n) + pw * n)
     # HBM bytes the reference-state gain streams per launch: u+ and u- of f0
     # (two complex values) per (task, p, q, point)
     gain_stream = ntask * B * n * 4 * (4 if cfg.dtype.__name__ == 'float32' else 8) \
@@ -514,6 +522,7 @@ def main(argv=None):
     stages = dict(zip(plan.STAGES, (m / nc for m in stage_ms)))
     gain_ms = stages['gain']
     achieved = gain_flops / (gain_ms * 1e-3) / 1e12
+    executed = gain_exec / (gain_ms * 1e-3) / 1e12
     coll_ms = sum(stage_ms) / nc
     q_survey = (4 + 5 * B) * fft_flops(n) + 26 * B * n   # per Q, reference-structured count
     value = Ne * K * n * args.steps / wall
@@ -542,6 +551,9 @@ def main(argv=None):
                      'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / PEAK_FP32_TFLOPS, 'traffic': traffic,
                      'flops_per_launch': gain_flops, 'avg_launch_ms': gain_ms,
+                     'executed_flops_per_launch': gain_exec, 'executed_tflops': executed,
+                     'executed_frac': executed / PEAK_FP32_TFLOPS,
+                     'peak_without_packed_fp32': PEAK_FP32_TFLOPS / 2,
                      'streamed_bytes_per_launch': gain_stream,
                      'streamed_GBs': gain_stream / (gain_ms * 1e-3) / 1e9,
                      'streamed_frac_of_hbm': gain_stream / (gain_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},

@@ -83,6 +83,7 @@ SIGNATURES = {
     'dgfs_collision_plan_destroy': (_i, [_vp]),
     'dgfs_collision_plan_query': (_i, [_vp, _ip, _ip, _ip, _szp]),
     'dgfs_collision_plan_set_qgroups': (_i, [_vp, _i]),
+    'dgfs_collision_plan_set_schedule': (_i, [_vp, _i, _i]),
     'dgfs_collide': (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _i,
                           _vp]),
     'dgfs_collision_reference_set': (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp]),
@@ -99,6 +100,11 @@ SIGNATURES = {
     'dgfs_limit_positivity': (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'dgfs_transport_work_bytes': (_sz, [_i]),
     'dgfs_transport_rhs': (_i, [C.POINTER(Transport), _vp, _vp, _d, _vp]),
+    'dgfs_transport_batch_create': (_i, [C.POINTER(Transport), _i, C.POINTER(_l),
+                                         C.POINTER(_vp)]),
+    'dgfs_transport_batch_destroy': (_i, [_vp]),
+    'dgfs_transport_batch_rhs': (_i, [_vp, _vp, _vp, _vp]),
+    'dgfs_transport_batch_nw': (_i, [_vp, _dp]),
     'dgfs_matop': (_i, [_i, _i, _i, _l, _vp, _vp, _vp, _i, _vp]),
     'dgfs_axnpby': (_i, [_i, _l, _i, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _vp]),
     'dgfs_axnpby_compensated': (_i, [_l, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d,

@@ -10,10 +10,14 @@ dgfs1D/std/std.py:37-423).  Design:
   - the collisions of all problems that share a velocity grid (N, M, N_rho,
     L, prefactor) run as ONE batched dgfs_collide launch sequence over a task
     table that spans those problems (tasks may address any slice of the group);
-* transport runs per problem (its elements, basis and walls differ);
+* the transport of every problem runs as one batched launch sequence
+  (dgfs_transport_batch_rhs: three launches for the whole batch, each
+  problem with its own elements, basis and walls);
 * a whole time step can be captured once in a HIP graph and replayed;
-* ``autotune`` times the q-group count of every collision plan on its real
-  task table and keeps the fastest (dgfs_collision_plan_set_qgroups).
+* ``autotune`` times, for every collision plan on its real task table, the
+  gain kernel's schedules -- q-group count, residue split (the smallest that
+  fits LDS or twice it) and work-group order (XCD-aware or dispatch order) --
+  and keeps the fastest (dgfs_collision_plan_set_qgroups, _set_schedule).
 """
 import time
 
@@ -24,6 +28,7 @@ from .collision import CollisionPlan, task_table
 from .dictionary import Dictionary
 from .std.scattering import _fold_same_slice
 from .std.std import DGFSStd1D
+from .transport import BatchTransport
 
 R0 = 8.3144598
 
@@ -112,7 +117,7 @@ def make_problems(seed=0, count=64):
 class BatchedStd1D:
     """Flat-tensor batch of DGFSStd1D problems sharing one dt and scheme."""
 
-    def __init__(self, specs, device, precision='float', dt=2e-4):
+    def __init__(self, specs, device, precision='float', dt=2e-4, batched_transport=True):
         import torch
         self.torch = torch
         self.device = device
@@ -134,6 +139,10 @@ class BatchedStd1D:
         self.dt = float(dt)
         self.time = 0.0
         self.points = sum(s.K * s.Ne * s.vm.vsize() for s in self.solvers)
+        self.btransport = None
+        if batched_transport:
+            self.btransport = BatchTransport([s.transport for s in self.solvers],
+                                             self.offsets[:-1])
         self._groups()
         from .std.integrator import SSPRK2IntegratorStd
         self.odes = SSPRK2IntegratorStd(self.rhs, (1, 1, self.total), self.tdtype, device=device)
@@ -170,9 +179,12 @@ class BatchedStd1D:
             i = j
 
     def rhs(self, t, u_in, u_out):
-        for k, s in enumerate(self.solvers):
-            a, b = int(self.offsets[k]), int(self.offsets[k + 1])
-            s.transport(t, u_in[a:b], u_out[a:b])
+        if self.btransport is not None:
+            self.btransport(u_in, u_out)
+        else:
+            for k, s in enumerate(self.solvers):
+                a, b = int(self.offsets[k]), int(self.offsets[k + 1])
+                s.transport(t, u_in[a:b], u_out[a:b])
         for g in self.groups:
             g['plan'](u_in[g['lo']:g['hi']], u_out[g['lo']:g['hi']])
 
@@ -202,29 +214,34 @@ class BatchedStd1D:
         self.graph.replay()          # the captured step itself has not run yet
         self.time += self.dt
 
-    def autotune(self, candidates=(1, 2, 3, 4, 6, 12), reps=3):
-        """Per collision plan: time dgfs_collide for each q-group count that
-        divides M and keep the fastest.  Returns {N: (best, {qg: ms})}."""
+    def autotune(self, candidates=(1, 2, 3, 4, 6, 12), splits=(1, 2), remaps=(1, 0), reps=3):
+        """Per collision plan: time dgfs_collide for every schedule (q-group
+        count dividing M, residue-split multiplier, work-group order) and keep
+        the fastest.  Returns {N: ((qg, split, remap), {schedule: ms})}."""
         torch = self.torch
         out = {}
         scratch = torch.zeros_like(self.u)
         for g in self.groups:
             plan = g['plan']
+            u, Q = self.u[g['lo']:g['hi']], scratch[g['lo']:g['hi']]
             times = {}
             for qg in candidates:
                 if plan.M % qg:
                     continue
-                plan.set_qgroups(qg)
-                u, Q = self.u[g['lo']:g['hi']], scratch[g['lo']:g['hi']]
-                plan(u, Q)
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                for _ in range(reps):
-                    plan(u, Q)
-                torch.cuda.synchronize()
-                times[qg] = (time.perf_counter() - t0) / reps * 1e3
+                for sm in splits:
+                    for rm in remaps:
+                        plan.set_qgroups(qg)
+                        plan.set_schedule(sm, rm)
+                        plan(u, Q)
+                        torch.cuda.synchronize()
+                        t0 = time.perf_counter()
+                        for _ in range(reps):
+                            plan(u, Q)
+                        torch.cuda.synchronize()
+                        times[(qg, sm, rm)] = (time.perf_counter() - t0) / reps * 1e3
             best = min(times, key=times.get)
-            plan.set_qgroups(best)
+            plan.set_qgroups(best[0])
+            plan.set_schedule(best[1], best[2])
             out[g['N']] = (best, times)
         return out
 

@@ -94,6 +94,13 @@ class CollisionPlan:
         """Split the M sphere directions of the gain kernel into qg groups."""
         _lib.check(_lib.lib().dgfs_collision_plan_set_qgroups(self._h, int(qg)))
 
+    def set_schedule(self, split_mult=1, xcd_remap=1):
+        """The f = g gain kernel's residue split (1: smallest that fits LDS,
+        2: twice that) and work-group order (1: XCD-aware, 0: dispatch order)."""
+        _lib.check(_lib.lib().dgfs_collision_plan_set_schedule(self._h, int(split_mult),
+                                                               int(xcd_remap)))
+        self.schedule = (int(split_mult), int(xcd_remap))
+
     def set_tasks(self, K, Ne, tasks):
         """tasks: list of (out_slice, f_terms, g_terms), terms = [(slice, coef), ...].
         Output slices are u-layout slice indices mode*Ne + elem."""

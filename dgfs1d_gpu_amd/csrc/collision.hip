@@ -261,6 +261,19 @@ extern "C" int dgfs_collision_plan_set_qgroups(dgfs_collision_plan_t plan, int q
     return 0;
 }
 
+extern "C" int dgfs_collision_plan_set_schedule(dgfs_collision_plan_t plan, int split_mult,
+                                                int xcd_remap) {
+    auto* pl = static_cast<CollisionPlan*>(plan);
+    DGFS_REQUIRE(pl, "collision_plan_set_schedule: null plan");
+    DGFS_REQUIRE(split_mult == 1 || split_mult == 2,
+                 "collision_plan_set_schedule: split_mult must be 1 or 2");
+    DGFS_REQUIRE(xcd_remap == 0 || xcd_remap == 1,
+                 "collision_plan_set_schedule: xcd_remap must be 0 or 1");
+    pl->gain_smul = split_mult;
+    pl->gain_remap = xcd_remap;
+    return 0;
+}
+
 extern "C" int dgfs_collide(dgfs_collision_plan_t plan, const void* u, const void* ug, void* Q,
                             int K, int Ne,
                             int ntask, const int* task_nterm, const int* task_slice,

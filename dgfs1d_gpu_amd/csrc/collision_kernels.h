@@ -298,7 +298,7 @@ template <typename T, int N, int S, int NW, int MODE = kGainAccumulate>
 __global__ void __launch_bounds__(round64(N * (N / S)))
 k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
        const T* __restrict__ nyq, int M, int Nrho, int QG, cplx<T>* __restrict__ acc_out,
-       UPair<T>* __restrict__ ucache) {
+       UPair<T>* __restrict__ ucache, int remap) {
     static_assert(MODE == kGainAccumulate || NW == 1, "reference-state modes transform one function");
     using E = Fft3Engine<T, N, S, round64(N * (N / S)), +1>;
     using NL = NyLayout<N, S>;
@@ -307,7 +307,7 @@ k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
     __shared__ cplx<T> V[E::LDS_ELEMS];
     __shared__ T Ny[NW * NL::SIZE];
     const int nblk = gridDim.x;
-    const int lb = xcd_remap(blockIdx.x, nblk);
+    const int lb = remap ? xcd_remap(blockIdx.x, nblk) : int(blockIdx.x);
     const int qg = lb % QG;
     const int r = (lb / QG) % S;
     const int p = (lb / (QG * S)) % Nrho;
@@ -400,11 +400,21 @@ k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
                         const UPair<T>* src = ucache +
                             ((((size_t(t) * Nrho + p) * M + q) * N) * S + r) * size_t(E::NPEN) +
                             threadIdx.x;
-                        sfor<0, N>([&](auto I) {
-                            cplx<T> up, um;
-                            unpack(0, I, o[I], up, um);
-                            const UPair<T> c0 = load_pair_nt(src + size_t(int(I)) * S * E::NPEN);
-                            acc[I] = acc[I] + up * (um + c0.m) + um * c0.p;
+                        // the stored pairs are read in chunks of CH points, all
+                        // CH loads issued before the first is used
+                        constexpr int CH = (N % 4 == 0) ? 4 : 2;
+                        sfor<0, N / CH>([&](auto C) {
+                            UPair<T> c0[CH];
+                            sfor<0, CH>([&](auto k) {
+                                c0[k] = load_pair_nt(src + size_t(int(C) * CH + int(k)) * S *
+                                                               E::NPEN);
+                            });
+                            sfor<0, CH>([&](auto k) {
+                                constexpr int i = int(C) * CH + int(k);
+                                cplx<T> up, um;
+                                unpack(0, std::integral_constant<int, i>{}, o[i], up, um);
+                                acc[i] = acc[i] + up * (um + c0[k].m) + um * c0[k].p;
+                            });
                         });
                     } else if constexpr (NW == 1) {
                         sfor<0, N>([&](auto I) {
@@ -615,6 +625,10 @@ k_line_output(const T* __restrict__ u, const T* __restrict__ u0, const double* _
 struct CollisionPlan {
     int N, M, Nrho, dtype, variant, ntask_max, ngroup_max;
     int SA, SB, SC, SD, QG;
+    // gain schedule (autotuner knobs): residue split of the f = g gain kernel
+    // (1: the smallest split whose LDS image fits, 2: twice that), and the
+    // XCD-aware work-group order (1) or the dispatch order (0)
+    int gain_smul = 1, gain_remap = 1;
     int QG_cap = 0;         // q groups the acc scratch holds
     double prefac;
     size_t n;
@@ -655,6 +669,8 @@ template <typename T, int N>
 struct Impl {
     static constexpr int S1 = pick_split_gain<T, N, 1>();   // gain residue split, f = g
     static constexpr int S2 = pick_split_gain<T, N, 2>();   // gain residue split, bilinear
+    // the f = g gain's alternative schedule: twice the split (when it divides N)
+    static constexpr int S1x = (N % (2 * S1) == 0) ? 2 * S1 : S1;
     static constexpr int PL = planes_per_wg(N);
 
     // Q0 == nullptr: Q += Q(f, g).  Q0 != nullptr (reference-state form, ug the
@@ -676,23 +692,30 @@ struct Impl {
                            pl->Nrho, nfg, (T*)pl->nyq);
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 2, st);
-        const int S = fg_same ? S1 : S2;
+        const bool alt = fg_same && !dev && pl->gain_smul == 2 && S1x != S1;
+        const int S = alt ? S1x : (fg_same ? S1 : S2);
         const int nb = ntask * pl->Nrho * S * pl->QG;
+        const int rm = pl->gain_remap;
         if (dev)
             hipLaunchKernelGGL((k_gain<T, N, S1, 1, kGainDelta>), dim3(nb),
                                dim3(round64(N * (N / S1))), 0, st, (const cplx<T>*)pl->Fhat,
                                (const cplx<T>*)pl->phase, (const T*)pl->nyq, pl->M, pl->Nrho,
-                               pl->QG, (cplx<T>*)pl->acc, (UPair<T>*)pl->ucache);
+                               pl->QG, (cplx<T>*)pl->acc, (UPair<T>*)pl->ucache, rm);
+        else if (alt)
+            hipLaunchKernelGGL((k_gain<T, N, S1x, 1>), dim3(nb), dim3(round64(N * (N / S1x))), 0,
+                               st, (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->phase,
+                               (const T*)pl->nyq, pl->M, pl->Nrho, pl->QG, (cplx<T>*)pl->acc,
+                               (UPair<T>*)nullptr, rm);
         else if (fg_same)
             hipLaunchKernelGGL((k_gain<T, N, S1, 1>), dim3(nb), dim3(round64(N * (N / S1))), 0,
                                st, (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->phase,
                                (const T*)pl->nyq, pl->M, pl->Nrho, pl->QG, (cplx<T>*)pl->acc,
-                               (UPair<T>*)nullptr);
+                               (UPair<T>*)nullptr, rm);
         else
             hipLaunchKernelGGL((k_gain<T, N, S2, 2>), dim3(nb), dim3(round64(N * (N / S2))), 0,
                                st, (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->phase,
                                (const T*)pl->nyq, pl->M, pl->Nrho, pl->QG, (cplx<T>*)pl->acc,
-                               (UPair<T>*)nullptr);
+                               (UPair<T>*)nullptr, rm);
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 3, st);
         if (dev)
@@ -756,7 +779,7 @@ struct Impl {
         hipLaunchKernelGGL((k_gain<T, N, S1, 1, kGainStore>), dim3(nb), dim3(round64(N * (N / S1))),
                            0, st, (const cplx<T>*)pl->Fhat0, (const cplx<T>*)pl->phase,
                            (const T*)pl->nyq, pl->M, pl->Nrho, pl->QG, (cplx<T>*)nullptr,
-                           (UPair<T>*)pl->ucache);
+                           (UPair<T>*)pl->ucache, pl->gain_remap);
         DGFS_HIP_CHECK(hipGetLastError());
         return 0;
     }

@@ -12,6 +12,9 @@
 //
 // All reductions accumulate in FP64 with a fixed, deterministic order
 // (block tree + ordered sum of block partials).
+#include <algorithm>
+#include <vector>
+
 #include "dgfs_common.h"
 #include "dgfs_hip.h"
 
@@ -228,14 +231,10 @@ DGFS_DEV T ghost_value(const WallDev& w, double nw, T ul, int v, T cx, T cy, T c
 // Fused transport (std.py:283-348): for element e and velocity v
 //   ux[l][e] = invjac[e] * ( -c (D u)[l] - jL[e+1] gRD[l] + jR[e] gLD[l] )
 template <typename T>
-__global__ void __launch_bounds__(kBlk)
-k_advect(AdvDev a, const T* __restrict__ u, const T* __restrict__ cx, const T* __restrict__ cy,
-         const T* __restrict__ cz, WallDev wl, WallDev wr, const double* __restrict__ nw,
-         T* __restrict__ ux) {
-    const long gid = long(blockIdx.x) * kBlk + threadIdx.x;
-    if (gid >= long(a.Ne) * a.n) return;
-    const int e = int(gid / a.n);
-    const int v = int(gid - long(e) * a.n);
+DGFS_DEV void advect_point(const AdvDev& a, const T* __restrict__ u, const T* __restrict__ cx,
+                           const T* __restrict__ cy, const T* __restrict__ cz, const WallDev& wl,
+                           const WallDev& wr, double nw0, double nw1, T* __restrict__ ux, int e,
+                           int v) {
     const int K = a.K, Ne = a.Ne, n = a.n;
     const T c = cx[v];
     T uk[kMaxK];
@@ -249,9 +248,9 @@ k_advect(AdvDev a, const T* __restrict__ u, const T* __restrict__ cx, const T* _
         uLp = br;   // element e's right-face value is the left state of face e+1
     }
     if (e > 0) uLe = face_value(a, u, a.BR, e - 1, v);
-    else uLe = ghost_value<T>(wl, nw[0], uRe, v, c, cy[v], cz[v]);
+    else uLe = ghost_value<T>(wl, nw0, uRe, v, c, cy[v], cz[v]);
     if (e < Ne - 1) uRp = face_value(a, u, a.BL, e + 1, v);
-    else uRp = ghost_value<T>(wr, nw[1], uLp, v, c, cy[v], cz[v]);
+    else uRp = ghost_value<T>(wr, nw1, uLp, v, c, cy[v], cz[v]);
     const T ac = fabs(c);
     const T ij = T(a.invjac[e]);
     if (a.De != nullptr) {
@@ -295,6 +294,103 @@ k_advect(AdvDev a, const T* __restrict__ u, const T* __restrict__ cx, const T* _
         r += -jL_p * T(a.gRD[l]) + jR_e * T(a.gLD[l]);              // totalFlux
         ux[(size_t(l) * Ne + e) * n + v] = r * ij;
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlk)
+k_advect(AdvDev a, const T* __restrict__ u, const T* __restrict__ cx, const T* __restrict__ cy,
+         const T* __restrict__ cz, WallDev wl, WallDev wr, const double* __restrict__ nw,
+         T* __restrict__ ux) {
+    const long gid = long(blockIdx.x) * kBlk + threadIdx.x;
+    if (gid >= long(a.Ne) * a.n) return;
+    const int e = int(gid / a.n);
+    const int v = int(gid - long(e) * a.n);
+    advect_point<T>(a, u, cx, cy, cz, wl, wr, nw[0], nw[1], ux, e, v);
+}
+
+// ------------------------------------------------------- batched transport
+// Independent problems of one dtype (no reference counterpart): problem p
+// reads u + off and writes ux + off; its walls' parameters are fixed at
+// batch creation.  Three launches for the batch: the diffuse-wall partial
+// sums (grid: blocks x problems), the wall densities (one block per
+// problem), and the transport over the concatenated (problem, element,
+// velocity) range.
+struct ProbDev {
+    AdvDev a;
+    WallDev wl, wr;
+    const void* cx;
+    const void* cy;
+    const void* cz;
+    long off;       // offset of the problem's (K, Ne, Nv) state, in elements
+    long start;     // first index of the problem in the concatenated Ne*Nv range
+    int nb;         // wall partial blocks (0: no diffuse wall)
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlk)
+k_bc_partial_batch(const ProbDev* __restrict__ P, int nbmax, const T* __restrict__ u,
+                   double* __restrict__ part) {
+    __shared__ double sh[4 * kBlk];
+    const ProbDev& q = P[blockIdx.y];
+    if (int(blockIdx.x) >= q.nb) return;              // uniform across the block
+    const AdvDev& a = q.a;
+    const T* up = u + q.off;
+    const T* cx = static_cast<const T*>(q.cx);
+    const int v = blockIdx.x * kBlk + threadIdx.x;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (v < a.n) {
+        const T c = cx[v];
+        for (int s = 0; s < 2; ++s) {
+            const WallDev& w = s == 0 ? q.wl : q.wr;
+            if (w.kind != DGFS_BC_DIFFUSE && w.kind != DGFS_BC_DIFFUSE_EXPR) continue;
+            const T ul = s == 0 ? face_value(a, up, a.BL, 0, v) : face_value(a, up, a.BR, a.Ne - 1, v);
+            const double fac = (double(c) - w.ushift) * w.nl;
+            if (fac >= 0.0) acc[2 * s] = fac * double(ul) * a.cw;
+            else acc[2 * s + 1] = fac * double(wall_inflow<T>(w, v, c, static_cast<const T*>(q.cy)[v],
+                                                               static_cast<const T*>(q.cz)[v])) * a.cw;
+        }
+    }
+    block_reduce<4>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < 4; ++i) part[(size_t(blockIdx.y) * nbmax + blockIdx.x) * 4 + i] = acc[i];
+}
+
+__global__ void __launch_bounds__(kBlk)
+k_bc_final_batch(const ProbDev* __restrict__ P, int nbmax, const double* __restrict__ part,
+                 double* __restrict__ nw) {
+    const int p = blockIdx.x;
+    if (P[p].nb == 0) return;                          // uniform across the block
+    __shared__ double sh[4 * kBlk];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const double* pp = part + size_t(p) * nbmax * 4;
+    for (int b = threadIdx.x; b < P[p].nb; b += kBlk)
+        for (int i = 0; i < 4; ++i) acc[i] += pp[b * 4 + i];
+    block_reduce<4>(acc, sh);
+    if (threadIdx.x == 0) {
+        nw[2 * p] = acc[1] != 0.0 ? -acc[0] / acc[1] : 0.0;
+        nw[2 * p + 1] = acc[3] != 0.0 ? -acc[2] / acc[3] : 0.0;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlk)
+k_advect_batch(const ProbDev* __restrict__ P, int nprob, long total, const T* __restrict__ u,
+               const double* __restrict__ nw, T* __restrict__ ux) {
+    const long gid = long(blockIdx.x) * kBlk + threadIdx.x;
+    if (gid >= total) return;
+    int lo = 0, hi = nprob - 1;                        // last problem with start <= gid
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (P[mid].start <= gid) lo = mid;
+        else hi = mid - 1;
+    }
+    const ProbDev& q = P[lo];
+    const long loc = gid - q.start;
+    const int e = int(loc / q.a.n);
+    const int v = int(loc - long(e) * q.a.n);
+    advect_point<T>(q.a, u + q.off, static_cast<const T*>(q.cx), static_cast<const T*>(q.cy),
+                    static_cast<const T*>(q.cz), q.wl, q.wr, nw[2 * lo], nw[2 * lo + 1],
+                    ux + q.off, e, v);
 }
 
 // ---------------------------------------------------------------- moments
@@ -458,6 +554,134 @@ extern "C" int dgfs_transport_rhs(const dgfs_transport_t* tp, const void* u, voi
     if (tp->dtype == DGFS_F64) return transport_rhs_t<double>(tp, u, ux, t, stream);
     DGFS_REQUIRE(tp->dtype == DGFS_F32, "transport_rhs: bad dtype");
     return transport_rhs_t<float>(tp, u, ux, t, stream);
+}
+
+namespace {
+
+struct TransportBatch {
+    int dtype = 0, nprob = 0, nbmax = 0;
+    long total = 0;
+    ProbDev* P = nullptr;       // device [nprob]
+    double* part = nullptr;     // device [nprob][nbmax][4]
+    double* nw = nullptr;       // device [nprob][2]
+};
+
+int check_transport(const dgfs_transport_t* tp) {
+    DGFS_REQUIRE(tp->K > 0 && tp->K <= kMaxK && tp->Ne > 0 && tp->Nv > 0,
+                 "transport: bad sizes K=%d Ne=%d Nv=%d", tp->K, tp->Ne, tp->Nv);
+    DGFS_REQUIRE(tp->D && tp->BL && tp->BR && tp->gLD && tp->gRD && tp->invjac && tp->cx &&
+                 tp->cy && tp->cz, "transport: null table");
+    for (const dgfs_wall_t* w : {&tp->left, &tp->right}) {
+        DGFS_REQUIRE(w->kind >= DGFS_BC_COPY && w->kind <= DGFS_BC_VANISH,
+                     "transport: bad wall kind %d", w->kind);
+        DGFS_REQUIRE(!(w->kind == DGFS_BC_DIFFUSE || w->kind == DGFS_BC_INLET) || w->f0,
+                     "transport: wall state f0 missing");
+    }
+    return 0;
+}
+
+void free_batch(TransportBatch* b) {
+    if (!b) return;
+    for (void* p : {(void*)b->P, (void*)b->part, (void*)b->nw})
+        if (p) (void)hipFree(p);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" int dgfs_transport_batch_create(const dgfs_transport_t* tps, int nprob,
+                                           const long* offset, dgfs_transport_batch_t* out) {
+    DGFS_REQUIRE(out, "transport_batch_create: null output handle");
+    *out = nullptr;
+    DGFS_REQUIRE(tps && offset && nprob > 0, "transport_batch_create: bad arguments");
+    const int dtype = tps[0].dtype;
+    DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "transport_batch_create: bad dtype");
+    std::vector<ProbDev> host(nprob);
+    long total = 0;
+    int nbmax = 1;
+    for (int p = 0; p < nprob; ++p) {
+        const dgfs_transport_t* tp = tps + p;
+        if (int st = check_transport(tp)) return st;
+        DGFS_REQUIRE(tp->dtype == dtype, "transport_batch_create: problem %d has another dtype", p);
+        DGFS_REQUIRE(offset[p] >= 0, "transport_batch_create: negative offset");
+        DGFS_REQUIRE(tp->left.kind != DGFS_BC_DIFFUSE_EXPR && tp->right.kind != DGFS_BC_DIFFUSE_EXPR,
+                     "transport_batch_create: problem %d has an expression wall (its parameters "
+                     "change with t; use dgfs_transport_rhs)", p);
+        ProbDev& q = host[p];
+        q.a = AdvDev{tp->K, tp->Ne, tp->Nv, tp->D, tp->BL, tp->BR, tp->gLD, tp->gRD, tp->invjac,
+                     tp->cw, tp->Dm, tp->De};
+        const dgfs_wall_t* ws[2] = {&tp->left, &tp->right};
+        WallDev* wd[2] = {&q.wl, &q.wr};
+        bool diffuse = false;
+        for (int s = 0; s < 2; ++s) {
+            *wd[s] = WallDev{ws[s]->kind, ws[s]->nl, ws[s]->ushift, ws[s]->f0, ws[s]->ux, ws[s]->uy,
+                             ws[s]->uz, ws[s]->T};
+            diffuse = diffuse || ws[s]->kind == DGFS_BC_DIFFUSE;
+        }
+        q.cx = tp->cx;
+        q.cy = tp->cy;
+        q.cz = tp->cz;
+        q.off = offset[p];
+        q.start = total;
+        q.nb = diffuse ? (tp->Nv + kBlk - 1) / kBlk : 0;
+        nbmax = std::max(nbmax, q.nb);
+        total += long(tp->Ne) * tp->Nv;
+    }
+    auto* b = new TransportBatch();
+    b->dtype = dtype;
+    b->nprob = nprob;
+    b->nbmax = nbmax;
+    b->total = total;
+    if (hipMalloc(&b->P, sizeof(ProbDev) * nprob) != hipSuccess ||
+        hipMalloc(&b->part, sizeof(double) * 4 * size_t(nprob) * nbmax) != hipSuccess ||
+        hipMalloc(&b->nw, sizeof(double) * 2 * nprob) != hipSuccess) {
+        free_batch(b);
+        set_error("transport_batch_create: allocation failed");
+        return 1;
+    }
+    if (hipMemcpy(b->P, host.data(), sizeof(ProbDev) * nprob, hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemset(b->nw, 0, sizeof(double) * 2 * nprob) != hipSuccess) {
+        free_batch(b);
+        set_error("transport_batch_create: upload failed");
+        return 1;
+    }
+    *out = b;
+    return 0;
+}
+
+extern "C" int dgfs_transport_batch_destroy(dgfs_transport_batch_t batch) {
+    free_batch(static_cast<TransportBatch*>(batch));
+    return 0;
+}
+
+template <typename T>
+static int transport_batch_t(TransportBatch* b, const void* u, void* ux, hipStream_t st) {
+    hipLaunchKernelGGL((k_bc_partial_batch<T>), dim3(b->nbmax, b->nprob), dim3(kBlk), 0, st,
+                       b->P, b->nbmax, (const T*)u, b->part);
+    DGFS_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_bc_final_batch, dim3(b->nprob), dim3(kBlk), 0, st, b->P, b->nbmax,
+                       b->part, b->nw);
+    DGFS_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL((k_advect_batch<T>), dim3((b->total + kBlk - 1) / kBlk), dim3(kBlk), 0, st,
+                       b->P, b->nprob, b->total, (const T*)u, b->nw, (T*)ux);
+    DGFS_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int dgfs_transport_batch_rhs(dgfs_transport_batch_t batch, const void* u, void* ux,
+                                        hipStream_t stream) {
+    auto* b = static_cast<TransportBatch*>(batch);
+    DGFS_REQUIRE(b && u && ux, "transport_batch_rhs: null argument");
+    if (b->dtype == DGFS_F64) return transport_batch_t<double>(b, u, ux, stream);
+    return transport_batch_t<float>(b, u, ux, stream);
+}
+
+extern "C" int dgfs_transport_batch_nw(dgfs_transport_batch_t batch, double* nw_host) {
+    auto* b = static_cast<TransportBatch*>(batch);
+    DGFS_REQUIRE(b && nw_host, "transport_batch_nw: null argument");
+    DGFS_HIP_CHECK(hipMemcpy(nw_host, b->nw, sizeof(double) * 2 * b->nprob, hipMemcpyDeviceToHost));
+    return 0;
 }
 
 extern "C" int dgfs_matop(int dtype, int Kout, int Kin, long ncol, const double* A,

@@ -125,6 +125,51 @@ class Transport:
         return self.keep['nw'].cpu().numpy()
 
 
+class BatchTransport:
+    """The transport of many independent problems of one dtype in three
+    launches (dgfs_transport_batch_*): ``transports`` are the problems'
+    Transport objects (their walls evaluated at t = 0; expression walls are
+    refused), ``offsets`` the start of each problem's state in the flat
+    arrays the call receives."""
+
+    def __init__(self, transports, offsets):
+        n = len(transports)
+        if n == 0 or len(offsets) != n:
+            raise ValueError('one offset per transport')
+        arr = (_lib.Transport * n)()
+        for i, tr in enumerate(transports):
+            tr._fill(tr.tp.left, tr.left.wall_params(0.0))
+            tr._fill(tr.tp.right, tr.right.wall_params(0.0))
+            arr[i] = tr.tp
+        off = (_lib.C.c_long * n)(*[int(o) for o in offsets])
+        self._keep = transports                  # their device tables must outlive the batch
+        self._h = _lib.C.c_void_p()
+        _lib.check(_lib.lib().dgfs_transport_batch_create(arr, n, off, _lib.C.byref(self._h)))
+        self.n = n
+
+    def __call__(self, u, ux, stream=None):
+        _lib.check(_lib.lib().dgfs_transport_batch_rhs(self._h, _lib.ptr(u), _lib.ptr(ux),
+                                                       _lib.stream_ptr(stream)))
+
+    def wall_densities(self):
+        """(nprob, 2) n_w of the last call -- host sync."""
+        out = (_lib.C.c_double * (2 * self.n))()
+        _lib.check(_lib.lib().dgfs_transport_batch_nw(
+            self._h, _lib.C.cast(out, _lib.C.POINTER(_lib.C.c_double))))
+        return np.array(out[:]).reshape(self.n, 2)
+
+    def close(self):
+        if self._h:
+            _lib.lib().dgfs_transport_batch_destroy(self._h)
+            self._h = _lib.C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Limiter:
     """Linear minmod limiter (dgfs_limit_linear; reference abstd.py:402-416):
     cell means and the linear slope by the basis' operators, then the

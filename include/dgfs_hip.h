@@ -63,6 +63,13 @@ int dgfs_collision_plan_query(dgfs_collision_plan_t plan, int* split, int* qgrou
  * when needed (synchronises the device): call between launches, not while
  * capturing a graph. */
 int dgfs_collision_plan_set_qgroups(dgfs_collision_plan_t plan, int qgroups);
+/* Tuning knobs of the f = g gain kernel (no reference counterpart):
+ * split_mult = 1 uses the smallest residue split S whose LDS image fits a
+ * work-group, 2 uses 2S where 2S divides N (more, smaller work-groups, each
+ * reloading the spectrum); xcd_remap = 1 orders work-groups so that those of
+ * one task share an XCD's L2, 0 keeps the dispatch order.  Results are the
+ * same up to summation order.  Defaults (1, 1). */
+int dgfs_collision_plan_set_schedule(dgfs_collision_plan_t plan, int split_mult, int xcd_remap);
 
 /* Batched collision: replaces the per-node loop
  *     for m: U/V transform; for r in sigModes[m], e: sm.fs(d_f, d_g, d_ux, e, r, m)
@@ -251,6 +258,26 @@ size_t dgfs_transport_work_bytes(int Nv);
  * stage time (kept for time-dependent boundaries evaluated on the host). */
 int dgfs_transport_rhs(const dgfs_transport_t* tp, const void* u, void* ux, double t,
                        hipStream_t stream);
+
+/* Batched transport (no reference counterpart: the reference runs one deck
+ * per process, dgfs1D/std/std.py:37-423).  The transport of dgfs_transport_rhs
+ * for nprob independent problems of one dtype, each described by its own
+ * dgfs_transport_t (K, Ne, Nv, operators, velocity grid, walls; its work and
+ * nw fields are not used), with state u + offset[p] and output ux + offset[p]
+ * (offsets in elements of the dtype).  The walls' parameters are read at
+ * creation: expression walls (DGFS_BC_DIFFUSE_EXPR, time-dependent) are
+ * refused.  dgfs_transport_batch_rhs runs three launches for the whole batch
+ * (diffuse-wall partial sums of every problem, their wall densities, the
+ * transport of every (problem, element, velocity)); it allocates nothing and
+ * may be captured in a graph.  dgfs_transport_batch_nw copies the 2*nprob
+ * wall densities of the last call to the host (synchronising). */
+typedef void* dgfs_transport_batch_t;
+int dgfs_transport_batch_create(const dgfs_transport_t* tps, int nprob, const long* offset,
+                                dgfs_transport_batch_t* batch);
+int dgfs_transport_batch_destroy(dgfs_transport_batch_t batch);
+int dgfs_transport_batch_rhs(dgfs_transport_batch_t batch, const void* u, void* ux,
+                             hipStream_t stream);
+int dgfs_transport_batch_nw(dgfs_transport_batch_t batch, double* nw_host);
 
 /* out[k][j] (+)= sum_m A[k][m] in[m][j], j < ncol: the GiMMiK operators of
  * dgfs1D/util.py:20-25 (fwdTrans, bwdTrans, invMass, U/V, interpolation).

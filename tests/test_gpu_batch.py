@@ -1,12 +1,15 @@
 """Batched uneven workload: many problems in flat tensors with one collision
-launch sequence per velocity grid, HIP-graph capture and the q-group
-autotuner.
+launch sequence per velocity grid, one batched transport launch sequence,
+HIP-graph capture and the q-group autotuner.
+
+* The batched transport equals each problem's own transport exactly.
 
 * Batched SSP-RK2 steps equal the problems stepped one by one (FP64, 1e-12).
 * Every replay of the captured step equals an eager step from the same state
   (several replays, eager work and synchronisations in between).
-* The autotuner keeps results unchanged (any q-group count is exact up to
-  summation order: FP64 1e-12).
+* Every gain schedule (q-group count, residue split, work-group order) and
+  the autotuner's choice keep results unchanged (exact up to summation
+  order: FP64 1e-12).
 """
 import numpy as np
 import pytest
@@ -39,6 +42,28 @@ def test_batched_equals_individual(cuda):
         assert err < 1e-12
 
 
+def test_batched_transport_equals_per_problem(cuda):
+    """dgfs_transport_batch_rhs (three launches for the batch) against each
+    problem's own dgfs_transport_rhs on a perturbed state: identical outputs
+    and diffuse-wall densities (same arithmetic and summation order)."""
+    import torch
+    b = _batch(cuda)
+    g = torch.Generator(device='cpu').manual_seed(5)
+    u = (b.u * (1 + 0.1 * (2 * torch.rand(b.u.numel(), generator=g, dtype=torch.float64)
+                           - 1).to(cuda))).contiguous()
+    out_b = torch.zeros_like(u)
+    b.btransport(u, out_b)
+    nw_b = b.btransport.wall_densities()
+    out_p = torch.zeros_like(u)
+    for k, s in enumerate(b.solvers):
+        a, e = int(b.offsets[k]), int(b.offsets[k + 1])
+        s.transport(0.0, u[a:e], out_p[a:e])
+        nw = s.transport.wall_density()
+        assert np.array_equal(nw, nw_b[k]), (k, nw, nw_b[k])
+    torch.cuda.synchronize()
+    assert torch.equal(out_b, out_p)
+
+
 def test_graph_replay_matches_eager(cuda):
     import torch
     b = _batch(cuda)
@@ -64,6 +89,16 @@ def test_autotune_keeps_results(cuda):
     ref = torch.zeros_like(b.u)
     for g in b.groups:
         g['plan'](b.u[g['lo']:g['hi']], ref[g['lo']:g['hi']])
+    # every schedule (residue split, work-group order) gives the same result
+    for sm in (1, 2):
+        for rm in (1, 0):
+            got = torch.zeros_like(b.u)
+            for g in b.groups:
+                g['plan'].set_schedule(sm, rm)
+                g['plan'](b.u[g['lo']:g['hi']], got[g['lo']:g['hi']])
+            err = float((got - ref).abs().max() / ref.abs().max())
+            print('schedule', sm, rm, err)
+            assert err < 1e-12
     res = b.autotune(candidates=(1, 2, 3, 6), reps=2)
     print('autotune', res)
     got = torch.zeros_like(b.u)
