@@ -294,13 +294,13 @@ enum GainMode { kGainAccumulate = 0, kGainStore = 1, kGainDelta = 2 };
 //   ucache[t][p][q][x][r][P] (P = this thread's pencil) instead.
 // MODE kGainDelta (NW = 1, F^ = h^): acc = sum_q u+h (u-h + u-f0) + u-h u+f0
 //   with u+-_f0 read from ucache.
-template <typename T, int N, int S, int NW, int MODE = kGainAccumulate>
+template <typename T, int N, int S, int NW, int MODE = kGainAccumulate, int PAD = 1>
 __global__ void __launch_bounds__(round64(N * (N / S)))
 k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
        const T* __restrict__ nyq, int M, int Nrho, int QG, cplx<T>* __restrict__ acc_out,
        UPair<T>* __restrict__ ucache, int remap) {
     static_assert(MODE == kGainAccumulate || NW == 1, "reference-state modes transform one function");
-    using E = Fft3Engine<T, N, S, round64(N * (N / S)), +1>;
+    using E = Fft3Engine<T, N, S, round64(N * (N / S)), +1, PAD>;
     using NL = NyLayout<N, S>;
     constexpr int n = N * N * N, NN = N * N;
     constexpr int NS = N / S, BP = NL::BP;
@@ -669,7 +669,8 @@ template <typename T, int N>
 struct Impl {
     static constexpr int S1 = pick_split_gain<T, N, 1>();   // gain residue split, f = g
     static constexpr int S2 = pick_split_gain<T, N, 2>();   // gain residue split, bilinear
-    // the f = g gain's alternative schedule: twice the split (when it divides N)
+    // the f = g gain's alternative schedule: twice the split (when it divides
+    // N) with an unpadded LDS cube, so that two work-groups may share a CU
     static constexpr int S1x = (N % (2 * S1) == 0) ? 2 * S1 : S1;
     static constexpr int PL = planes_per_wg(N);
 
@@ -702,7 +703,8 @@ struct Impl {
                                (const cplx<T>*)pl->phase, (const T*)pl->nyq, pl->M, pl->Nrho,
                                pl->QG, (cplx<T>*)pl->acc, (UPair<T>*)pl->ucache, rm);
         else if (alt)
-            hipLaunchKernelGGL((k_gain<T, N, S1x, 1>), dim3(nb), dim3(round64(N * (N / S1x))), 0,
+            hipLaunchKernelGGL((k_gain<T, N, S1x, 1, kGainAccumulate, 0>), dim3(nb),
+                               dim3(round64(N * (N / S1x))), 0,
                                st, (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->phase,
                                (const T*)pl->nyq, pl->M, pl->Nrho, pl->QG, (cplx<T>*)pl->acc,
                                (UPair<T>*)nullptr, rm);

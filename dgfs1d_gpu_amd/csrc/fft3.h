@@ -129,14 +129,17 @@ struct RegDFT {
 };
 
 // ------------------------------------------------------------- 3-pass engine
-template <typename T, int N_, int S_, int NT_, int DIR_>
+// PAD = 1 pads each LDS row by one element (bank spread); PAD = 0 packs the
+// cube (smaller, so that more work-groups fit a CU, at the cost of bank
+// conflicts in the first pass's stores).
+template <typename T, int N_, int S_, int NT_, int DIR_, int PAD = 1>
 struct Fft3Engine {
     static constexpr int N = N_;
     static constexpr int S = S_;
     static constexpr int NT = NT_;
     static constexpr int DIR = DIR_;
     static constexpr int NS = N / S;
-    static constexpr int NSP = NS + 1;          // padded LDS row (bank spread)
+    static constexpr int NSP = NS + PAD;        // LDS row length
     static constexpr int NPEN = N * NS;         // pass-2 columns == pass-3 pencils
     static constexpr int LDS_ELEMS = N * N * NSP;
     static_assert(N % S == 0, "S must divide N");

@@ -65,8 +65,9 @@ int dgfs_collision_plan_query(dgfs_collision_plan_t plan, int* split, int* qgrou
 int dgfs_collision_plan_set_qgroups(dgfs_collision_plan_t plan, int qgroups);
 /* Tuning knobs of the f = g gain kernel (no reference counterpart):
  * split_mult = 1 uses the smallest residue split S whose LDS image fits a
- * work-group, 2 uses 2S where 2S divides N (more, smaller work-groups, each
- * reloading the spectrum); xcd_remap = 1 orders work-groups so that those of
+ * work-group, 2 uses 2S where 2S divides N with an unpadded LDS cube (more,
+ * smaller work-groups, up to two per CU, each reloading the spectrum);
+ * xcd_remap = 1 orders work-groups so that those of
  * one task share an XCD's L2, 0 keeps the dispatch order.  Results are the
  * same up to summation order.  Defaults (1, 1). */
 int dgfs_collision_plan_set_schedule(dgfs_collision_plan_t plan, int split_mult, int xcd_remap);
