@@ -573,28 +573,39 @@ k_line_output(const T* __restrict__ u, const T* __restrict__ u0, const double* _
     for (int t = ta; t < tb; ++t) {
         const cplx<T>* hl = H + (size_t(t) * NH) * n + yz;
         if constexpr (DEV) {
-            // scheduling barriers keep each phase's loads after the previous
-            // phase, so that one pencil's values are live at a time
+            // the task's f terms in the outer loop and the line's points in
+            // the inner one, so that no per-point value array stays live
+            const int nt = tv.nterm[2 * t];
+            const int* sl = tv.slice + size_t(t) * 2 * tv.K;
+            const T* cf = static_cast<const T*>(tv.coef) + size_t(t) * 2 * tv.K;
             sfor<0, N>([&](auto K1) { a[K1] = hl[2 * size_t(n) + size_t(K1) * NN]; });
-            RegDFT<T, N, +1>::template run<1, 0>(a, o);             // L[h]
-            __builtin_amdgcn_sched_barrier(0);
-            sfor<0, N>([&](auto X) {
-                loss[X] += task_value<T>(tv, u0, t, 0, X * NN + yz) * o[X].x;
-            });
-            __builtin_amdgcn_sched_barrier(0);
+            RegDFT<T, N, +1>::template run<1, 0>(a, o);             // L[h]: loss += f0 L[h]
+            for (int k = 0; k < nt; ++k) {
+                const T c = cf[k];
+                const T* p0 = u0 + size_t(sl[k]) * n + yz;
+                sfor<0, N>([&](auto X) { loss[X] += c * p0[size_t(X) * NN] * o[X].x; });
+            }
             sfor<0, N>([&](auto K1) { a[K1] = hl[size_t(K1) * NN]; });
-            RegDFT<T, N, +1>::template run<1, 0>(a, o);             // L[f]
-            __builtin_amdgcn_sched_barrier(0);
-            sfor<0, N>([&](auto X) {
-                const T f = task_value<T>(tv, u, t, 0, X * NN + yz);
-                const T f0 = task_value<T>(tv, u0, t, 0, X * NN + yz);
-                loss[X] += (f - f0) * o[X].x;
-            });
-            __builtin_amdgcn_sched_barrier(0);
+            RegDFT<T, N, +1>::template run<1, 0>(a, o);             // L[f]: loss += h L[f]
+            for (int k = 0; k < nt; ++k) {
+                const T c = cf[k];
+                const T* p = u + size_t(sl[k]) * n + yz;
+                const T* p0 = u0 + size_t(sl[k]) * n + yz;
+                sfor<0, N>([&](auto X) {
+                    loss[X] += c * (p[size_t(X) * NN] - p0[size_t(X) * NN]) * o[X].x;
+                });
+            }
         } else {
             sfor<0, N>([&](auto K1) { a[K1] = hl[size_t(K1) * NN]; });
             RegDFT<T, N, +1>::template run<1, 0>(a, o);
-            sfor<0, N>([&](auto X) { loss[X] += task_value<T>(tv, u, t, 0, X * NN + yz) * o[X].x; });
+            const int nt = tv.nterm[2 * t];
+            const int* sl = tv.slice + size_t(t) * 2 * tv.K;
+            const T* cf = static_cast<const T*>(tv.coef) + size_t(t) * 2 * tv.K;
+            for (int k = 0; k < nt; ++k) {
+                const T c = cf[k];
+                const T* p = u + size_t(sl[k]) * n + yz;
+                sfor<0, N>([&](auto X) { loss[X] += c * p[size_t(X) * NN] * o[X].x; });
+            }
         }
     }
     sfor<0, N>([&](auto I) { a[I] = mk(T(0), T(0)); });
