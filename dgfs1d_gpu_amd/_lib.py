@@ -142,10 +142,10 @@ SIGNATURES = {
     'dgfs_esbgk_work_bytes': (_sz, [_i, _i]),
     'dgfs_bgk_work_bytes': (_sz, [_i, _i]),
     'dgfs_esbgk_conservative_work_bytes': (_sz, [_i, _i]),
-    'dgfs_esbgk_conservative': (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp,
-                                     _vp, _vp, _d, _d, _d, _d, _vp, _vp, _vp, _vp]),
-    'dgfs_bgk_conservative': (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp,
-                                   _vp, _vp, _d, _d, _d, _vp, _vp, _vp, _vp]),
+    'dgfs_esbgk_conservative': (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp,
+                                     _vp, _vp, _vp, _d, _d, _d, _d, _vp, _vp, _vp, _vp]),
+    'dgfs_bgk_conservative': (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp,
+                                   _vp, _vp, _vp, _d, _d, _d, _vp, _vp, _vp, _vp]),
     'dgfs_esbgk': (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                         _d, _d, _d, _d, _vp, _vp, _vp]),
 }

@@ -27,6 +27,7 @@ namespace dgfs {
 namespace {
 
 constexpr int kRB = 256;
+constexpr int kNchMax = 64;    // work-groups per task of the Newton solves, at most
 
 // block-wide sum of NV doubles in a fixed order (wave shuffles, then the four
 // wave totals); every thread receives the totals.  sh holds NV * 4 doubles.
@@ -232,23 +233,28 @@ DGFS_DEV ShkState shk_state(const double (&m)[kShkMom], double cw) {
 
 // per-task constants of f_S and the relaxation rate, from the summed raw
 // moments: cst[t][16] = {ux, uy, uz, 1/T, rho/(pi T)^1.5,
-// (1-Pr)/5 * 4/(rho T^2), qx, qy, qz, prefac rho T^(1-omega)}  (mako:50-141)
+// (1-Pr)/5 * 4/(rho T^2), qx, qy, qz, prefac rho T^(1-omega),
+// sum (c-u)_i (c-u)_j f (xx yy zz xy yz zx)}  (mako:50-141)
 constexpr int kShkCst = 16;
 
-// With cnt non-null it also zeroes the task's arrival counters cnt[i][t],
-// i < ncnt, and its status word (read by the Newton solves that follow on the stream).
+// With flags non-null it also zeroes the task's work-group flags
+// flags[i][t][0 .. kNchMax), i < nflag, and its status word (read by the
+// Newton solves that follow on the stream).
 __global__ void __launch_bounds__(kRB)
 k_shk_final(int nb, ShkParams P, const double* __restrict__ part, double* __restrict__ cst,
-            unsigned* __restrict__ cnt = nullptr, int ncnt = 0, int* __restrict__ status = nullptr) {
+            unsigned* __restrict__ flags = nullptr, int nflag = 0,
+            int* __restrict__ status = nullptr) {
     __shared__ double sh[kShkMom * 4];
     const int t = blockIdx.x;
     double m[kShkMom];
     reduce_partials<kShkMom>(part + size_t(t) * nb * kShkMom, nb, m, sh);
-    if (threadIdx.x != 0) return;
-    if (cnt) {
-        for (int i = 0; i < ncnt; ++i) cnt[size_t(i) * gridDim.x + t] = 0u;
-        status[t] = 0;
+    if (flags) {
+        for (int i = 0; i < nflag; ++i)
+            for (int j = threadIdx.x; j < kNchMax; j += kRB)
+                flags[(size_t(i) * gridDim.x + t) * kNchMax + j] = 0u;
+        if (threadIdx.x == 0) status[t] = 0;
     }
+    if (threadIdx.x != 0) return;
     const ShkState st = shk_state(m, P.cw);
     double* c = cst + size_t(t) * kShkCst;
     c[0] = st.ux;
@@ -261,6 +267,13 @@ k_shk_final(int nb, ShkParams P, const double* __restrict__ part, double* __rest
     c[7] = st.qy;
     c[8] = st.qz;
     c[9] = P.prefac * (st.rho / pow(st.T, P.omega - 1.0));
+    // sum (c-u)_i (c-u)_j f (xx yy zz xy yz zx), for the ES-BGK solve
+    c[10] = m[4] - m[0] * st.ux * st.ux;
+    c[11] = m[5] - m[0] * st.uy * st.uy;
+    c[12] = m[6] - m[0] * st.uz * st.uz;
+    c[13] = m[7] - m[0] * st.ux * st.uy;
+    c[14] = m[8] - m[0] * st.uy * st.uz;
+    c[15] = m[9] - m[0] * st.uz * st.ux;
 }
 
 // Q[gslice[g]] += sum_{t in group g} prefac rho T^{1-omega} (f_S(t) - f_t)
@@ -396,68 +409,248 @@ k_es_apply(int K, int n, const int* __restrict__ nterm, const int* __restrict__ 
 
 // ---------------------------------------------- multi-work-group Newton solves
 // The Newton solves of 'bgk-gll' and 'esbgk-gll' split each task's velocity
-// sums over nch work-groups: grid (nch, ntask), launched cooperatively, so
-// every work-group of the grid is resident at once.  Pass k writes one
-// partial vector per work-group into npart[t][k & 1][ch][NV]; the task's
-// work-groups then meet at the task's arrival counter (each adds 1 per pass;
-// the counter is zeroed by k_shk_final before the launch and never reset
-// inside it), and every one of them adds the task's partials in the same
-// order and solves the same small system, so all hold identical iterates and
-// leave the loop together.  The partials are double-buffered: buffer k & 1 is
-// written again only at pass k + 2, which a work-group reaches after the
-// meeting of pass k + 1, and every work-group arrives there only after it has
-// read pass k's partials.  The wait is bounded: a meeting that does not fill
-// within kMeetSpins polls (possible only if the grid were not resident) sets
-// kStMeetTimeout and ends the solve, so every wave exits.
-constexpr int kNchMax = 32;
-constexpr unsigned kMeetSpins = 1u << 22;
+// sums over nch work-groups: grid (nch, ntask).  Each thread keeps up to
+// kNewtonPts of its velocity points in registers across the iterations.
+// Pass k writes one partial vector per work-group into npart[t][k & 1][b][NV]
+// and raises the work-group's flag flags[t][b] to k + 1 (a release store);
+// the task's work-groups wait until all nch flags of the task have reached
+// k + 1, acquire once, and every one of them adds the task's partials in the
+// same order and solves the same small system, so all hold identical iterates
+// and leave the loop together.  The flags are zeroed by k_shk_final before
+// the launch and only grow inside it.  The partials are double-buffered:
+// buffer k & 1 is written again only at pass k + 2, which a work-group
+// reaches after the meeting of pass k + 1, and a work-group raises its flag
+// to k + 2 only after it has read pass k's partials.
+// Progress needs every work-group of a task resident at once: the host sizes
+// each launch to the device's co-resident capacity for the kernel and
+// launches it plainly (a cooperative launch adds about 17 us per launch on
+// the MI355X; a grid within the capacity is resident as soon as the device is
+// free of other work, and work-groups that kernels of other streams keep out
+// start once those finish).  The wait is bounded by kMeetTicks of the 100 MHz
+// real-time clock: a meeting that does not fill in time sets kStMeetTimeout
+// and ends the solve, so every wave exits.
+constexpr int kNewtonPts = 8;
+constexpr unsigned long long kMeetTicks = 100000000ull;   // 1 s
 
 // per-task status bits written by the Newton kernels
 constexpr int kStNonFinite = 1;      // a residual or iterate became NaN or infinite
 constexpr int kStNotConverged = 2;   // the iteration limit was reached above tolerance
 constexpr int kStMeetTimeout = 4;    // a work-group meeting gave up waiting
 
-DGFS_DEV bool task_meet(unsigned* cnt, unsigned target) {
+// Raise this work-group's flag to pass1, then wait until all nch flags of the
+// task have reached it: wave 0 polls (one flag per lane, without acquiring:
+// an acquire at device scope invalidates the L2 of the XCD each time) and
+// acquires once; the other waves follow through the barrier.  false on
+// timeout.
+DGFS_DEV bool task_meet(unsigned* flags, unsigned pass1, int nch) {
     __shared__ int ok;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();                  // this work-group's partial is visible
-        atomicAdd(cnt, 1u);
-        unsigned seen = __hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        for (unsigned s = 0; seen < target && s < kMeetSpins; ++s) {
+    __syncthreads();                      // the partial's stores (wave 0) come first
+    if (threadIdx.x < 64) {
+        const int l = threadIdx.x;
+        if (l == 0)
+            __hip_atomic_store(flags + blockIdx.x, pass1, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long start = wall_clock64();
+        bool done = false;
+        for (;;) {
+            bool mine = true;
+            for (int j = l; j < nch; j += 64)
+                mine = mine && __hip_atomic_load(flags + j, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT) >= pass1;
+            if (__all(mine)) {
+                done = true;
+                break;
+            }
+            if (wall_clock64() - start > kMeetTicks) break;
             __builtin_amdgcn_s_sleep(1);
-            seen = __hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         }
-        ok = seen >= target;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (l == 0) ok = done;
     }
     __syncthreads();
-    __threadfence();                      // the other work-groups' partials are visible
     return ok != 0;
 }
 
-// Sum v over the task's work-groups (pass `pass`); every thread receives the
-// task totals.  base: the task's npart block [2][nch][NV]; tot: NV doubles of LDS.
+// Wave reduce-scatter of NV <= 64 values: lane l returns the sum over the
+// wave's 64 lanes of value l % NP (NP = NV rounded up to a power of two;
+// zero for l % NP >= NV).  Each step sends half of the remaining values to
+// the partner lane, NP - 1 + log2(64 / NP) shuffles in all instead of
+// 6 NV for a butterfly per value.
 template <int NV>
-DGFS_DEV bool task_sum(double (&v)[NV], double* sh, double* tot, double* base, int nch,
-                       int pass, unsigned* cnt) {
-    block_sum_all<NV>(v, sh);
+DGFS_DEV double wave_reduce_scatter(const double (&v)[NV]) {
+    constexpr int NP = NV <= 1 ? 1 : NV <= 2 ? 2 : NV <= 4 ? 4 : NV <= 8 ? 8 : NV <= 16 ? 16
+                     : NV <= 32 ? 32 : 64;
+    static_assert(NV <= 64, "at most 64 values");
+    const int l = threadIdx.x & 63;
+    double a[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) a[i] = i < NV ? v[i] : 0.0;
+#pragma unroll
+    for (int h = NP / 2; h >= 1; h >>= 1) {
+        const bool up = (l & h) != 0;
+#pragma unroll
+        for (int j = 0; j < h; ++j) {
+            const double send = up ? a[j] : a[j + h];
+            const double keep = up ? a[j + h] : a[j];
+            a[j] = keep + __shfl_xor(send, h);
+        }
+    }
+#pragma unroll
+    for (int h = NP; h < 64; h <<= 1) a[0] += __shfl_xor(a[0], h);
+    return a[0];
+}
+
+// Sum v over the task's work-groups (pass `pass`); every thread receives the
+// task totals, added in a fixed order.  base: the task's npart block
+// [2][nch][NV]; flags: the task's kNchMax flags; sh: kRB doubles of LDS; pl:
+// kNchMax * NV doubles of LDS (the task's partials, read in one round trip).
+template <int NV>
+DGFS_DEV bool task_sum(double (&v)[NV], double* sh, double* pl, double* base, int nch,
+                       int pass, unsigned* flags) {
+    constexpr int NW = kRB / 64;
+    constexpr int NL = (kNchMax * NV + kRB - 1) / kRB;   // partial loads per thread
+    const double r = wave_reduce_scatter<NV>(v);
+    sh[threadIdx.x] = r;                  // [wave][lane]
+    __syncthreads();
     double* buf = base + size_t(pass & 1) * nch * NV;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < NV; ++i) buf[size_t(blockIdx.x) * NV + i] = v[i];
-    const bool ok = task_meet(cnt, unsigned(pass + 1) * unsigned(nch));
-    if (threadIdx.x < NV) {
-        double s = 0.0;
-        for (int b = 0; b < nch; ++b) s += buf[size_t(b) * NV + threadIdx.x];
-        tot[threadIdx.x] = s;
+    if (threadIdx.x < NV) {               // wave 0, the wave that raises the flag
+        double s = sh[threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) s += sh[w * 64 + threadIdx.x];
+        buf[size_t(blockIdx.x) * NV + threadIdx.x] = s;
+    }
+    const bool ok = task_meet(flags, unsigned(pass + 1), nch);
+    const int tot = nch * NV;
+    double x[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {        // every load in flight before the first use
+        const int j = threadIdx.x + k * kRB;
+        x[k] = j < tot ? buf[j] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int j = threadIdx.x + k * kRB;
+        if (j < tot) pl[j] = x[k];
     }
     __syncthreads();
-    for (int i = 0; i < NV; ++i) v[i] = tot[i];
+    if (threadIdx.x < NV) {
+        double s = 0.0;
+        for (int b = 0; b < nch; ++b) s += pl[b * NV + threadIdx.x];
+        sh[threadIdx.x] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = sh[i];
     __syncthreads();
     return ok;
 }
 
 // residual max that keeps a NaN (fmax would drop it)
 DGFS_DEV double res_max(double res, double r) { return (r > res || r != r) ? r : res; }
+
+// 1/x from the hardware reciprocal and two Newton-Raphson steps (each
+// doubles the correct bits): a short dependency chain for the small solves
+// below, whose Newton iterations correct any last-bit difference.
+DGFS_DEV double rcp_f64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// v of lane `lane` (a constant) in every lane of the wave
+DGFS_DEV double readlane_f64(double v, int lane) {
+    const unsigned long long b = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane(unsigned(b), lane);
+    const unsigned hi = __builtin_amdgcn_readlane(unsigned(b >> 32), lane);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | lo);
+}
+
+// Solves J x = J[:, R] (J is R x (R + 1), the right-hand side in the last
+// column) by Gaussian elimination with partial pivoting.  The pivot row is
+// found and moved by compare-and-swap against every row below it, so that
+// every index is a compile-time constant and J stays in registers; the pivot
+// is the first row of largest magnitude, as a scan with a strict comparison
+// picks it.  J is destroyed.
+template <int R>
+DGFS_DEV void solve_pivoted(double (&J)[R][R + 1], double (&x)[R]) {
+    double rp[R];                         // reciprocals of the pivots
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+#pragma unroll
+        for (int i = k + 1; i < R; ++i) {
+            const bool sw = fabs(J[i][k]) > fabs(J[k][k]);
+#pragma unroll
+            for (int j = k; j <= R; ++j) {
+                const double a = J[k][j], b = J[i][j];
+                J[k][j] = sw ? b : a;
+                J[i][j] = sw ? a : b;
+            }
+        }
+        rp[k] = rcp_f64(J[k][k]);
+#pragma unroll
+        for (int i = k + 1; i < R; ++i) {
+            const double f = J[i][k] * rp[k];
+#pragma unroll
+            for (int j = k; j <= R; ++j) J[i][j] -= f * J[k][j];
+        }
+    }
+#pragma unroll
+    for (int i = R - 1; i >= 0; --i) {
+        double acc = J[i][R];
+#pragma unroll
+        for (int j = i + 1; j < R; ++j) acc -= J[i][j] * x[j];
+        x[i] = acc * rp[i];
+    }
+}
+
+// Solves A x = b for a symmetric definite A (positive or negative) given by
+// its upper triangle packed row by row (a[0] = A00, a[1] = A01, ...), by the
+// LDL^T factorisation, which needs no pivoting for a definite matrix.  The
+// factor overwrites a (L_ij, i > j, in the place of A_ji; D on the diagonal).
+// All indices are compile-time constants.
+template <int R>
+DGFS_DEV constexpr int packed(int i, int j) { return i * R - i * (i - 1) / 2 + (j - i); }
+
+template <int R>
+DGFS_DEV void solve_ldlt(double (&a)[R * (R + 1) / 2], const double (&b)[R], double (&x)[R]) {
+    double rd[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        double e[R];                                   // e_k = L_jk D_k
+        double dj = a[packed<R>(j, j)];
+#pragma unroll
+        for (int k = 0; k < j; ++k) {
+            e[k] = a[packed<R>(k, j)] * a[packed<R>(k, k)];
+            dj -= a[packed<R>(k, j)] * e[k];
+        }
+        a[packed<R>(j, j)] = dj;
+        rd[j] = rcp_f64(dj);
+#pragma unroll
+        for (int i = j + 1; i < R; ++i) {
+            double s = a[packed<R>(j, i)];
+#pragma unroll
+            for (int k = 0; k < j; ++k) s -= a[packed<R>(k, i)] * e[k];
+            a[packed<R>(j, i)] = s * rd[j];
+        }
+    }
+    double y[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        double s = b[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) s -= a[packed<R>(k, i)] * y[k];
+        y[i] = s;
+    }
+#pragma unroll
+    for (int i = R - 1; i >= 0; --i) {
+        double s = y[i] * rd[i];
+#pragma unroll
+        for (int k = i + 1; k < R; ++k) s -= a[packed<R>(i, k)] * x[k];
+        x[i] = s;
+    }
+}
 
 // --------------------------------------------------------- conservative BGK
 // 'bgk-gll' (std/scattering.py:445-706, bgk-gll.mako:83-229): per task the
@@ -466,104 +659,302 @@ DGFS_DEV double res_max(double res, double r) { return (r > res || r != r) ? r :
 // by Newton's method from a = (rho/(pi T)^1.5, 1/T, 0, 0, 0); then
 // Q += prefac rho T^{1-omega} (fe - f).  At most kBgkIters iterations, as the
 // reference's loop (scattering.py:647-695), which raises on a NaN residual.
+// Each pass forms the sums v[i] = sum phi_i fe and v[5 + 5 i + j] =
+// sum phi_i d_j, phi = {1, c_x, c_y, c_z, |c|^2}, d = dfe/da =
+// fe {1/a0, -|C|^2, C_x, C_y, C_z}; for the ES-BGK model (ES = true) also
+// v[30 + k] = sum C_i C_j fe (xx yy zz xy yz zx), which the converged pass
+// leaves in est[t][10 + k] for the ES-BGK solve.
 constexpr int kBgkIters = 20;
 constexpr double kBgkTol = 1e-12;
 constexpr int kBgkSums = 30;
 
-template <typename T>
+template <bool ES>
+struct BgkSums { static constexpr int value = ES ? kBgkSums + 6 : kBgkSums; };
+
+// Per-task constants of the BGK solve from cst (k_shk_final's).
+struct BgkTask {
+    double ux, uy, uz, m[5], sc[5], a[5];
+    DGFS_DEV explicit BgkTask(const double* c) {
+        ux = c[0];
+        uy = c[1];
+        uz = c[2];
+        const double Tn = 1.0 / c[3];
+        const double rho = c[4] * pow(kPi * Tn, 1.5);
+        m[0] = rho;
+        m[1] = rho * ux;
+        m[2] = rho * uy;
+        m[3] = rho * uz;
+        m[4] = 1.5 * rho * Tn + rho * (ux * ux + uy * uy + uz * uz);
+        // residual scales: |rho| (a task's f may enter with a negative
+        // coefficient, e.g. the basis' U/V terms, so that rho, a0 and the rate
+        // are negative)
+        const double ar = fabs(rho);
+        sc[0] = ar;
+        sc[1] = sc[2] = sc[3] = ar * sqrt(Tn);
+        sc[4] = ar * Tn;
+        a[0] = c[4];
+        a[1] = c[3];
+        a[2] = a[3] = a[4] = 0.0;
+    }
+};
+
+// One Newton update of a from the pass's sums: 1 = converged (a unchanged),
+// -1 = non-finite residual, 0 = updated.
+template <int NV>
+DGFS_DEV int bgk_update(const double (&v)[NV], const BgkTask& k, double cw, double (&a)[5]) {
+    double F[5], J[5][6], res = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        F[i] = cw * v[i] - k.m[i];
+        res = res_max(res, fabs(F[i]) / k.sc[i]);
+#pragma unroll
+        for (int j = 0; j < 5; ++j) J[i][j] = cw * v[5 + 5 * i + j];
+        J[i][5] = -F[i];
+    }
+    if (!isfinite(res)) return -1;
+    if (res < kBgkTol) return 1;
+    double dlt[5];
+    solve_pivoted<5>(J, dlt);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) a[i] += dlt[i];
+    return 0;
+}
+
+// est[t] = {ux, uy, uz, a0..a4, rate, iterations, (ES) sum C_i C_j fe}
+template <int NV>
+DGFS_DEV void bgk_finish(const BgkTask& k, const double (&a)[5], const double (&v)[NV], int it,
+                         bool conv, int st, const double* c, double* e, int* status) {
+    if (!conv && st == 0) st = kStNotConverged;
+    for (int i = 0; i < 5; ++i)
+        if (!isfinite(a[i])) st |= kStNonFinite;
+    e[0] = k.ux;
+    e[1] = k.uy;
+    e[2] = k.uz;
+    for (int i = 0; i < 5; ++i) e[3 + i] = a[i];
+    e[8] = c[9];
+    e[9] = double(it);
+    for (int i = kBgkSums; i < NV; ++i) e[10 + i - kBgkSums] = v[i];
+    *status |= st;
+}
+
+template <typename T, bool ES>
 __global__ void __launch_bounds__(kRB)
 k_bgk_newton(int n, int t0, const T* __restrict__ cx, const T* __restrict__ cy,
              const T* __restrict__ cz, double cw, const double* __restrict__ cst,
-             double* __restrict__ npart /*[task][2][nch][30]*/, unsigned* __restrict__ cnt,
-             double* __restrict__ est /*[task][16]: ux uy uz a0..a4 rate iters*/,
-             int* __restrict__ status) {
-    __shared__ double sh[kBgkSums * 4];
-    __shared__ double tot[kBgkSums];
-    const int nch = gridDim.x, ch = blockIdx.x, t = t0 + blockIdx.y;
+             double* __restrict__ npart /*[task][2][nch][NV]*/, unsigned* __restrict__ flags,
+             double* __restrict__ est /*[task][16]*/, int* __restrict__ status) {
+    constexpr int NV = BgkSums<ES>::value;
+    __shared__ double sh[kRB];
+    __shared__ double pl[kNchMax * NV];
+    const int nch = gridDim.x, t = t0 + blockIdx.y;
     const double* c = cst + size_t(t) * kShkCst;
-    const double ux = c[0], uy = c[1], uz = c[2], Tn = 1.0 / c[3];
-    const double rho = c[4] * pow(kPi * Tn, 1.5);
-    const double m[5] = {rho, rho * ux, rho * uy, rho * uz,
-                         1.5 * rho * Tn + rho * (ux * ux + uy * uy + uz * uz)};
-    // residual scales: |rho| (a task's f may enter with a negative coefficient,
-    // e.g. the basis' U/V terms, so that rho, a0 and the rate are negative)
-    const double ar = fabs(rho);
-    const double sc[5] = {ar, ar * sqrt(Tn), ar * sqrt(Tn), ar * sqrt(Tn), ar * Tn};
-    double a[5] = {c[4], c[3], 0.0, 0.0, 0.0};
-    double* base = npart + size_t(t) * 2 * nch * kBgkSums;
+    const BgkTask tk(c);
+    const double ux = tk.ux, uy = tk.uy, uz = tk.uz;
+    double a[5] = {tk.a[0], tk.a[1], tk.a[2], tk.a[3], tk.a[4]};
+    double* base = npart + size_t(t) * 2 * nch * NV;
+    unsigned* fl = flags + size_t(t) * kNchMax;
+    // this thread's points p0 + k * stride; the first kNewtonPts stay in registers
+    const int p0 = blockIdx.x * kRB + threadIdx.x, stride = nch * kRB;
+    T px[kNewtonPts], py[kNewtonPts], pz[kNewtonPts];
+#pragma unroll
+    for (int k = 0; k < kNewtonPts; ++k) {
+        const int p = p0 + k * stride;
+        px[k] = p < n ? cx[p] : T(0);
+        py[k] = p < n ? cy[p] : T(0);
+        pz[k] = p < n ? cz[p] : T(0);
+    }
     int it = 0, st = 0;
     bool conv = false;
+    double v[NV];
     for (; it < kBgkIters; ++it) {
-        double v[kBgkSums];
-        for (int i = 0; i < kBgkSums; ++i) v[i] = 0.0;
-        for (int p = ch * kRB + threadIdx.x; p < n; p += nch * kRB) {
-            const double x = cx[p], y = cy[p], z = cz[p];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] = 0.0;
+        auto point = [&](double x, double y, double z) {
             const double Cx = x - ux, Cy = y - uy, Cz = z - uz;
             const double C2 = Cx * Cx + Cy * Cy + Cz * Cz;
             const double fe = a[0] * exp(-a[1] * C2 + a[2] * Cx + a[3] * Cy + a[4] * Cz);
             const double phi[5] = {1.0, x, y, z, x * x + y * y + z * z};
             const double d[5] = {fe / a[0], -fe * C2, fe * Cx, fe * Cy, fe * Cz};
+#pragma unroll
             for (int i = 0; i < 5; ++i) {
                 v[i] += phi[i] * fe;
+#pragma unroll
                 for (int j = 0; j < 5; ++j) v[5 + 5 * i + j] += phi[i] * d[j];
             }
-        }
-        if (!task_sum<kBgkSums>(v, sh, tot, base, nch, it, cnt + t)) {
+            if constexpr (ES) {
+                v[30] += fe * Cx * Cx;
+                v[31] += fe * Cy * Cy;
+                v[32] += fe * Cz * Cz;
+                v[33] += fe * Cx * Cy;
+                v[34] += fe * Cy * Cz;
+                v[35] += fe * Cz * Cx;
+            }
+        };
+#pragma unroll
+        for (int k = 0; k < kNewtonPts; ++k)
+            if (p0 + k * stride < n) point(px[k], py[k], pz[k]);
+        for (int p = p0 + kNewtonPts * stride; p < n; p += stride) point(cx[p], cy[p], cz[p]);
+        if (!task_sum<NV>(v, sh, pl, base, nch, it, fl)) {
             st |= kStMeetTimeout;
             break;
         }
-        double F[5], J[5][6], res = 0.0;
-        for (int i = 0; i < 5; ++i) {
-            F[i] = cw * v[i] - m[i];
-            res = res_max(res, fabs(F[i]) / sc[i]);
-            for (int j = 0; j < 5; ++j) J[i][j] = cw * v[5 + 5 * i + j];
-            J[i][5] = -F[i];
-        }
-        if (!isfinite(res)) {
+        const int r = bgk_update<NV>(v, tk, cw, a);
+        if (r < 0) {
             st |= kStNonFinite;
             break;
         }
-        if (res < kBgkTol) {
+        if (r > 0) {
             conv = true;
             break;
         }
-        // Gaussian elimination with partial pivoting on [J | -F]
-        for (int k = 0; k < 5; ++k) {
-            int piv = k;
-            for (int i = k + 1; i < 5; ++i)
-                if (fabs(J[i][k]) > fabs(J[piv][k])) piv = i;
-            if (piv != k)
-                for (int j = 0; j < 6; ++j) {
-                    const double tmp = J[k][j];
-                    J[k][j] = J[piv][j];
-                    J[piv][j] = tmp;
-                }
-            for (int i = k + 1; i < 5; ++i) {
-                const double f = J[i][k] / J[k][k];
-                for (int j = k; j < 6; ++j) J[i][j] -= f * J[k][j];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        bgk_finish<NV>(tk, a, v, it, conv, st, c, est + size_t(t) * kEsCst, status + t);
+}
+
+// 'bgk-gll' on a tensor velocity grid, c(p) = (c0[i], c0[j], c0[k]) at
+// p = (i N + j) N + k: fe = a0 Ex(c_x) Ey(c_y) Ez(c_z) with
+// Ex(x) = exp(-a1 (x - ux)^2 + a2 (x - ux)) (Ey, Ez alike), so the sum over
+// the N^3 points of any polynomial in C = c - u times fe is a0 times products
+// of the one-dimensional moments X_p = sum_i (c0_i - ux)^p Ex(c0_i), p <= 4
+// (and Y_p, Z_p).  One wave per task forms the 15 moments (grid_n <= 64),
+// rebuilds the sums of k_bgk_newton from them (phi and d written in C) and
+// takes the same Newton steps: no pass over N^3 points and no meeting
+// between work-groups.  The sums equal the point sums up to rounding.
+constexpr int kTensorMaxN = 64;   // largest grid_n of the tensor kernel
+
+template <typename T, bool ES>
+__global__ void __launch_bounds__(64)
+k_bgk_newton_tensor(int N, int t0, const T* __restrict__ c0, double cw,
+                    const double* __restrict__ cst, double* __restrict__ est,
+                    int* __restrict__ status) {
+    constexpr int NV = BgkSums<ES>::value;
+    __shared__ double sC[3][kTensorMaxN], sE[3][kTensorMaxN];
+    const int t = t0 + blockIdx.x, l = threadIdx.x;
+    const double* c = cst + size_t(t) * kShkCst;
+    const BgkTask tk(c);
+    const double ux = tk.ux, uy = tk.uy, uz = tk.uz;
+    const double u2 = ux * ux + uy * uy + uz * uz;
+    double a[5] = {tk.a[0], tk.a[1], tk.a[2], tk.a[3], tk.a[4]};
+    int it = 0, st = 0;
+    bool conv = false;
+    double v[NV];
+    for (; it < kBgkIters; ++it) {
+        // per node and axis the offset C and weight E = exp(-a1 C^2 + a_(2+d) C)
+        // to LDS; lane 4 q + g (q < 15: moment (d, p) = (q / 5, q % 5);
+        // g < 4: nodes g, g + 4, ...) sums C^p E; the quad adds its four
+        // parts and lane 4 q hands moment q to every lane
+        for (int i = l; i < N; i += 64) {
+            const double x = double(c0[i]);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double C = x - (d == 0 ? ux : d == 1 ? uy : uz);
+                sC[d][i] = C;
+                sE[d][i] = exp(-a[1] * C * C + a[2 + d] * C);
             }
         }
-        double dlt[5];
-        for (int i = 4; i >= 0; --i) {
-            double acc = J[i][5];
-            for (int j = i + 1; j < 5; ++j) acc -= J[i][j] * dlt[j];
-            dlt[i] = acc / J[i][i];
+        __syncthreads();
+        double part = 0.0;
+        {
+            const int q = l >> 2, g = l & 3;
+            const int d = q < 15 ? q / 5 : 0, pw = q % 5;
+            if (q < 15) {
+                double s0 = 0.0, s1 = 0.0;
+                int i = g;
+                for (; i + 4 < N; i += 8) {
+                    const double C0 = sC[d][i], C1 = sC[d][i + 4];
+                    double w0 = sE[d][i], w1 = sE[d][i + 4];
+                    if (pw > 0) { w0 *= C0; w1 *= C1; }
+                    if (pw > 1) { w0 *= C0; w1 *= C1; }
+                    if (pw > 2) { w0 *= C0; w1 *= C1; }
+                    if (pw > 3) { w0 *= C0; w1 *= C1; }
+                    s0 += w0;
+                    s1 += w1;
+                }
+                if (i < N) {
+                    const double C0 = sC[d][i];
+                    double w0 = sE[d][i];
+                    if (pw > 0) w0 *= C0;
+                    if (pw > 1) w0 *= C0;
+                    if (pw > 2) w0 *= C0;
+                    if (pw > 3) w0 *= C0;
+                    s0 += w0;
+                }
+                part = s0 + s1;
+            }
+            part += __shfl_xor(part, 1);
+            part += __shfl_xor(part, 2);
         }
-        for (int i = 0; i < 5; ++i) a[i] += dlt[i];
+        __syncthreads();                    // sC, sE are rewritten by the next pass
+        double m[15];                       // X_0..4, Y_0..4, Z_0..4
+#pragma unroll
+        for (int q = 0; q < 15; ++q) m[q] = readlane_f64(part, 4 * q);
+        const double* X = m;
+        const double* Y = m + 5;
+        const double* Z = m + 10;
+        // centred monomial sums sum Cx^a Cy^b Cz^c fe = a0 X_a Y_b Z_c
+        auto M = [&](int i, int j, int k) { return a[0] * X[i] * Y[j] * Z[k]; };
+        const double n0 = M(0, 0, 0);
+        const double nx = M(1, 0, 0), ny = M(0, 1, 0), nz = M(0, 0, 1);
+        const double nxx = M(2, 0, 0), nyy = M(0, 2, 0), nzz = M(0, 0, 2);
+        const double nxy = M(1, 1, 0), nyz = M(0, 1, 1), nzx = M(1, 0, 1);
+        const double c2 = nxx + nyy + nzz;                          // sum C^2 fe
+        const double c2x = M(3, 0, 0) + M(1, 2, 0) + M(1, 0, 2);    // sum C^2 Cx fe
+        const double c2y = M(2, 1, 0) + M(0, 3, 0) + M(0, 1, 2);
+        const double c2z = M(2, 0, 1) + M(0, 2, 1) + M(0, 0, 3);
+        const double c4 = M(4, 0, 0) + M(0, 4, 0) + M(0, 0, 4) +
+                          2.0 * (M(2, 2, 0) + M(0, 2, 2) + M(2, 0, 2));
+        // phi = {1, Cx + ux, Cy + uy, Cz + uz, C^2 + 2 u.C + |u|^2}
+        v[0] = n0;
+        v[1] = nx + ux * n0;
+        v[2] = ny + uy * n0;
+        v[3] = nz + uz * n0;
+        v[4] = c2 + 2.0 * (ux * nx + uy * ny + uz * nz) + u2 * n0;
+        // d_0 = fe / a0
+        const double ra0 = rcp_f64(a[0]);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) v[5 + 5 * i] = v[i] * ra0;
+        // d_1 = -C^2 fe
+        v[6] = -c2;
+        v[11] = -(c2x + ux * c2);
+        v[16] = -(c2y + uy * c2);
+        v[21] = -(c2z + uz * c2);
+        v[26] = -(c4 + 2.0 * (ux * c2x + uy * c2y + uz * c2z) + u2 * c2);
+        // d_2 = Cx fe, d_3 = Cy fe, d_4 = Cz fe
+        v[7] = nx;
+        v[12] = nxx + ux * nx;
+        v[17] = nxy + uy * nx;
+        v[22] = nzx + uz * nx;
+        v[27] = c2x + 2.0 * (ux * nxx + uy * nxy + uz * nzx) + u2 * nx;
+        v[8] = ny;
+        v[13] = nxy + ux * ny;
+        v[18] = nyy + uy * ny;
+        v[23] = nyz + uz * ny;
+        v[28] = c2y + 2.0 * (ux * nxy + uy * nyy + uz * nyz) + u2 * ny;
+        v[9] = nz;
+        v[14] = nzx + ux * nz;
+        v[19] = nyz + uy * nz;
+        v[24] = nzz + uz * nz;
+        v[29] = c2z + 2.0 * (ux * nzx + uy * nyz + uz * nzz) + u2 * nz;
+        if constexpr (ES) {
+            v[30] = nxx;
+            v[31] = nyy;
+            v[32] = nzz;
+            v[33] = nxy;
+            v[34] = nyz;
+            v[35] = nzx;
+        }
+        const int r = bgk_update<NV>(v, tk, cw, a);
+        if (r < 0) {
+            st |= kStNonFinite;
+            break;
+        }
+        if (r > 0) {
+            conv = true;
+            break;
+        }
     }
-    if (!conv && st == 0) st = kStNotConverged;
-    for (int i = 0; i < 5; ++i)
-        if (!isfinite(a[i])) st |= kStNonFinite;
-    if (ch == 0 && threadIdx.x == 0) {
-        double* e = est + size_t(t) * kEsCst;
-        e[0] = ux;
-        e[1] = uy;
-        e[2] = uz;
-        for (int i = 0; i < 5; ++i) e[3 + i] = a[i];
-        e[8] = c[9];
-        e[9] = double(it);
-        status[t] |= st;
-    }
+    if (l == 0) bgk_finish<NV>(tk, a, v, it, conv, st, c, est + size_t(t) * kEsCst, status + t);
 }
 
 // Q[gslice[g]] += sum_{t in group g} prefac rho T^{1-omega} (fe(t) - f_t)
@@ -599,13 +990,37 @@ k_bgk_apply(int K, int n, const int* __restrict__ nterm, const int* __restrict__
 // G = p0 exp(sum_{l=1..9} p_l phi_l(c)), phi = {c_x, c_y, c_z, c_x^2, c_y^2,
 // c_z^2, c_x c_y, c_y c_z, c_z c_x}, with those ten moments is found by
 // Newton's method (at most 40 iterations, residual < 1e-12 of the scales).
-// The Jacobian cw sum phi_k phi_l G (phi_0 = 1) is symmetric: 55 sums.
-// The task's work-groups share the sums as k_bgk_newton's do (pass 0: the
-// tensor, pass k >= 1: Newton iteration k - 1); in each work-group thread 0
-// solves the 10 x 10 system and the others read the result from LDS.
+// The Jacobian cw sum phi_k phi_l G (phi_0 = 1) is symmetric, and its 55
+// entries are 35 distinct sums cw sum x^a y^b z^c G (a + b + c <= 4).
+// The task's work-groups share the sums as k_bgk_newton's do (pass k:
+// Newton iteration k), and every thread solves the 10 x 10 system in
+// registers.  T_ij needs no pass of its own: f's part comes from the raw
+// moments summed before the solves, fe's from the BGK solve's last pass.
 constexpr int kEsgIters = 40;
 constexpr int kEsgCst = 16;
-constexpr int kEsgSums = 55;
+constexpr int kEsgPairs = 55;      // (i, j), i <= j, of the ten monomials phi
+constexpr int kEsgMono = 35;       // distinct products phi_i phi_j: x^a y^b z^c, a + b + c <= 4
+// partial sums per work-group of the ES-BGK path's two solves, at most
+constexpr int kEsNewtonSums = BgkSums<true>::value > kEsgMono ? BgkSums<true>::value : kEsgMono;
+
+// The 35 monomials x^a y^b z^c of degree <= 4, by degree, then by a, b
+// descending: (0,0,0), (1,0,0), (0,1,0), (0,0,1), (2,0,0), (1,1,0), ...
+// Monomial i > 0 is monomial kMonoParent[i] times variable kMonoVar[i]
+// (0 = x, 1 = y, 2 = z), its parent listed before it.
+constexpr int kMonoParent[kEsgMono] = {0,  0,  0,  0,  1,  2,  3,  2,  3,  3,  4,  5,
+                                       6,  7,  8,  9,  7,  8,  9,  9,  10, 11, 12, 13,
+                                       14, 15, 16, 17, 18, 19, 16, 17, 18, 19, 19};
+constexpr int kMonoVar[kEsgMono] = {0, 0, 1, 2, 0, 0, 0, 1, 1, 2, 0, 0, 0, 0, 0, 0, 1, 1,
+                                    1, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2};
+// z exponent of each monomial
+constexpr int kMonoZ[kEsgMono] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2, 0, 0, 1, 0, 1, 2, 0, 1,
+                                  2, 3, 0, 0, 1, 0, 1, 2, 0, 1, 2, 3, 0, 1, 2, 3, 4};
+// monomial of phi_i phi_j for the pairs (i, j), i <= j, row by row, with
+// phi = {1, x, y, z, x^2, y^2, z^2, xy, yz, zx}
+constexpr int kPairMono[kEsgPairs] = {
+    0,  1,  2,  3,  4,  7,  9,  5,  8,  6,  4,  5,  6,  10, 13, 15, 11, 14, 12,
+    7,  8,  11, 16, 18, 13, 17, 14, 9,  12, 17, 19, 14, 18, 15, 20, 23, 25, 21,
+    24, 22, 30, 32, 26, 31, 27, 34, 28, 33, 29, 23, 27, 24, 32, 28, 25};
 
 DGFS_DEV void esg_phi(double x, double y, double z, double (&ph)[10]) {
     ph[0] = 1.0; ph[1] = x; ph[2] = y; ph[3] = z;
@@ -613,132 +1028,334 @@ DGFS_DEV void esg_phi(double x, double y, double z, double (&ph)[10]) {
     ph[7] = x * y; ph[8] = y * z; ph[9] = z * x;
 }
 
+// Per-task targets of the ES-BGK solve: m (the ten moments), their residual
+// scales and the start q (the continuous Gaussian with the target's
+// covariance Sig = cw S / rho, G = rho / ((2 pi)^1.5 sqrt(det Sig))
+// exp(-(c-u)^T Sig^-1 (c-u) / 2) written in the monomials, whose discrete
+// moments differ from the target by the quadrature error only; if Sig is not
+// positive definite, the Maxwellian of (rho, u, T): the reference's start,
+// esbgk-gll.mako:261-270).  S = sum C_i C_j ((1 - 1/Pr) f + (1/Pr) fe): f's
+// part from the raw moments (cst[10..15]), fe's from the BGK solve's last
+// pass (ebgk[10..15]).
+struct EsgTask {
+    double m[10], sc[10], q[10];
+    DGFS_DEV void init(const double* c, const double* b, double cw, double Pr) {
+        const double ux = c[0], uy = c[1], uz = c[2], Tn = 1.0 / c[3];
+        const double rho = c[4] * pow(kPi * Tn, 1.5);
+        double S[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) S[k] = (1.0 - 1.0 / Pr) * c[10 + k] + (1.0 / Pr) * b[10 + k];
+        m[0] = rho;
+        m[1] = rho * ux;
+        m[2] = rho * uy;
+        m[3] = rho * uz;
+        m[4] = rho * ux * ux + cw * S[0];
+        m[5] = rho * uy * uy + cw * S[1];
+        m[6] = rho * uz * uz + cw * S[2];
+        m[7] = rho * ux * uy + cw * S[3];
+        m[8] = rho * uy * uz + cw * S[4];
+        m[9] = rho * uz * ux + cw * S[5];
+        const double ar = fabs(rho), sT = sqrt(Tn);
+        sc[0] = ar;
+        sc[1] = sc[2] = sc[3] = ar * sT;
+#pragma unroll
+        for (int k = 4; k < 10; ++k) sc[k] = ar * Tn;
+        const double sxx = cw * S[0] / rho, syy = cw * S[1] / rho, szz = cw * S[2] / rho;
+        const double sxy = cw * S[3] / rho, syz = cw * S[4] / rho, szx = cw * S[5] / rho;
+        // adjugate of Sig (symmetric)
+        const double Axx = syy * szz - syz * syz, Ayy = sxx * szz - szx * szx;
+        const double Azz = sxx * syy - sxy * sxy, Axy = syz * szx - sxy * szz;
+        const double Ayz = sxy * szx - sxx * syz, Azx = sxy * syz - syy * szx;
+        const double det = sxx * Axx + sxy * Axy + szx * Azx;
+        if (sxx > 0.0 && Azz > 0.0 && det > 0.0 && isfinite(det)) {
+            const double Pxx = Axx / det, Pyy = Ayy / det, Pzz = Azz / det;
+            const double Pxy = Axy / det, Pyz = Ayz / det, Pzx = Azx / det;
+            const double Pux = Pxx * ux + Pxy * uy + Pzx * uz;
+            const double Puy = Pxy * ux + Pyy * uy + Pyz * uz;
+            const double Puz = Pzx * ux + Pyz * uy + Pzz * uz;
+            q[0] = rho / (pow(2.0 * kPi, 1.5) * sqrt(det)) *
+                   exp(-0.5 * (ux * Pux + uy * Puy + uz * Puz));
+            q[1] = Pux;
+            q[2] = Puy;
+            q[3] = Puz;
+            q[4] = -0.5 * Pxx;
+            q[5] = -0.5 * Pyy;
+            q[6] = -0.5 * Pzz;
+            q[7] = -Pxy;
+            q[8] = -Pyz;
+            q[9] = -Pzx;
+        } else {
+            q[0] = c[4] * exp(-(ux * ux + uy * uy + uz * uz) / Tn);
+            q[1] = 2.0 * ux / Tn;
+            q[2] = 2.0 * uy / Tn;
+            q[3] = 2.0 * uz / Tn;
+            q[4] = q[5] = q[6] = -1.0 / Tn;
+            q[7] = q[8] = q[9] = 0.0;
+        }
+    }
+};
+
+// One Newton update of q from the monomial sums v: 1 = converged (q
+// unchanged), -1 = non-finite residual, 0 = updated.  J_ij = cw sum phi_i
+// phi_j G is the Jacobian in (p0 / q0, p1, .., p9) (d G / d p0 = G / p0),
+// symmetric and definite: the step solves J d' = -F and scales d'_0 by q0.
+DGFS_DEV int esg_update(const double (&v)[kEsgMono], const EsgTask& k, double cw, double (&q)[10]) {
+    double a[kEsgPairs], F[10], d[10], res = 0.0;
+    sfor<0, kEsgPairs>([&](auto K) {
+        constexpr int i = K;
+        a[i] = cw * v[kPairMono[i]];
+    });
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        F[i] = a[i] - k.m[i];                             // cw sum phi_i G
+        res = res_max(res, fabs(F[i]) / k.sc[i]);
+        F[i] = -F[i];
+    }
+    if (!isfinite(res)) return -1;
+    if (res < kBgkTol) return 1;
+    solve_ldlt<10>(a, F, d);
+    q[0] += q[0] * d[0];
+#pragma unroll
+    for (int i = 1; i < 10; ++i) q[i] += d[i];
+    return 0;
+}
+
+DGFS_DEV void esg_finish(const double (&q)[10], int it, bool conv, int st, const double* c,
+                         double* e, int* status) {
+    if (!conv && st == 0) st = kStNotConverged;
+    for (int i = 0; i < 10; ++i)
+        if (!isfinite(q[i])) st |= kStNonFinite;
+    for (int i = 0; i < 10; ++i) e[i] = q[i];
+    e[10] = c[9];
+    e[11] = double(it);
+    *status |= st;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kRB)
-k_esg_newton(int K, int n, int t0, const int* __restrict__ nterm, const int* __restrict__ slice,
-             const T* __restrict__ coef, const T* __restrict__ u, const T* __restrict__ cx,
-             const T* __restrict__ cy, const T* __restrict__ cz, double cw, double Pr,
-             const double* __restrict__ cst, const double* __restrict__ ebgk,
-             double* __restrict__ npart /*[task][2][nch][55]*/, unsigned* __restrict__ cnt,
-             double* __restrict__ eesg /*[task][16]: p0..p9 rate iters*/,
+k_esg_newton(int n, int t0, const T* __restrict__ cx, const T* __restrict__ cy,
+             const T* __restrict__ cz, double cw, double Pr, const double* __restrict__ cst,
+             const double* __restrict__ ebgk, double* __restrict__ npart /*[task][2][nch][35]*/,
+             unsigned* __restrict__ flags, double* __restrict__ eesg /*[task][16]*/,
              int* __restrict__ status) {
-    __shared__ double sh[kEsgSums * 4];
-    __shared__ double tot[kEsgSums];
-    __shared__ double sp[11];
-    const int nch = gridDim.x, ch = blockIdx.x, t = t0 + blockIdx.y;
+    __shared__ double sh[kRB];
+    __shared__ double pl[kNchMax * kEsgMono];
+    const int nch = gridDim.x, t = t0 + blockIdx.y;
     const double* c = cst + size_t(t) * kShkCst;
-    const double* b = ebgk + size_t(t) * kEsCst;
-    const double ux = c[0], uy = c[1], uz = c[2], Tn = 1.0 / c[3];
-    const double rho = c[4] * pow(kPi * Tn, 1.5);
-    double* base = npart + size_t(t) * 2 * nch * kEsgSums;
-    int st = 0;
-    // pass 0: tensor moments of (1 - 1/Pr) f + (1/Pr) fe
-    double S[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int p = ch * kRB + threadIdx.x; p < n; p += nch * kRB) {
-        const double Cx = double(cx[p]) - ux, Cy = double(cy[p]) - uy, Cz = double(cz[p]) - uz;
-        const double fe = b[3] * exp(-b[4] * (Cx * Cx + Cy * Cy + Cz * Cz) + b[5] * Cx +
-                                     b[6] * Cy + b[7] * Cz);
-        const double w = (1.0 - 1.0 / Pr) * shk_task_value(nterm, slice, coef, K, n, u, t, p) +
-                         (1.0 / Pr) * fe;
-        S[0] += w * Cx * Cx; S[1] += w * Cy * Cy; S[2] += w * Cz * Cz;
-        S[3] += w * Cx * Cy; S[4] += w * Cy * Cz; S[5] += w * Cz * Cx;
+    EsgTask tk;
+    tk.init(c, ebgk + size_t(t) * kEsCst, cw, Pr);
+    double q[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) q[i] = tk.q[i];
+    double* base = npart + size_t(t) * 2 * nch * kEsgMono;
+    unsigned* fl = flags + size_t(t) * kNchMax;
+    const int p0 = blockIdx.x * kRB + threadIdx.x, stride = nch * kRB;
+    T px[kNewtonPts], py[kNewtonPts], pz[kNewtonPts];
+#pragma unroll
+    for (int k = 0; k < kNewtonPts; ++k) {
+        const int p = p0 + k * stride;
+        px[k] = p < n ? cx[p] : T(0);
+        py[k] = p < n ? cy[p] : T(0);
+        pz[k] = p < n ? cz[p] : T(0);
     }
-    const bool met = task_sum<6>(S, sh, tot, base, nch, 0, cnt + t);
-    const double m[10] = {rho, rho * ux, rho * uy, rho * uz,
-                          rho * ux * ux + cw * S[0], rho * uy * uy + cw * S[1],
-                          rho * uz * uz + cw * S[2], rho * ux * uy + cw * S[3],
-                          rho * uy * uz + cw * S[4], rho * uz * ux + cw * S[5]};
-    const double ar = fabs(rho), sT = sqrt(Tn);
-    const double sc[10] = {ar, ar * sT, ar * sT, ar * sT, ar * Tn, ar * Tn, ar * Tn,
-                           ar * Tn, ar * Tn, ar * Tn};
-    // start: the Maxwellian of (rho, u, T) written in the monomials
-    double q[10] = {c[4] * exp(-(ux * ux + uy * uy + uz * uz) / Tn), 2.0 * ux / Tn, 2.0 * uy / Tn,
-                    2.0 * uz / Tn, -1.0 / Tn, -1.0 / Tn, -1.0 / Tn, 0.0, 0.0, 0.0};
-    int it = 0;
+    int it = 0, st = 0;
     bool conv = false;
-    if (!met) st |= kStMeetTimeout;
-    for (; met && it < kEsgIters; ++it) {
-        double v[kEsgSums];
-        for (int i = 0; i < kEsgSums; ++i) v[i] = 0.0;
-        for (int p = ch * kRB + threadIdx.x; p < n; p += nch * kRB) {
-            double ph[10];
-            esg_phi(cx[p], cy[p], cz[p], ph);
-            double e = 0.0;
-            for (int l = 1; l < 10; ++l) e += q[l] * ph[l];
-            const double G = q[0] * exp(e);
-            int k = 0;
-            for (int i = 0; i < 10; ++i)
-                for (int j = i; j < 10; ++j) v[k++] += ph[i] * ph[j] * G;
-        }
-        if (!task_sum<kEsgSums>(v, sh, tot, base, nch, it + 1, cnt + t)) {
+    for (; it < kEsgIters; ++it) {
+        // the sums sum_p x^a y^b z^c G(p) over the 35 monomials: every
+        // product phi_i phi_j is one of them
+        double v[kEsgMono];
+#pragma unroll
+        for (int i = 0; i < kEsgMono; ++i) v[i] = 0.0;
+        auto point = [&](double x, double y, double z) {
+            const double e = q[1] * x + q[2] * y + q[3] * z + q[4] * (x * x) + q[5] * (y * y) +
+                             q[6] * (z * z) + q[7] * (x * y) + q[8] * (y * z) + q[9] * (z * x);
+            const double xyz[3] = {x, y, z};
+            double g[kEsgMono];
+            g[0] = q[0] * exp(e);
+            sfor<1, kEsgMono>([&](auto I) {
+                constexpr int i = I;
+                g[i] = g[kMonoParent[i]] * xyz[kMonoVar[i]];
+            });
+#pragma unroll
+            for (int i = 0; i < kEsgMono; ++i) v[i] += g[i];
+        };
+#pragma unroll
+        for (int k = 0; k < kNewtonPts; ++k)
+            if (p0 + k * stride < n) point(px[k], py[k], pz[k]);
+        for (int p = p0 + kNewtonPts * stride; p < n; p += stride) point(cx[p], cy[p], cz[p]);
+        if (!task_sum<kEsgMono>(v, sh, pl, base, nch, it, fl)) {
             st |= kStMeetTimeout;
             break;
         }
-        if (threadIdx.x == 0) {
-            double J[10][11], res = 0.0;
-            int k = 0;
-            for (int i = 0; i < 10; ++i)
-                for (int j = i; j < 10; ++j) {
-                    J[i][j] = J[j][i] = cw * v[k];
-                    ++k;
-                }
-            for (int i = 0; i < 10; ++i) {
-                const double F = J[i][0] - m[i];          // cw sum phi_i G
-                res = res_max(res, fabs(F) / sc[i]);
-                J[i][10] = -F;
-                J[i][0] /= q[0];                          // d G / d p0 = G / p0
-            }
-            sp[10] = res;
-            if (isfinite(res) && res >= kBgkTol) {
-                for (int kk = 0; kk < 10; ++kk) {
-                    int piv = kk;
-                    for (int i = kk + 1; i < 10; ++i)
-                        if (fabs(J[i][kk]) > fabs(J[piv][kk])) piv = i;
-                    if (piv != kk)
-                        for (int j = 0; j < 11; ++j) {
-                            const double tmp = J[kk][j];
-                            J[kk][j] = J[piv][j];
-                            J[piv][j] = tmp;
-                        }
-                    for (int i = kk + 1; i < 10; ++i) {
-                        const double f = J[i][kk] / J[kk][kk];
-                        for (int j = kk; j < 11; ++j) J[i][j] -= f * J[kk][j];
-                    }
-                }
-                double d[10];
-                for (int i = 9; i >= 0; --i) {
-                    double acc = J[i][10];
-                    for (int j = i + 1; j < 10; ++j) acc -= J[i][j] * d[j];
-                    d[i] = acc / J[i][i];
-                }
-                for (int i = 0; i < 10; ++i) sp[i] = q[i] + d[i];
-            } else {
-                for (int i = 0; i < 10; ++i) sp[i] = q[i];
-            }
-        }
-        __syncthreads();
-        const double res = sp[10];
-        for (int i = 0; i < 10; ++i) q[i] = sp[i];
-        __syncthreads();
-        if (!isfinite(res)) {
+        const int r = esg_update(v, tk, cw, q);
+        if (r < 0) {
             st |= kStNonFinite;
             break;
         }
-        if (res < kBgkTol) {
+        if (r > 0) {
             conv = true;
             break;
         }
     }
-    if (!conv && st == 0) st = kStNotConverged;
-    for (int i = 0; i < 10; ++i)
-        if (!isfinite(q[i])) st |= kStNonFinite;
-    if (ch == 0 && threadIdx.x == 0) {
-        double* e = eesg + size_t(t) * kEsgCst;
-        for (int i = 0; i < 10; ++i) e[i] = q[i];
-        e[10] = c[9];
-        e[11] = double(it);
-        status[t] |= st;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        esg_finish(q, it, conv, st, c, eesg + size_t(t) * kEsgCst, status + t);
+}
+
+// 'esbgk-gll' on a tensor velocity grid (grid_n <= kTensorMaxN; layout as
+// k_bgk_newton_tensor's), one work-group per task and no meeting.  For a
+// column (x_i, y_j) the Gaussian is G = q0 Ax(x) Ay(y) exp(q7 x y)
+// Az(z) exp(beta z), Ax(x) = exp(q1 x + q4 x^2) (Ay, Az alike),
+// beta = q8 y + q9 x.  With the nodes z_k = z0 + k h + dz_k (dz_k the
+// rounding of the stored nodes off the uniform ones, zero up to rounding for
+// FP64 grids), exp(beta z_k) = exp(beta z0) rho^k (1 + beta dz_k) to
+// (beta dz_k)^2 / 2, rho = exp(beta h), so the column's moments
+// W_c = sum_k z_k^c G are Horner sums in rho of the node tables
+// z_k^c Az(z_k) and z_k^c Az(z_k) dz_k: N^3 fused multiply-adds and N^2 + 3N
+// exponentials per pass instead of N^3 exponentials.  Then
+// sum x^a y^b z^c G = sum over columns of x^a y^b W_c.
+constexpr int kEsgTensorMaxN = 48;   // largest grid_n of the tensor ES-BGK kernel
+constexpr int kEsgRedG = 7;          // groups per value in its block sums
+
+template <typename T, int NT, int CB>   // NT threads; CB columns per thread and Horner sweep
+__global__ void __launch_bounds__(NT)
+k_esg_newton_tensor(int N, int t0, const T* __restrict__ c0, double cw, double Pr,
+                    const double* __restrict__ cst, const double* __restrict__ ebgk,
+                    double* __restrict__ eesg, int* __restrict__ status) {
+    static_assert(kEsgMono * kEsgRedG <= NT, "one thread per (value, group)");
+    static_assert(3 * kEsgTensorMaxN <= NT, "one thread per node and axis");
+    __shared__ double zc[kEsgTensorMaxN], dz[kEsgTensorMaxN];
+    __shared__ double Ax[kEsgTensorMaxN], Ay[kEsgTensorMaxN];
+    __shared__ double BD[kEsgTensorMaxN][10];   // z^c Az, then z^c Az dz (c <= 4)
+    __shared__ double red[kEsgMono][NT];        // per-thread monomial sums
+    __shared__ double part[kEsgMono * kEsgRedG];
+    __shared__ EsgTask tk;                      // the task's targets, kept out of registers
+    const int t = t0 + blockIdx.x, tid = threadIdx.x;
+    const double* c = cst + size_t(t) * kShkCst;
+    if (tid == 0) tk.init(c, ebgk + size_t(t) * kEsCst, cw, Pr);
+    for (int k = tid; k < N; k += NT) zc[k] = double(c0[k]);
+    __syncthreads();
+    double q[10];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) q[i] = tk.q[i];
+    const double z0 = zc[0], h = N > 1 ? (zc[N - 1] - zc[0]) / double(N - 1) : 0.0;
+    for (int k = tid; k < N; k += NT) dz[k] = zc[k] - (z0 + double(k) * h);
+    const int ncol = N * N;
+    int it = 0, st = 0;
+    bool conv = false;
+    for (; it < kEsgIters; ++it) {
+        __syncthreads();                  // the previous pass's readers are done
+        // node tables, one thread per (axis, node)
+        if (tid < 3 * N) {
+            const int d = tid / N, k = tid - (tid / N) * N;
+            const double x = zc[k];
+            const double w = exp(q[1 + d] * x + q[4 + d] * x * x);
+            if (d == 0) {
+                Ax[k] = w;
+            } else if (d == 1) {
+                Ay[k] = w;
+            } else {
+                double wp = w;
+#pragma unroll
+                for (int p = 0; p < 5; ++p) {
+                    BD[k][p] = wp;
+                    BD[k][5 + p] = wp * dz[k];
+                    wp *= x;
+                }
+            }
+        }
+        __syncthreads();
+        double v[kEsgMono];
+#pragma unroll
+        for (int m = 0; m < kEsgMono; ++m) v[m] = 0.0;
+        for (int col0 = tid * CB; col0 < ncol; col0 += NT * CB) {
+            double beta[CB], rho[CB], H[CB][10];
+#pragma unroll
+            for (int b = 0; b < CB; ++b) {
+                const int col = min(col0 + b, ncol - 1);
+                const double x = zc[col / N], y = zc[col % N];
+                beta[b] = q[8] * y + q[9] * x;
+                rho[b] = exp(beta[b] * h);
+#pragma unroll
+                for (int r = 0; r < 10; ++r) H[b][r] = 0.0;
+            }
+#pragma unroll 2
+            for (int k = N - 1; k >= 0; --k) {
+                double row[10];
+#pragma unroll
+                for (int r = 0; r < 10; ++r) row[r] = BD[k][r];
+#pragma unroll
+                for (int b = 0; b < CB; ++b)
+#pragma unroll
+                    for (int r = 0; r < 10; ++r) H[b][r] = fma(H[b][r], rho[b], row[r]);
+            }
+            // v[x^a y^b z^c] += x^a y^b W_c, W_c = sum_k z_k^c G the column's moment
+#pragma unroll
+            for (int b = 0; b < CB; ++b) {
+                const int col = col0 + b;
+                if (col >= ncol) break;
+                const int i = col / N, j = col - (col / N) * N;
+                const double x = zc[i], y = zc[j];
+                const double pre = q[0] * Ax[i] * Ay[j] * exp(q[7] * x * y + beta[b] * z0);
+                double W[5];
+#pragma unroll
+                for (int p = 0; p < 5; ++p) W[p] = pre * fma(beta[b], H[b][5 + p], H[b][p]);
+                const double xyz[3] = {x, y, 1.0};
+                double g[kEsgMono];
+                sfor<0, kEsgMono>([&](auto I) {
+                    constexpr int m = I;
+                    constexpr int var = kMonoVar[m];
+                    if constexpr (m == 0) {
+                        g[0] = 1.0;
+                    } else if constexpr (var == 2) {
+                        g[m] = g[kMonoParent[m]];            // z^c enters through W_c
+                    } else {
+                        g[m] = g[kMonoParent[m]] * xyz[var];
+                    }
+                });
+                sfor<0, kEsgMono>([&](auto I) {
+                    constexpr int m = I;
+                    v[m] = fma(g[m], W[kMonoZ[m]], v[m]);
+                });
+            }
+        }
+        // block sums through an LDS transpose, in a fixed order: thread (m, g),
+        // m < 35, g < kEsgRedG, adds value m of threads g, g + kEsgRedG, ...;
+        // then thread m adds the kEsgRedG parts
+#pragma unroll
+        for (int m = 0; m < kEsgMono; ++m) red[m][tid] = v[m];
+        __syncthreads();
+        if (tid < kEsgMono * kEsgRedG) {
+            const int m = tid / kEsgRedG, g = tid - m * kEsgRedG;
+            double a0 = 0.0, a1 = 0.0;
+            int e = g;
+            for (; e + kEsgRedG < NT; e += 2 * kEsgRedG) {
+                a0 += red[m][e];
+                a1 += red[m][e + kEsgRedG];
+            }
+            if (e < NT) a0 += red[m][e];
+            part[tid] = a0 + a1;
+        }
+        __syncthreads();
+        if (tid < kEsgMono) {
+            double a0 = 0.0;
+#pragma unroll
+            for (int g = 0; g < kEsgRedG; ++g) a0 += part[tid * kEsgRedG + g];
+            red[tid][0] = a0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kEsgMono; ++i) v[i] = red[i][0];
+        const int r = esg_update(v, tk, cw, q);
+        if (r < 0) {
+            st |= kStNonFinite;
+            break;
+        }
+        if (r > 0) {
+            conv = true;
+            break;
+        }
     }
+    if (tid == 0) esg_finish(q, it, conv, st, c, eesg + size_t(t) * kEsgCst, status + t);
 }
 
 // Q[gslice[g]] += sum_{t in group g} prefac rho T^{1-omega} (G(t) - f_t)
@@ -1070,6 +1687,8 @@ int coresident_groups(const void* kernel) {
     struct Entry { const void* k; int dev, groups; };
     static Entry cache[32];
     static int used = 0;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     for (int i = 0; i < used; ++i)
@@ -1082,56 +1701,81 @@ int coresident_groups(const void* kernel) {
     return per * ncu;
 }
 
-template <typename... A>
-hipError_t launch_coop(const void* kernel, dim3 grid, hipStream_t st, A... args) {
-    void* p[] = {static_cast<void*>(&args)...};
-    return hipLaunchCooperativeKernel(kernel, grid, dim3(kRB), p, 0, st);
-}
-
-// Newton work-group layout: nch work-groups per task, tasks in launches of
-// `per` so that each launch's grid is resident
+// Newton work-group layout: nch work-groups per task (enough to keep every
+// velocity point in registers, within kNchMax and the co-resident capacity),
+// tasks in launches of `per` so that each launch's grid is resident at once
 struct NewtonGrid { int nch, per; };
 
-NewtonGrid newton_grid(const void* kernel, int ntask, int nb) {
+NewtonGrid newton_grid(const void* kernel, int ntask, int n) {
     const int cap = coresident_groups(kernel);
     if (cap <= 0) return NewtonGrid{0, 0};
-    const int nch = std::max(1, std::min(std::min(kNchMax, nb), cap / ntask));
+    const int want = (n + kNewtonPts * kRB - 1) / (kNewtonPts * kRB);
+    const int nch = std::max(1, std::min(std::min(kNchMax, want), cap / ntask));
     return NewtonGrid{nch, std::max(1, cap / nch)};
 }
 
-// status and arrival counters behind the sums of the Newton solves
+// partials and flags behind the sums of the Newton solves: per task two
+// buffers of kNchMax partial vectors, then two sets of kNchMax flags (the
+// BGK solve's and the ES-BGK solve's) in kNchMax doubles
 size_t newton_work_doubles(int ntask, int nsums) {
-    return size_t(ntask) * 2 * kNchMax * nsums + size_t(ntask);   // + 2 uint counters per task
+    return size_t(ntask) * 2 * kNchMax * nsums + size_t(ntask) * kNchMax;
 }
 
-template <typename T>
-int bgk_newton(int Nv, int ntask, const void* cx, const void* cy, const void* cz, double cw,
-               const double* cst, double* npart, unsigned* cnt, double* est, int* status,
-               hipStream_t stream) {
-    const void* k = reinterpret_cast<const void*>(&k_bgk_newton<T>);
-    const NewtonGrid g = newton_grid(k, ntask, (Nv + kRB - 1) / kRB);
-    DGFS_REQUIRE(g.nch > 0, "conservative BGK: cannot size the cooperative Newton launch");
+// The BGK solves: with 0 < grid_n <= kTensorMaxN the tensor-grid kernel (one
+// wave per task; the nodes c0 are cz[0 .. grid_n), as cz[k] = c0[k] at
+// i = j = 0), otherwise the point-sum kernel.
+template <typename T, bool ES>
+int bgk_newton(int Nv, int grid_n, int ntask, const void* cx, const void* cy, const void* cz,
+               double cw, const double* cst, double* npart, unsigned* flags, double* est,
+               int* status, hipStream_t stream) {
+    if (grid_n > 0 && grid_n <= kTensorMaxN) {
+        for (int t0 = 0; t0 < ntask; t0 += 65535) {
+            const int nt = std::min(65535, ntask - t0);
+            hipLaunchKernelGGL((k_bgk_newton_tensor<T, ES>), dim3(nt), dim3(64), 0, stream,
+                               grid_n, t0, (const T*)cz, cw, cst, est, status);
+            DGFS_HIP_CHECK(hipGetLastError());
+        }
+        return 0;
+    }
+    const void* k = reinterpret_cast<const void*>(&k_bgk_newton<T, ES>);
+    const NewtonGrid g = newton_grid(k, ntask, Nv);
+    DGFS_REQUIRE(g.nch > 0, "conservative BGK: cannot size the Newton launch");
     for (int t0 = 0; t0 < ntask; t0 += g.per) {
         const int nt = std::min(g.per, ntask - t0);
-        DGFS_HIP_CHECK(launch_coop(k, dim3(g.nch, nt), stream, Nv, t0, (const T*)cx,
-                                   (const T*)cy, (const T*)cz, cw, cst, npart, cnt, est, status));
+        hipLaunchKernelGGL((k_bgk_newton<T, ES>), dim3(g.nch, nt), dim3(kRB), 0, stream, Nv, t0,
+                           (const T*)cx, (const T*)cy, (const T*)cz, cw, cst, npart, flags, est,
+                           status);
+        DGFS_HIP_CHECK(hipGetLastError());
     }
     return 0;
 }
 
+// The ES-BGK solves: the tensor-grid kernel as for the BGK solves, otherwise
+// the point-sum kernel.
+constexpr int kEsgTensorNT = 256, kEsgTensorCB = 4;
 template <typename T>
-int esg_newton(int K, int Nv, int ntask, const int* nterm, const int* slice, const void* coef,
-               const void* u, const void* cx, const void* cy, const void* cz, double cw,
-               double Pr, const double* cst, const double* ebgk, double* npart, unsigned* cnt,
-               double* eesg, int* status, hipStream_t stream) {
+int esg_newton(int Nv, int grid_n, int ntask, const void* cx, const void* cy, const void* cz,
+               double cw, double Pr, const double* cst, const double* ebgk, double* npart,
+               unsigned* flags, double* eesg, int* status, hipStream_t stream) {
+    if (grid_n > 0 && grid_n <= kEsgTensorMaxN) {
+        for (int t0 = 0; t0 < ntask; t0 += 65535) {
+            const int nt = std::min(65535, ntask - t0);
+            hipLaunchKernelGGL((k_esg_newton_tensor<T, kEsgTensorNT, kEsgTensorCB>), dim3(nt),
+                               dim3(kEsgTensorNT), 0, stream, grid_n, t0, (const T*)cz, cw, Pr,
+                               cst, ebgk, eesg, status);
+            DGFS_HIP_CHECK(hipGetLastError());
+        }
+        return 0;
+    }
     const void* k = reinterpret_cast<const void*>(&k_esg_newton<T>);
-    const NewtonGrid g = newton_grid(k, ntask, (Nv + kRB - 1) / kRB);
-    DGFS_REQUIRE(g.nch > 0, "conservative ES-BGK: cannot size the cooperative Newton launch");
+    const NewtonGrid g = newton_grid(k, ntask, Nv);
+    DGFS_REQUIRE(g.nch > 0, "conservative ES-BGK: cannot size the Newton launch");
     for (int t0 = 0; t0 < ntask; t0 += g.per) {
         const int nt = std::min(g.per, ntask - t0);
-        DGFS_HIP_CHECK(launch_coop(k, dim3(g.nch, nt), stream, K, Nv, t0, nterm, slice,
-                                   (const T*)coef, (const T*)u, (const T*)cx, (const T*)cy,
-                                   (const T*)cz, cw, Pr, cst, ebgk, npart, cnt, eesg, status));
+        hipLaunchKernelGGL(k_esg_newton<T>, dim3(g.nch, nt), dim3(kRB), 0, stream, Nv, t0,
+                           (const T*)cx, (const T*)cy, (const T*)cz, cw, Pr, cst, ebgk, npart,
+                           flags, eesg, status);
+        DGFS_HIP_CHECK(hipGetLastError());
     }
     return 0;
 }
@@ -1174,7 +1818,7 @@ extern "C" size_t dgfs_bgk_work_bytes(int ntask, int Nv) {
             newton_work_doubles(ntask, kBgkSums)) * sizeof(double);
 }
 
-extern "C" int dgfs_bgk_conservative(int dtype, int K, int Ne, int Nv, int ntask,
+extern "C" int dgfs_bgk_conservative(int dtype, int K, int Ne, int Nv, int grid_n, int ntask,
                                      const int* task_nterm, const int* task_slice,
                                      const void* task_coef, int ngroup, const int* group_start,
                                      const int* group_slice, const void* u, const void* cx,
@@ -1184,6 +1828,8 @@ extern "C" int dgfs_bgk_conservative(int dtype, int K, int Ne, int Nv, int ntask
     {
         DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "dtype must be DGFS_F64 or DGFS_F32");
         DGFS_REQUIRE(K > 0 && Ne > 0 && Nv > 0 && ntask >= 0 && ngroup >= 0, "bad sizes");
+        DGFS_REQUIRE(grid_n == 0 || (grid_n > 0 && long(grid_n) * grid_n * grid_n == Nv),
+                     "grid_n must be 0 or the cube root of Nv");
         if (ntask == 0 || ngroup == 0) return 0;
         DGFS_REQUIRE(ntask <= 65535 && ngroup <= 65535, "too many tasks");
         DGFS_REQUIRE(task_nterm && task_slice && task_coef && group_start && group_slice && u &&
@@ -1195,16 +1841,16 @@ extern "C" int dgfs_bgk_conservative(int dtype, int K, int Ne, int Nv, int ntask
         double* cst = part + size_t(ntask) * nb * kShkMom;
         double* est = cst + size_t(ntask) * kShkCst;
         double* npart = est + size_t(ntask) * kEsCst;
-        unsigned* cnt = reinterpret_cast<unsigned*>(npart + size_t(ntask) * 2 * kNchMax * kBgkSums);
+        unsigned* flags = reinterpret_cast<unsigned*>(npart + size_t(ntask) * 2 * kNchMax * kBgkSums);
         const bool f64 = dtype == DGFS_F64;
         (f64 ? shk_partial<double> : shk_partial<float>)(K, Nv, nb, ntask, task_nterm, task_slice,
                                                          task_coef, u, cx, cy, cz, part, stream);
         DGFS_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_shk_final, dim3(ntask), dim3(kRB), 0, stream, nb, P,
-                           (const double*)part, cst, cnt, 1, status);
+                           (const double*)part, cst, flags, 1, status);
         DGFS_HIP_CHECK(hipGetLastError());
-        const int rc = (f64 ? bgk_newton<double> : bgk_newton<float>)(
-            Nv, ntask, cx, cy, cz, cw, cst, npart, cnt, est, status, stream);
+        const int rc = (f64 ? bgk_newton<double, false> : bgk_newton<float, false>)(
+            Nv, grid_n, ntask, cx, cy, cz, cw, cst, npart, flags, est, status, stream);
         if (rc) return rc;
         (f64 ? bgk_apply<double> : bgk_apply<float>)(K, Nv, nb, ngroup, task_nterm, task_slice,
                                                      task_coef, group_start, group_slice, u, cx,
@@ -1217,10 +1863,10 @@ extern "C" int dgfs_bgk_conservative(int dtype, int K, int Ne, int Nv, int ntask
 extern "C" size_t dgfs_esbgk_conservative_work_bytes(int ntask, int Nv) {
     const int nb = (Nv + kRB - 1) / kRB;
     return (size_t(ntask) * (nb * kShkMom + kShkCst + kEsCst + kEsgCst) +
-            newton_work_doubles(ntask, kEsgSums)) * sizeof(double);
+            newton_work_doubles(ntask, kEsNewtonSums)) * sizeof(double);
 }
 
-extern "C" int dgfs_esbgk_conservative(int dtype, int K, int Ne, int Nv, int ntask,
+extern "C" int dgfs_esbgk_conservative(int dtype, int K, int Ne, int Nv, int grid_n, int ntask,
                                        const int* task_nterm, const int* task_slice,
                                        const void* task_coef, int ngroup, const int* group_start,
                                        const int* group_slice, const void* u, const void* cx,
@@ -1231,6 +1877,8 @@ extern "C" int dgfs_esbgk_conservative(int dtype, int K, int Ne, int Nv, int nta
         DGFS_REQUIRE(dtype == DGFS_F64 || dtype == DGFS_F32, "dtype must be DGFS_F64 or DGFS_F32");
         DGFS_REQUIRE(K > 0 && Ne > 0 && Nv > 0 && ntask >= 0 && ngroup >= 0, "bad sizes");
         DGFS_REQUIRE(Pr > 0.0, "esbgk_conservative: Pr must be positive");
+        DGFS_REQUIRE(grid_n == 0 || (grid_n > 0 && long(grid_n) * grid_n * grid_n == Nv),
+                     "grid_n must be 0 or the cube root of Nv");
         if (ntask == 0 || ngroup == 0) return 0;
         DGFS_REQUIRE(ntask <= 65535 && ngroup <= 65535, "too many tasks");
         DGFS_REQUIRE(task_nterm && task_slice && task_coef && group_start && group_slice && u &&
@@ -1243,20 +1891,21 @@ extern "C" int dgfs_esbgk_conservative(int dtype, int K, int Ne, int Nv, int nta
         double* ebgk = cst + size_t(ntask) * kShkCst;
         double* eesg = ebgk + size_t(ntask) * kEsCst;
         double* npart = eesg + size_t(ntask) * kEsgCst;
-        unsigned* cnt = reinterpret_cast<unsigned*>(npart + size_t(ntask) * 2 * kNchMax * kEsgSums);
+        unsigned* flags = reinterpret_cast<unsigned*>(
+            npart + size_t(ntask) * 2 * kNchMax * kEsNewtonSums);
         const bool f64 = dtype == DGFS_F64;
         (f64 ? shk_partial<double> : shk_partial<float>)(K, Nv, nb, ntask, task_nterm, task_slice,
                                                          task_coef, u, cx, cy, cz, part, stream);
         DGFS_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_shk_final, dim3(ntask), dim3(kRB), 0, stream, nb, P,
-                           (const double*)part, cst, cnt, 2, status);
+                           (const double*)part, cst, flags, 2, status);
         DGFS_HIP_CHECK(hipGetLastError());
-        int rc = (f64 ? bgk_newton<double> : bgk_newton<float>)(
-            Nv, ntask, cx, cy, cz, cw, cst, npart, cnt, ebgk, status, stream);
+        int rc = (f64 ? bgk_newton<double, true> : bgk_newton<float, true>)(
+            Nv, grid_n, ntask, cx, cy, cz, cw, cst, npart, flags, ebgk, status, stream);
         if (rc) return rc;
         rc = (f64 ? esg_newton<double> : esg_newton<float>)(
-            K, Nv, ntask, task_nterm, task_slice, task_coef, u, cx, cy, cz, cw, Pr, cst, ebgk,
-            npart, cnt + ntask, eesg, status, stream);
+            Nv, grid_n, ntask, cx, cy, cz, cw, Pr, cst, ebgk, npart,
+            flags + size_t(ntask) * kNchMax, eesg, status, stream);
         if (rc) return rc;
         (f64 ? esg_apply<double> : esg_apply<float>)(K, Nv, nb, ngroup, task_nterm, task_slice,
                                                      task_coef, group_start, group_slice, u, cx,

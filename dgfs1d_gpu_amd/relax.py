@@ -126,6 +126,25 @@ class MixtureProjection:
 NEWTON_NONFINITE, NEWTON_NOT_CONVERGED, NEWTON_WAIT_TIMEOUT = 1, 2, 4
 
 
+def tensor_grid_n(cx, cy, cz, n):
+    """N if the n points (cx, cy, cz) are the tensor grid c0[i], c0[j], c0[k]
+    at p = (i N + j) N + k (the layout of std/velocitymesh.py), compared
+    exactly; 0 otherwise."""
+    n = int(n)
+    N = int(round(n ** (1.0 / 3.0)))
+    if N < 1 or N ** 3 != n:
+        return 0
+    x, y, z = (np.asarray(a.detach().cpu().numpy() if hasattr(a, 'detach') else a).ravel()
+               for a in (cx, cy, cz))
+    if x.size != n or y.size != n or z.size != n:
+        return 0
+    c0 = z[:N]
+    p = np.arange(n)
+    ok = (np.array_equal(x, c0[p // (N * N)]) and np.array_equal(y, c0[(p // N) % N])
+          and np.array_equal(z, c0[p % N]))
+    return N if ok else 0
+
+
 class Shakov:
     """Batched Shakov relaxation term Q += prefac rho T^{1-omega} (f_S - f);
     with target='esbgk' the ES-BGK term (dgfs_esbgk), f_S replaced by the
@@ -140,10 +159,13 @@ class Shakov:
     an infinity, as the reference does (std/scattering.py:689, :1084); a solve
     that stops at the iteration limit is logged, as the reference stops there
     without an error.  check=False leaves the call free of host
-    synchronisation; ``check_status()`` then reads the last call's status."""
+    synchronisation; ``check_status()`` then reads the last call's status.
+    grid_n: the Newton solves' velocity layout (dgfs_bgk_conservative's
+    grid_n); None tests the points once (tensor_grid_n), 0 sums over the
+    points as given."""
 
     def __init__(self, n, cx, cy, cz, cw, prefac, omega, Pr, tdtype, device, target='shakov',
-                 check=True):
+                 check=True, grid_n=None):
         fns = {'shakov': 'dgfs_shakov', 'esbgk': 'dgfs_esbgk', 'bgk': 'dgfs_bgk_conservative',
                'esbgk-conservative': 'dgfs_esbgk_conservative'}
         if target not in fns:
@@ -152,6 +174,12 @@ class Shakov:
         self._wfn = ('dgfs_bgk' if target == 'bgk' else fns[target]) + '_work_bytes'
         self.newton = target in ('bgk', 'esbgk-conservative')
         self.check = bool(check)
+        if not self.newton:
+            self.grid_n = 0
+        elif grid_n is None:
+            self.grid_n = tensor_grid_n(cx, cy, cz, n)
+        else:
+            self.grid_n = int(grid_n)
         import torch
         self.torch = torch
         self.n = int(n)
@@ -188,15 +216,16 @@ class Shakov:
         if u.numel() < nmin or Q.numel() < nmin:
             raise ValueError('Shakov operands smaller than K*Ne*Nv')
         pr = () if self._fn == 'dgfs_bgk_conservative' else (self.Pr,)
-        st = ()
+        st, gn = (), ()
         if self.newton:
+            gn = (self.grid_n,)
             if self._status is None or self._status.numel() < t['ntask']:
                 self._status = self.torch.zeros(max(1, t['ntask']), dtype=self.torch.int32,
                                                 device=self.device)
             st = (_lib.ptr(self._status),)
             self._ntask_last = t['ntask']
         _lib.check(getattr(_lib.lib(), self._fn)(
-            _lib.dtype_code(self.tdtype), t['K'], t['Ne'], self.n, t['ntask'],
+            _lib.dtype_code(self.tdtype), t['K'], t['Ne'], self.n, *gn, t['ntask'],
             _lib.ptr(t['nterm']), _lib.ptr(t['slice']), _lib.ptr(t['coef']), t['ngroup'],
             _lib.ptr(t['gstart']), _lib.ptr(t['gslice']), _lib.ptr(u), _lib.ptr(self.cx),
             _lib.ptr(self.cy), _lib.ptr(self.cz), self.cw, self.prefac, self.omega, *pr,

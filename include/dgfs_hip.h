@@ -411,12 +411,20 @@ size_t dgfs_esbgk_work_bytes(int ntask, int Nv);
  * a = (rho/(pi T)^1.5, 1/T, 0, 0, 0), at most 20 iterations, until every
  * moment residual is below 1e-12 of its scale), then
  *   Q[group slice] += prefac rho T^{1-omega} (fe - f_t).
- * FP64 moments and solve.  Each task's velocity sums are split over up to 32
- * work-groups of one cooperative launch.  status: device array of ntask ints,
+ * FP64 moments and solve.  grid_n > 0 declares cx, cy, cz the tensor grid
+ * of grid_n^3 = Nv points, cx[p] = c0[i], cy[p] = c0[j], cz[p] = c0[k] at
+ * p = (i grid_n + j) grid_n + k (the reference's velocity mesh, whose points
+ * are always laid out so; grid_n <= 64 for the tensor kernels, larger grids
+ * take the point sums): the Newton sums then factor into one-dimensional
+ * sums over c0 (= cz[0 .. grid_n)), one wave per task.  grid_n = 0: any
+ * points; each task's velocity sums are split over up to 64 work-groups of
+ * one launch sized to be resident at once (the work-groups of a task meet
+ * once per iteration).  status: device array of ntask ints,
  * overwritten with per-task bits: 1 = a residual or the parameters became NaN
  * or infinite (the reference raises 'NaN encountered', scattering.py:689),
  * 2 = not converged within the iteration limit (the reference stops there
- * silently), 4 = a work-group wait gave up.  The call itself does not
+ * silently), 4 = a work-group wait gave up after one second (other kernels
+ * held the device's compute units that long).  The call itself does not
  * synchronise; the caller reads status when it needs the verdict.
  * work: dgfs_bgk_work_bytes(ntask, Nv) bytes. */
 size_t dgfs_bgk_work_bytes(int ntask, int Nv);
@@ -429,17 +437,24 @@ size_t dgfs_bgk_work_bytes(int ntask, int Nv);
  * momentum and second moments equal rho, rho u, rho (u_i u_j + T_ij)
  * (Newton's method, at most 40 iterations); Q += prefac rho T^{1-omega} (G - f),
  * prefac the caller's (t0 Pr p0 / mu(T0) in the reference).  Pr > 0.
+ * grid_n: as dgfs_bgk_conservative's; on the tensor grid the ES-BGK sums are
+ * taken column by column, the z-sums as Horner sums of a geometric factor
+ * (one work-group per task).  The ES-BGK Newton starts from the
+ * continuous Gaussian with the target covariance (the reference starts from
+ * the Maxwellian of rho, u, T, esbgk-gll.mako:261-270; the root is the same).
  * status: as dgfs_bgk_conservative's, the bits of both solves combined.
  * work: dgfs_esbgk_conservative_work_bytes(ntask, Nv) bytes. */
 size_t dgfs_esbgk_conservative_work_bytes(int ntask, int Nv);
-int dgfs_esbgk_conservative(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nterm,
-                            const int* task_slice, const void* task_coef, int ngroup,
+int dgfs_esbgk_conservative(int dtype, int K, int Ne, int Nv, int grid_n, int ntask,
+                            const int* task_nterm, const int* task_slice,
+                            const void* task_coef, int ngroup,
                             const int* group_start, const int* group_slice, const void* u,
                             const void* cx, const void* cy, const void* cz, double cw,
                             double prefac, double omega, double Pr, void* Q, int* status,
                             double* work, hipStream_t stream);
-int dgfs_bgk_conservative(int dtype, int K, int Ne, int Nv, int ntask, const int* task_nterm,
-                          const int* task_slice, const void* task_coef, int ngroup,
+int dgfs_bgk_conservative(int dtype, int K, int Ne, int Nv, int grid_n, int ntask,
+                          const int* task_nterm, const int* task_slice,
+                          const void* task_coef, int ngroup,
                           const int* group_start, const int* group_slice, const void* u,
                           const void* cx, const void* cy, const void* cz, double cw,
                           double prefac, double omega, void* Q, int* status, double* work,

@@ -82,16 +82,22 @@ def test_new_entry_points_reject_bad_arguments():
                         1.0, 1.0, 0.81, 0.5, None, None, None) == 2
     assert b'null pointer' in L.dgfs_last_error()
     assert L.dgfs_esbgk_work_bytes(2, 600) == 2 * (3 * (13 + 6) + 32) * 8
-    assert L.dgfs_bgk_conservative(0, 1, 1, 8, 1, None, None, None, 1, None, None, None, None,
+    assert L.dgfs_bgk_conservative(0, 1, 1, 8, 0, 1, None, None, None, 1, None, None, None, None,
                                    None, None, 1.0, 1.0, 0.81, None, None, None, None) == 2
     assert b'null pointer' in L.dgfs_last_error()
+    # grid_n must be 0 or the cube root of Nv
+    assert L.dgfs_bgk_conservative(0, 1, 1, 8, 3, 1, None, None, None, 1, None, None, None, None,
+                                   None, None, 1.0, 1.0, 0.81, None, None, None, None) == 2
+    assert b'cube root' in L.dgfs_last_error()
     # per task: 3 blocks x 13 moment partials, 32 constants, then the Newton
-    # partials (2 buffers x 32 work-groups x 30 sums) and one double holding
-    # the two arrival counters
-    assert L.dgfs_bgk_work_bytes(2, 600) == (2 * (3 * 13 + 32) + 2 * (2 * 32 * 30 + 1)) * 8
-    assert L.dgfs_esbgk_conservative(0, 1, 1, 8, 1, None, None, None, 1, None, None, None, None,
-                                     None, None, 1.0, 1.0, 0.81, -1.0, None, None, None,
+    # partials (2 buffers x 64 work-groups x 30 sums) and 64 doubles holding
+    # the two sets of 64 work-group flags
+    assert L.dgfs_bgk_work_bytes(2, 600) == (2 * (3 * 13 + 32) + 2 * (2 * 64 * 30 + 64)) * 8
+    assert L.dgfs_esbgk_conservative(0, 1, 1, 8, 2, 1, None, None, None, 1, None, None, None,
+                                     None, None, None, 1.0, 1.0, 0.81, -1.0, None, None, None,
                                      None) == 2
     assert b'Pr must be positive' in L.dgfs_last_error()
+    # ES-BGK: the larger of its BGK solve's 36 sums (30 + the fe tensor) and
+    # its own 35 monomial sums per work-group
     assert L.dgfs_esbgk_conservative_work_bytes(2, 600) == \
-        (2 * (3 * 13 + 48) + 2 * (2 * 32 * 55 + 1)) * 8
+        (2 * (3 * 13 + 48) + 2 * (2 * 64 * 36 + 64)) * 8

@@ -146,17 +146,21 @@ def test_axnpby_compensated(cuda):
     assert np.all(up.cpu().numpy() == 1.0)
 
 
-def _newton_model(cuda, td, N, target, Pr=0.6, check=True):
+def _newton_model(cuda, td, N, target, Pr=0.6, check=True, grid_n=None):
     from dgfs1d_gpu_amd.relax import Shakov
     cv, cw = _grid(N)
-    return Shakov(N ** 3, _dev(cv[0], td, cuda), _dev(cv[1], td, cuda), _dev(cv[2], td, cuda),
-                  cw, 1.0, 0.81, Pr, td, cuda, target=target, check=check), cv, cw
+    sh = Shakov(N ** 3, _dev(cv[0], td, cuda), _dev(cv[1], td, cuda), _dev(cv[2], td, cuda),
+                cw, 1.0, 0.81, Pr, td, cuda, target=target, check=check, grid_n=grid_n)
+    assert sh.grid_n == (N if grid_n is None else grid_n)   # the test grid is a tensor grid
+    return sh, cv, cw
 
 
+@pytest.mark.parametrize('grid_n', [None, 0], ids=['tensor', 'points'])
 @pytest.mark.parametrize('target', ['bgk', 'esbgk-conservative'])
-def test_newton_targets_node_by_node(cuda, target):
+def test_newton_targets_node_by_node(cuda, target, grid_n):
     """'bgk-gll' (std/scattering.py:445-706) and 'esbgk-gll' (:752-1185): the
-    device Newton solves, with each task's sums spread over many work-groups,
+    device Newton solves (the BGK sums factored on the tensor grid, or summed
+    over the points with each task's sums spread over many work-groups)
     against the oracle's MINPACK solves node by node (FP64, 12 tasks as the
     Couette deck's K = 3, Ne = 4), and the moments of the device targets
     against the conditions that define them: mass, momentum and energy of f
@@ -164,7 +168,7 @@ def test_newton_targets_node_by_node(cuda, target):
     import torch
     td = torch.float64
     N, K, Ne, Pr = 16, 3, 4, 0.6
-    sh, cv, cw = _newton_model(cuda, td, N, target, Pr)
+    sh, cv, cw = _newton_model(cuda, td, N, target, Pr, grid_n=grid_n)
     u = _state(cv, K, Ne, seed=21)
     tasks = [(s, [(s, 1.0)]) for s in range(K * Ne)]
     sh.set_tasks(K, Ne, [(o, f, f) for o, f in tasks])
@@ -206,7 +210,41 @@ def test_newton_targets_node_by_node(cuda, target):
 
 
 @pytest.mark.parametrize('target', ['bgk', 'esbgk-conservative'])
-def test_newton_nan_raises(cuda, target):
+def test_newton_tensor_matches_points_fp32(cuda, target):
+    """The tensor-grid Newton kernels (one-dimensional moments for BGK,
+    column-wise Horner sums with the node-rounding correction for ES-BGK)
+    against the point-sum kernels on an FP32 grid whose spacing is not a
+    binary fraction (so the stored nodes are not exactly uniform), N = 24,
+    12 tasks: the collision terms agree to FP32 output rounding, and the
+    device targets keep f's moments."""
+    import torch
+    from dgfs1d_gpu_amd.relax import Shakov
+    td = torch.float32
+    N, K, Ne, Pr = 24, 3, 4, 2.0 / 3.0
+    cv, cw = _grid(N, L=5.2371)
+    cv32 = cv.astype(np.float32)
+    dev = [_dev(cv32[i], td, cuda) for i in range(3)]
+    u = _state(cv, K, Ne, seed=7).astype(np.float32)
+    out = {}
+    for gn in (None, 0):
+        sh = Shakov(N ** 3, *dev, cw, 1.0, 0.81, Pr, td, cuda, target=target, grid_n=gn)
+        assert sh.grid_n == (N if gn is None else 0)
+        sh.set_tasks(K, Ne, [(s_, [(s_, 1.0)], [(s_, 1.0)]) for s_ in range(K * Ne)])
+        Q = torch.zeros(K * Ne * N ** 3, dtype=td, device=cuda)
+        sh(_dev(u, td, cuda), Q)
+        out[gn] = Q.double().cpu().numpy().reshape(K * Ne, -1)
+    err = np.abs(out[None] - out[0]).max() / np.abs(out[0]).max()
+    phi = np.vstack([np.ones(cv.shape[1]), cv32.astype(np.float64),
+                     (cv32.astype(np.float64) ** 2).sum(0)]) * cw
+    mom = max(np.abs(phi @ out[g].T).max() / (np.abs(phi) @ np.abs(out[g].T)).max()
+              for g in out)
+    print(target, 'tensor vs points', err, 'moments of Q', mom)
+    assert err < 2e-6 and mom < 1e-6
+
+
+@pytest.mark.parametrize('grid_n', [None, 0], ids=['tensor', 'points'])
+@pytest.mark.parametrize('target', ['bgk', 'esbgk-conservative'])
+def test_newton_nan_raises(cuda, target, grid_n):
     """A task whose slice is zero has no temperature: its Newton solve meets a
     NaN, the status says so and the call raises, as the reference's
     RuntimeError('NaN encountered') (std/scattering.py:689, :1084).  The other
@@ -214,7 +252,7 @@ def test_newton_nan_raises(cuda, target):
     import torch
     td = torch.float64
     N, K, Ne = 8, 1, 3
-    sh, cv, cw = _newton_model(cuda, td, N, target, check=False)
+    sh, cv, cw = _newton_model(cuda, td, N, target, check=False, grid_n=grid_n)
     u = _state(cv, K, Ne, seed=5)
     u[0, 1] = 0.0
     sh.set_tasks(K, Ne, [(s, [(s, 1.0)], [(s, 1.0)]) for s in range(Ne)])

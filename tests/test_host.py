@@ -100,6 +100,27 @@ def test_velocity_mesh_matches_oracle():
     assert np.abs(p.sz() - o.sz).max() == 0.0
 
 
+def test_tensor_grid_detection():
+    """The Newton solves' tensor layout (relax.tensor_grid_n): the velocity
+    mesh's points are the tensor grid of its 1D nodes, in FP64 and rounded to
+    FP32; a permuted, perturbed or non-cubic point set is not."""
+    from dgfs1d_gpu_amd.std.velocitymesh import DGFSVelocityMeshStd
+    from dgfs1d_gpu_amd.relax import tensor_grid_n
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cfg = Dictionary.load(os.path.join(root, 'dgfs1d_gpu_amd', 'configs', 'couette_small.ini'))
+    cv = DGFSVelocityMeshStd(cfg).cv()
+    n = cv.shape[1]
+    N = int(round(n ** (1 / 3)))
+    assert tensor_grid_n(cv[0], cv[1], cv[2], n) == N
+    c32 = cv.astype(np.float32)
+    assert tensor_grid_n(c32[0], c32[1], c32[2], n) == N
+    assert tensor_grid_n(cv[1], cv[0], cv[2], n) == 0          # x and y swapped
+    bad = cv[2].copy()
+    bad[n // 2] += 1e-12
+    assert tensor_grid_n(cv[0], cv[1], bad, n) == 0
+    assert tensor_grid_n(cv[0][:-1], cv[1][:-1], cv[2][:-1], n - 1) == 0
+
+
 def test_mesh_types():
     from dgfs1d_gpu_amd.mesh import Mesh
     base = '[mesh]\nxlo=0\nxhi=2\nNe=4\nH0=2\n'
