@@ -453,6 +453,154 @@ k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
     }
 }
 
+// k_gain on the lane-pair engine (Fft3Pair): the same work-group grid,
+// products, acc layout and stored-pair layout as k_gain with one transform
+// (f = g, or the reference-state modes), with 2 N (N / S) threads of at most
+// 128 VGPRs, so that four waves share each SIMD instead of two.  Lane h of
+// pencil pair P holds the a1-outputs (and acc entries) o1 = (N/2) h + k.
+constexpr int kPairDeltaChunk = 4;   // stored pairs in flight per lane (register budget)
+
+template <typename T, int N, int S, int MODE = kGainAccumulate>
+__global__ void __launch_bounds__(2 * N * (N / S))
+k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
+            const T* __restrict__ nyq, int M, int Nrho, int QG, cplx<T>* __restrict__ acc_out,
+            UPair<T>* __restrict__ ucache, int remap) {
+    using E = Fft3Pair<T, N, S, +1>;
+    using NL = NyLayout<N, S>;
+    constexpr int n = N * N * N, NN = N * N;
+    constexpr int NS = N / S, BP = NL::BP, H = E::H;
+    __shared__ cplx<T> V[E::LDS_ELEMS];
+    __shared__ T Ny[NL::SIZE];
+    const int nblk = gridDim.x;
+    const int lb = remap ? xcd_remap(blockIdx.x, nblk) : int(blockIdx.x);
+    const int qg = lb % QG;
+    const int r = (lb / QG) % S;
+    const int p = (lb / (QG * S)) % Nrho;
+    const int t = lb / (QG * S * Nrho);
+    const int h = threadIdx.x & 1, P = threadIdx.x >> 1;
+    __shared__ typename E::Split sp;      // the residue split's twiddles, uniform per group
+    if (threadIdx.x == 0) sp.init(r);     // (read after the engine's opening barrier)
+
+    const cplx<T>* Ff = Fhat + size_t(t) * 2 * n;
+    const int qchunk = (M + QG - 1) / QG;
+    const int q0 = qg * qchunk;
+    const int q1 = (q0 + qchunk < M) ? q0 + qchunk : M;
+
+    cplx<T> acc[H];
+    sfor<0, H>([&](auto I) { acc[I] = mk(T(0), T(0)); });
+
+    auto map1 = [](int pp, int& i1, int& i2) { i1 = pp / N; i2 = pp - (pp / N) * N; };  // (l1, l2)
+    auto map3 = [](int Q, int& o2, int& j) { o2 = Q / NS; j = Q - (Q / NS) * NS; };      // (y, j)
+
+    for (int q = q0; q < q1; ++q) {
+        const cplx<T>* ph = phase + (size_t(p) * M + q) * 3 * N;   // [axis][l]
+        // the transform's base pointer, opaque to the compiler once per q, so
+        // that pass 1's N strided addresses are formed as they are used
+        const cplx<T>* Fq = Ff;
+        asm volatile("" : "+s"(Fq));
+        E::run(V, sp,
+            [&] {
+                // stage this q's Nyquist imaginary parts for residue r
+                for (int i = threadIdx.x; i < 6 * N * NS + 2 * NN; i += E::NT) {
+                    if (i < 6 * N * NS) {
+                        // A+- [y][j] and B+- [j][x] (both N*NS per sign)
+                        const int seg = i / (N * NS), e = i - seg * (N * NS);   // seg 0..5
+                        if (seg >= 4) continue;
+                        const T* src = nyq + (((size_t(t) * Nrho + p) * M + q) * 6 + seg) * NN;
+                        if (seg < 2) {       // A: e = y*NS + j
+                            const int y = e / NS, j = e - y * NS;
+                            Ny[NL::A + seg * N * NS + e] = src[y * N + S * j + r];
+                        } else {             // B: e = j*N + x
+                            const int j = e / N, x = e - j * N;
+                            Ny[NL::B + (seg - 2) * NS * BP + j * BP + x] =
+                                src[(S * j + r) * N + x];
+                        }
+                    } else {
+                        // C+- [y][x]
+                        const int k = i - 6 * N * NS;
+                        const int sg = k / NN, e = k - sg * NN;
+                        const T* src = nyq + (((size_t(t) * Nrho + p) * M + q) * 6 + 4 + sg) * NN;
+                        Ny[NL::C + sg * N * BP + (e / N) * BP + e % N] = src[e];
+                    }
+                }
+            },
+            map1,
+            [&](int i1, int i2, int i3) {
+                const cplx<T> e = ph[i1] * ph[N + i2] * ph[2 * N + i3];
+                return (e.x + e.y) * Fq[(size_t(i3) * N + i1) * N + i2];
+            },
+            map3,
+            [&](int o2, int j, const cplx<T>* o) {
+                const int z = S * j + r;
+                const T sy = (o2 & 1) ? T(-1) : T(1), sz = (z & 1) ? T(-1) : T(1);
+                // o[k] is the output at x = H h + k; H is even, so x and k
+                // have the same parity
+                auto unpack = [&](auto K, cplx<T> v, cplx<T>& up, cplx<T>& um) {
+                    const int x = H * h + int(K);
+                    constexpr T sx = (int(K) & 1) ? T(-1) : T(1);
+                    const T beta = sx * Ny[NL::A + o2 * NS + j] + sy * Ny[NL::B + j * BP + x] +
+                                   sz * Ny[NL::C + o2 * BP + x];
+                    const T delta = sx * Ny[NL::A + N * NS + o2 * NS + j] +
+                                    sy * Ny[NL::B + NS * BP + j * BP + x] +
+                                    sz * Ny[NL::C + N * BP + o2 * BP + x];
+                    // w = (1 + i) v = u+ + i u-
+                    const T wr = v.x - v.y, wi = v.x + v.y;
+                    up = mk(wr + delta, beta);
+                    um = mk(wi - beta, delta);
+                };
+                // stored pairs [t][p][q][k][r][P][h] (x = H h + k): one
+                // contiguous run of 2 NPEN pairs per (k, r), read by the
+                // work-group's lanes in order
+                const size_t cbase =
+                    ((((size_t(t) * Nrho + p) * M + q) * H) * S + r) * size_t(E::NT) +
+                    threadIdx.x;
+                if constexpr (MODE == kGainStore) {
+                    UPair<T>* dst = ucache + cbase;
+                    sfor<0, H>([&](auto K) {
+                        UPair<T> v;
+                        unpack(K, o[K], v.p, v.m);
+                        store_pair_nt(dst + size_t(int(K)) * S * E::NT, v);
+                    });
+                } else if constexpr (MODE == kGainDelta) {
+                    const UPair<T>* src = ucache + cbase;
+                    constexpr int CH = kPairDeltaChunk;
+                    sfor<0, H / CH>([&](auto C) {
+                        UPair<T> c0[CH];
+                        sfor<0, CH>([&](auto k) {
+                            c0[k] = load_pair_nt(src + size_t(int(C) * CH + int(k)) * S * E::NT);
+                        });
+                        sfor<0, CH>([&](auto k) {
+                            constexpr int i = int(C) * CH + int(k);
+                            cplx<T> up, um;
+                            unpack(std::integral_constant<int, i>{}, o[i], up, um);
+                            acc[i] = acc[i] + up * (um + c0[k].m) + um * c0[k].p;
+                        });
+                        asm volatile("" ::: "memory");   // one chunk of loads in flight
+                    });
+                } else {
+                    sfor<0, H>([&](auto K) {
+                        cplx<T> up, um;
+                        unpack(K, o[K], up, um);
+                        acc[K] = acc[K] + up * um;
+                    });
+                }
+            });
+    }
+    if constexpr (MODE == kGainStore) return;
+    int o2, j;
+    map3(P, o2, j);
+    // forward transform along x of the pair's pencil (first axis of
+    // FFT3(acc_p)); lane h receives k1 = 2 m + h
+    cplx<T> out[H];
+    pair_dft_dif<T, N, -1>(acc, out, h);
+    // acc layout [t][p][qg][k1][r][y][j] with z = S*j + r
+    cplx<T>* dst = acc_out + ((size_t(t) * Nrho + p) * QG + qg) * n;
+    sfor<0, H>([&](auto K) {
+        const int k1 = 2 * int(K) + h;
+        dst[((size_t(k1) * S + r) * N + o2) * NS + j] = out[K];
+    });
+}
+
 // ================================================================ kernel C
 // One work-group per plane (t, k1).  Its PL slots of N threads take radii
 // p = slot, slot+PL, ...:  G^(k1,:,:) = sum_p wtab_p(k1,:,:) FFT2_{y,z}(sum_qg acc[t][p][qg][k1]),
@@ -684,6 +832,10 @@ struct Impl {
     // N) with an unpadded LDS cube, so that two work-groups may share a CU
     static constexpr int S1x = (N % (2 * S1) == 0) ? 2 * S1 : S1;
     static constexpr int PL = planes_per_wg(N);
+    // the f = g gain on the lane-pair engine (four waves per SIMD): FP32 at
+    // N = 32, where its 2 N (N / S1) = 1024 threads fit one work-group
+    static constexpr bool kPair = std::is_same<T, float>::value && N == 32 &&
+                                  2 * N * (N / S1) <= 1024;
 
     // Q0 == nullptr: Q += Q(f, g).  Q0 != nullptr (reference-state form, ug the
     // reference state u0 whose F0^ and u+-_f0 the plan holds): Q += Q0 +
@@ -708,11 +860,24 @@ struct Impl {
         const int S = alt ? S1x : (fg_same ? S1 : S2);
         const int nb = ntask * pl->Nrho * S * pl->QG;
         const int rm = pl->gain_remap;
-        if (dev)
-            hipLaunchKernelGGL((k_gain<T, N, S1, 1, kGainDelta>), dim3(nb),
-                               dim3(round64(N * (N / S1))), 0, st, (const cplx<T>*)pl->Fhat,
-                               (const cplx<T>*)pl->phase, (const T*)pl->nyq, pl->M, pl->Nrho,
-                               pl->QG, (cplx<T>*)pl->acc, (UPair<T>*)pl->ucache, rm);
+        if (dev) {
+            if constexpr (kPair)
+                hipLaunchKernelGGL((k_gain_pair<T, N, S1, kGainDelta>), dim3(nb),
+                                   dim3(2 * N * (N / S1)), 0, st, (const cplx<T>*)pl->Fhat,
+                                   (const cplx<T>*)pl->phase, (const T*)pl->nyq, pl->M, pl->Nrho,
+                                   pl->QG, (cplx<T>*)pl->acc, (UPair<T>*)pl->ucache, rm);
+            else
+                hipLaunchKernelGGL((k_gain<T, N, S1, 1, kGainDelta>), dim3(nb),
+                                   dim3(round64(N * (N / S1))), 0, st, (const cplx<T>*)pl->Fhat,
+                                   (const cplx<T>*)pl->phase, (const T*)pl->nyq, pl->M, pl->Nrho,
+                                   pl->QG, (cplx<T>*)pl->acc, (UPair<T>*)pl->ucache, rm);
+        } else if (fg_same && !alt && kPair) {
+            if constexpr (kPair)
+                hipLaunchKernelGGL((k_gain_pair<T, N, S1, kGainAccumulate>), dim3(nb),
+                                   dim3(2 * N * (N / S1)), 0, st, (const cplx<T>*)pl->Fhat,
+                                   (const cplx<T>*)pl->phase, (const T*)pl->nyq, pl->M, pl->Nrho,
+                                   pl->QG, (cplx<T>*)pl->acc, (UPair<T>*)nullptr, rm);
+        }
         else if (alt)
             hipLaunchKernelGGL((k_gain<T, N, S1x, 1, kGainAccumulate, 0>), dim3(nb),
                                dim3(round64(N * (N / S1x))), 0,
@@ -789,10 +954,17 @@ struct Impl {
                            1, (T*)pl->nyq);
         DGFS_HIP_CHECK(hipGetLastError());
         const int nb = ntask * pl->Nrho * S1 * pl->QG;
-        hipLaunchKernelGGL((k_gain<T, N, S1, 1, kGainStore>), dim3(nb), dim3(round64(N * (N / S1))),
-                           0, st, (const cplx<T>*)pl->Fhat0, (const cplx<T>*)pl->phase,
-                           (const T*)pl->nyq, pl->M, pl->Nrho, pl->QG, (cplx<T>*)nullptr,
-                           (UPair<T>*)pl->ucache, pl->gain_remap);
+        // the store pass lays the pairs out for the delta kernel that reads them
+        if constexpr (kPair)
+            hipLaunchKernelGGL((k_gain_pair<T, N, S1, kGainStore>), dim3(nb),
+                               dim3(2 * N * (N / S1)), 0, st, (const cplx<T>*)pl->Fhat0,
+                               (const cplx<T>*)pl->phase, (const T*)pl->nyq, pl->M, pl->Nrho,
+                               pl->QG, (cplx<T>*)nullptr, (UPair<T>*)pl->ucache, pl->gain_remap);
+        else
+            hipLaunchKernelGGL((k_gain<T, N, S1, 1, kGainStore>), dim3(nb),
+                               dim3(round64(N * (N / S1))), 0, st, (const cplx<T>*)pl->Fhat0,
+                               (const cplx<T>*)pl->phase, (const T*)pl->nyq, pl->M, pl->Nrho,
+                               pl->QG, (cplx<T>*)nullptr, (UPair<T>*)pl->ucache, pl->gain_remap);
         DGFS_HIP_CHECK(hipGetLastError());
         return 0;
     }

@@ -251,6 +251,147 @@ struct Fft3Engine {
     }
 };
 
+// ------------------------------------------------------- lane-pair variant
+// The partner lane's value (lanes 2m <-> 2m + 1), a DPP quad permutation.
+DGFS_DEV float pair_swap(float v) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+}
+DGFS_DEV double pair_swap(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp(int(unsigned(b)), 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(int(unsigned(b >> 32)), 0xB1, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(unsigned(hi)) << 32) |
+                                          unsigned(lo));
+}
+template <typename T>
+DGFS_DEV cplx<T> pair_swap(cplx<T> v) { return mk(pair_swap(v.x), pair_swap(v.y)); }
+
+// Length-L DFT shared by a lane pair, decimation in time: lane h holds
+// x[k] = in[2 k + h], k < L/2, and receives out[k] = X[(L/2) h + k].
+template <typename T, int L, int DIR>
+DGFS_DEV void pair_dft_dit(const cplx<T>* x, cplx<T>* out, int h) {
+    constexpr int H = L / 2;
+    cplx<T> y[H];
+    RegDFT<T, H, DIR>::template run<1, 0>(x, y);
+    sfor<0, H>([&](auto K) {
+        constexpr int k = K;
+        const cplx<T> w = twiddle<T, long(DIR) * k, L>(y[k]);   // lane 1: w^k Y_1[k]
+        y[k] = h ? w : y[k];
+    });
+    sfor<0, H>([&](auto K) {
+        const cplx<T> r = pair_swap(y[K]);
+        out[K] = h ? r - y[K] : y[K] + r;   // X[k] = Y0 + w^k Y1, X[k + H] = Y0 - w^k Y1
+    });
+}
+
+// Length-L DFT shared by a lane pair, decimation in frequency: lane h holds
+// a[k] = in[(L/2) h + k], k < L/2, and receives out[m] = X[2 m + h].
+template <typename T, int L, int DIR>
+DGFS_DEV void pair_dft_dif(const cplx<T>* a, cplx<T>* out, int h) {
+    constexpr int H = L / 2;
+    cplx<T> d[H];
+    sfor<0, H>([&](auto K) {
+        constexpr int k = K;
+        const cplx<T> r = pair_swap(a[k]);
+        const cplx<T> s = h ? r - a[k] : a[k] + r;   // lane 1: in[k] - in[k + H]
+        d[k] = h ? twiddle<T, long(DIR) * k, L>(s) : s;
+    });
+    RegDFT<T, H, DIR>::template run<1, 0>(d, out);
+}
+
+// Fft3Engine's transform with 2 NPEN threads: every pass-2 column and pass-3
+// pencil is shared by a lane pair (pair_dft_dit), and pass 1 folds its
+// residue split as it loads, so that no thread holds a whole length-N
+// vector; twice the waves of Fft3Engine on the same LDS cube.  Pass 3
+// leaves lane h of pair P the outputs o1 = (N/2) h + k, k < N/2.
+template <typename T, int N_, int S_, int DIR_, int PAD = 1>
+struct Fft3Pair : Fft3Engine<T, N_, S_, 2 * N_ * (N_ / S_), DIR_, PAD> {
+    using B = Fft3Engine<T, N_, S_, 2 * N_ * (N_ / S_), DIR_, PAD>;
+    using B::N;
+    using B::S;
+    using B::NS;
+    using B::NPEN;
+    using B::NT;
+    using B::DIR;
+    using typename B::Split;
+    static constexpr int H = N / 2;
+    static_assert(N % 2 == 0 && H % 2 == 0, "pair transforms need N divisible by 4");
+
+    // sp may live in LDS (its twiddles are the same for the whole work-group).
+    // The fold runs in chunks of CK outputs with a compiler barrier between
+    // them, so that at most S CK loads are in flight per thread.
+    template <class Map1, class Load>
+    static DGFS_DEV void pass1(cplx<T>* V, const Split& sp, Map1&& map1, Load&& load) {
+        constexpr int CK = NS % 4 == 0 ? 4 : 1;
+        for (int p = threadIdx.x; p < N * N; p += NT) {
+            int i1, i2;
+            map1(p, i1, i2);
+            cplx<T> b[NS];
+            sfor<0, NS / CK>([&](auto Ci) {
+                sfor<0, CK>([&](auto Ki) {
+                    constexpr int m = int(Ci) * CK + int(Ki);
+                    cplx<T> acc = load(i1, i2, m);
+                    sfor<1, S>([&](auto Si) {
+                        constexpr int s = Si;
+                        acc = acc + load(i1, i2, m + s * NS) * sp.wS[s];
+                    });
+                    b[m] = (m == 0) ? acc : acc * sp.wN[m];
+                });
+                asm volatile("" ::: "memory");
+            });
+            cplx<T> o[NS];
+            RegDFT<T, NS, DIR>::template run<1, 0>(b, o);
+            sfor<0, NS>([&](auto J) { V[B::vidx(i1, i2, J)] = o[J]; });
+        }
+    }
+
+    // The LDS bases below pass through an empty asm statement: the compiler
+    // then forms each address from one base per pass instead of keeping
+    // H addresses alive across the caller's loop.
+    static DGFS_DEV int opaque(int v) {
+        asm volatile("" : "+v"(v));
+        return v;
+    }
+
+    static DGFS_DEV void pass2(cplx<T>* V) {
+        const int h = threadIdx.x & 1, c = threadIdx.x >> 1;    // c < NPEN
+        const int i1 = c / NS, j = c - (c / NS) * NS;
+        const int bi = opaque(B::vidx(i1, h, j));               // + 2 k rows
+        const int bo = opaque(B::vidx(i1, H * h, j));           // + k rows
+        cplx<T> x[H], o[H];
+        sfor<0, H>([&](auto K) { x[K] = V[bi + 2 * int(K) * B::NSP]; });
+        pair_dft_dit<T, N, DIR>(x, o, h);
+        sfor<0, H>([&](auto K) { V[bo + int(K) * B::NSP] = o[K]; });
+    }
+
+    // pencil P = thread / 2 along a1; lane h gets o[k] = output o1 = H h + k
+    template <class Map3>
+    static DGFS_DEV void pass3(const cplx<T>* V, Map3&& map3, int& o2, int& j, cplx<T>* o) {
+        const int h = threadIdx.x & 1;
+        map3(int(threadIdx.x >> 1), o2, j);
+        const int bi = opaque(B::vidx(h, o2, j));               // + 2 k planes
+        cplx<T> x[H];
+        sfor<0, H>([&](auto K) { x[K] = V[bi + 2 * int(K) * N * B::NSP]; });
+        pair_dft_dit<T, N, DIR>(x, o, h);
+    }
+
+    template <class Pro, class Map1, class Load, class Map3, class Epi>
+    static DGFS_DEV void run(cplx<T>* V, const Split& sp, Pro&& prologue, Map1&& map1,
+                             Load&& load, Map3&& map3, Epi&& epi) {
+        __syncthreads();
+        prologue();
+        pass1(V, sp, map1, load);
+        __syncthreads();
+        pass2(V);
+        __syncthreads();
+        int o2, j;
+        cplx<T> o[H];
+        pass3(V, map3, o2, j, o);
+        epi(o2, j, o);
+    }
+};
+
 struct IdentityPre {
     template <class I, class V>
     DGFS_DEV V operator()(I, V v) const { return v; }
