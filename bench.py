@@ -30,15 +30,15 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3      # MI355X published dense FP32
 PEAK_HBM_GBS = 8000.0
 
-# HBM bytes per k_gain dispatch, from two rocprofv3 counter passes over this
+# HBM bytes per gain dispatch (k_gain_pair at N=32 FP32), from two rocprofv3 counter passes over this
 # script (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what
 # the same counter reads for a device-to-device copy of 512 MiB (--copy-calibration):
 # FETCH x 2.000, WRITE x 1.000.  Keyed by workload; re-measured whenever the
 # kernel changes, null for workloads not measured.
 GAIN_TRAFFIC_KIB = {
-    'couette Ne=4 K=3 N=32^3 M=12 Nrho=16 ssp-rk2 dt=0.0005': (32022.9, 2.000, 98304.0, 1.000),
+    'couette Ne=4 K=3 N=32^3 M=12 Nrho=16 ssp-rk2 dt=0.0005': (35808.4, 2.000, 107520.0, 1.000),
     'couette Ne=4 K=3 N=32^3 M=12 Nrho=16 ssp-rk2 dt=0.0005 reference-state R=40':
-        (621858.8, 2.000, 98304.0, 1.000),
+        (626691.3, 2.000, 110583.8, 1.000),
 }
 
 
@@ -549,7 +549,7 @@ def main(argv=None):
             'pgroups': q['pgroups'],
             'collision_form': 'direct' if ref is None else
             'reference-state, FP64 Q(f0,f0) refreshed every %d calls' % ref['R']},
-        'roofline': {'bound': 'fp32-valu', 'kernel': 'k_gain', 'achieved': achieved,
+        'roofline': {'bound': 'fp32-valu', 'kernel': 'k_gain_pair', 'achieved': achieved,
                      'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / PEAK_FP32_TFLOPS, 'traffic': traffic,
                      'flops_per_launch': gain_flops, 'avg_launch_ms': gain_ms,
