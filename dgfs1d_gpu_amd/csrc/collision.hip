@@ -160,7 +160,19 @@ extern "C" int dgfs_collision_plan_create(int N, int M, int Nrho, double L, cons
         }
         return 0;
     };
+    // the gain's z-axis factors: with e = A B (B the z phase), Re e + Im e =
+    // A.x (B.x + B.y) + A.y (B.x - B.y); [Nrho][M][N] pairs (B.x + B.y, B.x - B.y)
+    std::vector<double> pz(size_t(Nrho) * M * N * 2);
+    for (int p = 0; p < Nrho; ++p)
+        for (int q = 0; q < M; ++q)
+            for (int i = 0; i < N; ++i) {
+                const size_t o = (((size_t(p) * M + q) * 3 + 2) * N + i) * 2;
+                const size_t d = ((size_t(p) * M + q) * N + i) * 2;
+                pz[d] = ph[o] + ph[o + 1];
+                pz[d + 1] = ph[o] - ph[o + 1];
+            }
     int rc = upload(&pl->phase, ph);
+    if (!rc) rc = upload(&pl->phz, pz);
     if (!rc) rc = upload(&pl->wtab, wt);
     if (!rc) rc = upload(&pl->bb2, b2);
     if (rc) { dgfs_collision_plan_destroy(pl); return rc; }
@@ -225,7 +237,7 @@ extern "C" int dgfs_collision_plan_destroy(dgfs_collision_plan_t plan) {
     auto* pl = static_cast<CollisionPlan*>(plan);
     if (!pl) return 0;
     free_events(pl);
-    for (void* p : {pl->phase, pl->wtab, pl->bb2, pl->Fhat, pl->acc, pl->Iw, pl->H, pl->nyq,
+    for (void* p : {pl->phase, pl->phz, pl->wtab, pl->bb2, pl->Fhat, pl->acc, pl->Iw, pl->H, pl->nyq,
                     pl->Fhat0, pl->ucache})
         if (p) (void)hipFree(p);
     delete pl;

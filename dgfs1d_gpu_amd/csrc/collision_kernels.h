@@ -463,8 +463,8 @@ constexpr int kPairDeltaChunk = 4;   // stored pairs in flight per lane (registe
 template <typename T, int N, int S, int MODE = kGainAccumulate>
 __global__ void __launch_bounds__(2 * N * (N / S))
 k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
-            const T* __restrict__ nyq, int M, int Nrho, int QG, cplx<T>* __restrict__ acc_out,
-            UPair<T>* __restrict__ ucache, int remap) {
+            const cplx<T>* __restrict__ phz, const T* __restrict__ nyq, int M, int Nrho, int QG,
+            cplx<T>* __restrict__ acc_out, UPair<T>* __restrict__ ucache, int remap) {
     using E = Fft3Pair<T, N, S, +1>;
     using NL = NyLayout<N, S>;
     constexpr int n = N * N * N, NN = N * N;
@@ -478,8 +478,6 @@ k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
     const int p = (lb / (QG * S)) % Nrho;
     const int t = lb / (QG * S * Nrho);
     const int h = threadIdx.x & 1, P = threadIdx.x >> 1;
-    __shared__ typename E::Split sp;      // the residue split's twiddles, uniform per group
-    if (threadIdx.x == 0) sp.init(r);     // (read after the engine's opening barrier)
 
     const cplx<T>* Ff = Fhat + size_t(t) * 2 * n;
     const int qchunk = (M + QG - 1) / QG;
@@ -494,12 +492,12 @@ k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
 
     for (int q = q0; q < q1; ++q) {
         const cplx<T>* ph = phase + (size_t(p) * M + q) * 3 * N;   // [axis][l]
+        const cplx<T>* pzq = phz + (size_t(p) * M + q) * N;          // [l3]
         // the transform's base pointer, opaque to the compiler once per q, so
         // that pass 1's N strided addresses are formed as they are used
         const cplx<T>* Fq = Ff;
         asm volatile("" : "+s"(Fq));
-        E::run(V, sp,
-            [&] {
+        auto prologue = [&] {
                 // stage this q's Nyquist imaginary parts for residue r
                 for (int i = threadIdx.x; i < 6 * N * NS + 2 * NN; i += E::NT) {
                     if (i < 6 * N * NS) {
@@ -523,14 +521,13 @@ k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
                         Ny[NL::C + sg * N * BP + (e / N) * BP + e % N] = src[e];
                     }
                 }
-            },
-            map1,
-            [&](int i1, int i2, int i3) {
-                const cplx<T> e = ph[i1] * ph[N + i2] * ph[2 * N + i3];
-                return (e.x + e.y) * Fq[(size_t(i3) * N + i1) * N + i2];
-            },
-            map3,
-            [&](int o2, int j, const cplx<T>* o) {
+            };
+        auto load = [&](int i1, int i2, int i3) {
+                // Re e + Im e of e = ph_x ph_y ph_z from the z factors' pair
+                const cplx<T> a = ph[i1] * ph[N + i2], z = pzq[i3];
+                return (a.x * z.x + a.y * z.y) * Fq[(size_t(i3) * N + i1) * N + i2];
+            };
+        auto epi = [&](int o2, int j, const cplx<T>* o) {
                 const int z = S * j + r;
                 const T sy = (o2 & 1) ? T(-1) : T(1), sz = (z & 1) ? T(-1) : T(1);
                 // o[k] is the output at x = H h + k; H is even, so x and k
@@ -584,7 +581,12 @@ k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
                         acc[K] = acc[K] + up * um;
                     });
                 }
-            });
+            };
+        // the residue r is uniform per work-group: one instantiation per r, so
+        // that the engine's split twiddles are compile-time constants
+        sfor<0, S>([&](auto Ri) {
+            if (r == int(Ri)) E::template run<int(Ri)>(V, prologue, map1, load, map3, epi);
+        });
     }
     if constexpr (MODE == kGainStore) return;
     int o2, j;
@@ -792,6 +794,7 @@ struct CollisionPlan {
     double prefac;
     size_t n;
     void* phase = nullptr;  // [Nrho][M][3][N] cplx<T>
+    void* phz = nullptr;    // [Nrho][M][N] cplx<T>: (cos + sin, cos - sin) of the z phase
     void* wtab = nullptr;   // [Nrho][k3][k1][k2] T
     void* bb2 = nullptr;    // [l3][l1][l2] T
     void* Fhat = nullptr;   // [ntask][2][n] cplx<T>
@@ -832,8 +835,10 @@ struct Impl {
     // N) with an unpadded LDS cube, so that two work-groups may share a CU
     static constexpr int S1x = (N % (2 * S1) == 0) ? 2 * S1 : S1;
     static constexpr int PL = planes_per_wg(N);
-    // the f = g gain on the lane-pair engine (four waves per SIMD): FP32 at
-    // N = 32, where its 2 N (N / S1) = 1024 threads fit one work-group
+    // the f = g gain on the lane-pair engine (four waves per SIMD instead of
+    // two): FP32 at N = 32, where its 2 N (N / S1) = 1024 threads fit one
+    // work-group.  (FP64 at N = 32, S1 = 8, gains nothing: its 104 KiB
+    // work-group leaves one per CU either way; 3.40 against 3.29 ms per call.)
     static constexpr bool kPair = std::is_same<T, float>::value && N == 32 &&
                                   2 * N * (N / S1) <= 1024;
 
@@ -864,7 +869,8 @@ struct Impl {
             if constexpr (kPair)
                 hipLaunchKernelGGL((k_gain_pair<T, N, S1, kGainDelta>), dim3(nb),
                                    dim3(2 * N * (N / S1)), 0, st, (const cplx<T>*)pl->Fhat,
-                                   (const cplx<T>*)pl->phase, (const T*)pl->nyq, pl->M, pl->Nrho,
+                                   (const cplx<T>*)pl->phase, (const cplx<T>*)pl->phz,
+                                   (const T*)pl->nyq, pl->M, pl->Nrho,
                                    pl->QG, (cplx<T>*)pl->acc, (UPair<T>*)pl->ucache, rm);
             else
                 hipLaunchKernelGGL((k_gain<T, N, S1, 1, kGainDelta>), dim3(nb),
@@ -875,7 +881,8 @@ struct Impl {
             if constexpr (kPair)
                 hipLaunchKernelGGL((k_gain_pair<T, N, S1, kGainAccumulate>), dim3(nb),
                                    dim3(2 * N * (N / S1)), 0, st, (const cplx<T>*)pl->Fhat,
-                                   (const cplx<T>*)pl->phase, (const T*)pl->nyq, pl->M, pl->Nrho,
+                                   (const cplx<T>*)pl->phase, (const cplx<T>*)pl->phz,
+                                   (const T*)pl->nyq, pl->M, pl->Nrho,
                                    pl->QG, (cplx<T>*)pl->acc, (UPair<T>*)nullptr, rm);
         }
         else if (alt)
@@ -958,7 +965,8 @@ struct Impl {
         if constexpr (kPair)
             hipLaunchKernelGGL((k_gain_pair<T, N, S1, kGainStore>), dim3(nb),
                                dim3(2 * N * (N / S1)), 0, st, (const cplx<T>*)pl->Fhat0,
-                               (const cplx<T>*)pl->phase, (const T*)pl->nyq, pl->M, pl->Nrho,
+                               (const cplx<T>*)pl->phase, (const cplx<T>*)pl->phz,
+                               (const T*)pl->nyq, pl->M, pl->Nrho,
                                pl->QG, (cplx<T>*)nullptr, (UPair<T>*)pl->ucache, pl->gain_remap);
         else
             hipLaunchKernelGGL((k_gain<T, N, S1, 1, kGainStore>), dim3(nb),

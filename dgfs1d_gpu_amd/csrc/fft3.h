@@ -279,9 +279,12 @@ DGFS_DEV void pair_dft_dit(const cplx<T>* x, cplx<T>* out, int h) {
         const cplx<T> w = twiddle<T, long(DIR) * k, L>(y[k]);   // lane 1: w^k Y_1[k]
         y[k] = h ? w : y[k];
     });
+    // X[k] = Y0 + w^k Y1 (lane 0), X[k + H] = Y0 - w^k Y1 (lane 1): one fma
+    // with the lane's sign
+    const T sg = h ? T(-1) : T(1);
     sfor<0, H>([&](auto K) {
         const cplx<T> r = pair_swap(y[K]);
-        out[K] = h ? r - y[K] : y[K] + r;   // X[k] = Y0 + w^k Y1, X[k + H] = Y0 - w^k Y1
+        out[K] = mk(y[K].x * sg + r.x, y[K].y * sg + r.y);   // contracted to fma
     });
 }
 
@@ -291,10 +294,12 @@ template <typename T, int L, int DIR>
 DGFS_DEV void pair_dft_dif(const cplx<T>* a, cplx<T>* out, int h) {
     constexpr int H = L / 2;
     cplx<T> d[H];
+    const T sg = h ? T(-1) : T(1);
     sfor<0, H>([&](auto K) {
         constexpr int k = K;
         const cplx<T> r = pair_swap(a[k]);
-        const cplx<T> s = h ? r - a[k] : a[k] + r;   // lane 1: in[k] - in[k + H]
+        // lane 0: in[k] + in[k + H]; lane 1: in[k] - in[k + H]
+        const cplx<T> s = mk(a[k].x * sg + r.x, a[k].y * sg + r.y);
         d[k] = h ? twiddle<T, long(DIR) * k, L>(s) : s;
     });
     RegDFT<T, H, DIR>::template run<1, 0>(d, out);
@@ -318,11 +323,12 @@ struct Fft3Pair : Fft3Engine<T, N_, S_, 2 * N_ * (N_ / S_), DIR_, PAD> {
     static constexpr int H = N / 2;
     static_assert(N % 2 == 0 && H % 2 == 0, "pair transforms need N divisible by 4");
 
-    // sp may live in LDS (its twiddles are the same for the whole work-group).
-    // The fold runs in chunks of CK outputs with a compiler barrier between
-    // them, so that at most S CK loads are in flight per thread.
-    template <class Map1, class Load>
-    static DGFS_DEV void pass1(cplx<T>* V, const Split& sp, Map1&& map1, Load&& load) {
+    // The residue R is a template parameter, so that the split's twiddles
+    // w_S^(s R) and w_N^(m R) are compile-time constants (twiddle<> folds the
+    // trivial ones).  The fold runs in chunks of CK outputs with a compiler
+    // barrier between them, so that at most S CK loads are in flight per thread.
+    template <int R, class Map1, class Load>
+    static DGFS_DEV void pass1(cplx<T>* V, Map1&& map1, Load&& load) {
         constexpr int CK = NS % 4 == 0 ? 4 : 1;
         for (int p = threadIdx.x; p < N * N; p += NT) {
             int i1, i2;
@@ -334,9 +340,9 @@ struct Fft3Pair : Fft3Engine<T, N_, S_, 2 * N_ * (N_ / S_), DIR_, PAD> {
                     cplx<T> acc = load(i1, i2, m);
                     sfor<1, S>([&](auto Si) {
                         constexpr int s = Si;
-                        acc = acc + load(i1, i2, m + s * NS) * sp.wS[s];
+                        acc = acc + twiddle<T, long(DIR) * s * R, S>(load(i1, i2, m + s * NS));
                     });
-                    b[m] = (m == 0) ? acc : acc * sp.wN[m];
+                    b[m] = twiddle<T, long(DIR) * m * R, N>(acc);
                 });
                 asm volatile("" ::: "memory");
             });
@@ -376,12 +382,12 @@ struct Fft3Pair : Fft3Engine<T, N_, S_, 2 * N_ * (N_ / S_), DIR_, PAD> {
         pair_dft_dit<T, N, DIR>(x, o, h);
     }
 
-    template <class Pro, class Map1, class Load, class Map3, class Epi>
-    static DGFS_DEV void run(cplx<T>* V, const Split& sp, Pro&& prologue, Map1&& map1,
-                             Load&& load, Map3&& map3, Epi&& epi) {
+    template <int R, class Pro, class Map1, class Load, class Map3, class Epi>
+    static DGFS_DEV void run(cplx<T>* V, Pro&& prologue, Map1&& map1, Load&& load, Map3&& map3,
+                             Epi&& epi) {
         __syncthreads();
         prologue();
-        pass1(V, sp, map1, load);
+        pass1<R>(V, map1, load);
         __syncthreads();
         pass2(V);
         __syncthreads();
