@@ -286,6 +286,35 @@ DGFS_DEV void store_pair_nt(UPair<T>* p, const UPair<T>& v) {
     __builtin_nontemporal_store(v.m.y, q + 3);
 }
 
+// Raw buffer views of device arrays.  A buffer access adds a per-lane VGPR
+// byte offset and a uniform SGPR byte offset to the descriptor's base, so a
+// stream with compile-time strides needs no 64-bit address arithmetic per
+// access.  Word 3 of the gfx9 descriptor selects the 32-bit data format;
+// accesses at or beyond `bytes` from the base read 0 and drop stores.
+DGFS_DEV __amdgpu_buffer_rsrc_t buffer_view(const void* base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, int(bytes),
+                                             0x00020000);
+}
+constexpr int kBufNT = 2;   // cache-policy operand of a non-temporal access
+// (the loaded vectors are reinterpreted whole: extracting their elements one
+// by one has been seen to lose the upper dwords)
+DGFS_DEV cplx<float> buf_load_c(__amdgpu_buffer_rsrc_t r, int voff, int soff, float) {
+    return __builtin_bit_cast(cplx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+DGFS_DEV cplx<double> buf_load_c(__amdgpu_buffer_rsrc_t r, int voff, int soff, double) {
+    return __builtin_bit_cast(cplx<double>,
+                              __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+DGFS_DEV UPair<float> buf_load_pair_nt(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(UPair<float>,
+                              __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, kBufNT));
+}
+DGFS_DEV void buf_store_pair_nt(__amdgpu_buffer_rsrc_t r, int voff, int soff,
+                                const UPair<float>& u) {
+    using V4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, u), r, voff, soff, kBufNT);
+}
+
 enum GainMode { kGainAccumulate = 0, kGainStore = 1, kGainDelta = 2 };
 
 // Gain: for (task, radius p, z-residue r, q-group), acc = sum_q t1 t2 + t3 t4
@@ -455,17 +484,22 @@ k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
 
 // k_gain on the lane-pair engine (Fft3Pair): the same work-group grid,
 // products, acc layout and stored-pair layout as k_gain with one transform
-// (f = g, or the reference-state modes), with 2 N (N / S) threads of at most
-// 128 VGPRs, so that four waves share each SIMD instead of two.  Lane h of
-// pencil pair P holds the a1-outputs (and acc entries) o1 = (N/2) h + k.
+// (f = g, or the reference-state modes), with 2 N (N / S) threads.  FP32 at
+// N = 32 (S = 2, padded cube): 1024 threads of at most 128 VGPRs, four waves
+// per SIMD instead of two.  FP64 at N = 32 (S = 4, unpadded 128 KiB cube):
+// 512 threads of at most 256 VGPRs, two waves per SIMD instead of half a
+// wave.  Lane h of pencil pair P holds the a1-outputs (and acc entries)
+// o1 = (N/2) h + k.
 constexpr int kPairDeltaChunk = 4;   // stored pairs in flight per lane (register budget)
 
-template <typename T, int N, int S, int MODE = kGainAccumulate>
+template <typename T, int N, int S, int MODE = kGainAccumulate, int PAD = 1>
 __global__ void __launch_bounds__(2 * N * (N / S))
 k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
             const cplx<T>* __restrict__ phz, const T* __restrict__ nyq, int M, int Nrho, int QG,
             cplx<T>* __restrict__ acc_out, UPair<T>* __restrict__ ucache, int remap) {
-    using E = Fft3Pair<T, N, S, +1>;
+    static_assert(MODE == kGainAccumulate || std::is_same<T, float>::value,
+                  "the stored reference transforms are FP32");
+    using E = Fft3Pair<T, N, S, +1, PAD>;
     using NL = NyLayout<N, S>;
     constexpr int n = N * N * N, NN = N * N;
     constexpr int NS = N / S, BP = NL::BP, H = E::H;
@@ -493,10 +527,9 @@ k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
     for (int q = q0; q < q1; ++q) {
         const cplx<T>* ph = phase + (size_t(p) * M + q) * 3 * N;   // [axis][l]
         const cplx<T>* pzq = phz + (size_t(p) * M + q) * N;          // [l3]
-        // the transform's base pointer, opaque to the compiler once per q, so
-        // that pass 1's N strided addresses are formed as they are used
-        const cplx<T>* Fq = Ff;
-        asm volatile("" : "+s"(Fq));
+        // pass 1 reads F[l3][l1][l2] at the pencil's lane offset plus a
+        // compile-time plane offset
+        const auto Fb = buffer_view(Ff, unsigned(n * sizeof(cplx<T>)));
         auto prologue = [&] {
                 // stage this q's Nyquist imaginary parts for residue r
                 for (int i = threadIdx.x; i < 6 * N * NS + 2 * NN; i += E::NT) {
@@ -525,7 +558,9 @@ k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
         auto load = [&](int i1, int i2, int i3) {
                 // Re e + Im e of e = ph_x ph_y ph_z from the z factors' pair
                 const cplx<T> a = ph[i1] * ph[N + i2], z = pzq[i3];
-                return (a.x * z.x + a.y * z.y) * Fq[(size_t(i3) * N + i1) * N + i2];
+                const cplx<T> f = buf_load_c(Fb, (i1 * N + i2) * int(sizeof(cplx<T>)),
+                                             i3 * NN * int(sizeof(cplx<T>)), T(0));
+                return (a.x * z.x + a.y * z.y) * f;
             };
         auto epi = [&](int o2, int j, const cplx<T>* o) {
                 const int z = S * j + r;
@@ -547,24 +582,26 @@ k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
                 };
                 // stored pairs [t][p][q][k][r][P][h] (x = H h + k): one
                 // contiguous run of 2 NPEN pairs per (k, r), read by the
-                // work-group's lanes in order
-                const size_t cbase =
-                    ((((size_t(t) * Nrho + p) * M + q) * H) * S + r) * size_t(E::NT) +
-                    threadIdx.x;
+                // work-group's lanes in order; point k sits k S NT pairs
+                // after the view's base
+                constexpr int PB = int(sizeof(UPair<T>));
+                const auto Ub = buffer_view(
+                    ucache + ((((size_t(t) * Nrho + p) * M + q) * H) * S + r) * size_t(E::NT),
+                    unsigned(H * S * E::NT * PB));
+                const int lane_off = int(threadIdx.x) * PB;
                 if constexpr (MODE == kGainStore) {
-                    UPair<T>* dst = ucache + cbase;
                     sfor<0, H>([&](auto K) {
                         UPair<T> v;
                         unpack(K, o[K], v.p, v.m);
-                        store_pair_nt(dst + size_t(int(K)) * S * E::NT, v);
+                        buf_store_pair_nt(Ub, lane_off, int(K) * S * E::NT * PB, v);
                     });
                 } else if constexpr (MODE == kGainDelta) {
-                    const UPair<T>* src = ucache + cbase;
                     constexpr int CH = kPairDeltaChunk;
                     sfor<0, H / CH>([&](auto C) {
                         UPair<T> c0[CH];
                         sfor<0, CH>([&](auto k) {
-                            c0[k] = load_pair_nt(src + size_t(int(C) * CH + int(k)) * S * E::NT);
+                            c0[k] = buf_load_pair_nt(Ub, lane_off,
+                                                     (int(C) * CH + int(k)) * S * E::NT * PB);
                         });
                         sfor<0, CH>([&](auto k) {
                             constexpr int i = int(C) * CH + int(k);
@@ -835,12 +872,18 @@ struct Impl {
     // N) with an unpadded LDS cube, so that two work-groups may share a CU
     static constexpr int S1x = (N % (2 * S1) == 0) ? 2 * S1 : S1;
     static constexpr int PL = planes_per_wg(N);
-    // the f = g gain on the lane-pair engine (four waves per SIMD instead of
-    // two): FP32 at N = 32, where its 2 N (N / S1) = 1024 threads fit one
-    // work-group.  (FP64 at N = 32, S1 = 8, gains nothing: its 104 KiB
-    // work-group leaves one per CU either way; 3.40 against 3.29 ms per call.)
+    // the f = g gain on the lane-pair engine: FP32 at N = 32 on the padded
+    // cube of split S1 (1024 threads, four waves per SIMD); FP64 at N = 32 on
+    // an unpadded cube of split 4 (512 threads, two waves per SIMD, where
+    // k_gain at S1 = 8 runs half a wave per SIMD).  The reference-state modes
+    // (FP32 only) keep S1.
     static constexpr bool kPair = std::is_same<T, float>::value && N == 32 &&
                                   2 * N * (N / S1) <= 1024;
+    static constexpr bool kPair64 = std::is_same<T, double>::value && N == 32;
+    static constexpr int SP = kPair64 ? 4 : S1;          // the f = g pair kernel's split
+    static constexpr int PADP = kPair64 ? 0 : 1;
+    static_assert(!kPair64 || N * N * (N / SP) * 16 + NyLayout<N, SP>::SIZE * 8 <= 160 * 1024,
+                  "the FP64 pair kernel's LDS image exceeds a CU");
 
     // Q0 == nullptr: Q += Q(f, g).  Q0 != nullptr (reference-state form, ug the
     // reference state u0 whose F0^ and u+-_f0 the plan holds): Q += Q0 +
@@ -862,7 +905,8 @@ struct Impl {
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 2, st);
         const bool alt = fg_same && !dev && pl->gain_smul == 2 && S1x != S1;
-        const int S = alt ? S1x : (fg_same ? S1 : S2);
+        const bool pair = fg_same && !alt && (kPair || kPair64);
+        const int S = alt ? S1x : (pair && !dev ? SP : (fg_same ? S1 : S2));
         const int nb = ntask * pl->Nrho * S * pl->QG;
         const int rm = pl->gain_remap;
         if (dev) {
@@ -877,10 +921,10 @@ struct Impl {
                                    dim3(round64(N * (N / S1))), 0, st, (const cplx<T>*)pl->Fhat,
                                    (const cplx<T>*)pl->phase, (const T*)pl->nyq, pl->M, pl->Nrho,
                                    pl->QG, (cplx<T>*)pl->acc, (UPair<T>*)pl->ucache, rm);
-        } else if (fg_same && !alt && kPair) {
-            if constexpr (kPair)
-                hipLaunchKernelGGL((k_gain_pair<T, N, S1, kGainAccumulate>), dim3(nb),
-                                   dim3(2 * N * (N / S1)), 0, st, (const cplx<T>*)pl->Fhat,
+        } else if (pair) {
+            if constexpr (kPair || kPair64)
+                hipLaunchKernelGGL((k_gain_pair<T, N, SP, kGainAccumulate, PADP>), dim3(nb),
+                                   dim3(2 * N * (N / SP)), 0, st, (const cplx<T>*)pl->Fhat,
                                    (const cplx<T>*)pl->phase, (const cplx<T>*)pl->phz,
                                    (const T*)pl->nyq, pl->M, pl->Nrho,
                                    pl->QG, (cplx<T>*)pl->acc, (UPair<T>*)nullptr, rm);

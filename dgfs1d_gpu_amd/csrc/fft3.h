@@ -267,6 +267,10 @@ DGFS_DEV double pair_swap(double v) {
 template <typename T>
 DGFS_DEV cplx<T> pair_swap(cplx<T> v) { return mk(pair_swap(v.x), pair_swap(v.y)); }
 
+// c ? a : b on values (a conditional on two lvalues selects their addresses)
+template <typename T>
+DGFS_DEV T vsel(bool c, T a, T b) { return c ? a : b; }
+
 // Length-L DFT shared by a lane pair, decimation in time: lane h holds
 // x[k] = in[2 k + h], k < L/2, and receives out[k] = X[(L/2) h + k].
 template <typename T, int L, int DIR>
@@ -277,7 +281,9 @@ DGFS_DEV void pair_dft_dit(const cplx<T>* x, cplx<T>* out, int h) {
     sfor<0, H>([&](auto K) {
         constexpr int k = K;
         const cplx<T> w = twiddle<T, long(DIR) * k, L>(y[k]);   // lane 1: w^k Y_1[k]
-        y[k] = h ? w : y[k];
+        // a select of values: `h ? w : y[k]` on the structs would select
+        // between their addresses and keep y out of registers
+        y[k] = mk(vsel(h, w.x, y[k].x), vsel(h, w.y, y[k].y));
     });
     // X[k] = Y0 + w^k Y1 (lane 0), X[k + H] = Y0 - w^k Y1 (lane 1): one fma
     // with the lane's sign
