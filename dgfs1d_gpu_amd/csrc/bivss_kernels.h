@@ -294,7 +294,8 @@ int bi_launch(BiPlan* pl, const void* up, const void* uq, void* Q, TaskView tv, 
         tc.coef = static_cast<const T*>(tv.coef) + size_t(c0) * 2 * tv.K;
         const int XG = bi_xgroups<N>(nt, NXY);
         const long npl = long(nt) * 2 * N;
-        hipLaunchKernelGGL((k_fwd_plane<T, N, false>), dim3((npl + PL - 1) / PL), dim3(PL * N),
+        constexpr int FP = fwd_planes_per_wg(N);
+        hipLaunchKernelGGL((k_fwd_plane<T, N, false>), dim3((npl + FP - 1) / FP), dim3(FP * N),
                            0, st, (const T*)up, (const T*)uq, tc, 2, npl, (cplx<T>*)pl->Iw);
         DGFS_HIP_CHECK(hipGetLastError());
         const long nli = long(nt) * 2 * NN;
@@ -331,7 +332,8 @@ int bi_launch(BiPlan* pl, const void* up, const void* uq, void* Q, TaskView tv, 
         DGFS_HIP_CHECK(hipGetLastError());
     }
     const long nlo = long(ngroup) * NN;
-    hipLaunchKernelGGL((k_line_output<T, N, false>), dim3((nlo + 255) / 256), dim3(256), 0, st,
+    hipLaunchKernelGGL((k_line_output<T, N, false>),
+                       dim3((nlo * line_lanes(N) + kLineWG - 1) / kLineWG), dim3(kLineWG), 0, st,
                        (const T*)up, (const T*)nullptr, (const double*)nullptr, tv,
                        (const cplx<T>*)pl->H, nlo, gstart, gslice, T(pl->prefac), (T*)Q);
     DGFS_HIP_CHECK(hipGetLastError());

@@ -97,38 +97,55 @@ DGFS_DEV void lds_fft_col(cplx<T>* P, int i) {
 //     Iw[t][which][x][l3][l2]
 //     DEV (reference-state form): ug is the reference state u0; which = 0
 //     loads h = f - f0, which = 1 loads s = f + f0 (both with the f terms)
+// planes per work-group of k_fwd_plane: two at N >= 32, so that a task set's
+// N planes per task spread over many CUs
+constexpr int fwd_planes_per_wg(int N) { return N >= 32 ? 2 : planes_per_wg(N); }
+
 template <typename T, int N, bool DEV>
-__global__ void __launch_bounds__(planes_per_wg(N) * N)
+__global__ void __launch_bounds__(fwd_planes_per_wg(N) * N)
 k_fwd_plane(const T* __restrict__ u, const T* __restrict__ ug, TaskView tv, int nfg,
             long nplanes, cplx<T>* __restrict__ Iw) {
-    constexpr int PL = planes_per_wg(N), NT = PL * N, NN = N * N;
+    constexpr int PL = fwd_planes_per_wg(N), NN = N * N;
     __shared__ cplx<T> L[PL * N * (N + 1)];
     const long plane0 = long(blockIdx.x) * PL;
-    for (int i = threadIdx.x; i < PL * NN; i += NT) {
-        const int pl = i / NN, yz = i - pl * NN;
+    const int pl = threadIdx.x / N, c = threadIdx.x - pl * N;
+    cplx<T>* P = L + pl * N * (N + 1);
+    {
+        // thread c of plane pl gathers the points (y, z = c), y < N: the task's
+        // terms in the outer loop, the N loads of a term issued together
+        T v[N];
+        sfor<0, N>([&](auto I) { v[I] = T(0); });
         const long pid = plane0 + pl;
-        cplx<T> v = mk(T(0), T(0));
         if (pid < nplanes) {
             const int x = int(pid % N);
             const long tw = pid / N;
             const int which = int(tw % nfg), t = int(tw / nfg);
+            auto gather = [&](const T* src, int wh, T sign) {
+                const int nt = tv.nterm[2 * t + wh];
+                const int* sl = tv.slice + (size_t(t) * 2 + wh) * tv.K;
+                const T* cf = static_cast<const T*>(tv.coef) + (size_t(t) * 2 + wh) * tv.K;
+                for (int a = 0; a < nt; ++a) {
+                    const T ca = sign * cf[a];
+                    const T* b = src + size_t(sl[a]) * tv.n + size_t(x) * NN + c;
+                    sfor<0, N>([&](auto I) { v[I] += ca * b[int(I) * N]; });
+                }
+            };
             if constexpr (DEV) {
-                const T f = task_value<T>(tv, u, t, 0, x * NN + yz);
-                const T f0 = task_value<T>(tv, ug, t, 0, x * NN + yz);
-                v.x = which ? f + f0 : f - f0;
+                // which = 0: h = f - f0; which = 1: s = f + f0 (f terms of u, u0)
+                gather(u, 0, T(1));
+                gather(ug, 0, which ? T(1) : T(-1));
             } else {
-                v.x = task_value<T>(tv, which ? ug : u, t, which, x * NN + yz);
+                gather(which ? ug : u, which, T(1));
             }
         }
-        L[pl * N * (N + 1) + (yz / N) * (N + 1) + yz % N] = v;
+        sfor<0, N>([&](auto I) { P[int(I) * (N + 1) + c] = mk(v[I], T(0)); });
     }
     __syncthreads();
-    const int pl = threadIdx.x / N, i = threadIdx.x - pl * N;
-    cplx<T>* P = L + pl * N * (N + 1);
-    lds_fft_row<T, N, -1>(P, i);     // z -> l3
+    lds_fft_row<T, N, -1>(P, c);     // z -> l3
     __syncthreads();
-    lds_fft_col<T, N, -1>(P, i);     // y -> l2
+    lds_fft_col<T, N, -1>(P, c);     // y -> l2
     __syncthreads();
+    constexpr int NT = PL * N;
     for (int k = threadIdx.x; k < PL * NN; k += NT) {
         const int q = k / NN, r = k - q * NN;
         const long pid = plane0 + q;
@@ -671,14 +688,20 @@ k_plane_weight(const cplx<T>* __restrict__ acc_in, const T* __restrict__ wtab,
         __syncthreads();
         if (live) {
             const cplx<T>* src = acc_in + ((size_t(t) * Nrho + p) * QG) * n + size_t(k1) * NN;
-            for (int pos = c; pos < NN; pos += N) {
-                cplx<T> v = src[pos];
-                for (int g = 1; g < QG; ++g) v = v + src[size_t(g) * n + pos];
-                // stored position (r, y, j) with z = SB*j + r
-                const int r = pos / (N * NSB), rem = pos - r * N * NSB;
-                const int y = rem / NSB, z = (rem - y * NSB) * SB + r;
-                P[y * (N + 1) + z] = v;
-            }
+            // the thread's N positions pos = c + N i, all loads of a q-group in
+            // flight together
+            cplx<T> v[N];
+            sfor<0, N>([&](auto I) { v[I] = src[c + int(I) * N]; });
+            for (int g = 1; g < QG; ++g)
+                sfor<0, N>([&](auto I) { v[I] = v[I] + src[size_t(g) * n + c + int(I) * N]; });
+            // stored position pos = (r N + y) NSB + j with z = SB j + r: j is the
+            // thread's c mod NSB and r N + y = c / NSB + SB i
+            const int j = c % NSB, rb = c / NSB;
+            sfor<0, N>([&](auto I) {
+                const int ry = rb + SB * int(I);
+                const int r = ry / N, y = ry - r * N;
+                P[y * (N + 1) + SB * j + r] = v[I];
+            });
         }
         __syncthreads();
         if (live) lds_fft_row<T, N, -1>(P, c);     // z -> k3
@@ -741,72 +764,87 @@ k_plane_weight(const cplx<T>* __restrict__ acc_in, const T* __restrict__ wtab,
 // DEV (reference-state form, u0 the reference state, Q0 = Q(f0, f0) per
 // output slice in FP64): loss = sum_t (h_t L[f_t] + f0_t L[h_t]) and
 // Q[gslice[g]] += Q0[gslice[g]] + prefac (gain - loss), gain = G(h, s).
+// LANES = 2 (N divisible by 4): a lane pair per line; lane h loads the k1 =
+// 2 k + h inputs of each inverse (pair_dft_dit) and owns the points
+// x = (N/2) h + k, so a task set's few lines keep twice the waves busy.
+constexpr int line_lanes(int N) { return N % 4 == 0 ? 2 : 1; }
+constexpr int kLineWG = 64;   // threads per work-group of k_line_output
+
 template <typename T, int N, bool DEV>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kLineWG)
 k_line_output(const T* __restrict__ u, const T* __restrict__ u0, const double* __restrict__ Q0,
               TaskView tv, const cplx<T>* __restrict__ H, long nlines,
               const int* __restrict__ gstart, const int* __restrict__ gslice, T prefac,
               T* __restrict__ Q) {
     constexpr int NN = N * N, n = N * NN, NH = DEV ? 3 : 2;
-    const long gid = long(blockIdx.x) * 256 + threadIdx.x;
+    constexpr int LN = line_lanes(N), PN = N / LN;
+    const long tid = long(blockIdx.x) * kLineWG + threadIdx.x;
+    const long gid = tid / LN;                    // both lanes of a pair share the line
+    const int h = LN == 2 ? int(tid & 1) : 0;
     if (gid >= nlines) return;
     const int g = int(gid / NN), yz = int(gid - long(g) * NN);
     const int ta = gstart[g], tb = gstart[g + 1];
-    cplx<T> a[N], o[N];
-    T loss[N];
-    sfor<0, N>([&](auto I) { loss[I] = T(0); });
+    const int x0 = PN * h;                        // this lane's points x = x0 + k
+    auto kin = [&](int k) { return LN == 2 ? 2 * k + h : k; };   // input k1 of slot k
+    // o = Re-usable inverse transform over k1 of a[] (slots as kin)
+    auto inverse = [&](const cplx<T>* a, cplx<T>* o) {
+        if constexpr (LN == 2)
+            pair_dft_dit<T, N, +1>(a, o, h);
+        else
+            RegDFT<T, N, +1>::template run<1, 0>(a, o);
+    };
+    cplx<T> a[PN], o[PN];
+    T loss[PN];
+    sfor<0, PN>([&](auto I) { loss[I] = T(0); });
     // loss terms first, task by task (one inverse pencil live at a time),
     // then the gain planes summed over the group and inverted once
     for (int t = ta; t < tb; ++t) {
         const cplx<T>* hl = H + (size_t(t) * NH) * n + yz;
+        const int nt = tv.nterm[2 * t];
+        const int* sl = tv.slice + size_t(t) * 2 * tv.K;
+        const T* cf = static_cast<const T*>(tv.coef) + size_t(t) * 2 * tv.K;
         if constexpr (DEV) {
             // the task's f terms in the outer loop and the line's points in
             // the inner one, so that no per-point value array stays live
-            const int nt = tv.nterm[2 * t];
-            const int* sl = tv.slice + size_t(t) * 2 * tv.K;
-            const T* cf = static_cast<const T*>(tv.coef) + size_t(t) * 2 * tv.K;
-            sfor<0, N>([&](auto K1) { a[K1] = hl[2 * size_t(n) + size_t(K1) * NN]; });
-            RegDFT<T, N, +1>::template run<1, 0>(a, o);             // L[h]: loss += f0 L[h]
+            sfor<0, PN>([&](auto K) { a[K] = hl[2 * size_t(n) + size_t(kin(K)) * NN]; });
+            inverse(a, o);                                          // L[h]: loss += f0 L[h]
             for (int k = 0; k < nt; ++k) {
                 const T c = cf[k];
-                const T* p0 = u0 + size_t(sl[k]) * n + yz;
-                sfor<0, N>([&](auto X) { loss[X] += c * p0[size_t(X) * NN] * o[X].x; });
+                const T* p0 = u0 + size_t(sl[k]) * n + size_t(x0) * NN + yz;
+                sfor<0, PN>([&](auto X) { loss[X] += c * p0[size_t(X) * NN] * o[X].x; });
             }
-            sfor<0, N>([&](auto K1) { a[K1] = hl[size_t(K1) * NN]; });
-            RegDFT<T, N, +1>::template run<1, 0>(a, o);             // L[f]: loss += h L[f]
+            sfor<0, PN>([&](auto K) { a[K] = hl[size_t(kin(K)) * NN]; });
+            inverse(a, o);                                          // L[f]: loss += h L[f]
             for (int k = 0; k < nt; ++k) {
                 const T c = cf[k];
-                const T* p = u + size_t(sl[k]) * n + yz;
-                const T* p0 = u0 + size_t(sl[k]) * n + yz;
-                sfor<0, N>([&](auto X) {
+                const T* p = u + size_t(sl[k]) * n + size_t(x0) * NN + yz;
+                const T* p0 = u0 + size_t(sl[k]) * n + size_t(x0) * NN + yz;
+                sfor<0, PN>([&](auto X) {
                     loss[X] += c * (p[size_t(X) * NN] - p0[size_t(X) * NN]) * o[X].x;
                 });
             }
         } else {
-            sfor<0, N>([&](auto K1) { a[K1] = hl[size_t(K1) * NN]; });
-            RegDFT<T, N, +1>::template run<1, 0>(a, o);
-            const int nt = tv.nterm[2 * t];
-            const int* sl = tv.slice + size_t(t) * 2 * tv.K;
-            const T* cf = static_cast<const T*>(tv.coef) + size_t(t) * 2 * tv.K;
+            sfor<0, PN>([&](auto K) { a[K] = hl[size_t(kin(K)) * NN]; });
+            inverse(a, o);
             for (int k = 0; k < nt; ++k) {
                 const T c = cf[k];
-                const T* p = u + size_t(sl[k]) * n + yz;
-                sfor<0, N>([&](auto X) { loss[X] += c * p[size_t(X) * NN] * o[X].x; });
+                const T* p = u + size_t(sl[k]) * n + size_t(x0) * NN + yz;
+                sfor<0, PN>([&](auto X) { loss[X] += c * p[size_t(X) * NN] * o[X].x; });
             }
         }
     }
-    sfor<0, N>([&](auto I) { a[I] = mk(T(0), T(0)); });
+    sfor<0, PN>([&](auto I) { a[I] = mk(T(0), T(0)); });
     for (int t = ta; t < tb; ++t) {
         const cplx<T>* hg = H + (size_t(t) * NH + 1) * n + yz;
-        sfor<0, N>([&](auto K1) { a[K1] = a[K1] + hg[size_t(K1) * NN]; });
+        sfor<0, PN>([&](auto K) { a[K] = a[K] + hg[size_t(kin(K)) * NN]; });
     }
-    RegDFT<T, N, +1>::template run<1, 0>(a, o);
-    T* dst = Q + size_t(gslice[g]) * n + yz;
+    inverse(a, o);
+    T* dst = Q + size_t(gslice[g]) * n + size_t(x0) * NN + yz;
     if constexpr (DEV) {
-        const double* q0 = Q0 + size_t(gslice[g]) * n + yz;
+        const double* q0 = Q0 + size_t(gslice[g]) * n + size_t(x0) * NN + yz;
         // in chunks of CH points, so the FP64 loads of Q0 are not all live at once
-        constexpr int CH = (N % 8 == 0) ? 8 : ((N % 4 == 0) ? 4 : 2);
-        sfor<0, N / CH>([&](auto C) {
+        constexpr int CH = (PN % 8 == 0) ? 8 : ((PN % 4 == 0) ? 4 : 2);
+        sfor<0, PN / CH>([&](auto C) {
             sfor<0, CH>([&](auto Xc) {
                 constexpr int X = int(C) * CH + int(Xc);
                 dst[size_t(X) * NN] += T(q0[size_t(X) * NN] +
@@ -815,7 +853,7 @@ k_line_output(const T* __restrict__ u, const T* __restrict__ u0, const double* _
             __builtin_amdgcn_sched_barrier(0);
         });
     } else {
-        sfor<0, N>([&](auto X) { dst[size_t(X) * NN] += prefac * (o[X].x - loss[X]); });
+        sfor<0, PN>([&](auto X) { dst[size_t(X) * NN] += prefac * (o[X].x - loss[X]); });
     }
 }
 
@@ -960,14 +998,15 @@ struct Impl {
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 4, st);
         const long nlo = long(ngroup) * NN;
+        const long nlb = (nlo * line_lanes(N) + kLineWG - 1) / kLineWG;
         if (dev)
-            hipLaunchKernelGGL((k_line_output<T, N, true>), dim3((nlo + 255) / 256), dim3(256), 0,
-                               st, (const T*)u, (const T*)ug, Q0, tv, (const cplx<T>*)pl->H, nlo,
+            hipLaunchKernelGGL((k_line_output<T, N, true>), dim3(nlb), dim3(kLineWG), 0, st,
+                               (const T*)u, (const T*)ug, Q0, tv, (const cplx<T>*)pl->H, nlo,
                                gstart, gslice, T(pl->prefac), (T*)Q);
         else
-            hipLaunchKernelGGL((k_line_output<T, N, false>), dim3((nlo + 255) / 256), dim3(256),
-                               0, st, (const T*)u, (const T*)ug, Q0, tv, (const cplx<T>*)pl->H,
-                               nlo, gstart, gslice, T(pl->prefac), (T*)Q);
+            hipLaunchKernelGGL((k_line_output<T, N, false>), dim3(nlb), dim3(kLineWG), 0, st,
+                               (const T*)u, (const T*)ug, Q0, tv, (const cplx<T>*)pl->H, nlo,
+                               gstart, gslice, T(pl->prefac), (T*)Q);
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 5, st);
         if (pl->ev && pl->ev_used < pl->ev_max) ++pl->ev_used;
@@ -980,12 +1019,13 @@ struct Impl {
                        int nfg, bool dev, cplx<T>* dst, hipStream_t st) {
         constexpr long NN = long(N) * N;
         const long npl = long(ntask) * nfg * N;
+        constexpr int FP = fwd_planes_per_wg(N);
         if (dev)
-            hipLaunchKernelGGL((k_fwd_plane<T, N, true>), dim3((npl + PL - 1) / PL), dim3(PL * N),
+            hipLaunchKernelGGL((k_fwd_plane<T, N, true>), dim3((npl + FP - 1) / FP), dim3(FP * N),
                                0, st, (const T*)u, (const T*)ug, tv, nfg, npl, (cplx<T>*)pl->Iw);
         else
-            hipLaunchKernelGGL((k_fwd_plane<T, N, false>), dim3((npl + PL - 1) / PL),
-                               dim3(PL * N), 0, st, (const T*)u, (const T*)ug, tv, nfg, npl,
+            hipLaunchKernelGGL((k_fwd_plane<T, N, false>), dim3((npl + FP - 1) / FP),
+                               dim3(FP * N), 0, st, (const T*)u, (const T*)ug, tv, nfg, npl,
                                (cplx<T>*)pl->Iw);
         DGFS_HIP_CHECK(hipGetLastError());
         const long nli = long(ntask) * nfg * NN;
