@@ -535,8 +535,12 @@ k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
     const int q0 = qg * qchunk;
     const int q1 = (q0 + qchunk < M) ? q0 + qchunk : M;
 
-    cplx<T> acc[H];
-    sfor<0, H>([&](auto I) { acc[I] = mk(T(0), T(0)); });
+    // Only Re acc reaches the output: the plane weights are real and even in
+    // k (functions of |l|, the Nyquist index its own negative), so the
+    // transform of i Im acc is Hermitian times i, weighted and inverted to a
+    // purely imaginary field, and the line stage keeps the real part.
+    T acc[H];
+    sfor<0, H>([&](auto I) { acc[I] = T(0); });
 
     auto map1 = [](int pp, int& i1, int& i2) { i1 = pp / N; i2 = pp - (pp / N) * N; };  // (l1, l2)
     auto map3 = [](int Q, int& o2, int& j) { o2 = Q / NS; j = Q - (Q / NS) * NS; };      // (y, j)
@@ -624,7 +628,9 @@ k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
                             constexpr int i = int(C) * CH + int(k);
                             cplx<T> up, um;
                             unpack(std::integral_constant<int, i>{}, o[i], up, um);
-                            acc[i] = acc[i] + up * (um + c0[k].m) + um * c0[k].p;
+                            // Re[up (um + u0-) + um u0+]
+                            acc[i] += up.x * (um.x + c0[k].m.x) - up.y * (um.y + c0[k].m.y) +
+                                      um.x * c0[k].p.x - um.y * c0[k].p.y;
                         });
                         asm volatile("" ::: "memory");   // one chunk of loads in flight
                     });
@@ -632,7 +638,7 @@ k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
                     sfor<0, H>([&](auto K) {
                         cplx<T> up, um;
                         unpack(K, o[K], up, um);
-                        acc[K] = acc[K] + up * um;
+                        acc[K] += up.x * um.x - up.y * um.y;             // Re[up um]
                     });
                 }
             };
@@ -647,8 +653,9 @@ k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
     map3(P, o2, j);
     // forward transform along x of the pair's pencil (first axis of
     // FFT3(acc_p)); lane h receives k1 = 2 m + h
-    cplx<T> out[H];
-    pair_dft_dif<T, N, -1>(acc, out, h);
+    cplx<T> in[H], out[H];
+    sfor<0, H>([&](auto K) { in[K] = mk(acc[K], T(0)); });
+    pair_dft_dif<T, N, -1>(in, out, h);
     // acc layout [t][p][qg][k1][r][y][j] with z = S*j + r
     cplx<T>* dst = acc_out + ((size_t(t) * Nrho + p) * QG + qg) * n;
     sfor<0, H>([&](auto K) {
