@@ -486,14 +486,16 @@ k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
         int o2, j;
         map3(threadIdx.x, o2, j);
         const T scale = NW == 1 ? T(1) : T(0.5);
-        // forward transform along x of this thread's pencil (first axis of
-        // FFT3(acc_p)); the (y, z) plane transforms follow in k_plane_weight
+        // forward transform along x of Re acc for this thread's pencil (first
+        // axis of FFT3(Re acc_p); only Re acc reaches the output, and its
+        // transform is Hermitian, so the planes k1 <= N/2 carry it); the
+        // (y, z) plane transforms follow in k_plane_weight
         cplx<T> tmp[N];
-        sfor<0, N>([&](auto I) { tmp[I] = scale * acc[I]; });
+        sfor<0, N>([&](auto I) { tmp[I] = mk(scale * acc[I].x, T(0)); });
         RegDFT<T, N, -1>::template run<1, 0>(tmp, acc);
-        // acc layout [t][p][qg][k1][r][y][j] with z = S*j + r
+        // acc layout [t][p][qg][k1][r][y][j] with z = S*j + r, k1 <= N/2
         cplx<T>* dst = acc_out + ((size_t(t) * Nrho + p) * QG + qg) * n;
-        sfor<0, N>([&](auto I) {
+        sfor<0, N / 2 + 1>([&](auto I) {
             dst[((size_t(I) * S + r) * N + o2) * NS + j] = acc[I];
         });
     }
@@ -656,16 +658,20 @@ k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
     cplx<T> in[H], out[H];
     sfor<0, H>([&](auto K) { in[K] = mk(acc[K], T(0)); });
     pair_dft_dif<T, N, -1>(in, out, h);
-    // acc layout [t][p][qg][k1][r][y][j] with z = S*j + r
+    // acc layout [t][p][qg][k1][r][y][j] with z = S*j + r; the transform of
+    // the real acc is Hermitian in k1, so only k1 <= N/2 is stored
     cplx<T>* dst = acc_out + ((size_t(t) * Nrho + p) * QG + qg) * n;
     sfor<0, H>([&](auto K) {
         const int k1 = 2 * int(K) + h;
-        dst[((size_t(k1) * S + r) * N + o2) * NS + j] = out[K];
+        if (k1 <= N / 2) dst[((size_t(k1) * S + r) * N + o2) * NS + j] = out[K];
     });
 }
 
 // ================================================================ kernel C
-// One work-group per plane (t, k1).  Its PL slots of N threads take radii
+// One work-group per plane (t, k1), k1 <= N/2: everything this kernel
+// transforms is the transform of a real field (Re acc_p, f, h, f0), so
+// planes N - k1 are the conjugates of planes k1 and are written from them.
+// Its PL slots of N threads take radii
 // p = slot, slot+PL, ...:  G^(k1,:,:) = sum_p wtab_p(k1,:,:) FFT2_{y,z}(sum_qg acc[t][p][qg][k1]),
 // reduced over the slots through LDS; then Hg = IFFT2(G^(k1)) (slot 0) and
 // Hl = IFFT2(bb2 G^_g(k1)) (slot 1).  H[t][0 = loss | 1 = gain][k1][y][z].
@@ -682,7 +688,8 @@ k_plane_weight(const cplx<T>* __restrict__ acc_in, const T* __restrict__ wtab,
     constexpr int NH = DEV ? 3 : 2;
     static_assert(PL >= NH, "plane kernel needs a slot per output plane");
     __shared__ cplx<T> L[PL * PS];
-    const int k1 = blockIdx.x % N, t = blockIdx.x / N;
+    constexpr int NK = N / 2 + 1;                   // planes computed per task
+    const int k1 = blockIdx.x % NK, t = blockIdx.x / NK;
     const int slot = threadIdx.x / N, c = threadIdx.x - slot * N;
     cplx<T>* P = L + slot * PS;
     const int NSB = N / SB;
@@ -757,10 +764,14 @@ k_plane_weight(const cplx<T>* __restrict__ acc_in, const T* __restrict__ wtab,
     __syncthreads();
     if (slot < NH) lds_fft_row<T, N, +1>(P, c);      // k3 -> z
     __syncthreads();
+    // H(N - k1, y, z) = conj H(k1, y, z) for 0 < k1 < N/2
+    const bool mirror = k1 > 0 && 2 * k1 < N;
     for (int i = threadIdx.x; i < NH * NN; i += NT) {
         const int which = i / NN, r = i - which * NN;     // slot 0 = gain, 1 = loss, 2 = L[h]
         const int dst = which == 0 ? 1 : (which == 1 ? 0 : 2);
-        H[((size_t(t) * NH + dst) * N + k1) * NN + r] = L[which * PS + (r / N) * (N + 1) + r % N];
+        const cplx<T> v = L[which * PS + (r / N) * (N + 1) + r % N];
+        H[((size_t(t) * NH + dst) * N + k1) * NN + r] = v;
+        if (mirror) H[((size_t(t) * NH + dst) * N + (N - k1)) * NN + r] = conj(v);
     }
 }
 
@@ -993,12 +1004,14 @@ struct Impl {
         DGFS_HIP_CHECK(hipGetLastError());
         timing_mark(pl, 3, st);
         if (dev)
-            hipLaunchKernelGGL((k_plane_weight<T, N, true>), dim3(ntask * N), dim3(PL * N), 0, st,
+            hipLaunchKernelGGL((k_plane_weight<T, N, true>), dim3(ntask * (N / 2 + 1)),
+                               dim3(PL * N), 0, st,
                                (const cplx<T>*)pl->acc, (const T*)pl->wtab,
                                (const cplx<T>*)pl->Fhat, (const cplx<T>*)pl->Fhat0,
                                (const T*)pl->bb2, fg_same, pl->Nrho, pl->QG, S, (cplx<T>*)pl->H);
         else
-            hipLaunchKernelGGL((k_plane_weight<T, N, false>), dim3(ntask * N), dim3(PL * N), 0,
+            hipLaunchKernelGGL((k_plane_weight<T, N, false>), dim3(ntask * (N / 2 + 1)),
+                               dim3(PL * N), 0,
                                st, (const cplx<T>*)pl->acc, (const T*)pl->wtab,
                                (const cplx<T>*)pl->Fhat, (const cplx<T>*)nullptr,
                                (const T*)pl->bb2, fg_same, pl->Nrho, pl->QG, S, (cplx<T>*)pl->H);
