@@ -86,19 +86,6 @@ k_proj_partial(int n, const T* __restrict__ Q, const T* __restrict__ cx,
         for (int i = 0; i < 5; ++i) part[(size_t(s) * nb + blockIdx.x) * 5 + i] = acc[i];
 }
 
-// multipliers lam = G (Phi Q) of slice s -> lam[s][8] (first 5 used)
-__global__ void __launch_bounds__(kRB)
-k_proj_final(int nb, Gram5 G, const double* __restrict__ part, double* __restrict__ lam) {
-    __shared__ double sh[5 * 4];
-    const int s = blockIdx.x;
-    double m[5];
-    reduce_partials<5>(part + size_t(s) * nb * 5, nb, m, sh);
-    if (threadIdx.x < 5) {
-        double a = 0.0;
-        for (int j = 0; j < 5; ++j) a += G.g[threadIdx.x * 5 + j] * m[j];
-        lam[size_t(s) * 8 + threadIdx.x] = a;
-    }
-}
 
 // mixture: per slice the two species' moments m0, m1 -> residuals of the
 // six constraints r = (m0[0], m1[0], M0 m0[1..3] + M1 m1[1..3],
@@ -136,6 +123,21 @@ k_proj_final_mix(int nb, Gram6 G, double M0, double M1, const double* __restrict
     }
 }
 
+// out[s][v] (+)= Q[s][v] - Phi(v)^T L for one point
+template <typename T>
+DGFS_DEV void proj_apply_point(int n, const T* __restrict__ Q, const T* __restrict__ cx,
+                               const T* __restrict__ cy, const T* __restrict__ cz,
+                               const double* L, T* __restrict__ out, int accumulate, int s,
+                               int v) {
+    const double x = cx[v], y = cy[v], z = cz[v];
+    const size_t o = size_t(s) * n + v;
+    const double qc = double(Q[o]) -
+                      (L[0] + L[1] * x + L[2] * y + L[3] * z + L[4] * (x * x + y * y + z * z));
+    if (accumulate) out[o] = T(double(out[o]) + qc);
+    else out[o] = T(qc);
+}
+
+// multipliers lam[s][8] (first 5 used) from memory (the mixture's)
 template <typename T>
 __global__ void __launch_bounds__(kRB)
 k_proj_apply(int n, const T* __restrict__ Q, const T* __restrict__ cx,
@@ -144,13 +146,29 @@ k_proj_apply(int n, const T* __restrict__ Q, const T* __restrict__ cx,
     const int s = blockIdx.y;
     const int v = blockIdx.x * kRB + threadIdx.x;
     if (v >= n) return;
-    const double* L = lam + size_t(s) * 8;
-    const double x = cx[v], y = cy[v], z = cz[v];
-    const size_t o = size_t(s) * n + v;
-    const double qc = double(Q[o]) -
-                      (L[0] + L[1] * x + L[2] * y + L[3] * z + L[4] * (x * x + y * y + z * z));
-    if (accumulate) out[o] = T(double(out[o]) + qc);
-    else out[o] = T(qc);
+    proj_apply_point<T>(n, Q, cx, cy, cz, lam + size_t(s) * 8, out, accumulate, s, v);
+}
+
+// lam = G (Phi Q) of slice s formed in every work-group: each sums the
+// slice's block partials in the same fixed order (no separate launch for
+// the 5 x 5 product)
+template <typename T>
+__global__ void __launch_bounds__(kRB)
+k_proj_apply_fused(int n, const T* __restrict__ Q, const T* __restrict__ cx,
+                   const T* __restrict__ cy, const T* __restrict__ cz, int nb, Gram5 G,
+                   const double* __restrict__ part, T* __restrict__ out, int accumulate) {
+    __shared__ double sh[5 * 4];
+    const int s = blockIdx.y;
+    double m[5], L[5];
+    reduce_partials<5>(part + size_t(s) * nb * 5, nb, m, sh);
+    for (int i = 0; i < 5; ++i) {
+        double a = 0.0;
+        for (int j = 0; j < 5; ++j) a += G.g[i * 5 + j] * m[j];
+        L[i] = a;
+    }
+    const int v = blockIdx.x * kRB + threadIdx.x;
+    if (v >= n) return;
+    proj_apply_point<T>(n, Q, cx, cy, cz, L, out, accumulate, s, v);
 }
 
 // ------------------------------------------------------------ Shakov model
@@ -1404,7 +1422,6 @@ int project_conservative_on(int dtype, int nslice, int Nv, const void* cx, const
     for (int i = 0; i < 25; ++i) G.g[i] = gram_inv[i];
     const int nb = (Nv + kRB - 1) / kRB;
     const dim3 grid(nb, nslice);
-    double* lam = work + size_t(nslice) * nb * 5;
     if (dtype == DGFS_F64)
         hipLaunchKernelGGL(k_proj_partial<double>, grid, dim3(kRB), 0, stream, Nv,
                            (const double*)Q, (const double*)cx, (const double*)cy,
@@ -1414,17 +1431,16 @@ int project_conservative_on(int dtype, int nslice, int Nv, const void* cx, const
                            (const float*)Q, (const float*)cx, (const float*)cy,
                            (const float*)cz, work);
     DGFS_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_proj_final, dim3(nslice), dim3(kRB), 0, stream, nb, G,
-                       (const double*)work, lam);
-    DGFS_HIP_CHECK(hipGetLastError());
     if (dtype == DGFS_F64)
-        hipLaunchKernelGGL(k_proj_apply<double>, grid, dim3(kRB), 0, stream, Nv,
+        hipLaunchKernelGGL(k_proj_apply_fused<double>, grid, dim3(kRB), 0, stream, Nv,
                            (const double*)Q, (const double*)cx, (const double*)cy,
-                           (const double*)cz, (const double*)lam, (double*)out, accumulate);
+                           (const double*)cz, nb, G, (const double*)work, (double*)out,
+                           accumulate);
     else
-        hipLaunchKernelGGL(k_proj_apply<float>, grid, dim3(kRB), 0, stream, Nv,
+        hipLaunchKernelGGL(k_proj_apply_fused<float>, grid, dim3(kRB), 0, stream, Nv,
                            (const float*)Q, (const float*)cx, (const float*)cy,
-                           (const float*)cz, (const double*)lam, (float*)out, accumulate);
+                           (const float*)cz, nb, G, (const double*)work, (float*)out,
+                           accumulate);
     DGFS_HIP_CHECK(hipGetLastError());
     return 0;
 }

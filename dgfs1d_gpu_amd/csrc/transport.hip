@@ -200,20 +200,6 @@ k_bc_partial(AdvDev a, const T* __restrict__ u, const T* __restrict__ cx, const 
         for (int i = 0; i < 4; ++i) part[blockIdx.x * 4 + i] = acc[i];
 }
 
-// nw[s] = -num/den (ordered sum of the block partials), single block
-__global__ void __launch_bounds__(kBlk)
-k_bc_final(int nblk, const double* __restrict__ part, double* __restrict__ nw) {
-    __shared__ double sh[4 * kBlk];
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < nblk; b += kBlk)
-        for (int i = 0; i < 4; ++i) acc[i] += part[b * 4 + i];
-    block_reduce<4>(acc, sh);
-    if (threadIdx.x == 0) {
-        nw[0] = acc[1] != 0.0 ? -acc[0] / acc[1] : 0.0;
-        nw[1] = acc[3] != 0.0 ? -acc[2] / acc[3] : 0.0;
-    }
-}
-
 template <typename T>
 DGFS_DEV T ghost_value(const WallDev& w, double nw, T ul, int v, T cx, T cy, T cz) {
     const double fac = (double(cx) - w.ushift) * w.nl;
@@ -296,16 +282,34 @@ DGFS_DEV void advect_point(const AdvDev& a, const T* __restrict__ u, const T* __
     }
 }
 
+// nbw > 0 (a diffuse wall): every work-group forms the wall densities
+// nw[s] = -num_s / den_s from the nbw block partials of k_bc_partial (an
+// ordered sum, so the same values in every work-group), and work-group 0
+// stores them in nw
 template <typename T>
 __global__ void __launch_bounds__(kBlk)
 k_advect(AdvDev a, const T* __restrict__ u, const T* __restrict__ cx, const T* __restrict__ cy,
-         const T* __restrict__ cz, WallDev wl, WallDev wr, const double* __restrict__ nw,
-         T* __restrict__ ux) {
+         const T* __restrict__ cz, WallDev wl, WallDev wr, int nbw,
+         const double* __restrict__ part, double* __restrict__ nw, T* __restrict__ ux) {
+    double n0 = 0.0, n1 = 0.0;
+    if (nbw > 0) {
+        __shared__ double sh[4 * kBlk];
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int b = threadIdx.x; b < nbw; b += kBlk)
+            for (int i = 0; i < 4; ++i) acc[i] += part[b * 4 + i];
+        block_reduce<4>(acc, sh);
+        n0 = acc[1] != 0.0 ? -acc[0] / acc[1] : 0.0;
+        n1 = acc[3] != 0.0 ? -acc[2] / acc[3] : 0.0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            nw[0] = n0;
+            nw[1] = n1;
+        }
+    }
     const long gid = long(blockIdx.x) * kBlk + threadIdx.x;
     if (gid >= long(a.Ne) * a.n) return;
     const int e = int(gid / a.n);
     const int v = int(gid - long(e) * a.n);
-    advect_point<T>(a, u, cx, cy, cz, wl, wr, nw[0], nw[1], ux, e, v);
+    advect_point<T>(a, u, cx, cy, cz, wl, wr, n0, n1, ux, e, v);
 }
 
 // ------------------------------------------------------- batched transport
@@ -519,13 +523,12 @@ static int transport_rhs_t(const ::dgfs_transport_t* tp, const void* u, void* ux
                            (const T*)tp->cx, (const T*)tp->cy, (const T*)tp->cz, w[0], w[1],
                            tp->work);
         DGFS_HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_bc_final, dim3(1), dim3(kBlk), 0, st, nb, tp->work, tp->nw);
-        DGFS_HIP_CHECK(hipGetLastError());
     }
+    // the wall densities are formed inside k_advect from the partials
     const long tot = long(tp->Ne) * tp->Nv;
     hipLaunchKernelGGL((k_advect<T>), dim3((tot + kBlk - 1) / kBlk), dim3(kBlk), 0, st, a,
                        (const T*)u, (const T*)tp->cx, (const T*)tp->cy, (const T*)tp->cz, w[0],
-                       w[1], tp->nw, (T*)ux);
+                       w[1], need ? nb : 0, (const double*)tp->work, tp->nw, (T*)ux);
     DGFS_HIP_CHECK(hipGetLastError());
     return 0;
 }
