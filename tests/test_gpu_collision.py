@@ -66,7 +66,7 @@ BILINEAR = [(0, [(0, 1.0)], [(1, 1.0)]), (1, [(2, 0.5), (3, -0.25)], [(1, 1.0), 
             (1, [(3, 1.0)], [(2, -1.0)]), (3, [(1, 0.7)], [(1, 0.7), (2, 0.1)])]
 
 
-@pytest.mark.parametrize('N,M,Nrho', [(8, 6, 4), (16, 6, 8), (32, 6, 4)])
+@pytest.mark.parametrize('N,M,Nrho', [(4, 6, 4), (6, 6, 4), (8, 6, 4), (16, 6, 8), (32, 6, 4)])
 @pytest.mark.parametrize('variant', [0, 1])
 def test_collision_fp64_nodal(cuda, N, M, Nrho, variant):
     from dgfs1d_gpu_amd import _lib
@@ -83,7 +83,7 @@ def test_collision_fp64_bilinear_grouped(cuda, N, M, Nrho):
     assert err < 1e-11
 
 
-@pytest.mark.parametrize('N,M,Nrho', [(16, 6, 8), (24, 6, 6), (32, 6, 8)])
+@pytest.mark.parametrize('N,M,Nrho', [(8, 6, 4), (12, 6, 4), (16, 6, 8), (24, 6, 6), (32, 6, 8)])
 def test_collision_fp32(cuda, N, M, Nrho):
     from dgfs1d_gpu_amd import _lib
     err = _run(cuda, N, M, Nrho, _lib.F32, _lib.VHS, NODAL)
