@@ -36,9 +36,9 @@ PEAK_HBM_GBS = 8000.0
 # FETCH x 2.000, WRITE x 1.000.  Keyed by workload; re-measured whenever the
 # kernel changes, null for workloads not measured.
 GAIN_TRAFFIC_KIB = {
-    'couette Ne=4 K=3 N=32^3 M=12 Nrho=16 ssp-rk2 dt=0.0005': (30959.2, 2.000, 52224.0, 1.000),
+    'couette Ne=4 K=3 N=32^3 M=12 Nrho=16 ssp-rk2 dt=0.0005': (30957.8, 2.000, 52224.0, 1.000),
     'couette Ne=4 K=3 N=32^3 M=12 Nrho=16 ssp-rk2 dt=0.0005 reference-state R=40':
-        (620826.2, 2.000, 52224.0, 1.000),
+        (629353.4, 2.000, 70656.0, 1.000),
 }
 
 

@@ -509,7 +509,7 @@ k_gain(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
 // 512 threads of at most 256 VGPRs, two waves per SIMD instead of half a
 // wave.  Lane h of pencil pair P holds the a1-outputs (and acc entries)
 // o1 = (N/2) h + k.
-constexpr int kPairDeltaChunk = 4;   // stored pairs in flight per lane (register budget)
+constexpr int kPairDeltaChunk = 8;   // stored pairs in flight per lane (register budget)
 
 template <typename T, int N, int S, int MODE = kGainAccumulate, int PAD = 1>
 __global__ void __launch_bounds__(2 * N * (N / S))
