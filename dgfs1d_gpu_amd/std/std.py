@@ -152,7 +152,8 @@ class DGFSStd1D:
             self.moments(self.dt, self.time, self.u)
         if self.distribution is not None:
             self.distribution(self.dt, self.time, self.u)
-        self.uprev.copy_(self.u)
+        if self.residual is not None:       # the previous state serves the residual only
+            self.uprev.copy_(self.u)
 
     def run(self, nsteps=None):
         """nsteps=None: the reference loop ``while time < tend`` (std.py:396-404,
