@@ -293,17 +293,27 @@ k_advect(AdvDev a, const T* __restrict__ u, const T* __restrict__ cx, const T* _
          const double* __restrict__ part, double* __restrict__ nw, T* __restrict__ ux) {
     double n0 = 0.0, n1 = 0.0;
     if (nbw > 0) {
-        __shared__ double sh[4 * kBlk];
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int b = threadIdx.x; b < nbw; b += kBlk)
-            for (int i = 0; i < 4; ++i) acc[i] += part[b * 4 + i];
-        block_reduce<4>(acc, sh);
-        n0 = acc[1] != 0.0 ? -acc[0] / acc[1] : 0.0;
-        n1 = acc[3] != 0.0 ? -acc[2] / acc[3] : 0.0;
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            nw[0] = n0;
-            nw[1] = n1;
+        // the first wave sums the partials (a fixed shuffle tree) and hands
+        // the two densities to the work-group through LDS
+        __shared__ double nws[2];
+        if (threadIdx.x < 64) {
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int b = threadIdx.x; b < nbw; b += 64)
+                for (int i = 0; i < 4; ++i) acc[i] += part[b * 4 + i];
+            for (int o = 32; o > 0; o >>= 1)
+                for (int i = 0; i < 4; ++i) acc[i] += __shfl_xor(acc[i], o);
+            if (threadIdx.x == 0) {
+                nws[0] = acc[1] != 0.0 ? -acc[0] / acc[1] : 0.0;
+                nws[1] = acc[3] != 0.0 ? -acc[2] / acc[3] : 0.0;
+                if (blockIdx.x == 0) {
+                    nw[0] = nws[0];
+                    nw[1] = nws[1];
+                }
+            }
         }
+        __syncthreads();
+        n0 = nws[0];
+        n1 = nws[1];
     }
     const long gid = long(blockIdx.x) * kBlk + threadIdx.x;
     if (gid >= long(a.Ne) * a.n) return;
