@@ -335,7 +335,7 @@ struct Fft3Pair : Fft3Engine<T, N_, S_, 2 * N_ * (N_ / S_), DIR_, PAD> {
     // barrier between them, so that at most S CK loads are in flight per thread.
     template <int R, class Map1, class Load>
     static DGFS_DEV void pass1(cplx<T>* V, Map1&& map1, Load&& load) {
-        constexpr int CK = NS % 4 == 0 ? 4 : 1;
+        constexpr int CK = std::is_same<T, float>::value && NS % 8 == 0 ? 8 : NS % 4 == 0 ? 4 : 1;
         for (int p = threadIdx.x; p < N * N; p += NT) {
             int i1, i2;
             map1(p, i1, i2);
