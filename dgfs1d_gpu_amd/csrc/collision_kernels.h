@@ -326,16 +326,14 @@ DGFS_DEV UPair<float> buf_load_pair_nt(__amdgpu_buffer_rsrc_t r, int voff, int s
     return __builtin_bit_cast(UPair<float>,
                               __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, kBufNT));
 }
-// A pair is written as two 8-byte stores.  As one 16-byte buffer store with an
-// SGPR offset, the compiler let the next VALU instruction overwrite the store's
-// data VGPRs without a wait state, and the stored values then differed from run
-// to run; stores of at most 8 bytes have no such data hazard.
-DGFS_DEV void buf_store_pair_nt(__amdgpu_buffer_rsrc_t r, int voff, int soff,
-                                const UPair<float>& u) {
-    using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, u.p), r, voff, soff, kBufNT);
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, u.m), r, voff + 8, soff,
-                                          kBufNT);
+// A pair is one 16-byte store with its whole offset in the VGPR and a zero
+// SGPR offset.  The compiler's wait-state check for the store-data hazard of
+// stores wider than 8 bytes skips buffer stores whose SGPR offset is a
+// register: with one, the next VALU instruction overwrote the store's data
+// VGPRs at once and the stored values differed from run to run.
+DGFS_DEV void buf_store_pair_nt(__amdgpu_buffer_rsrc_t r, int voff, const UPair<float>& u) {
+    using V4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, u), r, voff, 0, kBufNT);
 }
 
 enum GainMode { kGainAccumulate = 0, kGainStore = 1, kGainDelta = 2 };
@@ -622,7 +620,7 @@ k_gain_pair(const cplx<T>* __restrict__ Fhat, const cplx<T>* __restrict__ phase,
                     sfor<0, H>([&](auto K) {
                         UPair<T> v;
                         unpack(K, o[K], v.p, v.m);
-                        buf_store_pair_nt(Ub, lane_off, int(K) * S * E::NT * PB, v);
+                        buf_store_pair_nt(Ub, lane_off + int(K) * S * E::NT * PB, v);
                     });
                 } else if constexpr (MODE == kGainDelta) {
                     constexpr int CH = kPairDeltaChunk;
